@@ -1,0 +1,174 @@
+"""Benchmark of the MLS-MPM substep on one MI355X.
+
+One "step" is one reference frame: execute() = 2 substeps
+(mls-mpm/mls-mpm.ts:245-268).  The workload is the 300,000-particle dam break
+on an 80^3 grid: the timed region replays its frames from the initial state,
+resizing the box to 64 cells in x at frame 500 (when the run reaches it).
+Inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line with the throughput (particle-substeps/s), the roofline
+of the dominant kernel (algorithmic bytes / HIP-event kernel time) and the
+CPU baseline (the oracle's OpenMP restatement on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "webgpu-ocean_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/s @300k MLS-MPM, % HBM peak; batched & adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+SUBSTEPS_PER_FRAME = 2
+RESIZE_FRAME = 500
+
+
+def stage_bytes(name: str, n: int, g: int) -> int:
+    """Algorithmic HBM bytes of one launch of each kernel.
+
+    Particle fields: x 12 B, v 12 B, C 36 B; cells 16 B = (vx, vy, vz, mass).
+    These are the minimum bytes the data dependencies force per kernel, the
+    per-unit figures of the substep total 204 B/particle + 84 B/cell.
+    """
+    table = {
+        "clear_grid": 16 * g,                     # write 4 x i32
+        "p2g_1": 60 * n + 16 * g,                 # read x, v, C; write cells
+        "p2g_2": 48 * n + 28 * g,                 # read x, C; mass read + v RMW
+        "update_grid": 28 * g,                    # read 16, write 12
+        "g2p": (12 + 60) * n + 12 * g,            # read x; write x, v, C; gather v
+        "copy_position": 48 * n,                  # read x, v; write PosVel (24 useful)
+        "g2p_fused": (12 + 60 + 24) * n + 16 * g,  # g2p + grid update + PosVel
+    }
+    return table[name]
+
+
+def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
+    """Oracle (CPU restatement of the reference WGSL) timed on the host cores."""
+    import oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sc = oracle.Scene(rec, box)
+    t0 = time.perf_counter()
+    sc.substep(threads=threads)
+    one = time.perf_counter() - t0
+    k = max(2, min(400, int(seconds / max(one, 1e-4))))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        sc.substep(threads=threads)
+    dt = time.perf_counter() - t0
+    n = rec.shape[0]
+    return {
+        "value": n * k / dt,
+        "unit": "particle-substeps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{k} substeps of the same {n}-particle 80^3 scene "
+                  f"({dt:.1f} s, OpenMP over particles, atomic i32 P2G)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000, help="timed frames (2 substeps each)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=300_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-substeps", type=int, default=200)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
+              "supported (scenes would shard by x-slab with halo exchange; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    from mlsmpm import MLSMPMSimulator, scenes
+
+    rec, box = scenes.config1(n=args.particles)
+    n = rec.shape[0]
+    sim = MLSMPMSimulator(particles=rec, init_box_size=box)
+    g = sim.grid_count
+
+    # warmup on the same scene, then restart from the initial state
+    for _ in range(args.warmup):
+        sim.execute()
+    sim.synchronize()
+    sim.reset(n, box, particles=rec)
+    sim.synchronize()
+
+    t0 = time.perf_counter()
+    for f in range(args.steps):
+        if f == RESIZE_FRAME:
+            sim.change_box_size([64.0, box[1], box[2]])
+        sim.execute()
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    substeps = args.steps * SUBSTEPS_PER_FRAME
+    value = n * substeps / elapsed
+
+    # per-kernel HIP-event timing of the same path on the same stream
+    sim.reset(n, box, particles=rec)
+    stage_ms = sim.time_stages(args.stage_substeps)
+    avg_us = {k: 1e3 * v / args.stage_substeps for k, v in stage_ms.items()}
+    dom = max(avg_us, key=avg_us.get)
+    dom_bytes = stage_bytes(dom, n, g)
+    achieved = dom_bytes / (avg_us[dom] * 1e-6) / 1e9
+    substep_us = sum(avg_us.values())
+    total_bytes = 204 * n + 84 * g
+    sim.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (i32 fixed-point grid)",
+        "data": "synthetic",
+        "config": {
+            "workload": "MLS-MPM dam break, 300k particles, 80^3 grid, 2 substeps/frame, "
+                        "box x resized to 64 at frame 500",
+            "particles": n,
+            "grid": [int(b) for b in box],
+            "grid_cells": g,
+            "substeps_per_step": SUBSTEPS_PER_FRAME,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "avg_launch_us": avg_us[dom],
+        },
+        "stages_us": avg_us,
+        "substep_roofline": {
+            "bytes": total_bytes,
+            "sum_kernel_us": substep_us,
+            "achieved_GBs": total_bytes / (substep_us * 1e-6) / 1e9,
+            "wall_us_per_substep": 1e6 * elapsed / substeps,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(rec, box)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

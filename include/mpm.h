@@ -1,0 +1,141 @@
+/*
+ * mpm.h -- C ABI of libmpm355, an MI355X (gfx950) implementation of the
+ * MLS-MPM fluid substep of WebGPU-Ocean (mls-mpm/mls-mpm.ts).
+ *
+ * The ABI is what a Node N-API addon (or any FFI) binds to replace the
+ * reference's MLSMPMSimulator class.  Plain pointers and sizes only; every
+ * function returns an mpm_status (0 = OK) and never throws or aborts across
+ * the boundary.  mpm_last_error() returns a message for the calling thread.
+ *
+ * Threading: one simulator handle per host thread; no internal locking.  All
+ * device work of a handle is ordered on that handle's HIP stream.
+ *
+ * Layouts (identical to the reference's GPU buffers):
+ *   particle  80 B  position f32[3] @0, v f32[3] @16, C f32 3x3 column-major @32,
+ *                   each column xyz + 4 B pad     (mls-mpm/p2g_1.wgsl:1-5)
+ *   cell      16 B  vx, vy, vz, mass as i32 fixed point (mls-mpm/clearGrid.wgsl:1-6)
+ *   posvel    32 B  position f32[3] @0, v f32[3] @16    (mls-mpm/copyPosition.wgsl:7-10)
+ */
+#ifndef MPM355_H
+#define MPM355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+
+#define MPM_PARTICLE_STRUCT_SIZE 80 /* mlsmpmParticleStructSize, mls-mpm/mls-mpm.ts:10 */
+#define MPM_CELL_STRUCT_SIZE 16     /* cellStructSize, mls-mpm/mls-mpm.ts:16 */
+#define MPM_POSVEL_STRUCT_SIZE 32   /* main.ts:117 */
+
+typedef enum {
+    MPM_OK = 0,
+    MPM_ERR_INVALID_ARGUMENT = 1,
+    MPM_ERR_GRID_TOO_LARGE = 2, /* counterpart of the throw at mls-mpm/mls-mpm.ts:230-232 */
+    MPM_ERR_OUT_OF_MEMORY = 3,
+    MPM_ERR_DEVICE = 4,         /* a HIP runtime call failed */
+    MPM_ERR_NO_DEVICE = 5,
+} mpm_status;
+
+/* The reference's pipeline-override constants (mls-mpm/mls-mpm.ts:53-59). */
+typedef struct {
+    float stiffness;              /* 3.0  */
+    float rest_density;           /* 4.0  */
+    float dynamic_viscosity;      /* 0.1  */
+    float dt;                     /* 0.2  */
+    float fixed_point_multiplier; /* 1e7  */
+} mpm_params;
+
+/* The six reference kernels, in dispatch order (mls-mpm/mls-mpm.ts:248-265). */
+typedef enum {
+    MPM_STAGE_CLEAR_GRID = 0,   /* mls-mpm/clearGrid.wgsl   */
+    MPM_STAGE_P2G_1 = 1,        /* mls-mpm/p2g_1.wgsl       */
+    MPM_STAGE_P2G_2 = 2,        /* mls-mpm/p2g_2.wgsl       */
+    MPM_STAGE_UPDATE_GRID = 3,  /* mls-mpm/updateGrid.wgsl  */
+    MPM_STAGE_G2P = 4,          /* mls-mpm/g2p.wgsl         */
+    MPM_STAGE_COPY_POSITION = 5,/* mls-mpm/copyPosition.wgsl */
+    MPM_STAGE_COUNT = 6
+} mpm_stage;
+
+typedef struct mpm_sim mpm_sim;
+
+/* Library version (MPM_ABI_VERSION). */
+int mpm_abi_version(void);
+
+/* Message describing the calling thread's most recent error ("" if none). */
+const char *mpm_last_error(void);
+
+/* Fills *out with the reference constants of mls-mpm/mls-mpm.ts:53-59. */
+int mpm_default_params(mpm_params *out);
+
+/* Construction from a particle buffer, box and parameters.
+ * Replaces `new MLSMPMSimulator(particleBuffer, posvelBuffer, renderDiameter,
+ * device)` (mls-mpm/mls-mpm.ts:42-193) followed by `reset` (:225-243):
+ * particles = n records of 80 B on the host (may be NULL when n == 0);
+ * init_box = the reference's initBoxSize; the real box starts equal to it.
+ * params == NULL selects mpm_default_params.  Grid cells =
+ * ceil(bx)*ceil(by)*ceil(bz) (:229); MPM_ERR_GRID_TOO_LARGE replaces the
+ * 64^3 cap of :230-232 with the limit of 32-bit cell indices. */
+int mpm_create(const mpm_params *params, const void *particles, uint32_t n,
+               const float init_box[3], mpm_sim **out);
+
+/* Releases the handle and its device memory. NULL is accepted. */
+int mpm_destroy(mpm_sim *sim);
+
+/* reset(numParticles, initBoxSize) (mls-mpm/mls-mpm.ts:225-243) with the
+ * particle data supplied by the caller (the host side builds it, as
+ * initDambreak does at :195-223).  Sets real box = init box. */
+int mpm_reset(mpm_sim *sim, const void *particles, uint32_t n, const float init_box[3]);
+
+/* changeBoxSize(realBoxSize) (mls-mpm/mls-mpm.ts:270-275): ordered on the
+ * handle's stream after previously issued steps.  Each component must be
+ * finite, >= 1 and <= the init box. */
+int mpm_set_box(mpm_sim *sim, const float real_box[3]);
+
+/* Advances `substeps` substeps; execute(commandEncoder) (mls-mpm.ts:245-268)
+ * is substeps == 2.  Asynchronous on the handle's stream. */
+int mpm_step(mpm_sim *sim, uint32_t substeps);
+
+/* Runs one reference kernel exactly as dispatched at mls-mpm.ts:248-265
+ * (test hook for per-stage bit-exact comparison).  Asynchronous. */
+int mpm_run_stage(mpm_sim *sim, int stage);
+
+/* Waits for all work issued on the handle. */
+int mpm_synchronize(mpm_sim *sim);
+
+/* Position readout: n records of 32-B PosVel (the output of copyPosition,
+ * copyPosition.wgsl:15-21) into host memory.  Synchronizes. */
+int mpm_read_posvel(mpm_sim *sim, void *dst, uint32_t n);
+
+/* Device address of the N x 32-B PosVel buffer, for consumers that read it
+ * on the GPU as the reference's renderer does (render/sphere.wgsl:28-33). */
+int mpm_posvel_device_ptr(mpm_sim *sim, void **out);
+
+/* Particle state in the reference's 80-B layout (read synchronizes). */
+int mpm_read_particles(mpm_sim *sim, void *dst, uint32_t n);
+int mpm_write_particles(mpm_sim *sim, const void *src, uint32_t n);
+
+/* Grid accumulators, ncells x (vx, vy, vz, mass) i32 (read synchronizes). */
+int mpm_read_grid(mpm_sim *sim, int32_t *dst, uint64_t ncells);
+int mpm_write_grid(mpm_sim *sim, const int32_t *src, uint64_t ncells);
+
+/* numParticles and gridCount (mls-mpm/mls-mpm.ts:19-20). */
+int mpm_get_counts(mpm_sim *sim, uint32_t *num_particles, uint64_t *grid_count);
+
+/* Timing hooks (HIP events on the handle's stream, after a synchronize).
+ * mpm_time_steps: runs `frames` x `substeps_per_frame` substeps; *ms = total.
+ * mpm_time_stages: runs `substeps` substeps with events around every kernel
+ * of the production path; stage_ms[i] = summed duration of kernel slot i,
+ * names[i] (may be NULL) receives a static name; *count = slots used. */
+int mpm_time_steps(mpm_sim *sim, uint32_t frames, uint32_t substeps_per_frame, float *ms);
+int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, const char **names,
+                    int capacity, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+"""The C-ABI library loads and exports every entry point include/mpm.h
+declares; without a GPU the library reports MPM_ERR_NO_DEVICE instead of
+falling back to anything.  No compute runs here."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import mlsmpm
+from mlsmpm import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mpm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpm_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(_capi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    L = mlsmpm.load()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", mlsmpm.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(mpm_\w+)", out))
+    assert set(declared_functions()) <= exported
+
+
+def test_abi_version_and_default_params():
+    L = mlsmpm.load()
+    assert L.mpm_abi_version() == 1
+    p = mlsmpm.Params()
+    assert L.mpm_default_params(ctypes.byref(p)) == 0
+    assert (p.stiffness, p.rest_density, p.dt, p.fixed_point_multiplier) == (3.0, 4.0,
+                                                                             ctypes.c_float(0.2).value, 1e7)
+    assert abs(p.dynamic_viscosity - 0.1) < 1e-7
+
+
+def test_null_arguments_are_rejected_with_status_not_crash():
+    L = mlsmpm.load()
+    assert L.mpm_create(None, None, 0, _capi.f3([8, 8, 8]), None) == 1
+    assert b"NULL" in L.mpm_last_error()
+    assert L.mpm_step(None, 2) == 1
+    assert L.mpm_destroy(None) == 0
+    assert L.mpm_default_params(None) == 1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_no_device_is_reported_not_faked():
+    with pytest.raises(mlsmpm.MPMError) as ei:
+        mlsmpm.MLSMPMSimulator(init_box_size=(8, 8, 8))
+    assert ei.value.status == 5

@@ -1,0 +1,324 @@
+"""Pins the CPU oracle (oracle/mpm_oracle.c) without the reference running:
+closed forms of the quadratic B-spline, known-answer integers of the
+fixed-point encode, MLS-MPM invariants, and brute-force hand cases.
+
+The reference ships no tests or golden vectors (package.json:3-6), so these
+checks are what fix the oracle's arithmetic.  Each is chosen so that a
+plausible slip (a dropped term, a wrong sign or index, transposed C, a
+multiply instead of the WGSL division) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from mlsmpm import scenes
+
+MULT = 1e7
+f32 = np.float32
+
+
+# --- scalar helpers -------------------------------------------------------
+
+def test_encode_truncates_toward_zero_and_saturates():
+    assert oracle.encode(0.12345678) == 1234567         # 1234567.8 -> trunc
+    assert oracle.encode(-0.12345678) == -1234567       # toward zero, not floor
+    assert oracle.encode(1e10) == 2**31 - 1             # saturating, not UB
+    assert oracle.encode(-1e10) == -(2**31)
+    assert oracle.encode(float("nan")) == 0
+    assert oracle.encode(float("inf")) == 2**31 - 1
+    assert oracle.f32_to_i32(-0.99) == 0
+
+
+def test_decode_is_a_division_not_a_reciprocal_multiply():
+    # f32(i) / 1e7 and f32(i) * f32(1e-7) differ for these integers; the
+    # WGSL decodeFixedPoint divides (mls-mpm/p2g_2.wgsl:22-24).
+    ints = np.arange(1, 200000, 7, dtype=np.int64)
+    want = (ints.astype(f32) / f32(MULT)).astype(f32)
+    got = np.array([oracle.decode(int(i)) for i in ints[:2000]], dtype=f32)
+    assert np.array_equal(got, want[:2000])
+    mul = (ints.astype(f32) * f32(1e-7)).astype(f32)
+    assert np.any(mul != want), "test must separate division from multiply"
+
+
+def test_pow5_close_to_exact_fifth_power():
+    for r in [0.0, 0.2, 0.5, 1.0, 1.5, 2.0, 3.7]:
+        got = oracle.pow5(r)
+        want = float(f32(r)) ** 5
+        assert abs(got - want) <= 4 * abs(np.spacing(f32(want))) + 1e-45
+
+
+# --- quadratic B-spline closed forms ---------------------------------------
+
+def test_weights_at_cell_centre_are_exact():
+    for x in [0.5, 7.5, 31.5]:
+        assert list(oracle.weights(x)) == [0.125, 0.75, 0.125]
+
+
+def test_weights_partition_of_unity_and_moments():
+    rng = np.random.default_rng(0)
+    for x in rng.uniform(1.0, 60.0, size=500):
+        x = float(f32(x))
+        w = oracle.weights(x).astype(np.float64)
+        base = math.floor(x)
+        nodes = np.array([base - 1 + 0.5, base + 0.5, base + 1 + 0.5])
+        d = nodes - x
+        assert abs(w.sum() - 1.0) < 1e-6                  # sum w = 1
+        assert abs((w * d).sum()) < 1e-6                  # sum w d = 0
+        assert abs((w * d * d).sum() - 0.25) < 1e-6       # sum w d^2 = 1/4
+
+
+# --- p2g_1 -----------------------------------------------------------------
+
+def one_particle(pos, v=(0, 0, 0), C=None):
+    rec = scenes.records(np.array([pos], dtype=np.float64), np.array([v], dtype=np.float64))
+    if C is not None:
+        C = np.asarray(C, dtype=np.float32)  # C[col][row]
+        for c in range(3):
+            rec[0, 8 + 4 * c:8 + 4 * c + 3] = C[c]
+    return rec
+
+
+def test_p2g1_single_particle_at_cell_centre_known_integers():
+    box = [8.0, 8.0, 8.0]
+    sc = oracle.Scene(one_particle((3.5, 4.5, 2.5)), box)
+    sc.run_stage(0)
+    sc.run_stage(1)
+    m = sc.cells[:, 3].reshape(8, 8, 8)
+    # per-axis weights (0.125, 0.75, 0.125) -> products x 1e7, truncated
+    assert m[3, 4, 2] == 4218750            # 0.75^3
+    assert m[2, 4, 2] == m[3, 3, 2] == m[3, 4, 1] == 703125     # 0.125*0.75^2
+    assert m[2, 3, 2] == m[4, 4, 3] == 117187                   # 0.125^2*0.75 (117187.5)
+    assert m[2, 3, 1] == m[4, 5, 3] == 19531                    # 0.125^3 (19531.25)
+    assert m.sum() == 9999992
+    assert np.count_nonzero(m) == 27
+    assert np.all(sc.cells[:, :3] == 0)     # v = 0, C = 0: no momentum
+
+
+def test_p2g1_index_uses_init_box_strides():
+    # x-major index x*Y*Z + y*Z + z with Y, Z = i32(init_box.y/z) (p2g_1.wgsl:53-56)
+    box = [5.0, 6.0, 7.0]
+    sc = oracle.Scene(one_particle((2.5, 3.5, 4.5)), box)
+    sc.run_stage(1)
+    centre = 2 * 6 * 7 + 3 * 7 + 4
+    assert sc.cells[centre, 3] == 4218750
+    assert sc.cells[centre + 1, 3] == 703125      # z+1
+    assert sc.cells[centre + 7, 3] == 703125      # y+1
+    assert sc.cells[centre + 42, 3] == 703125     # x+1
+
+
+def test_p2g1_velocity_and_apic_momentum():
+    box = [8.0, 8.0, 8.0]
+    v = (2.0, -1.0, 0.5)       # exact binary fractions: w*v is exact
+    sc = oracle.Scene(one_particle((3.5, 4.5, 2.5), v), box)
+    sc.run_stage(1)
+    c = sc.cells.reshape(8, 8, 8, 4)
+    assert c[3, 4, 2, 0] == 8437500 and c[3, 4, 2, 1] == -4218750 and c[3, 4, 2, 2] == 2109375
+    # APIC: node momentum w*(v + C d); C = [[a,0,0],[0,0,0],[0,0,0]] as columns
+    # gives vx contribution w*a*dx, so +x and -x neighbours get opposite signs.
+    a = 0.5
+    sc2 = oracle.Scene(one_particle((3.5, 4.5, 2.5), (0, 0, 0), C=[[a, 0, 0], [0, 0, 0], [0, 0, 0]]), box)
+    sc2.run_stage(1)
+    c2 = sc2.cells.reshape(8, 8, 8, 4)
+    assert c2[4, 4, 2, 0] == 703125 // 2 and c2[2, 4, 2, 0] == -(703125 // 2)
+    assert np.all(c2[..., 1:3] == 0)
+    # C column 1 (d.y multiplier) feeds the y-neighbours, not x: catches a transpose
+    sc3 = oracle.Scene(one_particle((3.5, 4.5, 2.5), (0, 0, 0), C=[[0, 0, 0], [a, 0, 0], [0, 0, 0]]), box)
+    sc3.run_stage(1)
+    c3 = sc3.cells.reshape(8, 8, 8, 4)
+    assert c3[3, 5, 2, 0] == 703125 // 2 and c3[4, 4, 2, 0] == 0
+
+
+def test_p2g1_mass_and_momentum_conservation():
+    rng = np.random.default_rng(3)
+    n, box = 500, [24.0, 24.0, 24.0]
+    rec = scenes.small_scene(n, box, seed=3, velocity=1.0, affine=0.3)
+    sc = oracle.Scene(rec, box)
+    sc.run_stage(1)
+    total_mass = int(sc.cells[:, 3].astype(np.int64).sum())
+    assert n * MULT - 27 * n <= total_mass <= n * MULT
+    # APIC momentum conservation: sum_i w_i (v + C d_i) = v (first moment is 0)
+    mom = sc.cells[:, :3].astype(np.int64).sum(axis=0) / MULT
+    vsum = rec[:, 4:7].astype(np.float64).sum(axis=0)
+    assert np.all(np.abs(mom - vsum) < 27 * n * 3 / MULT + 1e-3 * n)
+
+
+# --- p2g_2 -----------------------------------------------------------------
+
+def test_p2g2_single_particle_at_rest_has_zero_stress():
+    box = [8.0, 8.0, 8.0]
+    sc = oracle.Scene(one_particle((3.5, 4.5, 2.5)), box)
+    sc.run_stage(1)
+    before = sc.cells.copy()
+    sc.run_stage(2)
+    # density = sum w*trunc(w*1e7)/1e7 ~ 0.59375^3 < rest density -> p = max(-0, <0) = 0
+    assert np.array_equal(sc.cells, before)
+
+
+def test_p2g2_uniform_lattice_density_closed_form():
+    # Lattice of spacing 1/2 aligned with the nodes: per axis sum of weights
+    # over particles = N(0) + 2N(.5) + 2N(1) = 0.75 + 1 + 0.25 = 2, so every
+    # interior node receives mass 8 and an interior particle's density is 8:
+    # p = 3*((8/4)^5 - 1) = 93.  One particle with velocity 0 and C = 0 then
+    # receives stress momentum M*w*d with M = -(1/8)*4*(-93 I)*0.2.
+    box = [16.0, 16.0, 16.0]
+    ax = np.arange(4.5, 11.5 + 1e-9, 0.5)
+    xs, ys, zs = np.meshgrid(ax, ax, ax, indexing="ij")
+    pos = np.stack([xs.ravel(), ys.ravel(), zs.ravel()], axis=1)
+    sc = oracle.Scene(scenes.records(pos), box)
+    sc.run_stage(1)
+    m = sc.cells[:, 3].reshape(16, 16, 16)
+    assert abs(m[8, 8, 8] - 8e7) <= 8 * 27       # truncation only
+    sc.run_stage(2)
+    # momentum added at node (9,8,8) by the particle at (8.5,8.5,8.5) etc. is
+    # antisymmetric: the lattice-interior momentum sums to ~0 by symmetry
+    v = sc.cells[:, :3].reshape(16, 16, 16, 3).astype(np.int64)
+    assert abs(int(v[8, 8, 8].sum())) <= 27 * 64 * 3
+    # the outward force on the block's faces is positive pressure
+    assert v[11, 8, 8, 0] > 0 and v[4, 8, 8, 0] < 0
+
+
+def test_p2g2_stress_formula_single_cell_hand_value():
+    # Pre-filled uniform grid mass 8 (x1e7) around one particle at a node:
+    # density = 8 exactly (sum w = 1 in f32 at the centre), pressure 93,
+    # volume 1/8; M = (-(1/8)*4) * (-93) * 0.2 = 9.3 on the diagonal.
+    # Node (+1,0,0): w = .125*.75*.75 = .0703125, d = (1,0,0):
+    # vx += trunc(9.3*.0703125*1e7)
+    box = [8.0, 8.0, 8.0]
+    sc = oracle.Scene(one_particle((3.5, 4.5, 2.5)), box)
+    sc.cells[:, 3] = 80000000
+    sc.run_stage(2)
+    c = sc.cells.reshape(8, 8, 8, 4)
+    M = f32(f32(f32(-f32(1.0) / f32(8.0)) * f32(4.0)) * f32(-93.0)) * f32(0.2)
+    want = int(f32(f32(M * f32(0.0703125)) * f32(1.0)) * f32(MULT))
+    assert c[4, 4, 2, 0] == want and c[2, 4, 2, 0] == -want
+    assert c[3, 5, 2, 1] == want and c[3, 4, 3, 2] == want
+    assert c[3, 4, 2, 0] == 0   # d = 0 at the centre node
+
+
+# --- updateGrid ------------------------------------------------------------
+
+def test_update_grid_hand_cases():
+    box = [8.0, 8.0, 8.0]
+    sc = oracle.Scene(np.zeros((0, 20), np.float32), box)
+    idx = lambda x, y, z: x * 64 + y * 8 + z
+    sc.cells[idx(4, 4, 4)] = [20000000, 0, -5000000, 10000000]   # v = (2, 0, -.5), m = 1
+    sc.cells[idx(1, 4, 4)] = [20000000, 20000000, 20000000, 10000000]   # x < 2 wall
+    sc.cells[idx(4, 6, 4)] = [20000000, 20000000, 20000000, 10000000]   # y > ceil(8)-3 = 5
+    sc.cells[idx(5, 5, 5)] = [123, 456, 789, 0]                          # mass 0: untouched
+    sc.cells[idx(5, 5, 4)] = [123, 456, 789, -5]                         # mass < 0: untouched
+    sc.run_stage(3)
+    gdt = f32(-0.3) * f32(0.2)
+    vy = int(f32(gdt) * f32(MULT))
+    assert list(sc.cells[idx(4, 4, 4)]) == [20000000, vy, -5000000, 10000000]
+    assert list(sc.cells[idx(1, 4, 4)]) == [0, int(f32(2 + gdt) * f32(MULT)), 20000000, 10000000]
+    assert list(sc.cells[idx(4, 6, 4)]) == [20000000, 0, 20000000, 10000000]
+    assert list(sc.cells[idx(5, 5, 5)]) == [123, 456, 789, 0]
+    assert list(sc.cells[idx(5, 5, 4)]) == [123, 456, 789, -5]
+
+
+def test_update_grid_wall_follows_real_box():
+    box = [8.0, 8.0, 8.0]
+    sc = oracle.Scene(np.zeros((0, 20), np.float32), box)
+    sc.change_box_size([8.0, 8.0, 6.5])     # ceil(6.5) - 3 = 4
+    z5 = 4 * 64 + 4 * 8 + 5
+    z4 = 4 * 64 + 4 * 8 + 4
+    sc.cells[z5] = [0, 0, 10000000, 10000000]
+    sc.cells[z4] = [0, 0, 10000000, 10000000]
+    sc.run_stage(3)
+    assert sc.cells[z5, 2] == 0 and sc.cells[z4, 2] == 10000000
+
+
+# --- g2p -------------------------------------------------------------------
+
+def _grid_field(box, fn):
+    X, Y, Z = [int(b) for b in box]
+    cells = np.zeros((X, Y, Z, 4), np.int32)
+    for x in range(X):
+        for y in range(Y):
+            for z in range(Z):
+                u = fn(np.array([x + 0.5, y + 0.5, z + 0.5]))
+                cells[x, y, z, :3] = [int(f32(f32(c) * f32(MULT))) for c in u]
+    return cells.reshape(-1, 4)
+
+
+def test_g2p_uniform_field_gives_that_velocity_and_zero_C():
+    box = [16.0, 16.0, 16.0]
+    u = np.array([0.25, -0.5, 0.125])
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(4, 8, size=(50, 3))
+    sc = oracle.Scene(scenes.records(pos), box)
+    sc.cells[:] = _grid_field(box, lambda x: u)
+    sc.run_stage(4)
+    p = sc.particles
+    # far from walls: no penalty (x_n within [3, real-4])
+    assert np.allclose(p[:, 4:7], u, atol=1e-6)
+    for c in range(3):
+        assert np.allclose(p[:, 8 + 4 * c:8 + 4 * c + 3], 0.0, atol=1e-6)
+    assert np.allclose(p[:, 0:3], (pos.astype(np.float32) + u.astype(np.float32) * 0.2), atol=1e-5)
+
+
+def test_g2p_linear_field_recovers_gradient_in_C():
+    # u(x) = A x  ->  v_p = A x_p and C = 4 sum w u (x_i - x_p)^T = A (D = I/4)
+    box = [16.0, 16.0, 16.0]
+    A = np.array([[0.01, 0.02, -0.01], [0.0, -0.02, 0.03], [0.015, 0.0, 0.01]])
+    rng = np.random.default_rng(6)
+    pos = rng.uniform(4, 8, size=(40, 3))
+    sc = oracle.Scene(scenes.records(pos), box)
+    sc.cells[:] = _grid_field(box, lambda x: A @ x)
+    sc.run_stage(4)
+    p = sc.particles
+    for i in range(len(pos)):
+        want_v = A @ pos[i].astype(np.float32)
+        assert np.allclose(p[i, 4:7], want_v, atol=2e-5)
+        C = np.stack([p[i, 8 + 4 * c:8 + 4 * c + 3] for c in range(3)], axis=0)  # C[col][row]
+        assert np.allclose(C.T, A, atol=2e-5)   # matrix C[row][col] = A
+
+
+def test_g2p_clamp_and_wall_penalty_hand_case():
+    box = [16.0, 16.0, 16.0]
+    sc = oracle.Scene(one_particle((1.1, 8.5, 13.9)), box)
+    vel = np.array([-1.0, 0.0, 1.0])
+    sc.cells[:] = _grid_field(box, lambda x: vel)
+    sc.run_stage(4)
+    p = sc.particles[0]
+    # x: 1.1 - 0.2 = 0.9 -> clamp lower 1.0; x_n = 1 - .6 = .4 < 3 -> v += .3*(3-.4)
+    assert p[0] == f32(1.0)
+    assert abs(p[4] - (-1.0 + 0.3 * (3 - 0.4))) < 1e-5
+    # z: 13.9 + 0.2 = 14.1 -> clamp to 16-2 = 14; x_n = 14.6 > 12 -> v += .3*(12-14.6)
+    assert p[2] == f32(14.0)
+    assert abs(p[6] - (1.0 + 0.3 * (12 - 14.6))) < 1e-5
+
+
+# --- whole substep ---------------------------------------------------------
+
+def test_openmp_substep_equals_serial_bit_for_bit():
+    box = [32.0, 32.0, 32.0]
+    rec = scenes.small_scene(3000, box, seed=11, velocity=0.5, affine=0.2)
+    a, b = oracle.Scene(rec, box), oracle.Scene(rec, box)
+    for _ in range(4):
+        a.substep(threads=0)
+        b.substep(threads=4)
+    assert np.array_equal(a.cells, b.cells)
+    assert np.array_equal(a.particles, b.particles)
+    assert np.array_equal(a.posvel, b.posvel)
+
+
+def test_rollout_is_sane_dam_break():
+    # 40 substeps of a small dam break: particles stay in the box, fall
+    # (centre of mass y decreases), spread in x, and total mass is conserved.
+    box = [32.0, 32.0, 32.0]
+    rec = scenes.small_scene(4000, box, seed=12)
+    sc = oracle.Scene(rec, box)
+    y0 = rec[:, 1].mean()
+    x0 = rec[:, 0].std()
+    for _ in range(40):
+        sc.substep()
+    p = sc.particles
+    assert np.all(np.isfinite(p))
+    assert np.all(p[:, 0:3] >= 1.0) and np.all(p[:, 0:3] <= 30.0)
+    assert p[:, 1].mean() < y0
+    assert p[:, 0].std() > x0
+    mass = int(sc.cells[:, 3].astype(np.int64).sum())
+    assert 4000 * MULT - 27 * 4000 <= mass <= 4000 * MULT

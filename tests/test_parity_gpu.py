@@ -1,0 +1,162 @@
+"""GPU parity: libmpm355 (through its C ABI) against the CPU oracle.
+
+Bar: i32 grid accumulators bit-exact; particle state and PosVel bit-exact too
+(both sides evaluate the same IEEE binary32 expressions in the same order,
+with no contraction), so any difference is a bug rather than a tolerance
+question.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mlsmpm import MLSMPMSimulator, MPMError, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(rec, box):
+    return MLSMPMSimulator(particles=rec, init_box_size=box), oracle.Scene(rec, box)
+
+
+def assert_same(sim, ref, what=("grid", "particles", "posvel"), ctx=""):
+    if "grid" in what:
+        g = sim.read_grid()
+        bad = np.nonzero(np.any(g != ref.cells, axis=1))[0]
+        assert bad.size == 0, f"{ctx}: {bad.size} cells differ, first {bad[:5]}: " \
+                              f"{g[bad[:3]].tolist()} vs {ref.cells[bad[:3]].tolist()}"
+    if "particles" in what:
+        p = sim.read_particles()
+        bad = np.nonzero(np.any(p.view(np.int32) != ref.particles.view(np.int32), axis=1))[0]
+        assert bad.size == 0, f"{ctx}: {bad.size} particles differ, first {bad[:5]}: " \
+                              f"{p[bad[:2]].tolist()} vs {ref.particles[bad[:2]].tolist()}"
+    if "posvel" in what:
+        pv = sim.read_posvel()
+        assert np.array_equal(pv.view(np.int32), ref.posvel.view(np.int32)), ctx
+
+
+SMALL = [
+    # n, box, seed, velocity, affine
+    (1, [8.0, 8.0, 8.0], 0, 0.0, 0.0),
+    (65, [16.0, 16.0, 16.0], 1, 1.0, 0.5),
+    (1000, [24.0, 20.0, 28.0], 2, 2.0, 0.3),
+    (5000, [32.0, 32.0, 32.0], 3, 0.5, 0.1),
+]
+
+
+@pytest.mark.parametrize("n,box,seed,vel,aff", SMALL)
+def test_each_reference_stage_bit_exact(n, box, seed, vel, aff):
+    rec = scenes.small_scene(n, box, seed=seed, velocity=vel, affine=aff) if n > 1 else \
+        scenes.records(np.array([[3.3, 4.6, 2.2]]), np.array([[0.5, -0.25, 1.0]]))
+    sim, ref = pair(rec, box)
+    with sim:
+        for sub in range(3):
+            for stage, what in [(0, ("grid",)), (1, ("grid",)), (2, ("grid",)),
+                                (3, ("grid",)), (4, ("particles",)), (5, ("posvel",))]:
+                sim.run_stage(stage)
+                ref.run_stage(stage)
+                assert_same(sim, ref, what, ctx=f"substep {sub} stage {stage}")
+
+
+@pytest.mark.parametrize("n,box,seed,vel,aff", SMALL)
+def test_execute_matches_oracle_substeps(n, box, seed, vel, aff):
+    rec = scenes.small_scene(max(n, 2), box, seed=seed, velocity=vel, affine=aff)
+    sim, ref = pair(rec, box)
+    with sim:
+        for frame in range(3):
+            sim.execute()
+            ref.execute(2)
+            assert_same(sim, ref, ("particles", "posvel"), ctx=f"frame {frame}")
+
+
+def test_config0_rollout_bit_exact_with_box_resize():
+    rec, box = scenes.config0()
+    sim, ref = pair(rec, box)
+    with sim:
+        for frame in range(12):
+            if frame == 6:
+                sim.change_box_size([64.0, 64.0, 48.5])
+                ref.change_box_size([64.0, 64.0, 48.5])
+            sim.execute()
+            ref.execute(2)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="cfg0 frame 12")
+
+
+def test_config1_full_size_substeps_bit_exact():
+    rec, box = scenes.config1()
+    sim, ref = pair(rec, box)
+    with sim:
+        for frame in range(2):
+            sim.execute()
+            ref.execute(2, threads=8)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="cfg1 frame 2")
+        # stage path from the same state: P2G accumulators bit-exact at 300k
+        for stage in (0, 1, 2):
+            sim.run_stage(stage)
+            ref.run_stage(stage)
+        assert_same(sim, ref, ("grid",), ctx="cfg1 p2g grid")
+        mass = int(ref.cells[:, 3].astype(np.int64).sum())
+        n = rec.shape[0]
+        assert n * 10**7 - 27 * n <= mass <= n * 10**7
+
+
+def test_config2_two_million_mass_and_one_substep():
+    rec, box = scenes.config2()
+    sim, ref = pair(rec, box)
+    with sim:
+        sim.run_stage(0)
+        sim.run_stage(1)
+        g = sim.read_grid()
+        n = rec.shape[0]
+        mass = int(g[:, 3].astype(np.int64).sum())
+        assert n * 10**7 - 27 * n <= mass <= n * 10**7
+        assert g[:, 3].max() < 2**31 - 1
+        sim.write_particles(rec)
+        sim.execute(1)
+        ref.substep(threads=8)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="cfg2 substep 1")
+
+
+def test_particles_outside_the_grid_are_dropped_like_the_oracle():
+    box = [12.0, 12.0, 12.0]
+    pos = np.array([[0.2, 5.0, 5.0], [11.8, 5.0, 5.0], [5.0, 5.0, 11.9], [5.0, -3.0, 5.0],
+                    [5.5, 5.5, 5.5], [40.0, 40.0, 40.0]])
+    rec = scenes.records(pos, np.full((6, 3), 0.25))
+    sim, ref = pair(rec, box)
+    with sim:
+        for stage in range(6):
+            sim.run_stage(stage)
+            ref.run_stage(stage)
+        assert_same(sim, ref, ("grid", "particles", "posvel"), ctx="out of grid")
+        sim.execute()
+        ref.execute(2)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="out of grid step")
+
+
+def test_empty_scene_and_reset():
+    box = [8.0, 8.0, 8.0]
+    with MLSMPMSimulator(init_box_size=box) as sim:
+        assert sim.num_particles == 0 and sim.grid_count == 512
+        sim.execute()
+        sim.synchronize()
+        rec = scenes.small_scene(300, [16.0, 16.0, 16.0], seed=9)
+        sim.reset(300, [16.0, 16.0, 16.0], particles=rec)
+        ref = oracle.Scene(rec, [16.0, 16.0, 16.0])
+        sim.execute()
+        ref.execute(2)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="after reset")
+        assert sim.grid_count == 4096
+
+
+def test_errors_are_status_codes():
+    with MLSMPMSimulator(init_box_size=(8, 8, 8)) as sim:
+        with pytest.raises(MPMError) as ei:
+            sim.reset(0, [2000.0, 2000.0, 2000.0], particles=np.zeros((0, 20), np.float32))
+        assert ei.value.status == 2            # grid too large
+        with pytest.raises(MPMError):
+            sim.change_box_size([float("nan"), 8, 8])
+        with pytest.raises(MPMError):
+            sim.run_stage(7)
+        # the handle stays usable after failed calls
+        sim.reset(10, [8.0, 8.0, 8.0], particles=scenes.small_scene(10, [8.0, 8.0, 8.0], 1))
+        sim.execute()
+        assert sim.read_posvel().shape == (10, 8)
