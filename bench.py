@@ -36,6 +36,7 @@ def stage_bytes(name: str, n: int, g: int) -> int:
     per-unit figures of the substep total 204 B/particle + 84 B/cell.
     """
     table = {
+        "bin": 156 * n + 16 * g,                  # keys, counting-sort scan, permute state
         "clear_grid": 16 * g,                     # write 4 x i32
         "p2g_1": 60 * n + 16 * g,                 # read x, v, C; write cells
         "p2g_2": 48 * n + 28 * g,                 # read x, C; mass read + v RMW
@@ -115,12 +116,14 @@ def main() -> int:
 
     # per-kernel HIP-event timing of the same path on the same stream
     sim.reset(n, box, particles=rec)
-    stage_ms = sim.time_stages(args.stage_substeps)
-    avg_us = {k: 1e3 * v / args.stage_substeps for k, v in stage_ms.items()}
-    dom = max(avg_us, key=avg_us.get)
+    stage = sim.time_stages(args.stage_substeps)
+    per_substep_us = {k: 1e3 * ms / args.stage_substeps for k, (ms, c) in stage.items()}
+    per_launch_us = {k: 1e3 * ms / max(c, 1) for k, (ms, c) in stage.items()}
+    dom = max(per_substep_us, key=per_substep_us.get)
     dom_bytes = stage_bytes(dom, n, g)
-    achieved = dom_bytes / (avg_us[dom] * 1e-6) / 1e9
-    substep_us = sum(avg_us.values())
+    achieved = dom_bytes / (per_launch_us[dom] * 1e-6) / 1e9
+    substep_us = sum(per_substep_us.values())
+    tuning = sim.tuning
     total_bytes = 204 * n + 84 * g
     sim.close()
 
@@ -144,6 +147,7 @@ def main() -> int:
             "grid": [int(b) for b in box],
             "grid_cells": g,
             "substeps_per_step": SUBSTEPS_PER_FRAME,
+            "timed_frames": args.steps,
         },
         "roofline": {
             "bound": "hbm",
@@ -154,9 +158,11 @@ def main() -> int:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "algorithmic_bytes_per_launch": dom_bytes,
-            "avg_launch_us": avg_us[dom],
+            "avg_launch_us": per_launch_us[dom],
         },
-        "stages_us": avg_us,
+        "stages_us_per_launch": per_launch_us,
+        "stages_us_per_substep": per_substep_us,
+        "tuning": tuning,
         "substep_roofline": {
             "bytes": total_bytes,
             "sum_kernel_us": substep_us,

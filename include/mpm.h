@@ -92,8 +92,8 @@ int mpm_destroy(mpm_sim *sim);
 int mpm_reset(mpm_sim *sim, const void *particles, uint32_t n, const float init_box[3]);
 
 /* changeBoxSize(realBoxSize) (mls-mpm/mls-mpm.ts:270-275): ordered on the
- * handle's stream after previously issued steps.  Each component must be
- * finite, >= 1 and <= the init box. */
+ * handle's stream after previously issued steps.  Components must be finite;
+ * the grid strides stay those of the init box, as in the reference. */
 int mpm_set_box(mpm_sim *sim, const float real_box[3]);
 
 /* Advances `substeps` substeps; execute(commandEncoder) (mls-mpm.ts:245-268)
@@ -119,7 +119,10 @@ int mpm_posvel_device_ptr(mpm_sim *sim, void **out);
 int mpm_read_particles(mpm_sim *sim, void *dst, uint32_t n);
 int mpm_write_particles(mpm_sim *sim, const void *src, uint32_t n);
 
-/* Grid accumulators, ncells x (vx, vy, vz, mass) i32 (read synchronizes). */
+/* Grid accumulators, ncells x (vx, vy, vz, mass) i32 (read synchronizes).
+ * After mpm_step on the default path the grid holds the last substep's P2G
+ * accumulators (updateGrid is applied inside g2p's LDS staging); after
+ * mpm_run_stage it holds that stage's output exactly as the reference's. */
 int mpm_read_grid(mpm_sim *sim, int32_t *dst, uint64_t ncells);
 int mpm_write_grid(mpm_sim *sim, const int32_t *src, uint64_t ncells);
 
@@ -128,12 +131,33 @@ int mpm_get_counts(mpm_sim *sim, uint32_t *num_particles, uint64_t *grid_count);
 
 /* Timing hooks (HIP events on the handle's stream, after a synchronize).
  * mpm_time_steps: runs `frames` x `substeps_per_frame` substeps; *ms = total.
- * mpm_time_stages: runs `substeps` substeps with events around every kernel
- * of the production path; stage_ms[i] = summed duration of kernel slot i,
- * names[i] (may be NULL) receives a static name; *count = slots used. */
+ * mpm_time_stages: runs `substeps` substeps eagerly with events around every
+ * kernel group of the production path; stage_ms[i] = summed duration of slot
+ * i, calls[i] (may be NULL) = its launches, names[i] (may be NULL) = a static
+ * name; *count = slots used.  No reference counterpart (the reference only
+ * times JS encoding, main.ts:209,278-279). */
 int mpm_time_steps(mpm_sim *sim, uint32_t frames, uint32_t substeps_per_frame, float *ms);
-int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, const char **names,
-                    int capacity, int *count);
+int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls,
+                    const char **names, int capacity, int *count);
+
+/* Launch configuration of the production substep (no reference counterpart:
+ * the reference fixes @workgroup_size(64) and one thread per item).
+ *   path                 0 = binned LDS-window kernels (default),
+ *                        1 = reference-shaped kernels (one launch per WGSL kernel)
+ *   particles_per_thread 1, 2 or 4: a P2G/G2P block covers 256 x this particles
+ *   window_cells         LDS window budget per block, 27..8192 cells
+ *   sort_period          re-bin particles every k >= 1 substeps
+ * Results do not depend on these (integer adds commute; per-particle float
+ * math is unchanged); only speed does. */
+typedef struct {
+    int path;
+    int particles_per_thread;
+    int window_cells;
+    int sort_period;
+} mpm_tuning;
+
+int mpm_get_tuning(mpm_sim *sim, mpm_tuning *out);
+int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in);
 
 #ifdef __cplusplus
 }

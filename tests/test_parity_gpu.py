@@ -119,8 +119,9 @@ def test_config2_two_million_mass_and_one_substep():
 def test_particles_outside_the_grid_are_dropped_like_the_oracle():
     box = [12.0, 12.0, 12.0]
     pos = np.array([[0.2, 5.0, 5.0], [11.8, 5.0, 5.0], [5.0, 5.0, 11.9], [5.0, -3.0, 5.0],
-                    [5.5, 5.5, 5.5], [40.0, 40.0, 40.0]])
-    rec = scenes.records(pos, np.full((6, 3), 0.25))
+                    [5.5, 5.5, 5.5], [40.0, 40.0, 40.0], [np.nan, 5.0, 5.0],
+                    [5.0, np.inf, 5.0], [3e6, 5.0, 5.0], [5.2, 5.7, 6.1]])
+    rec = scenes.records(pos, np.full((len(pos), 3), 0.25))
     sim, ref = pair(rec, box)
     with sim:
         for stage in range(6):
@@ -160,3 +161,44 @@ def test_errors_are_status_codes():
         sim.reset(10, [8.0, 8.0, 8.0], particles=scenes.small_scene(10, [8.0, 8.0, 8.0], 1))
         sim.execute()
         assert sim.read_posvel().shape == (10, 8)
+
+
+TUNINGS = [
+    dict(path=0, particles_per_thread=1, window_cells=2048, sort_period=1),
+    dict(path=0, particles_per_thread=2, window_cells=2048, sort_period=3),
+    dict(path=0, particles_per_thread=4, window_cells=8192, sort_period=2),
+    dict(path=0, particles_per_thread=2, window_cells=40, sort_period=1),   # global fallback
+    dict(path=1, particles_per_thread=2, window_cells=2048, sort_period=1),
+]
+
+
+@pytest.mark.parametrize("tun", TUNINGS, ids=lambda t: "-".join(str(v) for v in t.values()))
+def test_tunings_do_not_change_results(tun):
+    box = [40.0, 36.0, 44.0]
+    rec = scenes.small_scene(20000, box, seed=21, velocity=1.5, affine=0.4)
+    sim, ref = pair(rec, box)
+    with sim:
+        sim.set_tuning(**tun)
+        for frame in range(4):
+            if frame == 2:
+                sim.change_box_size([33.5, 36.0, 41.0])
+                ref.change_box_size([33.5, 36.0, 41.0])
+            sim.execute()
+            ref.execute(2)
+            assert_same(sim, ref, ("particles", "posvel"), ctx=f"{tun} frame {frame}")
+        # the per-stage hooks still work on the binned storage order
+        for stage in range(6):
+            sim.run_stage(stage)
+            ref.run_stage(stage)
+        assert_same(sim, ref, ("grid", "particles", "posvel"), ctx=f"{tun} stages")
+
+
+def test_time_stages_reports_every_kernel_slot():
+    rec, box = scenes.config0()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        t = sim.time_stages(8)
+        assert set(t) == {"bin", "clear_grid", "p2g_1", "p2g_2", "g2p_fused"}
+        assert all(ms > 0 for ms, calls in t.values())
+        assert t["p2g_1"][1] == 8
+        ms = sim.time_steps(4)
+        assert ms > 0

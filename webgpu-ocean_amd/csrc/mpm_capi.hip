@@ -78,10 +78,12 @@ void derive_uniforms(mpm::Uniforms &u, const float init_box[3], const float real
 
 void free_buffers(mpm_sim *sim) {
     if (sim->s.state) (void)hipFree(sim->s.state);
+    if (sim->s.id) (void)hipFree(sim->s.id);
     if (sim->s.grid) (void)hipFree(sim->s.grid);
     if (sim->s.posvel) (void)hipFree(sim->s.posvel);
     if (sim->staging) (void)hipFree(sim->staging);
     sim->s.state = nullptr;
+    sim->s.id = nullptr;
     sim->s.grid = nullptr;
     sim->s.posvel = nullptr;
     sim->staging = nullptr;
@@ -114,6 +116,7 @@ int reset_scene(mpm_sim *sim, const void *particles, uint32_t n, const float ini
     if (cap > sim->cap || G > sim->grid_cap) {
         free_buffers(sim);
         HIP_TRY(hipMalloc(&sim->s.state, sizeof(float) * mpm::NCOMP * (size_t)cap));
+        HIP_TRY(hipMalloc(&sim->s.id, sizeof(uint32_t) * (size_t)cap));
         HIP_TRY(hipMalloc(&sim->s.posvel, 2 * sizeof(float4) * (size_t)cap));
         HIP_TRY(hipMalloc(&sim->staging, 80 * (size_t)cap));
         HIP_TRY(hipMalloc(&sim->s.grid, sizeof(int4) * (size_t)(G ? G : 1)));
@@ -124,6 +127,7 @@ int reset_scene(mpm_sim *sim, const void *particles, uint32_t n, const float ini
     sim->s.n = n;
     sim->s.G = (uint32_t)G;
     sim->s.prm = sim->prm;
+    for (int a = 0; a < 3; ++a) sim->s.init_box[a] = init_box[a];
     derive_uniforms(sim->uni, init_box, init_box);
     HIP_TRY(mpm::launch_set_uniforms(sim->s.uni, sim->uni, sim->stream));
     HIP_TRY(hipMemsetAsync(sim->s.grid, 0, sizeof(int4) * (size_t)sim->s.G, sim->stream));
@@ -354,16 +358,43 @@ int mpm_time_steps(mpm_sim *sim, uint32_t frames, uint32_t substeps_per_frame, f
     return rc;
 }
 
-int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, const char **names,
-                    int capacity, int *count) {
+int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls,
+                    const char **names, int capacity, int *count) {
     if (!sim || !stage_ms || !count) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
     int rc = select_device(sim);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(sim->stream));
-    hipError_t e = sim->path.time_stages(sim->s, sim->stream, substeps, stage_ms, names,
+    hipError_t e = sim->path.time_stages(sim->s, sim->stream, substeps, stage_ms, calls, names,
                                          capacity, count);
     if (e != hipSuccess)
         return fail(MPM_ERR_DEVICE, std::string("time_stages: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int mpm_get_tuning(mpm_sim *sim, mpm_tuning *out) {
+    if (!sim || !out) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    const mpm::Tuning &t = sim->path.tuning();
+    out->path = t.path;
+    out->particles_per_thread = t.ppt;
+    out->window_cells = t.wmax;
+    out->sort_period = t.sort_period;
+    return MPM_OK;
+}
+
+int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in) {
+    if (!sim || !in) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = select_device(sim);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(sim->stream));
+    mpm::Tuning t;
+    t.path = in->path;
+    t.ppt = in->particles_per_thread;
+    t.wmax = in->window_cells;
+    t.sort_period = in->sort_period;
+    if (sim->path.set_tuning(t) != hipSuccess)
+        return fail(MPM_ERR_INVALID_ARGUMENT,
+                    "tuning out of range: path 0|1, particles_per_thread 1|2|4, "
+                    "window_cells 27..8192, sort_period >= 1");
     return MPM_OK;
 }
 

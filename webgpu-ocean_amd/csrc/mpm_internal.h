@@ -10,6 +10,7 @@ namespace mpm {
 // Device buffers of one scene.
 struct DeviceScene {
     float *state;        // NCOMP x stride floats (SoA particle state)
+    uint32_t *id;        // storage slot -> caller particle index
     uint32_t stride;     // >= n, multiple of 64
     uint32_t n;          // particles
     int4 *grid;          // G cells: (vx, vy, vz, mass) i32
@@ -17,6 +18,7 @@ struct DeviceScene {
     float4 *posvel;      // 2 x n float4: (position, 0), (v, 0)
     Uniforms *uni;       // device copy of the box uniforms
     Params prm;
+    float init_box[3];
 };
 
 // Reference-shaped kernels: one launch per WGSL kernel, one thread per
@@ -27,7 +29,8 @@ hipError_t launch_stage(int stage, const DeviceScene &s, hipStream_t st);
 // the stream like queue.writeBuffer (mls-mpm/mls-mpm.ts:274).
 hipError_t launch_set_uniforms(Uniforms *dst, const Uniforms &src, hipStream_t st);
 
-// SoA <-> 80-B reference layout conversion on the device.
+// SoA <-> 80-B reference layout conversion on the device; unpack resets the
+// storage order to the caller's (id = identity), pack writes caller order.
 hipError_t launch_unpack_particles(const float *packed, const DeviceScene &s, hipStream_t st);
 hipError_t launch_pack_particles(float *packed, const DeviceScene &s, hipStream_t st);
 

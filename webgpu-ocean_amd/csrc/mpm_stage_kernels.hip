@@ -220,27 +220,31 @@ __global__ __launch_bounds__(kBlock) void k_g2p(float *__restrict__ st, uint32_t
     }
 }
 
-// copyPosition -- mls-mpm/copyPosition.wgsl:15-21
+// copyPosition -- mls-mpm/copyPosition.wgsl:15-21 (PosVel in caller order)
 __global__ __launch_bounds__(kBlock) void k_copy_position(const float *__restrict__ st,
+                                                          const uint32_t *__restrict__ id,
                                                           uint32_t stride, uint32_t n,
                                                           float4 *__restrict__ posvel) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    posvel[2 * (size_t)i] = make_float4(st[PX * stride + i], st[PY * stride + i],
-                                        st[PZ * stride + i], 0.0f);
-    posvel[2 * (size_t)i + 1] = make_float4(st[VX * stride + i], st[VY * stride + i],
-                                            st[VZ * stride + i], 0.0f);
+    const size_t o = id[i];
+    posvel[2 * o] = make_float4(st[PX * stride + i], st[PY * stride + i], st[PZ * stride + i],
+                                0.0f);
+    posvel[2 * o + 1] = make_float4(st[VX * stride + i], st[VY * stride + i],
+                                    st[VZ * stride + i], 0.0f);
 }
 
 __global__ void k_set_uniforms(Uniforms *dst, Uniforms src) { *dst = src; }
 
 // 80-B reference record (20 floats) <-> SoA state.
 __global__ __launch_bounds__(kBlock) void k_unpack(const float *__restrict__ packed,
-                                                   float *__restrict__ st, uint32_t stride,
+                                                   float *__restrict__ st,
+                                                   uint32_t *__restrict__ id, uint32_t stride,
                                                    uint32_t n) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float *p = packed + 20 * (size_t)i;
+    id[i] = i;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         st[(PX + a) * stride + i] = p[a];
@@ -253,11 +257,12 @@ __global__ __launch_bounds__(kBlock) void k_unpack(const float *__restrict__ pac
 }
 
 __global__ __launch_bounds__(kBlock) void k_pack(float *__restrict__ packed,
-                                                 const float *__restrict__ st, uint32_t stride,
-                                                 uint32_t n) {
+                                                 const float *__restrict__ st,
+                                                 const uint32_t *__restrict__ id,
+                                                 uint32_t stride, uint32_t n) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    float *p = packed + 20 * (size_t)i;
+    float *p = packed + 20 * (size_t)id[i];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         p[a] = st[(PX + a) * stride + i];
@@ -302,7 +307,7 @@ hipError_t launch_stage(int stage, const DeviceScene &s, hipStream_t st) {
         break;
     case 5:
         if (s.n)
-            k_copy_position<<<blocks_for(s.n), kBlock, 0, st>>>(s.state, s.stride, s.n,
+            k_copy_position<<<blocks_for(s.n), kBlock, 0, st>>>(s.state, s.id, s.stride, s.n,
                                                                 s.posvel);
         break;
     default:
@@ -317,12 +322,12 @@ hipError_t launch_set_uniforms(Uniforms *dst, const Uniforms &src, hipStream_t s
 }
 
 hipError_t launch_unpack_particles(const float *packed, const DeviceScene &s, hipStream_t st) {
-    if (s.n) k_unpack<<<blocks_for(s.n), kBlock, 0, st>>>(packed, s.state, s.stride, s.n);
+    if (s.n) k_unpack<<<blocks_for(s.n), kBlock, 0, st>>>(packed, s.state, s.id, s.stride, s.n);
     return hipGetLastError();
 }
 
 hipError_t launch_pack_particles(float *packed, const DeviceScene &s, hipStream_t st) {
-    if (s.n) k_pack<<<blocks_for(s.n), kBlock, 0, st>>>(packed, s.state, s.stride, s.n);
+    if (s.n) k_pack<<<blocks_for(s.n), kBlock, 0, st>>>(packed, s.state, s.id, s.stride, s.n);
     return hipGetLastError();
 }
 

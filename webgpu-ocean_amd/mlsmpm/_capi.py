@@ -29,7 +29,8 @@ EXPORTED = (
     "mpm_destroy", "mpm_reset", "mpm_set_box", "mpm_step", "mpm_run_stage",
     "mpm_synchronize", "mpm_read_posvel", "mpm_posvel_device_ptr",
     "mpm_read_particles", "mpm_write_particles", "mpm_read_grid", "mpm_write_grid",
-    "mpm_get_counts", "mpm_time_steps", "mpm_time_stages",
+    "mpm_get_counts", "mpm_time_steps", "mpm_time_stages", "mpm_get_tuning",
+    "mpm_set_tuning",
 )
 
 
@@ -37,6 +38,15 @@ class MPMError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
         self.status = status
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("path", ctypes.c_int),
+        ("particles_per_thread", ctypes.c_int),
+        ("window_cells", ctypes.c_int),
+        ("sort_period", ctypes.c_int),
+    ]
 
 
 class Params(ctypes.Structure):
@@ -82,8 +92,10 @@ def load(path: str = LIB_PATH):
         "mpm_write_grid": ([P, P, u64], i32),
         "mpm_get_counts": ([P, ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
         "mpm_time_steps": ([P, u32, u32, ctypes.POINTER(ctypes.c_float)], i32),
-        "mpm_time_stages": ([P, u32, ctypes.POINTER(ctypes.c_float),
+        "mpm_time_stages": ([P, u32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32),
                              ctypes.POINTER(ctypes.c_char_p), i32, ctypes.POINTER(i32)], i32),
+        "mpm_get_tuning": ([P, ctypes.POINTER(Tuning)], i32),
+        "mpm_set_tuning": ([P, ctypes.POINTER(Tuning)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

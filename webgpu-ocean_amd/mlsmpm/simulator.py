@@ -148,10 +148,24 @@ class MLSMPMSimulator:
         return ms.value
 
     def time_stages(self, substeps: int) -> dict:
+        """{kernel slot: (total ms, launches)} over `substeps` eager substeps."""
         cap = 16
         ms = (ctypes.c_float * cap)()
+        calls = (ctypes.c_int * cap)()
         names = (ctypes.c_char_p * cap)()
         count = ctypes.c_int()
-        _capi.check(self._L.mpm_time_stages(self._h, substeps, ms, names, cap,
+        _capi.check(self._L.mpm_time_stages(self._h, substeps, ms, calls, names, cap,
                                             ctypes.byref(count)))
-        return {names[i].decode(): ms[i] for i in range(count.value)}
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(count.value)}
+
+    @property
+    def tuning(self) -> dict:
+        t = _capi.Tuning()
+        _capi.check(self._L.mpm_get_tuning(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _capi.Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning
+        cur.update(kw)
+        t = _capi.Tuning(**cur)
+        _capi.check(self._L.mpm_set_tuning(self._h, ctypes.byref(t)))
