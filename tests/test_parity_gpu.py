@@ -18,6 +18,12 @@ def pair(rec, box):
     return MLSMPMSimulator(particles=rec, init_box_size=box), oracle.Scene(rec, box)
 
 
+def bits_equal(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Element-wise bit equality of float32 arrays, except that every NaN
+    equals every NaN (IEEE 754 leaves NaN payload and sign propagation open)."""
+    return (a.view(np.int32) == b.view(np.int32)) | (np.isnan(a) & np.isnan(b))
+
+
 def assert_same(sim, ref, what=("grid", "particles", "posvel"), ctx=""):
     if "grid" in what:
         g = sim.read_grid()
@@ -26,12 +32,12 @@ def assert_same(sim, ref, what=("grid", "particles", "posvel"), ctx=""):
                               f"{g[bad[:3]].tolist()} vs {ref.cells[bad[:3]].tolist()}"
     if "particles" in what:
         p = sim.read_particles()
-        bad = np.nonzero(np.any(p.view(np.int32) != ref.particles.view(np.int32), axis=1))[0]
+        bad = np.nonzero(~np.all(bits_equal(p, ref.particles), axis=1))[0]
         assert bad.size == 0, f"{ctx}: {bad.size} particles differ, first {bad[:5]}: " \
                               f"{p[bad[:2]].tolist()} vs {ref.particles[bad[:2]].tolist()}"
     if "posvel" in what:
         pv = sim.read_posvel()
-        assert np.array_equal(pv.view(np.int32), ref.posvel.view(np.int32)), ctx
+        assert np.all(bits_equal(pv, ref.posvel)), ctx
 
 
 SMALL = [
