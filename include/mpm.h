@@ -143,10 +143,13 @@ int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls
 /* Launch configuration of the production substep (no reference counterpart:
  * the reference fixes @workgroup_size(64) and one thread per item).
  *   path                 0 = binned LDS-window kernels (default),
- *                        1 = reference-shaped kernels (one launch per WGSL kernel)
+ *                        1 = reference-shaped kernels (one launch per WGSL kernel;
+ *                            single simulators only)
  *   particles_per_thread 1, 2 or 4: a P2G/G2P block covers 256 x this particles
  *   window_cells         LDS window budget per block, 27..8192 cells
  *   sort_period          re-bin particles every k >= 1 substeps
+ *   grid_blocks          persistent blocks per launch, 0 = one block per chunk
+ *   xcd_remap            1 = each XCD walks a contiguous range of chunks
  * Results do not depend on these (integer adds commute; per-particle float
  * math is unchanged); only speed does. */
 typedef struct {
@@ -154,10 +157,48 @@ typedef struct {
     int particles_per_thread;
     int window_cells;
     int sort_period;
+    int grid_blocks;
+    int xcd_remap;
 } mpm_tuning;
 
 int mpm_get_tuning(mpm_sim *sim, mpm_tuning *out);
 int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in);
+
+/* Autotuner: times candidate tunings (particles_per_thread x window_cells x
+ * sort_period) for `substeps_per_trial` substeps each on a snapshot of the
+ * current state, restores the state and keeps the fastest.  No reference
+ * counterpart. */
+int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen);
+
+/* ---- Batched scenes (no reference counterpart: the reference runs one scene).
+ * Many independent scenes with their own particle counts, boxes and
+ * parameters, packed into shared buffers and advanced by ONE launch per stage
+ * (hipGraph-replayed substep).  Each scene's results equal a single
+ * simulator's on the same inputs. */
+typedef struct {
+    const void *particles;  /* n x 80-B records (host) */
+    uint32_t n;
+    float init_box[3];
+    mpm_params params;
+} mpm_scene;
+
+typedef struct mpm_batch mpm_batch;
+
+int mpm_batch_create(const mpm_scene *scenes, uint32_t count, mpm_batch **out);
+int mpm_batch_destroy(mpm_batch *batch);
+int mpm_batch_step(mpm_batch *batch, uint32_t substeps);
+int mpm_batch_set_box(mpm_batch *batch, uint32_t scene, const float real_box[3]);
+int mpm_batch_synchronize(mpm_batch *batch);
+int mpm_batch_get_counts(mpm_batch *batch, uint32_t scene, uint32_t *num_particles,
+                         uint64_t *grid_count);
+int mpm_batch_read_posvel(mpm_batch *batch, uint32_t scene, void *dst, uint32_t n);
+int mpm_batch_read_particles(mpm_batch *batch, uint32_t scene, void *dst, uint32_t n);
+int mpm_batch_get_tuning(mpm_batch *batch, mpm_tuning *out);
+int mpm_batch_set_tuning(mpm_batch *batch, const mpm_tuning *in);
+int mpm_batch_autotune(mpm_batch *batch, uint32_t substeps_per_trial, mpm_tuning *chosen);
+int mpm_batch_time_steps(mpm_batch *batch, uint32_t substeps, float *ms);
+int mpm_batch_time_stages(mpm_batch *batch, uint32_t substeps, float *stage_ms, int *calls,
+                          const char **names, int capacity, int *count);
 
 #ifdef __cplusplus
 }

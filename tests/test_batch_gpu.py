@@ -1,0 +1,102 @@
+"""Batched scenes: one launch per stage over many uneven scenes.  Each
+scene's results must equal the oracle's (and a single simulator's) on the
+same inputs, bit for bit, whatever the tuning the autotuner picks."""
+import numpy as np
+import pytest
+
+import oracle
+from mlsmpm import MLSMPMBatch, MLSMPMSimulator, Params, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def bits_equal(a, b):
+    return (a.view(np.int32) == b.view(np.int32)) | (np.isnan(a) & np.isnan(b))
+
+
+def params_of(phys):
+    p = Params(3.0, 4.0, 0.1, 0.2, 1e7)
+    for k, v in phys.items():
+        setattr(p, k, v)
+    return p
+
+
+def small_batch(seed=5, count=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(count):
+        edge = float(rng.choice([12, 16, 20, 28]))
+        box = [edge, float(rng.choice([12, 16, 24])), edge + 2]
+        n = int(rng.integers(1, 3000)) if s else 0   # scene 0 is empty
+        rec = scenes.small_scene(max(n, 1), box, seed=seed + s, velocity=1.0, affine=0.3)[:n]
+        phys = {"stiffness": float(rng.uniform(1, 5)),
+                "dynamic_viscosity": float(rng.uniform(0.02, 0.2))}
+        out.append((rec, box, phys))
+    return out
+
+
+def oracle_scene(rec, box, phys):
+    return oracle.Scene(rec, box, oracle.default_params(**phys))
+
+
+@pytest.mark.parametrize("tuning", [None, dict(particles_per_thread=1, sort_period=2,
+                                                grid_blocks=5, xcd_remap=0),
+                                    dict(particles_per_thread=4, window_cells=64)])
+def test_batch_matches_oracle_per_scene(tuning):
+    sc = small_batch()
+    refs = [oracle_scene(r, b, p) for r, b, p in sc]
+    with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in sc]) as bt:
+        if tuning:
+            bt.set_tuning(**tuning)
+        for frame in range(3):
+            if frame == 1:
+                bt.change_box_size(3, [10.5, 12.0, 13.0])
+                refs[3].change_box_size([10.5, 12.0, 13.0])
+            bt.step(2)
+            for r in refs:
+                r.execute(2)
+        for s, r in enumerate(refs):
+            assert np.all(bits_equal(bt.read_posvel(s), r.posvel)), f"scene {s} posvel"
+            assert np.all(bits_equal(bt.read_particles(s), r.particles)), f"scene {s} state"
+
+
+def test_autotuned_batch_equals_scenes_run_alone():
+    sc = scenes.config3(seed=3, scenes=12, n_lo=2000, n_hi=40000)
+    with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in sc]) as bt:
+        chosen = bt.autotune(2)
+        assert chosen["particles_per_thread"] in (1, 2, 4)
+        bt.step(4)
+        for s, (r, b, p) in enumerate(sc):
+            with MLSMPMSimulator(params=params_of(p), particles=r, init_box_size=b) as sim:
+                sim.execute(4)
+                assert np.all(bits_equal(bt.read_posvel(s), sim.read_posvel())), f"scene {s}"
+
+
+def test_config3_full_batch_two_substeps_equals_sequential():
+    sc = scenes.config3()
+    with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in sc]) as bt:
+        bt.step(2)
+        picks = [0, 17, 40, 63]
+        for s in picks:
+            r, b, p = sc[s]
+            with MLSMPMSimulator(params=params_of(p), particles=r, init_box_size=b) as sim:
+                sim.execute(2)
+                assert np.all(bits_equal(bt.read_posvel(s), sim.read_posvel())), f"scene {s}"
+        # one scene against the oracle as well
+        r, b, p = sc[picks[1]]
+        ref = oracle_scene(r, b, p)
+        ref.execute(2, threads=8)
+        assert np.all(bits_equal(bt.read_posvel(picks[1]), ref.posvel))
+
+
+def test_batch_errors():
+    from mlsmpm import MPMError
+    with pytest.raises(MPMError):
+        MLSMPMBatch([(np.zeros((0, 20), np.float32), [5000.0, 5000.0, 5000.0], None)])
+    sc = small_batch(count=2)
+    with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in sc]) as bt:
+        with pytest.raises(MPMError):
+            bt.read_posvel(5)
+        with pytest.raises(MPMError):
+            bt.set_tuning(path=1)        # reference-shaped path is single-scene only
+        bt.step(1)

@@ -1,0 +1,108 @@
+// mpm_batch.h -- device state and production substep of a batch of
+// independent MLS-MPM scenes (a single simulator is a batch of one).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "mpm_fused.h"
+#include "mpm_internal.h"
+
+namespace mpm {
+
+struct Tuning {
+    int path = 0;          // 0: binned LDS-window kernels, 1: reference-shaped stage kernels
+    int ppt = 2;           // particles per thread: chunk = 256 * ppt particles
+    int wmax = 2048;       // LDS window budget in cells
+    int sort_period = 1;   // re-bin every k substeps
+    int grid_blocks = 0;   // persistent blocks per launch (0: one block per chunk)
+    int xcd_remap = 1;     // contiguous chunk ranges per XCD
+};
+
+// Uniforms of a scene from its init and real boxes.
+Uniforms derive_uniforms(const float init_box[3], const float real_box[3]);
+
+struct SceneInit {
+    const void *particles;  // n x 80-B reference records (host)
+    uint32_t n;
+    float init_box[3];
+    Params prm;
+};
+
+class Batch {
+public:
+    Batch() = default;
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+    ~Batch() { release(); }
+
+    // (Re)creates the batch: packs every scene's particles, grid and keys.
+    hipError_t init(const SceneInit *scenes, uint32_t count, hipStream_t st);
+    void release();
+
+    uint32_t scenes() const { return (uint32_t)h_scenes_.size(); }
+    const SceneDesc &scene(uint32_t s) const { return h_scenes_[s]; }
+
+    hipError_t set_box(uint32_t s, const float real_box[3], hipStream_t st);
+    hipError_t substep(hipStream_t st);
+    hipError_t time_stages(hipStream_t st, unsigned substeps, float *ms, int *calls,
+                           const char **names, int capacity, int *count);
+
+    hipError_t read_posvel(uint32_t s, void *dst, uint32_t n, hipStream_t st);
+    hipError_t read_particles(uint32_t s, void *dst, uint32_t n, hipStream_t st);
+    hipError_t write_particles(uint32_t s, const void *src, uint32_t n, hipStream_t st);
+    hipError_t read_grid(uint32_t s, int32_t *dst, uint64_t ncells, hipStream_t st);
+    hipError_t write_grid(uint32_t s, const int32_t *src, uint64_t ncells, hipStream_t st);
+    void *posvel_ptr(uint32_t s) const { return posvel_ + 2 * (size_t)h_scenes_[s].poff; }
+
+    // View of scene 0 for the reference-shaped stage kernels.
+    DeviceScene stage_view() const;
+
+    const Tuning &tuning() const { return tun_; }
+    hipError_t set_tuning(const Tuning &t);
+    // Times candidate tunings on a snapshot of the current state (restored
+    // afterwards) and keeps the fastest.
+    hipError_t autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *chosen);
+
+private:
+    hipError_t ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys);
+    hipError_t upload_chunks();
+    BatchArgs args() const;
+    hipError_t launch_substep(hipStream_t st, bool sort, hipEvent_t *ev);
+    void drop_graphs();
+    bool sort_due() const;
+    void after_substep(bool sort);
+
+    std::vector<SceneDesc> h_scenes_;
+    std::vector<Chunk> h_chunks_;
+    SceneDesc *d_scenes_ = nullptr;
+    Chunk *d_chunks_ = nullptr;
+    size_t chunk_cap_ = 0;
+
+    uint32_t total_n_ = 0, stride_ = 0;
+    uint64_t total_cells_ = 0, total_keys_ = 0;
+    uint32_t cap_n_ = 0;
+    uint64_t cap_cells_ = 0, cap_keys_ = 0;
+    uint32_t scene_cap_ = 0;
+
+    float *state_ = nullptr, *spare_state_ = nullptr;
+    uint32_t *id_ = nullptr, *spare_id_ = nullptr;
+    int4 *grid_ = nullptr;
+    float4 *posvel_ = nullptr;
+    float *staging_ = nullptr;  // 80-B records of every particle
+    uint32_t *key_ = nullptr, *rank_ = nullptr, *count_ = nullptr, *offset_ = nullptr,
+             *block_sums_ = nullptr;
+
+    Tuning tun_{};
+    unsigned since_sort_ = 0;
+
+    struct GraphEntry {
+        bool sort;
+        float *state;
+        hipGraphExec_t exec;
+    };
+    GraphEntry graphs_[4] = {};
+    int ngraphs_ = 0;
+};
+
+}  // namespace mpm

@@ -1,0 +1,476 @@
+// mpm_batch.hip -- buffers, chunk table, hipGraph replay and autotuner of a
+// batch of scenes.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+#include "mpm_batch.h"
+
+namespace mpm {
+namespace {
+
+const char *const kFusedNames[5] = {"bin", "clear_grid", "p2g_1", "p2g_2", "g2p_fused"};
+const char *const kStageNames[6] = {"clear_grid", "p2g_1", "p2g_2",
+                                    "update_grid", "g2p", "copy_position"};
+
+int32_t f2i_sat_host(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)f;
+}
+
+template <typename T>
+void free_ptr(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+#define TRY(call)                          \
+    do {                                   \
+        hipError_t e_ = (call);            \
+        if (e_ != hipSuccess) return e_;   \
+    } while (0)
+
+}  // namespace
+
+// Uniforms derived from the two boxes (mls-mpm/p2g_1.wgsl:53-56,
+// updateGrid.wgsl:37-43).
+Uniforms derive_uniforms(const float init_box[3], const float real_box[3]) {
+    Uniforms u{};
+    for (int a = 0; a < 3; ++a) {
+        u.init_box[a] = init_box[a];
+        u.real_box[a] = real_box[a];
+        u.wall_hi[a] = f2i_sat_host(std::ceil(real_box[a]) - 3.0f);
+    }
+    u.Y = f2i_sat_host(init_box[1]);
+    u.Z = f2i_sat_host(init_box[2]);
+    u.pad = 0;
+    return u;
+}
+
+hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys) {
+    const uint32_t cap = std::max<uint32_t>(64u, (total_n + 63u) & ~63u);
+    if (cap <= cap_n_ && cells <= cap_cells_ && keys <= cap_keys_) return hipSuccess;
+    drop_graphs();
+    free_ptr(state_);
+    free_ptr(spare_state_);
+    free_ptr(id_);
+    free_ptr(spare_id_);
+    free_ptr(grid_);
+    free_ptr(posvel_);
+    free_ptr(staging_);
+    free_ptr(key_);
+    free_ptr(rank_);
+    free_ptr(count_);
+    free_ptr(offset_);
+    free_ptr(block_sums_);
+    cap_n_ = 0;
+    cap_cells_ = cap_keys_ = 0;
+    const size_t c = cap, g = std::max<uint64_t>(cells, 1), k = std::max<uint64_t>(keys, 1);
+    TRY(hipMalloc(&state_, sizeof(float) * NCOMP * c));
+    TRY(hipMalloc(&spare_state_, sizeof(float) * NCOMP * c));
+    TRY(hipMalloc(&id_, sizeof(uint32_t) * c));
+    TRY(hipMalloc(&spare_id_, sizeof(uint32_t) * c));
+    TRY(hipMalloc(&grid_, sizeof(int4) * g));
+    TRY(hipMalloc(&posvel_, 2 * sizeof(float4) * c));
+    TRY(hipMalloc(&staging_, 80 * c));
+    TRY(hipMalloc(&key_, sizeof(uint32_t) * c));
+    TRY(hipMalloc(&rank_, sizeof(uint32_t) * c));
+    TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
+    TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
+    TRY(hipMalloc(&block_sums_, sizeof(uint32_t) * (k / 4096 + 1)));
+    TRY(hipMemset(count_, 0, sizeof(uint32_t) * k));
+    cap_n_ = cap;
+    cap_cells_ = g;
+    cap_keys_ = k;
+    return hipSuccess;
+}
+
+hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
+    TRY(hipStreamSynchronize(st));
+    drop_graphs();
+    h_scenes_.assign(count, SceneDesc{});
+    uint64_t poff = 0, goff = 0, koff = 0;
+    for (uint32_t s = 0; s < count; ++s) {
+        SceneDesc &d = h_scenes_[s];
+        d.uni = derive_uniforms(sc[s].init_box, sc[s].init_box);
+        d.prm = sc[s].prm;
+        d.poff = (uint32_t)poff;
+        d.n = sc[s].n;
+        d.goff = (uint32_t)goff;
+        d.G = (uint32_t)((uint64_t)std::ceil(sc[s].init_box[0]) *
+                         (uint64_t)std::ceil(sc[s].init_box[1]) *
+                         (uint64_t)std::ceil(sc[s].init_box[2]));
+        scene_key_geometry(d);
+        d.koff = (uint32_t)koff;
+        poff += d.n;
+        goff += d.G;
+        koff += d.nkeys;
+    }
+    total_n_ = (uint32_t)poff;
+    total_cells_ = goff;
+    total_keys_ = koff;
+    TRY(ensure_capacity(total_n_, total_cells_, total_keys_));
+    stride_ = cap_n_;
+    if (count > scene_cap_) {
+        free_ptr(d_scenes_);
+        TRY(hipMalloc(&d_scenes_, sizeof(SceneDesc) * count));
+        scene_cap_ = count;
+    }
+    TRY(hipMemcpy(d_scenes_, h_scenes_.data(), sizeof(SceneDesc) * count,
+                  hipMemcpyHostToDevice));
+    TRY(hipMemsetAsync(grid_, 0, sizeof(int4) * std::max<uint64_t>(total_cells_, 1), st));
+    TRY(hipMemsetAsync(posvel_, 0, 2 * sizeof(float4) * (size_t)stride_, st));
+    for (uint32_t s = 0; s < count; ++s)
+        if (sc[s].n)
+            TRY(hipMemcpyAsync(staging_ + 20 * (size_t)h_scenes_[s].poff, sc[s].particles,
+                               80 * (size_t)sc[s].n, hipMemcpyHostToDevice, st));
+    DeviceScene all = stage_view();
+    all.n = total_n_;
+    TRY(launch_unpack_particles(staging_, all, st));
+    TRY(upload_chunks());
+    since_sort_ = 0;
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::upload_chunks() {
+    const uint32_t per = (uint32_t)kBlock * (uint32_t)tun_.ppt;
+    h_chunks_.clear();
+    for (uint32_t s = 0; s < h_scenes_.size(); ++s) {
+        const SceneDesc &d = h_scenes_[s];
+        for (uint32_t b = 0; b < d.n; b += per)
+            h_chunks_.push_back(Chunk{s, d.poff + b, std::min(per, d.n - b), 0});
+    }
+    if (h_chunks_.size() > chunk_cap_) {
+        free_ptr(d_chunks_);
+        TRY(hipMalloc(&d_chunks_, sizeof(Chunk) * h_chunks_.size()));
+        chunk_cap_ = h_chunks_.size();
+    }
+    if (!h_chunks_.empty())
+        TRY(hipMemcpy(d_chunks_, h_chunks_.data(), sizeof(Chunk) * h_chunks_.size(),
+                      hipMemcpyHostToDevice));
+    return hipSuccess;
+}
+
+DeviceScene Batch::stage_view() const {
+    DeviceScene v{};
+    v.state = state_;
+    v.id = id_;
+    v.stride = stride_;
+    v.grid = grid_;
+    v.posvel = posvel_;
+    if (!h_scenes_.empty()) {
+        const SceneDesc &d = h_scenes_[0];
+        v.n = d.n;
+        v.G = d.G;
+        v.prm = d.prm;
+        v.uni = reinterpret_cast<Uniforms *>(reinterpret_cast<char *>(d_scenes_) +
+                                             offsetof(SceneDesc, uni));
+        for (int a = 0; a < 3; ++a) v.init_box[a] = d.uni.init_box[a];
+    }
+    return v;
+}
+
+BatchArgs Batch::args() const {
+    BatchArgs a{};
+    a.scenes = d_scenes_;
+    a.chunks = d_chunks_;
+    a.nchunks = (uint32_t)h_chunks_.size();
+    a.state = state_;
+    a.id = id_;
+    a.stride = stride_;
+    a.grid = grid_;
+    a.posvel = posvel_;
+    a.ppt = tun_.ppt;
+    a.wmax = tun_.wmax;
+    a.grid_blocks = tun_.grid_blocks;
+    a.xcd_remap = tun_.xcd_remap;
+    return a;
+}
+
+hipError_t Batch::set_box(uint32_t s, const float real_box[3], hipStream_t st) {
+    SceneDesc &d = h_scenes_[s];
+    d.uni = derive_uniforms(d.uni.init_box, real_box);
+    return launch_set_scene_uniforms(d_scenes_ + s, d.uni, st);
+}
+
+bool Batch::sort_due() const {
+    return tun_.path == 0 && total_n_ > 0 && (since_sort_ % (unsigned)tun_.sort_period) == 0;
+}
+
+void Batch::after_substep(bool sort) {
+    if (sort) {
+        std::swap(state_, spare_state_);
+        std::swap(id_, spare_id_);
+        since_sort_ = 0;
+    }
+    ++since_sort_;
+}
+
+hipError_t Batch::launch_substep(hipStream_t st, bool sort, hipEvent_t *ev) {
+    int k = 0;
+    auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
+    if (tun_.path == 1) {
+        const DeviceScene v = stage_view();
+        for (int stage = 0; stage < 6; ++stage) {
+            TRY(mark());
+            TRY(launch_stage(stage, v, st));
+        }
+        return mark();
+    }
+    BatchArgs a = args();
+    TRY(mark());
+    if (sort) {
+        BinBuffers b{key_, rank_, count_, offset_, block_sums_, (uint32_t)total_keys_,
+                     spare_state_, spare_id_};
+        TRY(launch_bin(a, b, st));
+        a.state = spare_state_;
+        a.id = spare_id_;
+    }
+    TRY(mark());
+    TRY(launch_clear(grid_, (uint32_t)total_cells_, st));
+    TRY(mark());
+    TRY(launch_p2g1(a, st));
+    TRY(mark());
+    TRY(launch_p2g2(a, st));
+    TRY(mark());
+    TRY(launch_g2p(a, st));
+    return mark();
+}
+
+hipError_t Batch::substep(hipStream_t st) {
+    const bool sort = sort_due();
+    GraphEntry *g = nullptr;
+    for (int i = 0; i < ngraphs_; ++i)
+        if (graphs_[i].sort == sort && graphs_[i].state == state_) g = &graphs_[i];
+    if (!g) {
+        if (ngraphs_ == 4) drop_graphs();
+        hipGraph_t graph = nullptr;
+        TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_substep(st, sort, nullptr);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (le != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return le;
+        }
+        if (e != hipSuccess) return e;
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return e;
+        graphs_[ngraphs_] = GraphEntry{sort, state_, exec};
+        g = &graphs_[ngraphs_++];
+    }
+    TRY(hipGraphLaunch(g->exec, st));
+    after_substep(sort);
+    return hipSuccess;
+}
+
+hipError_t Batch::time_stages(hipStream_t st, unsigned substeps, float *ms, int *calls,
+                              const char **names, int capacity, int *count) {
+    const bool fused = tun_.path == 0;
+    const int nslots = fused ? 5 : 6;
+    if (capacity < nslots) return hipErrorInvalidValue;
+    hipEvent_t ev[7];
+    for (int i = 0; i <= nslots; ++i) TRY(hipEventCreate(&ev[i]));
+    for (int i = 0; i < nslots; ++i) {
+        ms[i] = 0.0f;
+        if (calls) calls[i] = 0;
+        if (names) names[i] = fused ? kFusedNames[i] : kStageNames[i];
+    }
+    hipError_t e = hipSuccess;
+    for (unsigned k = 0; k < substeps && e == hipSuccess; ++k) {
+        const bool sort = sort_due();
+        e = launch_substep(st, sort, ev);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[nslots]);
+        for (int i = 0; i < nslots && e == hipSuccess; ++i) {
+            if (fused && i == 0 && !sort) continue;
+            float t = 0.0f;
+            e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+            ms[i] += t;
+            if (calls) calls[i] += 1;
+        }
+        after_substep(sort);
+    }
+    for (int i = 0; i <= nslots; ++i) (void)hipEventDestroy(ev[i]);
+    *count = nslots;
+    return e;
+}
+
+hipError_t Batch::read_posvel(uint32_t s, void *dst, uint32_t n, hipStream_t st) {
+    if (n)
+        TRY(hipMemcpyAsync(dst, posvel_ptr(s), 32 * (size_t)n, hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::read_particles(uint32_t s, void *dst, uint32_t n, hipStream_t st) {
+    DeviceScene all = stage_view();
+    all.n = total_n_;
+    TRY(launch_pack_particles(staging_, all, st));
+    if (n)
+        TRY(hipMemcpyAsync(dst, staging_ + 20 * (size_t)h_scenes_[s].poff, 80 * (size_t)n,
+                           hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::write_particles(uint32_t s, const void *src, uint32_t n, hipStream_t st) {
+    // Every scene returns to caller order: pack all, replace one scene, unpack all.
+    DeviceScene all = stage_view();
+    all.n = total_n_;
+    TRY(launch_pack_particles(staging_, all, st));
+    if (n)
+        TRY(hipMemcpyAsync(staging_ + 20 * (size_t)h_scenes_[s].poff, src, 80 * (size_t)n,
+                           hipMemcpyHostToDevice, st));
+    TRY(launch_unpack_particles(staging_, all, st));
+    since_sort_ = 0;
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::read_grid(uint32_t s, int32_t *dst, uint64_t ncells, hipStream_t st) {
+    if (ncells)
+        TRY(hipMemcpyAsync(dst, grid_ + h_scenes_[s].goff, 16 * (size_t)ncells,
+                           hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::write_grid(uint32_t s, const int32_t *src, uint64_t ncells, hipStream_t st) {
+    if (ncells)
+        TRY(hipMemcpyAsync(grid_ + h_scenes_[s].goff, src, 16 * (size_t)ncells,
+                           hipMemcpyHostToDevice, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::set_tuning(const Tuning &t) {
+    if (t.path < 0 || t.path > 1) return hipErrorInvalidValue;
+    if (t.ppt != 1 && t.ppt != 2 && t.ppt != 4) return hipErrorInvalidValue;
+    if (t.wmax < 27 || t.wmax > 8192) return hipErrorInvalidValue;
+    if (t.sort_period < 1 || t.grid_blocks < 0 || t.xcd_remap < 0 || t.xcd_remap > 1)
+        return hipErrorInvalidValue;
+    if (t.path == 1 && h_scenes_.size() > 1) return hipErrorInvalidValue;
+    drop_graphs();
+    const bool rechunk = t.ppt != tun_.ppt;
+    tun_ = t;
+    return rechunk ? upload_chunks() : hipSuccess;
+}
+
+hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *chosen) {
+    TRY(hipStreamSynchronize(st));
+    // Snapshot of everything a substep changes.
+    const size_t sb = sizeof(float) * NCOMP * stride_, ib = sizeof(uint32_t) * stride_,
+                 gb = sizeof(int4) * std::max<uint64_t>(total_cells_, 1),
+                 pb = 2 * sizeof(float4) * (size_t)stride_;
+    char *snap = nullptr;
+    TRY(hipMalloc(&snap, sb + ib + gb + pb));
+    auto save = [&]() -> hipError_t {
+        TRY(hipMemcpyAsync(snap, state_, sb, hipMemcpyDeviceToDevice, st));
+        TRY(hipMemcpyAsync(snap + sb, id_, ib, hipMemcpyDeviceToDevice, st));
+        TRY(hipMemcpyAsync(snap + sb + ib, grid_, gb, hipMemcpyDeviceToDevice, st));
+        return hipMemcpyAsync(snap + sb + ib + gb, posvel_, pb, hipMemcpyDeviceToDevice, st);
+    };
+    const Tuning orig = tun_;
+    float *const orig_state = state_;
+    const unsigned orig_since = since_sort_;
+    hipError_t e = save();
+    Tuning best = orig;
+    float best_ms = 1e30f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const int ppts[3] = {1, 2, 4};
+    const int wmaxs[3] = {1024, 2048, 4096};
+    const int periods[3] = {1, 2, 4};
+    for (int pi = 0; pi < 3 && e == hipSuccess; ++pi)
+        for (int wi = 0; wi < 3 && e == hipSuccess; ++wi)
+            for (int si = 0; si < 3 && e == hipSuccess; ++si) {
+                Tuning t = orig;
+                t.path = 0;
+                t.ppt = ppts[pi];
+                t.wmax = wmaxs[wi];
+                t.sort_period = periods[si];
+                if ((e = set_tuning(t)) != hipSuccess) break;
+                // restore the snapshot into the current buffers
+                if (state_ != orig_state) {
+                    std::swap(state_, spare_state_);
+                    std::swap(id_, spare_id_);
+                }
+                e = hipMemcpyAsync(state_, snap, sb, hipMemcpyDeviceToDevice, st);
+                if (e == hipSuccess) e = hipMemcpyAsync(id_, snap + sb, ib, hipMemcpyDeviceToDevice, st);
+                since_sort_ = 0;
+                // eager launches: graph capture per candidate would dominate
+                for (unsigned k = 0; k < 2 && e == hipSuccess; ++k) {
+                    const bool sort = sort_due();
+                    e = launch_substep(st, sort, nullptr);
+                    after_substep(sort);
+                }
+                if (e == hipSuccess) e = hipEventRecord(e0, st);
+                for (unsigned k = 0; k < substeps_per_trial && e == hipSuccess; ++k) {
+                    const bool sort = sort_due();
+                    e = launch_substep(st, sort, nullptr);
+                    after_substep(sort);
+                }
+                if (e == hipSuccess) e = hipEventRecord(e1, st);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e == hipSuccess && ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
+            }
+    // restore state, tuning and binning phase
+    if (state_ != orig_state) {
+        std::swap(state_, spare_state_);
+        std::swap(id_, spare_id_);
+    }
+    hipError_t r = hipMemcpyAsync(state_, snap, sb, hipMemcpyDeviceToDevice, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(id_, snap + sb, ib, hipMemcpyDeviceToDevice, st);
+    if (r == hipSuccess) r = hipMemcpyAsync(grid_, snap + sb + ib, gb, hipMemcpyDeviceToDevice, st);
+    if (r == hipSuccess)
+        r = hipMemcpyAsync(posvel_, snap + sb + ib + gb, pb, hipMemcpyDeviceToDevice, st);
+    if (r == hipSuccess) r = hipStreamSynchronize(st);
+    (void)hipFree(snap);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    since_sort_ = orig_since;
+    if (e == hipSuccess) e = r;
+    if (e != hipSuccess) {
+        (void)set_tuning(orig);
+        return e;
+    }
+    e = set_tuning(best_ms < 1e30f ? best : orig);
+    if (chosen) *chosen = tun_;
+    return e;
+}
+
+void Batch::drop_graphs() {
+    for (int i = 0; i < ngraphs_; ++i) (void)hipGraphExecDestroy(graphs_[i].exec);
+    ngraphs_ = 0;
+}
+
+void Batch::release() {
+    drop_graphs();
+    free_ptr(state_);
+    free_ptr(spare_state_);
+    free_ptr(id_);
+    free_ptr(spare_id_);
+    free_ptr(grid_);
+    free_ptr(posvel_);
+    free_ptr(staging_);
+    free_ptr(key_);
+    free_ptr(rank_);
+    free_ptr(count_);
+    free_ptr(offset_);
+    free_ptr(block_sums_);
+    free_ptr(d_scenes_);
+    free_ptr(d_chunks_);
+    cap_n_ = 0;
+    cap_cells_ = cap_keys_ = 0;
+    scene_cap_ = 0;
+    chunk_cap_ = 0;
+    h_scenes_.clear();
+    h_chunks_.clear();
+}
+
+}  // namespace mpm

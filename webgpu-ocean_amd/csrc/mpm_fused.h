@@ -1,10 +1,12 @@
-// mpm_fused.h -- launch interface of the production (binned, LDS-windowed)
-// substep kernels.
+// mpm_fused.h -- launch interface of the production substep kernels.
+//
+// The kernels serve a *batch* of independent scenes (one scene is a batch of
+// one): particle slots, grid cells and bin keys of all scenes are packed into
+// single buffers, and every stage is ONE launch over a table of particle
+// chunks, each chunk lying inside one scene.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
-#include <cmath>
 
 #include "mpm_device.h"
 
@@ -13,47 +15,58 @@ namespace mpm {
 // Bin keys are tile-major: tiles of (2^kTileBits)^3 cells in x-major tile
 // order, cells x-major inside a tile.
 constexpr int kTileBits = 2;
+constexpr int kBlock = 256;
 
-struct KeyGeom {
-    int32_t dim[3];   // ceil(init_box): cells per axis used for keys
-    uint32_t nt[3];   // tiles per axis
-    uint64_t nkeys;   // nt0*nt1*nt2 * 2^(3*kTileBits)
+// Per-scene constants, device resident.
+struct SceneDesc {
+    Uniforms uni;        // boxes and derived strides / wall limits
+    Params prm;          // WGSL override constants of this scene
+    uint32_t poff, n;    // particle slots [poff, poff + n)
+    uint32_t goff, G;    // grid cells [goff, goff + G)
+    uint32_t koff;       // first bin key of the scene
+    int32_t kdim[3];     // ceil(init_box): cells per axis used for keys
+    uint32_t knt[3];     // key tiles per axis
+    uint32_t nkeys;      // knt0*knt1*knt2 * 2^(3*kTileBits)
 };
 
-KeyGeom make_key_geom(const float init_box[3]);
+// A run of consecutive particle slots of one scene, handled by one block.
+struct Chunk {
+    uint32_t scene, begin, count, pad;
+};
 
-// Buffers of the substep kernels.
-struct FusedArgs {
-    float *state;          // SoA particle state in bin order
-    uint32_t *id;          // storage slot -> caller particle index
+struct BatchArgs {
+    const SceneDesc *scenes;
+    const Chunk *chunks;
+    uint32_t nchunks;
+    float *state;        // SoA particle state, NCOMP x stride, bin order
+    uint32_t *id;        // slot -> global caller index (poff + local index)
     uint32_t stride;
-    uint32_t n;
-    int4 *grid;
-    uint32_t G;
-    float4 *posvel;        // indexed by caller particle index
-    const Uniforms *uni;
-    Params prm;
-    int ppt;               // particles per thread (1, 2, 4): chunk = 256 * ppt
-    int wmax;              // LDS window budget in cells (16 B each)
+    int4 *grid;          // all scenes' cells
+    float4 *posvel;      // 2 float4 per global caller index
+    int ppt;             // particles per thread: chunk = kBlock * ppt
+    int wmax;            // LDS window budget in cells (16 B each)
+    int grid_blocks;     // persistent blocks per launch (0: one per chunk)
+    int xcd_remap;       // give each XCD a contiguous range of chunks
 };
 
-// Binning (counting sort by cell key): reads state/id of FusedArgs and writes
-// the permuted copy to state_out/id_out.
-struct BinArgs {
-    KeyGeom kg;
-    uint32_t *key;         // n
-    uint32_t *rank;        // n
-    uint32_t *count;       // nkeys, zero on entry, left zero on exit
-    uint32_t *offset;      // nkeys
-    uint32_t *block_sums;  // ceil(nkeys / 4096)
+struct BinBuffers {
+    uint32_t *key, *rank;      // per slot
+    uint32_t *count, *offset;  // per key; count is zero on entry and exit
+    uint32_t *block_sums;      // ceil(nkeys / 4096)
+    uint32_t nkeys;            // all scenes
     float *state_out;
     uint32_t *id_out;
 };
 
-hipError_t launch_bin(const FusedArgs &a, const BinArgs &b, hipStream_t st);
-hipError_t launch_p2g1(const FusedArgs &a, hipStream_t st);
-hipError_t launch_p2g2(const FusedArgs &a, hipStream_t st);
-hipError_t launch_g2p(const FusedArgs &a, hipStream_t st);
-hipError_t launch_iota(uint32_t *id, uint32_t n, hipStream_t st);
+// Derives key geometry of one scene from its init box.
+void scene_key_geometry(SceneDesc &d);
+
+hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
+hipError_t launch_clear(int4 *grid, uint32_t cells, hipStream_t st);
+hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st);
+hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st);
+hipError_t launch_g2p(const BatchArgs &a, hipStream_t st);
+// Writes uniforms of one scene (stream-ordered, like queue.writeBuffer).
+hipError_t launch_set_scene_uniforms(SceneDesc *d, const Uniforms &u, hipStream_t st);
 
 }  // namespace mpm

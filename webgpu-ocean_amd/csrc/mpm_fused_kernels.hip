@@ -1,21 +1,24 @@
 // mpm_fused_kernels.hip -- the production substep for gfx950.
 //
-//   bin (radix/counting sort by tile-major cell key, every `sort_period`
-//   substeps) -> clear -> p2g_1 -> p2g_2 -> g2p(+updateGrid +copyPosition)
+//   bin (counting sort by tile-major cell key, every `sort_period` substeps)
+//   -> clear -> p2g_1 -> p2g_2 -> g2p (+updateGrid +copyPosition)
 //
-// Particles live in SoA storage permuted into bin order; id[] maps a storage
-// slot to the caller's particle index, so readout order never changes.
-// Every P2G/G2P block takes a contiguous chunk of stored particles and stages
-// the chunk's stencil bounding box ("window") in LDS:
+// Each stage is one launch over a table of particle chunks covering every
+// scene of the batch; blocks walk the table persistently.  Particles live in
+// SoA storage permuted into bin order; id[] maps a storage slot to the
+// caller's particle index, so readout order never changes.  A block stages
+// its chunk's stencil bounding box ("window") in LDS:
 //   p2g_1 / p2g_2: i32 fixed-point ds_add into the window, then one global
 //                  atomic per non-zero window cell and component;
-//   g2p:           window cells loaded once, updateGrid applied on load
+//   g2p:           window cells loaded once with updateGrid applied on load
 //                  (a pure per-cell function), 27-node gathers from LDS.
 // Integer adds commute, and every float expression is evaluated per particle
-// exactly as in the stage kernels, so results are identical to them.
-// Particles whose floor(x) is non-finite or beyond +-2^20, and whole chunks
-// whose window exceeds the LDS budget, take the global-memory path of the
-// stage kernels instead.
+// exactly as in the reference-shaped stage kernels, so results are identical
+// to them.  Particles whose floor(x) is non-finite or beyond +-2^20, and whole
+// chunks whose window exceeds the LDS budget, take the global-memory path of
+// the stage kernels instead.
+#include <cmath>
+
 #include "mpm_fused.h"
 
 #pragma clang fp contract(off)
@@ -23,9 +26,34 @@
 namespace mpm {
 namespace {
 
-constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kXcds = 8;
 constexpr float kTameLimit = 1048576.0f;  // |floor(x)| <= 2^20: exact int coordinates
+
+// ---------------------------------------------------------------------------
+// chunk walk: blocks stride over the chunk table; with xcd_remap, the chunks
+// are dealt so that the blocks of one XCD (block id mod 8) take a contiguous
+// range, keeping neighbouring windows in the same L2.
+
+struct ChunkWalk {
+    uint32_t t, end, step, per;
+    bool remap;
+};
+
+__device__ __forceinline__ ChunkWalk walk_begin(const BatchArgs &a) {
+    ChunkWalk w;
+    w.remap = a.xcd_remap != 0;
+    w.per = (a.nchunks + kXcds - 1) / kXcds;
+    w.end = w.remap ? w.per * kXcds : a.nchunks;
+    w.t = blockIdx.x;
+    w.step = gridDim.x;
+    return w;
+}
+
+__device__ __forceinline__ uint32_t walk_chunk(const ChunkWalk &w, uint32_t nchunks) {
+    uint32_t c = w.remap ? (w.t % kXcds) * w.per + w.t / kXcds : w.t;
+    return c < nchunks ? c : 0xffffffffu;
+}
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -41,7 +69,6 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Inclusive wave scan of unsigned values.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
@@ -51,8 +78,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
     return v;
 }
 
-// Exclusive scan over the block (kBlock threads); returns the thread's
-// exclusive prefix, *total receives the block sum.
+// Exclusive scan over the block; *total receives the block sum.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
     __shared__ uint32_t s_w[kBlock / kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -77,8 +103,8 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v) {
     return tot;
 }
 
-__device__ __forceinline__ void load_xvC(const float *__restrict__ st, uint32_t stride,
-                                         uint32_t i, float v[3], float C[3][3]) {
+__device__ __forceinline__ void load_vC(const float *__restrict__ st, uint32_t stride,
+                                        uint32_t i, float v[3], float C[3][3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] = st[(VX + a) * stride + i];
 #pragma unroll
@@ -88,7 +114,7 @@ __device__ __forceinline__ void load_xvC(const float *__restrict__ st, uint32_t 
 }
 
 // updateGrid on one cell (mls-mpm/updateGrid.wgsl:26-44); `id` is the cell's
-// linear index.  Returns the updated (vx, vy, vz).
+// linear index within its scene.
 __device__ __forceinline__ int4 update_cell(int4 c, int32_t id, int32_t Y, int32_t Z,
                                             int32_t wx, int32_t wy, int32_t wz,
                                             const Params &prm) {
@@ -120,15 +146,13 @@ __device__ __forceinline__ int32_t cell_linear(int32_t x, int32_t y, int32_t z, 
                      (uint32_t)z);
 }
 
-// Window = bounding box of the chunk's stencils, [lo, lo + dim).
+// Window = bounding box of the chunk's stencils, cells [lo, lo + dim).
 struct Window {
     int lo[3];
     int dim[3];
-    int vol;   // 0: no window (global path for the whole chunk)
+    int vol;  // 0: no window (global path for the whole chunk)
 };
 
-// Computes the block's window from the tame particles' integer base cells.
-// Must be called by all threads of the block.
 template <int PPT>
 __device__ __forceinline__ Window block_window(const int ic[PPT][3], const bool tame[PPT],
                                                int wmax) {
@@ -187,15 +211,16 @@ __device__ __forceinline__ void win_coords(const Window &w, int l, int &x, int &
     z = w.lo[2] + lz;
 }
 
-// Loads the chunk's positions, integer base cells and "tame" flags.
+// Loads a chunk's positions, integer base cells and "tame" flags.
 template <int PPT>
 __device__ __forceinline__ void load_chunk(const float *__restrict__ st, uint32_t stride,
-                                           uint32_t n, uint32_t base, float x[PPT][3],
-                                           int ic[PPT][3], bool valid[PPT], bool tame[PPT]) {
+                                           const Chunk &ch, float x[PPT][3], int ic[PPT][3],
+                                           bool valid[PPT], bool tame[PPT]) {
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        uint32_t i = base + threadIdx.x + k * kBlock;
-        valid[k] = i < n;
+        const uint32_t j = threadIdx.x + k * kBlock;
+        const uint32_t i = ch.begin + j;
+        valid[k] = j < ch.count;
         tame[k] = false;
         if (valid[k]) {
             bool t = true;
@@ -214,29 +239,33 @@ __device__ __forceinline__ void load_chunk(const float *__restrict__ st, uint32_
 // ---------------------------------------------------------------------------
 // binning sort
 
-__global__ __launch_bounds__(kBlock) void k_bin_count(const float *__restrict__ st,
-                                                      uint32_t stride, uint32_t n,
-                                                      KeyGeom kg, uint32_t *__restrict__ key,
-                                                      uint32_t *__restrict__ rank,
-                                                      uint32_t *__restrict__ count) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    int c[3];
+__global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a, BinBuffers b) {
+    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
+        const uint32_t c = walk_chunk(cw, a.nchunks);
+        if (c == 0xffffffffu) continue;
+        const Chunk ch = a.chunks[c];
+        const SceneDesc *sd = a.scenes + ch.scene;
+        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
+            const uint32_t i = ch.begin + j;
+            int cc[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float f = floorf(st[(PX + a) * stride + i]);
-        int v = (f >= 0.0f) ? ((f < (float)kg.dim[a]) ? (int)f : kg.dim[a] - 1) : 0;
-        c[a] = v;  // NaN compares false -> 0
+            for (int ax = 0; ax < 3; ++ax) {
+                const float f = floorf(a.state[(PX + ax) * a.stride + i]);
+                const int dim = sd->kdim[ax];
+                cc[ax] = (f >= 0.0f) ? ((f < (float)dim) ? (int)f : dim - 1) : 0;  // NaN -> 0
+            }
+            const uint32_t tile =
+                ((uint32_t)(cc[0] >> kTileBits) * sd->knt[1] + (uint32_t)(cc[1] >> kTileBits)) *
+                    sd->knt[2] +
+                (uint32_t)(cc[2] >> kTileBits);
+            constexpr int m = (1 << kTileBits) - 1;
+            const uint32_t local = ((uint32_t)(cc[0] & m) << (2 * kTileBits)) |
+                                   ((uint32_t)(cc[1] & m) << kTileBits) | (uint32_t)(cc[2] & m);
+            const uint32_t k = sd->koff + ((tile << (3 * kTileBits)) | local);
+            b.key[i] = k;
+            b.rank[i] = atomicAdd(&b.count[k], 1u);
+        }
     }
-    uint32_t tile = ((uint32_t)(c[0] >> kTileBits) * kg.nt[1] + (uint32_t)(c[1] >> kTileBits)) *
-                        kg.nt[2] +
-                    (uint32_t)(c[2] >> kTileBits);
-    constexpr int m = (1 << kTileBits) - 1;
-    uint32_t local = ((uint32_t)(c[0] & m) << (2 * kTileBits)) |
-                     ((uint32_t)(c[1] & m) << kTileBits) | (uint32_t)(c[2] & m);
-    uint32_t k = (tile << (3 * kTileBits)) | local;
-    key[i] = k;
-    rank[i] = atomicAdd(&count[k], 1u);
 }
 
 constexpr int kScanItems = 16;
@@ -290,97 +319,107 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ cou
         }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(
-    const float *__restrict__ st, const uint32_t *__restrict__ id, uint32_t stride, uint32_t n,
-    const uint32_t *__restrict__ key, const uint32_t *__restrict__ rank,
-    const uint32_t *__restrict__ offset, float *__restrict__ st2, uint32_t *__restrict__ id2) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t d = offset[key[i]] + rank[i];
+// Keys of scene s lie in [koff_s, koff_s + nkeys_s) and scenes are packed in
+// slot order, so the scattered slot stays inside the scene's slot range.
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers b) {
+    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
+        const uint32_t c = walk_chunk(cw, a.nchunks);
+        if (c == 0xffffffffu) continue;
+        const Chunk ch = a.chunks[c];
+        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
+            const uint32_t i = ch.begin + j;
+            const uint32_t d = b.offset[b.key[i]] + b.rank[i];
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) st2[c * stride + d] = st[c * stride + i];
-    id2[d] = id[i];
+            for (int k = 0; k < NCOMP; ++k) b.state_out[k * a.stride + d] = a.state[k * a.stride + i];
+            b.id_out[d] = a.id[i];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65)
 
 template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_p2g1_win(const float *__restrict__ st,
-                                                     uint32_t stride, uint32_t n,
-                                                     int32_t *__restrict__ grid, uint32_t G,
-                                                     const Uniforms *__restrict__ u, Params prm,
-                                                     int wmax) {
+__global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
     extern __shared__ int lds[];  // [mass | vx | vy | vz] x vol
-    const uint32_t base = blockIdx.x * (kBlock * PPT);
-    const int32_t Y = u->Y, Z = u->Z;
-    float x[PPT][3];
-    int ic[PPT][3];
-    bool valid[PPT], tame[PPT];
-    load_chunk<PPT>(st, stride, n, base, x, ic, valid, tame);
-    const Window w = block_window<PPT>(ic, tame, wmax);
-    const int vol = w.vol;
-    for (int l = threadIdx.x; l < 4 * vol; l += kBlock) lds[l] = 0;
-    __syncthreads();
+    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
+        const uint32_t c = walk_chunk(cw, a.nchunks);
+        if (c == 0xffffffffu) continue;
+        const Chunk ch = a.chunks[c];
+        const SceneDesc *sd = a.scenes + ch.scene;
+        const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
+        const uint32_t G = sd->G;
+        const float mult = sd->prm.mult;
+        int32_t *grid = (int32_t *)(a.grid + sd->goff);
+        float x[PPT][3];
+        int ic[PPT][3];
+        bool valid[PPT], tame[PPT];
+        load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
+        const Window w = block_window<PPT>(ic, tame, a.wmax);
+        const int vol = w.vol;
+        for (int l = threadIdx.x; l < 4 * vol; l += kBlock) lds[l] = 0;
+        __syncthreads();
 
 #pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        if (!valid[k]) continue;
-        const uint32_t i = base + threadIdx.x + k * kBlock;
-        float v[3], C[3][3];
-        load_xvC(st, stride, i, v, C);
-        Stencil s;
-        stencil_init(x[k], s);
-        const bool in_win = vol > 0 && tame[k];
-        const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
+        for (int k = 0; k < PPT; ++k) {
+            if (!valid[k]) continue;
+            const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
+            float v[3], C[3][3];
+            load_vC(a.state, a.stride, i, v, C);
+            Stencil s;
+            stencil_init(x[k], s);
+            const bool in_win = vol > 0 && tame[k];
+            const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
 #pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
+            for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
+                for (int gy = 0; gy < 3; ++gy)
 #pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    float weight = node_weight(s, gx, gy, gz);
-                    float dist[3];
-                    int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                    float mass_contrib = weight * 1.0f;
-                    float vel[3];
+                    for (int gz = 0; gz < 3; ++gz) {
+                        float weight = node_weight(s, gx, gy, gz);
+                        float dist[3];
+                        int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
+                        float mass_contrib = weight * 1.0f;
+                        float vel[3];
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        float q = (C[0][r] * dist[0] + C[1][r] * dist[1]) + C[2][r] * dist[2];
-                        vel[r] = mass_contrib * (v[r] + q);
+                        for (int r = 0; r < 3; ++r) {
+                            float q = (C[0][r] * dist[0] + C[1][r] * dist[1]) + C[2][r] * dist[2];
+                            vel[r] = mass_contrib * (v[r] + q);
+                        }
+                        const int32_t em = encode(mass_contrib, mult);
+                        const int32_t ex = encode(vel[0], mult);
+                        const int32_t ey = encode(vel[1], mult);
+                        const int32_t ez = encode(vel[2], mult);
+                        if (in_win) {
+                            const int l = l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz;
+                            atomicAdd(&lds[l], em);
+                            atomicAdd(&lds[vol + l], ex);
+                            atomicAdd(&lds[2 * vol + l], ey);
+                            atomicAdd(&lds[3 * vol + l], ez);
+                        } else if (in_grid(idx, G)) {
+                            int32_t *cell = grid + 4 * (size_t)idx;
+                            atomicAdd(cell + 3, em);
+                            atomicAdd(cell + 0, ex);
+                            atomicAdd(cell + 1, ey);
+                            atomicAdd(cell + 2, ez);
+                        }
                     }
-                    const int32_t em = encode(mass_contrib, prm.mult);
-                    const int32_t ex = encode(vel[0], prm.mult);
-                    const int32_t ey = encode(vel[1], prm.mult);
-                    const int32_t ez = encode(vel[2], prm.mult);
-                    if (in_win) {
-                        const int l = l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz;
-                        atomicAdd(&lds[l], em);
-                        atomicAdd(&lds[vol + l], ex);
-                        atomicAdd(&lds[2 * vol + l], ey);
-                        atomicAdd(&lds[3 * vol + l], ez);
-                    } else if (in_grid(idx, G)) {
-                        int32_t *cell = grid + 4 * (size_t)idx;
-                        atomicAdd(cell + 3, em);
-                        atomicAdd(cell + 0, ex);
-                        atomicAdd(cell + 1, ey);
-                        atomicAdd(cell + 2, ez);
-                    }
-                }
-    }
-    __syncthreads();
-    for (int l = threadIdx.x; l < vol; l += kBlock) {
-        int cx, cy, cz;
-        win_coords(w, l, cx, cy, cz);
-        const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-        if (!in_grid(gi, G)) continue;
-        int32_t *cell = grid + 4 * (size_t)gi;
-        const int32_t m = lds[l], vx = lds[vol + l], vy = lds[2 * vol + l],
-                      vz = lds[3 * vol + l];
-        if (m) atomicAdd(cell + 3, m);
-        if (vx) atomicAdd(cell + 0, vx);
-        if (vy) atomicAdd(cell + 1, vy);
-        if (vz) atomicAdd(cell + 2, vz);
+        }
+        __syncthreads();
+        for (int l = threadIdx.x; l < vol; l += kBlock) {
+            int cx, cy, cz;
+            win_coords(w, l, cx, cy, cz);
+            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+            if (!in_grid(gi, G)) continue;
+            int32_t *cell = grid + 4 * (size_t)gi;
+            const int32_t m = lds[l], vx = lds[vol + l], vy = lds[2 * vol + l],
+                          vz = lds[3 * vol + l];
+            if (m) atomicAdd(cell + 3, m);
+            if (vx) atomicAdd(cell + 0, vx);
+            if (vy) atomicAdd(cell + 1, vy);
+            if (vz) atomicAdd(cell + 2, vz);
+        }
+        __syncthreads();  // LDS is reused by the next chunk
     }
 }
 
@@ -388,118 +427,124 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(const float *__restrict__ s
 // p2g_2: density, pressure, viscous stress momentum (mls-mpm/p2g_2.wgsl:30-94)
 
 template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_p2g2_win(const float *__restrict__ st,
-                                                     uint32_t stride, uint32_t n,
-                                                     int32_t *__restrict__ grid, uint32_t G,
-                                                     const Uniforms *__restrict__ u, Params prm,
-                                                     int wmax) {
+__global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
     extern __shared__ int lds[];  // [decoded mass (f32) | vx | vy | vz] x vol
     float *dmass = reinterpret_cast<float *>(lds);
-    const uint32_t base = blockIdx.x * (kBlock * PPT);
-    const int32_t Y = u->Y, Z = u->Z;
-    float x[PPT][3];
-    int ic[PPT][3];
-    bool valid[PPT], tame[PPT];
-    load_chunk<PPT>(st, stride, n, base, x, ic, valid, tame);
-    const Window w = block_window<PPT>(ic, tame, wmax);
-    const int vol = w.vol;
-    for (int l = threadIdx.x; l < vol; l += kBlock) {
-        int cx, cy, cz;
-        win_coords(w, l, cx, cy, cz);
-        const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-        const int32_t m = in_grid(gi, G) ? grid[4 * (size_t)gi + 3] : 0;
-        dmass[l] = decode(m, prm.mult);
-        lds[vol + l] = 0;
-        lds[2 * vol + l] = 0;
-        lds[3 * vol + l] = 0;
-    }
-    __syncthreads();
+    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
+        const uint32_t c = walk_chunk(cw, a.nchunks);
+        if (c == 0xffffffffu) continue;
+        const Chunk ch = a.chunks[c];
+        const SceneDesc *sd = a.scenes + ch.scene;
+        const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
+        const uint32_t G = sd->G;
+        const Params prm = sd->prm;
+        int32_t *grid = (int32_t *)(a.grid + sd->goff);
+        float x[PPT][3];
+        int ic[PPT][3];
+        bool valid[PPT], tame[PPT];
+        load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
+        const Window w = block_window<PPT>(ic, tame, a.wmax);
+        const int vol = w.vol;
+        for (int l = threadIdx.x; l < vol; l += kBlock) {
+            int cx, cy, cz;
+            win_coords(w, l, cx, cy, cz);
+            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+            const int32_t m = in_grid(gi, G) ? grid[4 * (size_t)gi + 3] : 0;
+            dmass[l] = decode(m, prm.mult);
+            lds[vol + l] = 0;
+            lds[2 * vol + l] = 0;
+            lds[3 * vol + l] = 0;
+        }
+        __syncthreads();
 
 #pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        if (!valid[k]) continue;
-        const uint32_t i = base + threadIdx.x + k * kBlock;
-        float v[3], C[3][3];
-        load_xvC(st, stride, i, v, C);
-        Stencil s;
-        stencil_init(x[k], s);
-        const bool in_win = vol > 0 && tame[k];
-        const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
-        float density = 0.0f;
+        for (int k = 0; k < PPT; ++k) {
+            if (!valid[k]) continue;
+            const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
+            float v[3], C[3][3];
+            load_vC(a.state, a.stride, i, v, C);
+            Stencil s;
+            stencil_init(x[k], s);
+            const bool in_win = vol > 0 && tame[k];
+            const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
+            float density = 0.0f;
 #pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
+            for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
+                for (int gy = 0; gy < 3; ++gy)
 #pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    float weight = node_weight(s, gx, gy, gz);
-                    float dm;
-                    if (in_win) {
-                        dm = dmass[l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz];
-                    } else {
+                    for (int gz = 0; gz < 3; ++gz) {
+                        float weight = node_weight(s, gx, gy, gz);
+                        float dm;
+                        if (in_win) {
+                            dm = dmass[l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz];
+                        } else {
+                            float dist[3];
+                            int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
+                            dm = decode(in_grid(idx, G) ? grid[4 * (size_t)idx + 3] : 0, prm.mult);
+                        }
+                        density = density + dm * weight;
+                    }
+            float volume = 1.0f / density;
+            float r = density / prm.rest_density;
+            float r2 = r * r;
+            float r5 = (r2 * r2) * r;
+            float pw = prm.stiffness * (r5 - 1.0f);
+            float pressure = (pw > -0.0f) ? pw : -0.0f;
+            float M[3][3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr) {
+                    float stress = (cc == rr) ? -pressure : 0.0f;
+                    float strain = C[cc][rr] + C[rr][cc];
+                    stress = stress + prm.viscosity * strain;
+                    M[cc][rr] = ((-volume * 4.0f) * stress) * prm.dt;
+                }
+#pragma unroll
+            for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+                for (int gy = 0; gy < 3; ++gy)
+#pragma unroll
+                    for (int gz = 0; gz < 3; ++gz) {
+                        float weight = node_weight(s, gx, gy, gz);
                         float dist[3];
                         int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                        dm = decode(in_grid(idx, G) ? grid[4 * (size_t)idx + 3] : 0, prm.mult);
+                        float mom[3];
+#pragma unroll
+                        for (int rr = 0; rr < 3; ++rr)
+                            mom[rr] = ((M[0][rr] * weight) * dist[0] +
+                                       (M[1][rr] * weight) * dist[1]) +
+                                      (M[2][rr] * weight) * dist[2];
+                        const int32_t ex = encode(mom[0], prm.mult);
+                        const int32_t ey = encode(mom[1], prm.mult);
+                        const int32_t ez = encode(mom[2], prm.mult);
+                        if (in_win) {
+                            const int l = l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz;
+                            atomicAdd(&lds[vol + l], ex);
+                            atomicAdd(&lds[2 * vol + l], ey);
+                            atomicAdd(&lds[3 * vol + l], ez);
+                        } else if (in_grid(idx, G)) {
+                            int32_t *cell = grid + 4 * (size_t)idx;
+                            atomicAdd(cell + 0, ex);
+                            atomicAdd(cell + 1, ey);
+                            atomicAdd(cell + 2, ez);
+                        }
                     }
-                    density = density + dm * weight;
-                }
-        float volume = 1.0f / density;
-        float r = density / prm.rest_density;
-        float r2 = r * r;
-        float r5 = (r2 * r2) * r;
-        float pw = prm.stiffness * (r5 - 1.0f);
-        float pressure = (pw > -0.0f) ? pw : -0.0f;
-        float M[3][3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-                float stress = (c == rr) ? -pressure : 0.0f;
-                float strain = C[c][rr] + C[rr][c];
-                stress = stress + prm.viscosity * strain;
-                M[c][rr] = ((-volume * 4.0f) * stress) * prm.dt;
-            }
-#pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    float weight = node_weight(s, gx, gy, gz);
-                    float dist[3];
-                    int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                    float mom[3];
-#pragma unroll
-                    for (int rr = 0; rr < 3; ++rr)
-                        mom[rr] = ((M[0][rr] * weight) * dist[0] + (M[1][rr] * weight) * dist[1]) +
-                                  (M[2][rr] * weight) * dist[2];
-                    const int32_t ex = encode(mom[0], prm.mult);
-                    const int32_t ey = encode(mom[1], prm.mult);
-                    const int32_t ez = encode(mom[2], prm.mult);
-                    if (in_win) {
-                        const int l = l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz;
-                        atomicAdd(&lds[vol + l], ex);
-                        atomicAdd(&lds[2 * vol + l], ey);
-                        atomicAdd(&lds[3 * vol + l], ez);
-                    } else if (in_grid(idx, G)) {
-                        int32_t *cell = grid + 4 * (size_t)idx;
-                        atomicAdd(cell + 0, ex);
-                        atomicAdd(cell + 1, ey);
-                        atomicAdd(cell + 2, ez);
-                    }
-                }
-    }
-    __syncthreads();
-    for (int l = threadIdx.x; l < vol; l += kBlock) {
-        int cx, cy, cz;
-        win_coords(w, l, cx, cy, cz);
-        const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-        if (!in_grid(gi, G)) continue;
-        int32_t *cell = grid + 4 * (size_t)gi;
-        const int32_t vx = lds[vol + l], vy = lds[2 * vol + l], vz = lds[3 * vol + l];
-        if (vx) atomicAdd(cell + 0, vx);
-        if (vy) atomicAdd(cell + 1, vy);
-        if (vz) atomicAdd(cell + 2, vz);
+        }
+        __syncthreads();
+        for (int l = threadIdx.x; l < vol; l += kBlock) {
+            int cx, cy, cz;
+            win_coords(w, l, cx, cy, cz);
+            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+            if (!in_grid(gi, G)) continue;
+            int32_t *cell = grid + 4 * (size_t)gi;
+            const int32_t vx = lds[vol + l], vy = lds[2 * vol + l], vz = lds[3 * vol + l];
+            if (vx) atomicAdd(cell + 0, vx);
+            if (vy) atomicAdd(cell + 1, vy);
+            if (vz) atomicAdd(cell + 2, vz);
+        }
+        __syncthreads();
     }
 }
 
@@ -508,197 +553,191 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(const float *__restrict__ s
 // (mls-mpm/updateGrid.wgsl:23-46, g2p.wgsl:26-92, copyPosition.wgsl:15-21)
 
 template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_g2p_win(float *__restrict__ st,
-                                                    const uint32_t *__restrict__ id,
-                                                    uint32_t stride, uint32_t n,
-                                                    const int4 *__restrict__ grid, uint32_t G,
-                                                    float4 *__restrict__ posvel,
-                                                    const Uniforms *__restrict__ u, Params prm,
-                                                    int wmax) {
+__global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
     extern __shared__ float4 lvel[];  // decoded updated velocity x vol
-    const uint32_t base = blockIdx.x * (kBlock * PPT);
-    const int32_t Y = u->Y, Z = u->Z;
-    const int32_t wx = u->wall_hi[0], wy = u->wall_hi[1], wz = u->wall_hi[2];
-    float x[PPT][3];
-    int ic[PPT][3];
-    bool valid[PPT], tame[PPT];
-    load_chunk<PPT>(st, stride, n, base, x, ic, valid, tame);
-    const Window w = block_window<PPT>(ic, tame, wmax);
-    const int vol = w.vol;
-    for (int l = threadIdx.x; l < vol; l += kBlock) {
-        int cx, cy, cz;
-        win_coords(w, l, cx, cy, cz);
-        const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (in_grid(gi, G)) {
-            int4 c = update_cell(grid[gi], gi, Y, Z, wx, wy, wz, prm);
-            o = make_float4(decode(c.x, prm.mult), decode(c.y, prm.mult), decode(c.z, prm.mult),
-                            0.0f);
+    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
+        const uint32_t c = walk_chunk(cw, a.nchunks);
+        if (c == 0xffffffffu) continue;
+        const Chunk ch = a.chunks[c];
+        const SceneDesc *sd = a.scenes + ch.scene;
+        const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
+        const int32_t wx = sd->uni.wall_hi[0], wy = sd->uni.wall_hi[1], wz = sd->uni.wall_hi[2];
+        const uint32_t G = sd->G;
+        const Params prm = sd->prm;
+        const int4 *grid = a.grid + sd->goff;
+        float x[PPT][3];
+        int ic[PPT][3];
+        bool valid[PPT], tame[PPT];
+        load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
+        const Window w = block_window<PPT>(ic, tame, a.wmax);
+        const int vol = w.vol;
+        for (int l = threadIdx.x; l < vol; l += kBlock) {
+            int cx, cy, cz;
+            win_coords(w, l, cx, cy, cz);
+            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+            float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (in_grid(gi, G)) {
+                int4 cv = update_cell(grid[gi], gi, Y, Z, wx, wy, wz, prm);
+                o = make_float4(decode(cv.x, prm.mult), decode(cv.y, prm.mult),
+                                decode(cv.z, prm.mult), 0.0f);
+            }
+            lvel[l] = o;
         }
-        lvel[l] = o;
-    }
-    __syncthreads();
+        __syncthreads();
 
-    const float dt = prm.dt;
-    float rb[3] = {u->real_box[0], u->real_box[1], u->real_box[2]};
+        const float dt = prm.dt;
+        const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
 #pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        if (!valid[k]) continue;
-        const uint32_t i = base + threadIdx.x + k * kBlock;
-        Stencil s;
-        stencil_init(x[k], s);
-        const bool in_win = vol > 0 && tame[k];
-        const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
-        float v[3] = {0.0f, 0.0f, 0.0f};
-        float B[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        for (int k = 0; k < PPT; ++k) {
+            if (!valid[k]) continue;
+            const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
+            Stencil s;
+            stencil_init(x[k], s);
+            const bool in_win = vol > 0 && tame[k];
+            const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
+            float v[3] = {0.0f, 0.0f, 0.0f};
+            float B[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
 #pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
+            for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
+                for (int gy = 0; gy < 3; ++gy)
 #pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    float weight = node_weight(s, gx, gy, gz);
-                    float dist[3];
-                    int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                    float cv[3];
-                    if (in_win) {
-                        float4 o = lvel[l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz];
-                        cv[0] = o.x;
-                        cv[1] = o.y;
-                        cv[2] = o.z;
-                    } else {
-                        int4 c = make_int4(0, 0, 0, 0);
-                        if (in_grid(idx, G)) c = update_cell(grid[idx], idx, Y, Z, wx, wy, wz, prm);
-                        cv[0] = decode(c.x, prm.mult);
-                        cv[1] = decode(c.y, prm.mult);
-                        cv[2] = decode(c.z, prm.mult);
+                    for (int gz = 0; gz < 3; ++gz) {
+                        float weight = node_weight(s, gx, gy, gz);
+                        float dist[3];
+                        int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
+                        float cv[3];
+                        if (in_win) {
+                            float4 o = lvel[l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz];
+                            cv[0] = o.x;
+                            cv[1] = o.y;
+                            cv[2] = o.z;
+                        } else {
+                            int4 g4 = make_int4(0, 0, 0, 0);
+                            if (in_grid(idx, G))
+                                g4 = update_cell(grid[idx], idx, Y, Z, wx, wy, wz, prm);
+                            cv[0] = decode(g4.x, prm.mult);
+                            cv[1] = decode(g4.y, prm.mult);
+                            cv[2] = decode(g4.z, prm.mult);
+                        }
+                        float wv[3] = {cv[0] * weight, cv[1] * weight, cv[2] * weight};
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) B[cc][r] = B[cc][r] + wv[r] * dist[cc];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) v[r] = v[r] + wv[r];
                     }
-                    float wv[3] = {cv[0] * weight, cv[1] * weight, cv[2] * weight};
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc)
+            for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                        for (int r = 0; r < 3; ++r) B[cc][r] = B[cc][r] + wv[r] * dist[cc];
+                for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * cc + r) * a.stride + i] = B[cc][r] * 4.0f;
+            float xo[3], vo[3];
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) v[r] = v[r] + wv[r];
-                }
+            for (int ax = 0; ax < 3; ++ax) {
+                float xa = x[k][ax] + v[ax] * dt;
+                xo[ax] = fminf(fmaxf(xa, 1.0f), rb[ax] - 2.0f);
+            }
+            const float kk = 3.0f, wall_stiffness = 0.3f;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int r = 0; r < 3; ++r) st[(C00 + 3 * c + r) * stride + i] = B[c][r] * 4.0f;
-        float xo[3], vo[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float xa = x[k][a] + v[a] * dt;
-            xo[a] = fminf(fmaxf(xa, 1.0f), rb[a] - 2.0f);
+            for (int ax = 0; ax < 3; ++ax) {
+                float xn = xo[ax] + (v[ax] * dt) * kk;
+                float wall_max = rb[ax] - 4.0f;
+                float va = v[ax];
+                if (xn < 3.0f) va = va + wall_stiffness * (3.0f - xn);
+                if (xn > wall_max) va = va + wall_stiffness * (wall_max - xn);
+                vo[ax] = va;
+                a.state[(PX + ax) * a.stride + i] = xo[ax];
+                a.state[(VX + ax) * a.stride + i] = va;
+            }
+            const size_t o = a.id[i];
+            a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
+            a.posvel[2 * o + 1] = make_float4(vo[0], vo[1], vo[2], 0.0f);
         }
-        const float kk = 3.0f, wall_stiffness = 0.3f;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float xn = xo[a] + (v[a] * dt) * kk;
-            float wall_max = rb[a] - 4.0f;
-            float va = v[a];
-            if (xn < 3.0f) va = va + wall_stiffness * (3.0f - xn);
-            if (xn > wall_max) va = va + wall_stiffness * (wall_max - xn);
-            vo[a] = va;
-            st[(PX + a) * stride + i] = xo[a];
-            st[(VX + a) * stride + i] = va;
-        }
-        const uint32_t o = id[i];
-        posvel[2 * (size_t)o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
-        posvel[2 * (size_t)o + 1] = make_float4(vo[0], vo[1], vo[2], 0.0f);
+        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_iota(uint32_t *__restrict__ id, uint32_t n) {
-    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) id[i] = i;
+__global__ __launch_bounds__(kBlock) void k_clear(int4 *__restrict__ grid, uint32_t cells) {
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride)
+        grid[i] = make_int4(0, 0, 0, 0);
 }
+
+__global__ void k_set_scene_uniforms(SceneDesc *d, Uniforms u) { d->uni = u; }
 
 inline unsigned blocks_for(uint64_t count, uint64_t per) {
     return (unsigned)((count + per - 1) / per);
 }
 
+inline unsigned walk_blocks(const BatchArgs &a) {
+    unsigned nb = a.nchunks;
+    if (a.grid_blocks > 0 && (unsigned)a.grid_blocks < nb) nb = (unsigned)a.grid_blocks;
+    return nb ? nb : 1;
+}
+
 template <int PPT>
-hipError_t launch_win_kernels(int which, const FusedArgs &a, hipStream_t st) {
-    const unsigned nb = blocks_for(a.n, (uint64_t)kBlock * PPT);
+hipError_t launch_win(int which, const BatchArgs &a, hipStream_t st) {
+    if (!a.nchunks) return hipSuccess;
     const size_t lds = 16 * (size_t)a.wmax;
-    if (!a.n) return hipSuccess;
+    const unsigned nb = walk_blocks(a);
     switch (which) {
-    case 1:
-        k_p2g1_win<PPT><<<nb, kBlock, lds, st>>>(a.state, a.stride, a.n, (int32_t *)a.grid, a.G,
-                                                  a.uni, a.prm, a.wmax);
-        break;
-    case 2:
-        k_p2g2_win<PPT><<<nb, kBlock, lds, st>>>(a.state, a.stride, a.n, (int32_t *)a.grid, a.G,
-                                                  a.uni, a.prm, a.wmax);
-        break;
-    case 3:
-        k_g2p_win<PPT><<<nb, kBlock, lds, st>>>(a.state, a.id, a.stride, a.n, a.grid, a.G,
-                                                 a.posvel, a.uni, a.prm, a.wmax);
-        break;
-    default:
-        return hipErrorInvalidValue;
+    case 1: k_p2g1_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
+    case 2: k_p2g2_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
+    case 3: k_g2p_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_win_ppt(int which, const BatchArgs &a, hipStream_t st) {
+    switch (a.ppt) {
+    case 1: return launch_win<1>(which, a, st);
+    case 2: return launch_win<2>(which, a, st);
+    case 4: return launch_win<4>(which, a, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
 
-KeyGeom make_key_geom(const float init_box[3]) {
-    KeyGeom kg{};
-    uint64_t nkeys = 1;
+void scene_key_geometry(SceneDesc &d) {
+    uint64_t nk = 1;
     for (int a = 0; a < 3; ++a) {
-        kg.dim[a] = (int32_t)std::ceil(init_box[a]);
-        if (kg.dim[a] < 1) kg.dim[a] = 1;
-        kg.nt[a] = (uint32_t)((kg.dim[a] + (1 << kTileBits) - 1) >> kTileBits);
-        nkeys *= kg.nt[a];
+        int32_t dim = (int32_t)std::ceil(d.uni.init_box[a]);
+        d.kdim[a] = dim < 1 ? 1 : dim;
+        d.knt[a] = (uint32_t)((d.kdim[a] + (1 << kTileBits) - 1) >> kTileBits);
+        nk *= d.knt[a];
     }
-    kg.nkeys = nkeys << (3 * kTileBits);
-    return kg;
+    d.nkeys = (uint32_t)(nk << (3 * kTileBits));
 }
 
-hipError_t launch_bin(const FusedArgs &a, const BinArgs &b, hipStream_t st) {
-    if (!a.n) return hipSuccess;
-    k_bin_count<<<blocks_for(a.n, kBlock), kBlock, 0, st>>>(a.state, a.stride, a.n, b.kg, b.key,
-                                                            b.rank, b.count);
-    const unsigned nsb = blocks_for(b.kg.nkeys, kScanTile);
-    k_scan_reduce<<<nsb, kBlock, 0, st>>>(b.count, (uint32_t)b.kg.nkeys, b.block_sums);
+hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
+    if (!a.nchunks) return hipSuccess;
+    const unsigned nb = walk_blocks(a);
+    k_bin_count<<<nb, kBlock, 0, st>>>(a, b);
+    const unsigned nsb = blocks_for(b.nkeys, kScanTile);
+    k_scan_reduce<<<nsb, kBlock, 0, st>>>(b.count, b.nkeys, b.block_sums);
     k_scan_top<<<1, kBlock, 0, st>>>(b.block_sums, nsb);
-    k_scan_down<<<nsb, kBlock, 0, st>>>(b.count, b.offset, (uint32_t)b.kg.nkeys, b.block_sums);
-    k_bin_scatter<<<blocks_for(a.n, kBlock), kBlock, 0, st>>>(
-        a.state, a.id, a.stride, a.n, b.key, b.rank, b.offset, b.state_out, b.id_out);
+    k_scan_down<<<nsb, kBlock, 0, st>>>(b.count, b.offset, b.nkeys, b.block_sums);
+    k_bin_scatter<<<nb, kBlock, 0, st>>>(a, b);
     return hipGetLastError();
 }
 
-hipError_t launch_p2g1(const FusedArgs &a, hipStream_t st) {
-    switch (a.ppt) {
-    case 1: return launch_win_kernels<1>(1, a, st);
-    case 2: return launch_win_kernels<2>(1, a, st);
-    case 4: return launch_win_kernels<4>(1, a, st);
-    default: return hipErrorInvalidValue;
-    }
+hipError_t launch_clear(int4 *grid, uint32_t cells, hipStream_t st) {
+    if (!cells) return hipSuccess;
+    unsigned nb = blocks_for(cells, kBlock);
+    if (nb > 8192) nb = 8192;
+    k_clear<<<nb, kBlock, 0, st>>>(grid, cells);
+    return hipGetLastError();
 }
 
-hipError_t launch_p2g2(const FusedArgs &a, hipStream_t st) {
-    switch (a.ppt) {
-    case 1: return launch_win_kernels<1>(2, a, st);
-    case 2: return launch_win_kernels<2>(2, a, st);
-    case 4: return launch_win_kernels<4>(2, a, st);
-    default: return hipErrorInvalidValue;
-    }
-}
+hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st) { return launch_win_ppt(1, a, st); }
+hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st) { return launch_win_ppt(2, a, st); }
+hipError_t launch_g2p(const BatchArgs &a, hipStream_t st) { return launch_win_ppt(3, a, st); }
 
-hipError_t launch_g2p(const FusedArgs &a, hipStream_t st) {
-    switch (a.ppt) {
-    case 1: return launch_win_kernels<1>(3, a, st);
-    case 2: return launch_win_kernels<2>(3, a, st);
-    case 4: return launch_win_kernels<4>(3, a, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_iota(uint32_t *id, uint32_t n, hipStream_t st) {
-    if (n) k_iota<<<blocks_for(n, kBlock), kBlock, 0, st>>>(id, n);
+hipError_t launch_set_scene_uniforms(SceneDesc *d, const Uniforms &u, hipStream_t st) {
+    k_set_scene_uniforms<<<1, 1, 0, st>>>(d, u);
     return hipGetLastError();
 }
 

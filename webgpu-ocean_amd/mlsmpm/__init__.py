@@ -6,4 +6,5 @@ package does not touch the GPU; constructing a simulator does.
 """
 from ._capi import MPMError, Params, load, LIB_PATH, STAGES  # noqa: F401
 from .simulator import MLSMPMSimulator, default_params, PARTICLE_STRUCT_SIZE  # noqa: F401
+from .batch import MLSMPMBatch  # noqa: F401
 from . import scenes  # noqa: F401

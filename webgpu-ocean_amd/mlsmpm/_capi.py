@@ -30,7 +30,11 @@ EXPORTED = (
     "mpm_synchronize", "mpm_read_posvel", "mpm_posvel_device_ptr",
     "mpm_read_particles", "mpm_write_particles", "mpm_read_grid", "mpm_write_grid",
     "mpm_get_counts", "mpm_time_steps", "mpm_time_stages", "mpm_get_tuning",
-    "mpm_set_tuning",
+    "mpm_set_tuning", "mpm_autotune",
+    "mpm_batch_create", "mpm_batch_destroy", "mpm_batch_step", "mpm_batch_set_box",
+    "mpm_batch_synchronize", "mpm_batch_get_counts", "mpm_batch_read_posvel",
+    "mpm_batch_read_particles", "mpm_batch_get_tuning", "mpm_batch_set_tuning",
+    "mpm_batch_autotune", "mpm_batch_time_steps", "mpm_batch_time_stages",
 )
 
 
@@ -46,6 +50,8 @@ class Tuning(ctypes.Structure):
         ("particles_per_thread", ctypes.c_int),
         ("window_cells", ctypes.c_int),
         ("sort_period", ctypes.c_int),
+        ("grid_blocks", ctypes.c_int),
+        ("xcd_remap", ctypes.c_int),
     ]
 
 
@@ -56,6 +62,15 @@ class Params(ctypes.Structure):
         ("dynamic_viscosity", ctypes.c_float),
         ("dt", ctypes.c_float),
         ("fixed_point_multiplier", ctypes.c_float),
+    ]
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [
+        ("particles", ctypes.c_void_p),
+        ("n", ctypes.c_uint32),
+        ("init_box", ctypes.c_float * 3),
+        ("params", Params),
     ]
 
 
@@ -96,6 +111,21 @@ def load(path: str = LIB_PATH):
                              ctypes.POINTER(ctypes.c_char_p), i32, ctypes.POINTER(i32)], i32),
         "mpm_get_tuning": ([P, ctypes.POINTER(Tuning)], i32),
         "mpm_set_tuning": ([P, ctypes.POINTER(Tuning)], i32),
+        "mpm_autotune": ([P, u32, ctypes.POINTER(Tuning)], i32),
+        "mpm_batch_create": ([ctypes.POINTER(Scene), u32, ctypes.POINTER(P)], i32),
+        "mpm_batch_destroy": ([P], i32),
+        "mpm_batch_step": ([P, u32], i32),
+        "mpm_batch_set_box": ([P, u32, F3], i32),
+        "mpm_batch_synchronize": ([P], i32),
+        "mpm_batch_get_counts": ([P, u32, ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
+        "mpm_batch_read_posvel": ([P, u32, P, u32], i32),
+        "mpm_batch_read_particles": ([P, u32, P, u32], i32),
+        "mpm_batch_get_tuning": ([P, ctypes.POINTER(Tuning)], i32),
+        "mpm_batch_set_tuning": ([P, ctypes.POINTER(Tuning)], i32),
+        "mpm_batch_autotune": ([P, u32, ctypes.POINTER(Tuning)], i32),
+        "mpm_batch_time_steps": ([P, u32, ctypes.POINTER(ctypes.c_float)], i32),
+        "mpm_batch_time_stages": ([P, u32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32),
+                                   ctypes.POINTER(ctypes.c_char_p), i32, ctypes.POINTER(i32)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

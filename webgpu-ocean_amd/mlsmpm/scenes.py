@@ -118,3 +118,34 @@ def small_scene(n: int, box, seed: int, spacing: float = 0.55, velocity: float =
         for c in range(3):
             rec[:, 8 + 4 * c:8 + 4 * c + 3] = C[:, c, :]
     return rec
+
+
+BATCH_EDGES = (32, 48, 64, 96)
+
+
+def batch_block_capacity(edge: float, spacing: float = 0.55, frac: float = 0.6) -> int:
+    """Particles a dam-break block [3, 3+frac*(edge-6))^2 x [3, edge-3) holds."""
+    w = int(math.ceil(frac * (edge - 6) / spacing))
+    h = int(math.ceil((edge - 6) / spacing))
+    return w * w * h
+
+
+def config3(seed: int = 3, scenes: int = 64, n_lo: int = 5_000, n_hi: int = 200_000):
+    """Uneven batch: particle counts log-uniform in [n_lo, n_hi]; each cubic
+    box edge drawn from {32, 48, 64, 96} among the edges whose block holds
+    the scene's particles; stiffness ~ U[1,5], viscosity ~ U[0.02, 0.2].
+    Returns a list of (particles, box, {"stiffness", "dynamic_viscosity"})."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(scenes):
+        n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+        edges = [e for e in BATCH_EDGES if batch_block_capacity(e) >= n]
+        edge = float(edges[int(rng.integers(len(edges)))])
+        box = [edge, edge, edge]
+        lo = (3.0, 3.0, 3.0)
+        hi = (3.0 + 0.6 * (edge - 6), edge - 3.0, 3.0 + 0.6 * (edge - 6))
+        pts = jittered_block(lo, hi, 0.55, n, rng, 0.1)
+        phys = {"stiffness": float(rng.uniform(1.0, 5.0)),
+                "dynamic_viscosity": float(rng.uniform(0.02, 0.2))}
+        out.append((records(pts), box, phys))
+    return out

@@ -169,3 +169,8 @@ class MLSMPMSimulator:
         cur.update(kw)
         t = _capi.Tuning(**cur)
         _capi.check(self._L.mpm_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, substeps_per_trial: int = 4) -> dict:
+        t = _capi.Tuning()
+        _capi.check(self._L.mpm_autotune(self._h, substeps_per_trial, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in _capi.Tuning._fields_}
