@@ -28,6 +28,7 @@ def small_batch(seed=5, count=7):
         edge = float(rng.choice([12, 16, 20, 28]))
         box = [edge, float(rng.choice([12, 16, 24])), edge + 2]
         n = int(rng.integers(1, 3000)) if s else 0   # scene 0 is empty
+        n = min(n, scenes.small_scene_capacity(box))
         rec = scenes.small_scene(max(n, 1), box, seed=seed + s, velocity=1.0, affine=0.3)[:n]
         phys = {"stiffness": float(rng.uniform(1, 5)),
                 "dynamic_viscosity": float(rng.uniform(0.02, 0.2))}

@@ -102,6 +102,13 @@ def config2(seed: int = 2, n: int = 2_000_000):
     return records(np.concatenate([a, b])), box
 
 
+def small_scene_capacity(box, spacing: float = 0.55) -> int:
+    """Largest particle count small_scene() can place in `box`."""
+    frac = 0.6
+    hi = (3.0 + frac * (box[0] - 6), box[1] - 3.0, 3.0 + frac * (box[2] - 6))
+    return lattice((3.0, 3.0, 3.0), hi, spacing, 1 << 62).shape[0]
+
+
 def small_scene(n: int, box, seed: int, spacing: float = 0.55, velocity: float = 0.0,
                 affine: float = 0.0):
     """Small randomized scene for parity tests: jittered block plus optional
