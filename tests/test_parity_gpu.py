@@ -139,6 +139,30 @@ def test_particles_outside_the_grid_are_dropped_like_the_oracle():
         assert_same(sim, ref, ("particles", "posvel"), ctx="out of grid step")
 
 
+@pytest.mark.parametrize("path", [0, 1])
+def test_saturating_encode_and_non_finite_velocities(path):
+    # |w * v * 1e7| beyond 2^31 saturates; NaN converts to 0 (i32(f32) in WGSL)
+    box = [16.0, 16.0, 16.0]
+    pos = np.array([[5.5, 5.5, 5.5], [5.7, 5.2, 5.9], [8.1, 8.3, 8.2], [8.4, 8.9, 8.6],
+                    [10.2, 6.6, 7.7], [6.1, 9.5, 10.4], [7.3, 7.3, 7.3], [9.9, 9.1, 6.2]])
+    vel = np.array([[1e6, -1e6, 300.0], [-300.0, 2e9, -2e9], [np.inf, 0.0, -np.inf],
+                    [np.nan, 1.0, 1.0], [214.0, -214.5, 215.0], [-3e38, 3e38, 0.5],
+                    [0.0, 0.0, 0.0], [1.0, np.nan, -1.0]])
+    rec = scenes.records(pos, vel)
+    rec[:, 8] = np.array([1e7, -1e7, 0, 0, 3e3, 0, np.nan, 0], dtype=np.float32)  # C00
+    sim, ref = pair(rec, box)
+    with sim:
+        sim.set_tuning(path=path)
+        for stage in range(6):
+            sim.run_stage(stage)
+            ref.run_stage(stage)
+            assert_same(sim, ref, ("grid", "particles", "posvel"), ctx=f"stage {stage}")
+        for frame in range(2):
+            sim.execute()
+            ref.execute(2)
+            assert_same(sim, ref, ("particles", "posvel"), ctx=f"frame {frame}")
+
+
 def test_empty_scene_and_reset():
     box = [8.0, 8.0, 8.0]
     with MLSMPMSimulator(init_box_size=box) as sim:

@@ -30,12 +30,14 @@ struct Params {
     float stiffness, rest_density, viscosity, dt, mult;
 };
 
-// i32(f32): truncate toward zero, saturating, NaN -> 0.
+// i32(f32): truncate toward zero, saturating, NaN -> 0.  That is exactly
+// what gfx950's V_CVT_I32_F32 does, so the conversion is that one
+// instruction (C's (int)f leaves out-of-range values undefined and would
+// make the compiler emit range checks).
 __device__ __forceinline__ int32_t f2i_sat(float f) {
-    if (f != f) return 0;
-    if (f >= 2147483648.0f) return INT32_MAX;
-    if (f <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)f;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
 }
 
 // encodeFixedPoint -- mls-mpm/p2g_1.wgsl:15-17

@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "mpm_fused.h"
+#include "mpm_particle.h"
 
 #pragma clang fp contract(off)
 
@@ -111,31 +112,6 @@ __device__ __forceinline__ void load_vC(const float *__restrict__ st, uint32_t s
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int r = 0; r < 3; ++r) C[c][r] = st[(C00 + 3 * c + r) * stride + i];
-}
-
-// updateGrid on one cell (mls-mpm/updateGrid.wgsl:26-44); `id` is the cell's
-// linear index within its scene.
-__device__ __forceinline__ int4 update_cell(int4 c, int32_t id, int32_t Y, int32_t Z,
-                                            int32_t wx, int32_t wy, int32_t wz,
-                                            const Params &prm) {
-    if (c.w > 0) {
-        float fx = decode(c.x, prm.mult), fy = decode(c.y, prm.mult), fz = decode(c.z, prm.mult);
-        float m = decode(c.w, prm.mult);
-        fx = fx / m;
-        fy = fy / m;
-        fz = fz / m;
-        float gdt = -0.3f * prm.dt;
-        c.x = encode(fx, prm.mult);
-        c.y = encode(fy + gdt, prm.mult);
-        c.z = encode(fz, prm.mult);
-        int32_t x = id / Z / Y;
-        int32_t y = (id / Z) % Y;
-        int32_t z = id % Z;
-        if (x < 2 || x > wx) c.x = 0;
-        if (y < 2 || y > wy) c.y = 0;
-        if (z < 2 || z > wz) c.z = 0;
-    }
-    return c;
 }
 
 // Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
@@ -337,6 +313,100 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers 
 }
 
 // ---------------------------------------------------------------------------
+// sinks: where a particle's 27 node contributions go / come from
+
+struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x vol
+    int *lds;
+    int vol, l0, d1, d2;
+    __device__ __forceinline__ void add4(const Node3 &, int gx, int gy, int gz, int32_t m,
+                                         int32_t x, int32_t y, int32_t z) const {
+        const int l = l0 + (gx * d1 + gy) * d2 + gz;
+        atomicAdd(&lds[l], m);
+        atomicAdd(&lds[vol + l], x);
+        atomicAdd(&lds[2 * vol + l], y);
+        atomicAdd(&lds[3 * vol + l], z);
+    }
+};
+
+struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
+    int32_t *grid;
+    uint32_t G;
+    int32_t Y, Z;
+    __device__ __forceinline__ void add4(const Node3 &n, int gx, int gy, int gz, int32_t m,
+                                         int32_t x, int32_t y, int32_t z) const {
+        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+        if (!in_grid(idx, G)) return;
+        int32_t *cell = grid + 4 * (size_t)idx;
+        atomicAdd(cell + 3, m);
+        atomicAdd(cell + 0, x);
+        atomicAdd(cell + 1, y);
+        atomicAdd(cell + 2, z);
+    }
+};
+
+struct WinSink2 {  // p2g_2: decoded masses [0, vol), momentum ints [vol, 4 vol)
+    int *lds;
+    int vol, l0, d1, d2;
+    __device__ __forceinline__ float mass(const Node3 &, int gx, int gy, int gz) const {
+        return reinterpret_cast<const float *>(lds)[l0 + (gx * d1 + gy) * d2 + gz];
+    }
+    __device__ __forceinline__ void add3(const Node3 &, int gx, int gy, int gz, int32_t x,
+                                         int32_t y, int32_t z) const {
+        const int l = l0 + (gx * d1 + gy) * d2 + gz;
+        atomicAdd(&lds[vol + l], x);
+        atomicAdd(&lds[2 * vol + l], y);
+        atomicAdd(&lds[3 * vol + l], z);
+    }
+};
+
+struct GridSink2 {
+    int32_t *grid;
+    uint32_t G;
+    int32_t Y, Z;
+    float mult;
+    __device__ __forceinline__ float mass(const Node3 &n, int gx, int gy, int gz) const {
+        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+        return decode(in_grid(idx, G) ? grid[4 * (size_t)idx + 3] : 0, mult);
+    }
+    __device__ __forceinline__ void add3(const Node3 &n, int gx, int gy, int gz, int32_t x,
+                                         int32_t y, int32_t z) const {
+        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+        if (!in_grid(idx, G)) return;
+        int32_t *cell = grid + 4 * (size_t)idx;
+        atomicAdd(cell + 0, x);
+        atomicAdd(cell + 1, y);
+        atomicAdd(cell + 2, z);
+    }
+};
+
+struct WinSinkG {  // g2p: updated, decoded velocities staged in LDS
+    const float4 *lvel;
+    int l0, d1, d2;
+    __device__ __forceinline__ void vel(const Node3 &, int gx, int gy, int gz, float cv[3]) const {
+        const float4 o = lvel[l0 + (gx * d1 + gy) * d2 + gz];
+        cv[0] = o.x;
+        cv[1] = o.y;
+        cv[2] = o.z;
+    }
+};
+
+struct GridSinkG {  // g2p from the raw accumulators, updateGrid applied per node
+    const int4 *grid;
+    uint32_t G;
+    int32_t Y, Z;
+    int32_t wall_hi[3];
+    Params prm;
+    __device__ __forceinline__ void vel(const Node3 &n, int gx, int gy, int gz, float cv[3]) const {
+        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+        int4 c = make_int4(0, 0, 0, 0);
+        if (in_grid(idx, G)) c = update_cell_id(grid[idx], idx, Y, Z, wall_hi, prm);
+        cv[0] = decode(c.x, prm.mult);
+        cv[1] = decode(c.y, prm.mult);
+        cv[2] = decode(c.z, prm.mult);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65)
 
 template <int PPT>
@@ -366,44 +436,16 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
             const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
             float v[3], C[3][3];
             load_vC(a.state, a.stride, i, v, C);
-            Stencil s;
-            stencil_init(x[k], s);
-            const bool in_win = vol > 0 && tame[k];
-            const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
-#pragma unroll
-            for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-                for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                    for (int gz = 0; gz < 3; ++gz) {
-                        float weight = node_weight(s, gx, gy, gz);
-                        float dist[3];
-                        int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                        float mass_contrib = weight * 1.0f;
-                        float vel[3];
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) {
-                            float q = (C[0][r] * dist[0] + C[1][r] * dist[1]) + C[2][r] * dist[2];
-                            vel[r] = mass_contrib * (v[r] + q);
-                        }
-                        const int32_t em = encode(mass_contrib, mult);
-                        const int32_t ex = encode(vel[0], mult);
-                        const int32_t ey = encode(vel[1], mult);
-                        const int32_t ez = encode(vel[2], mult);
-                        if (in_win) {
-                            const int l = l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz;
-                            atomicAdd(&lds[l], em);
-                            atomicAdd(&lds[vol + l], ex);
-                            atomicAdd(&lds[2 * vol + l], ey);
-                            atomicAdd(&lds[3 * vol + l], ez);
-                        } else if (in_grid(idx, G)) {
-                            int32_t *cell = grid + 4 * (size_t)idx;
-                            atomicAdd(cell + 3, em);
-                            atomicAdd(cell + 0, ex);
-                            atomicAdd(cell + 1, ey);
-                            atomicAdd(cell + 2, ez);
-                        }
-                    }
+            Node3 n;
+            node3_init(x[k], n);
+            if (vol > 0 && tame[k]) {
+                const WinSink1 sink{lds, vol,
+                                    win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1),
+                                    w.dim[1], w.dim[2]};
+                p2g1_particle(n, v, C, mult, sink);
+            } else {
+                p2g1_particle(n, v, C, mult, GridSink1{grid, G, Y, Z});
+            }
         }
         __syncthreads();
         for (int l = threadIdx.x; l < vol; l += kBlock) {
@@ -461,76 +503,21 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
             const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
-            float v[3], C[3][3];
-            load_vC(a.state, a.stride, i, v, C);
-            Stencil s;
-            stencil_init(x[k], s);
-            const bool in_win = vol > 0 && tame[k];
-            const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
-            float density = 0.0f;
-#pragma unroll
-            for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-                for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                    for (int gz = 0; gz < 3; ++gz) {
-                        float weight = node_weight(s, gx, gy, gz);
-                        float dm;
-                        if (in_win) {
-                            dm = dmass[l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz];
-                        } else {
-                            float dist[3];
-                            int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                            dm = decode(in_grid(idx, G) ? grid[4 * (size_t)idx + 3] : 0, prm.mult);
-                        }
-                        density = density + dm * weight;
-                    }
-            float volume = 1.0f / density;
-            float r = density / prm.rest_density;
-            float r2 = r * r;
-            float r5 = (r2 * r2) * r;
-            float pw = prm.stiffness * (r5 - 1.0f);
-            float pressure = (pw > -0.0f) ? pw : -0.0f;
-            float M[3][3];
+            float C[3][3];
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                for (int rr = 0; rr < 3; ++rr) {
-                    float stress = (cc == rr) ? -pressure : 0.0f;
-                    float strain = C[cc][rr] + C[rr][cc];
-                    stress = stress + prm.viscosity * strain;
-                    M[cc][rr] = ((-volume * 4.0f) * stress) * prm.dt;
-                }
-#pragma unroll
-            for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-                for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                    for (int gz = 0; gz < 3; ++gz) {
-                        float weight = node_weight(s, gx, gy, gz);
-                        float dist[3];
-                        int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                        float mom[3];
-#pragma unroll
-                        for (int rr = 0; rr < 3; ++rr)
-                            mom[rr] = ((M[0][rr] * weight) * dist[0] +
-                                       (M[1][rr] * weight) * dist[1]) +
-                                      (M[2][rr] * weight) * dist[2];
-                        const int32_t ex = encode(mom[0], prm.mult);
-                        const int32_t ey = encode(mom[1], prm.mult);
-                        const int32_t ez = encode(mom[2], prm.mult);
-                        if (in_win) {
-                            const int l = l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz;
-                            atomicAdd(&lds[vol + l], ex);
-                            atomicAdd(&lds[2 * vol + l], ey);
-                            atomicAdd(&lds[3 * vol + l], ez);
-                        } else if (in_grid(idx, G)) {
-                            int32_t *cell = grid + 4 * (size_t)idx;
-                            atomicAdd(cell + 0, ex);
-                            atomicAdd(cell + 1, ey);
-                            atomicAdd(cell + 2, ez);
-                        }
-                    }
+                for (int r = 0; r < 3; ++r) C[cc][r] = a.state[(C00 + 3 * cc + r) * a.stride + i];
+            Node3 n;
+            node3_init(x[k], n);
+            if (vol > 0 && tame[k]) {
+                const WinSink2 sink{lds, vol,
+                                    win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1),
+                                    w.dim[1], w.dim[2]};
+                p2g2_particle(n, C, prm, sink);
+            } else {
+                p2g2_particle(n, C, prm, GridSink2{grid, G, Y, Z, prm.mult});
+            }
         }
         __syncthreads();
         for (int l = threadIdx.x; l < vol; l += kBlock) {
@@ -561,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
         const Chunk ch = a.chunks[c];
         const SceneDesc *sd = a.scenes + ch.scene;
         const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
-        const int32_t wx = sd->uni.wall_hi[0], wy = sd->uni.wall_hi[1], wz = sd->uni.wall_hi[2];
+        const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
         const uint32_t G = sd->G;
         const Params prm = sd->prm;
         const int4 *grid = a.grid + sd->goff;
@@ -571,13 +558,20 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
         const Window w = block_window<PPT>(ic, tame, a.wmax);
         const int vol = w.vol;
+        // Window cells inside [0,X) x [0,Y) x [0,Z) without index wrap-around
+        // decompose to their own coordinates; others like updateGrid.wgsl:37-39.
+        const bool nice = vol > 0 && w.lo[0] >= 0 && w.lo[1] >= 0 && w.lo[2] >= 0 &&
+                          w.lo[1] + w.dim[1] <= Y && w.lo[2] + w.dim[2] <= Z &&
+                          (int64_t)(w.lo[0] + w.dim[0]) * Y * Z <= (int64_t)INT32_MAX;
         for (int l = threadIdx.x; l < vol; l += kBlock) {
             int cx, cy, cz;
             win_coords(w, l, cx, cy, cz);
             const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
             float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (in_grid(gi, G)) {
-                int4 cv = update_cell(grid[gi], gi, Y, Z, wx, wy, wz, prm);
+                const int4 raw = grid[gi];
+                const int4 cv = nice ? update_cell_xyz(raw, cx, cy, cz, wall_hi, prm)
+                                     : update_cell_id(raw, gi, Y, Z, wall_hi, prm);
                 o = make_float4(decode(cv.x, prm.mult), decode(cv.y, prm.mult),
                                 decode(cv.z, prm.mult), 0.0f);
             }
@@ -585,74 +579,36 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
         }
         __syncthreads();
 
-        const float dt = prm.dt;
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
             const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
-            Stencil s;
-            stencil_init(x[k], s);
-            const bool in_win = vol > 0 && tame[k];
-            const int l0 = in_win ? win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1) : 0;
-            float v[3] = {0.0f, 0.0f, 0.0f};
-            float B[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-#pragma unroll
-            for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-                for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                    for (int gz = 0; gz < 3; ++gz) {
-                        float weight = node_weight(s, gx, gy, gz);
-                        float dist[3];
-                        int32_t idx = node_index(s, x[k], gx, gy, gz, Y, Z, dist);
-                        float cv[3];
-                        if (in_win) {
-                            float4 o = lvel[l0 + (gx * w.dim[1] + gy) * w.dim[2] + gz];
-                            cv[0] = o.x;
-                            cv[1] = o.y;
-                            cv[2] = o.z;
-                        } else {
-                            int4 g4 = make_int4(0, 0, 0, 0);
-                            if (in_grid(idx, G))
-                                g4 = update_cell(grid[idx], idx, Y, Z, wx, wy, wz, prm);
-                            cv[0] = decode(g4.x, prm.mult);
-                            cv[1] = decode(g4.y, prm.mult);
-                            cv[2] = decode(g4.z, prm.mult);
-                        }
-                        float wv[3] = {cv[0] * weight, cv[1] * weight, cv[2] * weight};
-#pragma unroll
-                        for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                            for (int r = 0; r < 3; ++r) B[cc][r] = B[cc][r] + wv[r] * dist[cc];
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) v[r] = v[r] + wv[r];
-                    }
+            Node3 n;
+            node3_init(x[k], n);
+            float v[3], B[3][3];
+            if (vol > 0 && tame[k]) {
+                const WinSinkG sink{lvel, win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1),
+                                    w.dim[1], w.dim[2]};
+                g2p_gather(n, sink, v, B);
+            } else {
+                g2p_gather(n, GridSinkG{grid, G, Y, Z, {wall_hi[0], wall_hi[1], wall_hi[2]}, prm},
+                           v, B);
+            }
+            float xo[3], C[3][3];
+            g2p_finish(x[k], rb, prm.dt, v, B, xo, C);
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * cc + r) * a.stride + i] = B[cc][r] * 4.0f;
-            float xo[3], vo[3];
+                for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * cc + r) * a.stride + i] = C[cc][r];
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
-                float xa = x[k][ax] + v[ax] * dt;
-                xo[ax] = fminf(fmaxf(xa, 1.0f), rb[ax] - 2.0f);
-            }
-            const float kk = 3.0f, wall_stiffness = 0.3f;
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                float xn = xo[ax] + (v[ax] * dt) * kk;
-                float wall_max = rb[ax] - 4.0f;
-                float va = v[ax];
-                if (xn < 3.0f) va = va + wall_stiffness * (3.0f - xn);
-                if (xn > wall_max) va = va + wall_stiffness * (wall_max - xn);
-                vo[ax] = va;
                 a.state[(PX + ax) * a.stride + i] = xo[ax];
-                a.state[(VX + ax) * a.stride + i] = va;
+                a.state[(VX + ax) * a.stride + i] = v[ax];
             }
             const size_t o = a.id[i];
             a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
-            a.posvel[2 * o + 1] = make_float4(vo[0], vo[1], vo[2], 0.0f);
+            a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
         }
         __syncthreads();
     }
