@@ -77,6 +77,8 @@ private:
     std::vector<Chunk> h_chunks_;
     SceneDesc *d_scenes_ = nullptr;
     Chunk *d_chunks_ = nullptr;
+    uint32_t *d_tile_rank_ = nullptr;
+    size_t tile_rank_cap_ = 0;
     size_t chunk_cap_ = 0;
 
     uint32_t total_n_ = 0, stride_ = 0;

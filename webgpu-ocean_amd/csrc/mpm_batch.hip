@@ -105,6 +105,7 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
                          (uint64_t)std::ceil(sc[s].init_box[2]));
         scene_key_geometry(d);
         d.koff = (uint32_t)koff;
+        d.rank_off = 0;  // set below
         poff += d.n;
         goff += d.G;
         koff += d.nkeys;
@@ -119,6 +120,18 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
         TRY(hipMalloc(&d_scenes_, sizeof(SceneDesc) * count));
         scene_cap_ = count;
     }
+    std::vector<uint32_t> ranks;
+    for (uint32_t s = 0; s < count; ++s) {
+        h_scenes_[s].rank_off = (uint32_t)ranks.size();
+        append_hilbert_ranks(h_scenes_[s], ranks);
+    }
+    if (ranks.size() > tile_rank_cap_) {
+        free_ptr(d_tile_rank_);
+        TRY(hipMalloc(&d_tile_rank_, sizeof(uint32_t) * ranks.size()));
+        tile_rank_cap_ = ranks.size();
+    }
+    TRY(hipMemcpy(d_tile_rank_, ranks.data(), sizeof(uint32_t) * ranks.size(),
+                  hipMemcpyHostToDevice));
     TRY(hipMemcpy(d_scenes_, h_scenes_.data(), sizeof(SceneDesc) * count,
                   hipMemcpyHostToDevice));
     TRY(hipMemsetAsync(grid_, 0, sizeof(int4) * std::max<uint64_t>(total_cells_, 1), st));
@@ -176,6 +189,7 @@ DeviceScene Batch::stage_view() const {
 BatchArgs Batch::args() const {
     BatchArgs a{};
     a.scenes = d_scenes_;
+    a.tile_rank = d_tile_rank_;
     a.chunks = d_chunks_;
     a.nchunks = (uint32_t)h_chunks_.size();
     a.state = state_;
@@ -465,6 +479,8 @@ void Batch::release() {
     free_ptr(block_sums_);
     free_ptr(d_scenes_);
     free_ptr(d_chunks_);
+    free_ptr(d_tile_rank_);
+    tile_rank_cap_ = 0;
     cap_n_ = 0;
     cap_cells_ = cap_keys_ = 0;
     scene_cap_ = 0;

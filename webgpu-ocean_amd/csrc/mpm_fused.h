@@ -8,12 +8,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "mpm_device.h"
 
 namespace mpm {
 
-// Bin keys are tile-major: tiles of (2^kTileBits)^3 cells in x-major tile
-// order, cells x-major inside a tile.
+// Bin keys are tile-major: tiles of (2^kTileBits)^3 cells ordered along a 3D
+// Hilbert curve (consecutive tiles share a face, so a run of particles in key
+// order covers a compact box), cells x-major inside a tile.
 constexpr int kTileBits = 2;
 constexpr int kBlock = 256;
 
@@ -27,6 +30,7 @@ struct SceneDesc {
     int32_t kdim[3];     // ceil(init_box): cells per axis used for keys
     uint32_t knt[3];     // key tiles per axis
     uint32_t nkeys;      // knt0*knt1*knt2 * 2^(3*kTileBits)
+    uint32_t rank_off;   // first entry of the scene's tile -> Hilbert rank table
 };
 
 // A run of consecutive particle slots of one scene, handled by one block.
@@ -36,6 +40,7 @@ struct Chunk {
 
 struct BatchArgs {
     const SceneDesc *scenes;
+    const uint32_t *tile_rank;  // per scene: x-major tile index -> Hilbert rank
     const Chunk *chunks;
     uint32_t nchunks;
     float *state;        // SoA particle state, NCOMP x stride, bin order
@@ -60,6 +65,9 @@ struct BinBuffers {
 
 // Derives key geometry of one scene from its init box.
 void scene_key_geometry(SceneDesc &d);
+
+// Appends the scene's tile -> Hilbert-rank table (knt0*knt1*knt2 entries).
+void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks);
 
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
 hipError_t launch_clear(int4 *grid, uint32_t cells, hipStream_t st);

@@ -17,7 +17,10 @@
 // to them.  Particles whose floor(x) is non-finite or beyond +-2^20, and whole
 // chunks whose window exceeds the LDS budget, take the global-memory path of
 // the stage kernels instead.
+#include <algorithm>
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "mpm_fused.h"
 #include "mpm_particle.h"
@@ -58,6 +61,14 @@ __device__ __forceinline__ uint32_t walk_chunk(const ChunkWalk &w, uint32_t nchu
 
 // ---------------------------------------------------------------------------
 // small helpers
+
+// Slot (within its chunk) of a thread's k-th particle.  Lanes of a wave are
+// 4*PPT slots apart, so in cell-sorted storage they mostly touch different
+// cells and their LDS adds rarely hit the same address in one instruction.
+template <int PPT>
+__device__ __forceinline__ uint32_t slot_of(int k) {
+    return (threadIdx.x & 63) * (4 * PPT) + (threadIdx.x >> 6) * PPT + k;
+}
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -194,7 +205,7 @@ __device__ __forceinline__ void load_chunk(const float *__restrict__ st, uint32_
                                            bool valid[PPT], bool tame[PPT]) {
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
-        const uint32_t j = threadIdx.x + k * kBlock;
+        const uint32_t j = slot_of<PPT>(k);
         const uint32_t i = ch.begin + j;
         valid[k] = j < ch.count;
         tame[k] = false;
@@ -230,10 +241,11 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a, BinBuffers b)
                 const int dim = sd->kdim[ax];
                 cc[ax] = (f >= 0.0f) ? ((f < (float)dim) ? (int)f : dim - 1) : 0;  // NaN -> 0
             }
-            const uint32_t tile =
-                ((uint32_t)(cc[0] >> kTileBits) * sd->knt[1] + (uint32_t)(cc[1] >> kTileBits)) *
-                    sd->knt[2] +
-                (uint32_t)(cc[2] >> kTileBits);
+            const uint32_t tile = a.tile_rank[sd->rank_off +
+                                              ((uint32_t)(cc[0] >> kTileBits) * sd->knt[1] +
+                                               (uint32_t)(cc[1] >> kTileBits)) *
+                                                  sd->knt[2] +
+                                              (uint32_t)(cc[2] >> kTileBits)];
             constexpr int m = (1 << kTileBits) - 1;
             const uint32_t local = ((uint32_t)(cc[0] & m) << (2 * kTileBits)) |
                                    ((uint32_t)(cc[1] & m) << kTileBits) | (uint32_t)(cc[2] & m);
@@ -433,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
+            const uint32_t i = ch.begin + slot_of<PPT>(k);
             float v[3], C[3][3];
             load_vC(a.state, a.stride, i, v, C);
             Node3 n;
@@ -502,7 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
+            const uint32_t i = ch.begin + slot_of<PPT>(k);
             float C[3][3];
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
@@ -583,7 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            const uint32_t i = ch.begin + threadIdx.x + k * kBlock;
+            const uint32_t i = ch.begin + slot_of<PPT>(k);
             Node3 n;
             node3_init(x[k], n);
             float v[3], B[3][3];
@@ -666,6 +678,54 @@ void scene_key_geometry(SceneDesc &d) {
         nk *= d.knt[a];
     }
     d.nkeys = (uint32_t)(nk << (3 * kTileBits));
+}
+
+// 3D Hilbert index of (x, y, z) in a 2^bits cube: J. Skilling, "Programming
+// the Hilbert curve", AIP Conf. Proc. 707, 381 (2004) -- axes to transposed
+// form, then the transposed bits interleaved most significant first.
+static uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+    uint32_t X[3] = {x, y, z};
+    if (bits <= 0) return 0;
+    const uint32_t M = 1u << (bits - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    for (int i = 0; i < 3; ++i) X[i] ^= t;
+    uint64_t h = 0;
+    for (int b = bits - 1; b >= 0; --b)
+        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+    return h;
+}
+
+void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks) {
+    const uint32_t n0 = d.knt[0], n1 = d.knt[1], n2 = d.knt[2];
+    const uint32_t nt = n0 * n1 * n2;
+    int bits = 0;
+    while ((1u << bits) < std::max(n0, std::max(n1, n2))) ++bits;
+    std::vector<std::pair<uint64_t, uint32_t>> order(nt);
+    for (uint32_t x = 0; x < n0; ++x)
+        for (uint32_t y = 0; y < n1; ++y)
+            for (uint32_t z = 0; z < n2; ++z) {
+                const uint32_t t = (x * n1 + y) * n2 + z;
+                order[t] = {hilbert3(x, y, z, bits), t};
+            }
+    std::sort(order.begin(), order.end());
+    const size_t base = ranks.size();
+    ranks.resize(base + nt);
+    for (uint32_t r = 0; r < nt; ++r) ranks[base + order[r].second] = r;
 }
 
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
