@@ -146,7 +146,8 @@ int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls
  *                        1 = reference-shaped kernels (one launch per WGSL kernel;
  *                            single simulators only)
  *   particles_per_thread 1, 2 or 4: a P2G/G2P block covers 256 x this particles
- *   window_cells         LDS window budget per block, 27..8192 cells
+ *   window_cells         LDS budget per block in cells; a block stages up to
+ *                        window_cells/216 (1..16) tile windows of 6^3 cells
  *   sort_period          re-bin particles every k >= 1 substeps
  *   grid_blocks          persistent blocks per launch, 0 = one block per chunk
  *   xcd_remap            1 = each XCD walks a contiguous range of chunks

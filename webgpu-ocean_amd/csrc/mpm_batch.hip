@@ -198,7 +198,7 @@ BatchArgs Batch::args() const {
     a.grid = grid_;
     a.posvel = posvel_;
     a.ppt = tun_.ppt;
-    a.wmax = tun_.wmax;
+    a.max_slots = std::min(16, std::max(1, tun_.wmax / 216));
     a.grid_blocks = tun_.grid_blocks;
     a.xcd_remap = tun_.xcd_remap;
     return a;
@@ -393,7 +393,7 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const int ppts[3] = {1, 2, 4};
-    const int wmaxs[3] = {1024, 2048, 4096};
+    const int wmaxs[3] = {1080, 2160, 3456};  // 5, 10, 16 tile windows
     const int periods[3] = {1, 2, 4};
     for (int pi = 0; pi < 3 && e == hipSuccess; ++pi)
         for (int wi = 0; wi < 3 && e == hipSuccess; ++wi)

@@ -49,7 +49,7 @@ struct BatchArgs {
     int4 *grid;          // all scenes' cells
     float4 *posvel;      // 2 float4 per global caller index
     int ppt;             // particles per thread: chunk = kBlock * ppt
-    int wmax;            // LDS window budget in cells (16 B each)
+    int max_slots;       // tile windows per block (1..16), 216 cells x 16 B each
     int grid_blocks;     // persistent blocks per launch (0: one per chunk)
     int xcd_remap;       // give each XCD a contiguous range of chunks
 };

@@ -7,16 +7,17 @@
 // scene of the batch; blocks walk the table persistently.  Particles live in
 // SoA storage permuted into bin order; id[] maps a storage slot to the
 // caller's particle index, so readout order never changes.  A block stages
-// its chunk's stencil bounding box ("window") in LDS:
-//   p2g_1 / p2g_2: i32 fixed-point ds_add into the window, then one global
+// the 6^3-cell windows (4^3 tile + halo) of the tiles its chunk occupies in
+// LDS:
+//   p2g_1 / p2g_2: i32 fixed-point ds_add into the windows, then one global
 //                  atomic per non-zero window cell and component;
 //   g2p:           window cells loaded once with updateGrid applied on load
 //                  (a pure per-cell function), 27-node gathers from LDS.
 // Integer adds commute, and every float expression is evaluated per particle
 // exactly as in the reference-shaped stage kernels, so results are identical
-// to them.  Particles whose floor(x) is non-finite or beyond +-2^20, and whole
-// chunks whose window exceeds the LDS budget, take the global-memory path of
-// the stage kernels instead.
+// to them.  Particles whose floor(x) is non-finite or far outside any grid,
+// and particles whose tile finds no free window slot, take the global-memory
+// path of the stage kernels instead.
 #include <algorithm>
 #include <cmath>
 #include <utility>
@@ -133,69 +134,82 @@ __device__ __forceinline__ int32_t cell_linear(int32_t x, int32_t y, int32_t z, 
                      (uint32_t)z);
 }
 
-// Window = bounding box of the chunk's stencils, cells [lo, lo + dim).
-struct Window {
-    int lo[3];
-    int dim[3];
-    int vol;  // 0: no window (global path for the whole chunk)
-};
+// ---------------------------------------------------------------------------
+// tile windows: each block registers the 4^3-cell tiles its particles occupy
+// in a small LDS table; tile slot s owns a fixed 6^3-cell window (the tile
+// plus a one-cell halo) at LDS cells [s*216, (s+1)*216).  A particle's
+// 27-node stencil always lies inside its own tile's window.
 
+constexpr int kTileEdge = 1 << kTileBits;            // 4
+constexpr int kWinEdge = kTileEdge + 2;              // 6
+constexpr int kWinCells = kWinEdge * kWinEdge * kWinEdge;  // 216
+constexpr int kMaxSlots = 16;
+constexpr uint32_t kNoTile = 0xffffffffu;
+constexpr int kTileCoordLimit = 512;                 // packed tile coords: 10 bits each
+
+__device__ __forceinline__ uint32_t tile_key(int tx, int ty, int tz) {
+    return ((uint32_t)(tx + kTileCoordLimit) << 20) | ((uint32_t)(ty + kTileCoordLimit) << 10) |
+           (uint32_t)(tz + kTileCoordLimit);
+}
+
+__device__ __forceinline__ void tile_origin(uint32_t key, int o[3]) {
+    // first cell of the slot's window: tile origin minus the halo
+    o[0] = ((int)(key >> 20) - kTileCoordLimit) * kTileEdge - 1;
+    o[1] = ((int)((key >> 10) & 1023u) - kTileCoordLimit) * kTileEdge - 1;
+    o[2] = ((int)(key & 1023u) - kTileCoordLimit) * kTileEdge - 1;
+}
+
+// Registers the tiles of the block's tame particles; slot[k] = the tile's slot
+// or -1 (table full: the particle takes the global path).  Returns the number
+// of slots in use.  Must be called by every thread of the block.
 template <int PPT>
-__device__ __forceinline__ Window block_window(const int ic[PPT][3], const bool tame[PPT],
-                                               int wmax) {
-    __shared__ int s_bb[6];
-    int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-#pragma unroll
-    for (int k = 0; k < PPT; ++k)
-        if (tame[k])
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = min(lo[a], ic[k][a]);
-                hi[a] = max(hi[a], ic[k][a]);
-            }
-    if (threadIdx.x < 6) s_bb[threadIdx.x] = threadIdx.x < 3 ? INT_MAX : INT_MIN;
+__device__ __forceinline__ int block_tiles(const int ic[PPT][3], const bool tame[PPT],
+                                           int max_slots, int slot[PPT], uint32_t *s_tiles) {
+    if (threadIdx.x < kMaxSlots) s_tiles[threadIdx.x] = kNoTile;
     __syncthreads();
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = wave_min(lo[a]);
-        hi[a] = wave_max(hi[a]);
-    }
-    if ((threadIdx.x & (kWave - 1)) == 0) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            atomicMin(&s_bb[a], lo[a]);
-            atomicMax(&s_bb[3 + a], hi[a]);
+    for (int k = 0; k < PPT; ++k) {
+        slot[k] = -1;
+        if (!tame[k]) continue;
+        const int tx = ic[k][0] >> kTileBits, ty = ic[k][1] >> kTileBits,
+                  tz = ic[k][2] >> kTileBits;
+        if (tx < -kTileCoordLimit || tx >= kTileCoordLimit || ty < -kTileCoordLimit ||
+            ty >= kTileCoordLimit || tz < -kTileCoordLimit || tz >= kTileCoordLimit)
+            continue;
+        const uint32_t key = tile_key(tx, ty, tz);
+        // Slots fill as a prefix: a thread tries slot s only after seeing
+        // slots 0..s-1 taken, and a taken slot is never released.
+        for (int sI = 0; sI < max_slots; ++sI) {
+            uint32_t cur = s_tiles[sI];
+            if (cur == kNoTile) cur = atomicCAS(&s_tiles[sI], kNoTile, key);
+            if (cur == kNoTile || cur == key) {
+                slot[k] = sI;
+                break;
+            }
         }
     }
     __syncthreads();
-    Window w;
-    int64_t vol = 1;
-    bool fits = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        int l = s_bb[a], h = s_bb[3 + a];
-        w.lo[a] = l - 1;
-        // tame coordinates lie in [-2^20, 2^20], so h - l + 3 cannot overflow
-        w.dim[a] = (l <= h) ? (h - l + 3) : 0;
-        fits = fits && w.dim[a] > 0 && w.dim[a] <= wmax;
-        vol *= fits ? w.dim[a] : 1;
-    }
-    w.vol = (fits && vol <= wmax) ? (int)vol : 0;
-    return w;
+    int used = 0;
+    for (int sI = 0; sI < max_slots; ++sI) used += s_tiles[sI] != kNoTile;
+    return used;
 }
 
-__device__ __forceinline__ int win_index(const Window &w, int x, int y, int z) {
-    return ((x - w.lo[0]) * w.dim[1] + (y - w.lo[1])) * w.dim[2] + (z - w.lo[2]);
+// LDS cell of node (gx,gy,gz) = l0 + (gx*6 + gy)*6 + gz for a particle whose
+// base cell is `ic` in tile slot `sl`.
+__device__ __forceinline__ int tile_l0(int sl, const int ic[3]) {
+    const int m = kTileEdge - 1;
+    return sl * kWinCells + ((ic[0] & m) * kWinEdge + (ic[1] & m)) * kWinEdge + (ic[2] & m);
 }
 
-__device__ __forceinline__ void win_coords(const Window &w, int l, int &x, int &y, int &z) {
-    int lz = l % w.dim[2];
-    int t = l / w.dim[2];
-    int ly = t % w.dim[1];
-    int lx = t / w.dim[1];
-    x = w.lo[0] + lx;
-    y = w.lo[1] + ly;
-    z = w.lo[2] + lz;
+// Cell coordinates of LDS cell l of the slot table.
+__device__ __forceinline__ void slot_cell(const uint32_t *s_tiles, int l, int &x, int &y,
+                                          int &z) {
+    const int sl = l / kWinCells, r = l - sl * kWinCells;
+    int o[3];
+    tile_origin(s_tiles[sl], o);
+    x = o[0] + r / (kWinEdge * kWinEdge);
+    y = o[1] + (r / kWinEdge) % kWinEdge;
+    z = o[2] + r % kWinEdge;
 }
 
 // Loads a chunk's positions, integer base cells and "tame" flags.
@@ -327,12 +341,12 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers 
 // ---------------------------------------------------------------------------
 // sinks: where a particle's 27 node contributions go / come from
 
-struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x vol
+struct WinSink1 {  // p2g_1 into the LDS windows: [mass | vx | vy | vz] x vol
     int *lds;
-    int vol, l0, d1, d2;
+    int vol, l0;
     __device__ __forceinline__ void add4(const Node3 &, int gx, int gy, int gz, int32_t m,
                                          int32_t x, int32_t y, int32_t z) const {
-        const int l = l0 + (gx * d1 + gy) * d2 + gz;
+        const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
         atomicAdd(&lds[l], m);
         atomicAdd(&lds[vol + l], x);
         atomicAdd(&lds[2 * vol + l], y);
@@ -358,13 +372,13 @@ struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
 
 struct WinSink2 {  // p2g_2: decoded masses [0, vol), momentum ints [vol, 4 vol)
     int *lds;
-    int vol, l0, d1, d2;
+    int vol, l0;
     __device__ __forceinline__ float mass(const Node3 &, int gx, int gy, int gz) const {
-        return reinterpret_cast<const float *>(lds)[l0 + (gx * d1 + gy) * d2 + gz];
+        return reinterpret_cast<const float *>(lds)[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
     }
     __device__ __forceinline__ void add3(const Node3 &, int gx, int gy, int gz, int32_t x,
                                          int32_t y, int32_t z) const {
-        const int l = l0 + (gx * d1 + gy) * d2 + gz;
+        const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
         atomicAdd(&lds[vol + l], x);
         atomicAdd(&lds[2 * vol + l], y);
         atomicAdd(&lds[3 * vol + l], z);
@@ -393,9 +407,9 @@ struct GridSink2 {
 
 struct WinSinkG {  // g2p: updated, decoded velocities staged in LDS
     const float4 *lvel;
-    int l0, d1, d2;
+    int l0;
     __device__ __forceinline__ void vel(const Node3 &, int gx, int gy, int gz, float cv[3]) const {
-        const float4 o = lvel[l0 + (gx * d1 + gy) * d2 + gz];
+        const float4 o = lvel[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
         cv[0] = o.x;
         cv[1] = o.y;
         cv[2] = o.z;
@@ -424,6 +438,7 @@ struct GridSinkG {  // g2p from the raw accumulators, updateGrid applied per nod
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
     extern __shared__ int lds[];  // [mass | vx | vy | vz] x vol
+    __shared__ uint32_t s_tiles[kMaxSlots];
     for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
         const uint32_t c = walk_chunk(cw, a.nchunks);
         if (c == 0xffffffffu) continue;
@@ -437,8 +452,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
         int ic[PPT][3];
         bool valid[PPT], tame[PPT];
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        const Window w = block_window<PPT>(ic, tame, a.wmax);
-        const int vol = w.vol;
+        int slot[PPT];
+        const int vol = kWinCells * block_tiles<PPT>(ic, tame, a.max_slots, slot, s_tiles);
         for (int l = threadIdx.x; l < 4 * vol; l += kBlock) lds[l] = 0;
         __syncthreads();
 
@@ -450,11 +465,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
             load_vC(a.state, a.stride, i, v, C);
             Node3 n;
             node3_init(x[k], n);
-            if (vol > 0 && tame[k]) {
-                const WinSink1 sink{lds, vol,
-                                    win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1),
-                                    w.dim[1], w.dim[2]};
-                p2g1_particle(n, v, C, mult, sink);
+            if (slot[k] >= 0) {
+                p2g1_particle(n, v, C, mult, WinSink1{lds, vol, tile_l0(slot[k], ic[k])});
             } else {
                 p2g1_particle(n, v, C, mult, GridSink1{grid, G, Y, Z});
             }
@@ -462,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
         __syncthreads();
         for (int l = threadIdx.x; l < vol; l += kBlock) {
             int cx, cy, cz;
-            win_coords(w, l, cx, cy, cz);
+            slot_cell(s_tiles, l, cx, cy, cz);
             const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
             if (!in_grid(gi, G)) continue;
             int32_t *cell = grid + 4 * (size_t)gi;
@@ -483,6 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
     extern __shared__ int lds[];  // [decoded mass (f32) | vx | vy | vz] x vol
+    __shared__ uint32_t s_tiles[kMaxSlots];
     float *dmass = reinterpret_cast<float *>(lds);
     for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
         const uint32_t c = walk_chunk(cw, a.nchunks);
@@ -497,11 +510,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
         int ic[PPT][3];
         bool valid[PPT], tame[PPT];
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        const Window w = block_window<PPT>(ic, tame, a.wmax);
-        const int vol = w.vol;
+        int slot[PPT];
+        const int vol = kWinCells * block_tiles<PPT>(ic, tame, a.max_slots, slot, s_tiles);
         for (int l = threadIdx.x; l < vol; l += kBlock) {
             int cx, cy, cz;
-            win_coords(w, l, cx, cy, cz);
+            slot_cell(s_tiles, l, cx, cy, cz);
             const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
             const int32_t m = in_grid(gi, G) ? grid[4 * (size_t)gi + 3] : 0;
             dmass[l] = decode(m, prm.mult);
@@ -522,11 +535,8 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
                 for (int r = 0; r < 3; ++r) C[cc][r] = a.state[(C00 + 3 * cc + r) * a.stride + i];
             Node3 n;
             node3_init(x[k], n);
-            if (vol > 0 && tame[k]) {
-                const WinSink2 sink{lds, vol,
-                                    win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1),
-                                    w.dim[1], w.dim[2]};
-                p2g2_particle(n, C, prm, sink);
+            if (slot[k] >= 0) {
+                p2g2_particle(n, C, prm, WinSink2{lds, vol, tile_l0(slot[k], ic[k])});
             } else {
                 p2g2_particle(n, C, prm, GridSink2{grid, G, Y, Z, prm.mult});
             }
@@ -534,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
         __syncthreads();
         for (int l = threadIdx.x; l < vol; l += kBlock) {
             int cx, cy, cz;
-            win_coords(w, l, cx, cy, cz);
+            slot_cell(s_tiles, l, cx, cy, cz);
             const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
             if (!in_grid(gi, G)) continue;
             int32_t *cell = grid + 4 * (size_t)gi;
@@ -554,6 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
     extern __shared__ float4 lvel[];  // decoded updated velocity x vol
+    __shared__ uint32_t s_tiles[kMaxSlots];
     for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
         const uint32_t c = walk_chunk(cw, a.nchunks);
         if (c == 0xffffffffu) continue;
@@ -568,18 +579,18 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
         int ic[PPT][3];
         bool valid[PPT], tame[PPT];
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        const Window w = block_window<PPT>(ic, tame, a.wmax);
-        const int vol = w.vol;
-        // Window cells inside [0,X) x [0,Y) x [0,Z) without index wrap-around
-        // decompose to their own coordinates; others like updateGrid.wgsl:37-39.
-        const bool nice = vol > 0 && w.lo[0] >= 0 && w.lo[1] >= 0 && w.lo[2] >= 0 &&
-                          w.lo[1] + w.dim[1] <= Y && w.lo[2] + w.dim[2] <= Z &&
-                          (int64_t)(w.lo[0] + w.dim[0]) * Y * Z <= (int64_t)INT32_MAX;
+        int slot[PPT];
+        const int vol = kWinCells * block_tiles<PPT>(ic, tame, a.max_slots, slot, s_tiles);
         for (int l = threadIdx.x; l < vol; l += kBlock) {
             int cx, cy, cz;
-            win_coords(w, l, cx, cy, cz);
+            slot_cell(s_tiles, l, cx, cy, cz);
             const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
             float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            // A cell inside [0,X) x [0,Y) x [0,Z) whose index did not wrap
+            // decomposes to its own coordinates; others like updateGrid.wgsl:37-39.
+            const bool nice = cx >= 0 && (uint32_t)cy < (uint32_t)Y &&
+                              (uint32_t)cz < (uint32_t)Z &&
+                              (int64_t)cx * Y * Z + (int64_t)cy * Z + cz < (int64_t)G;
             if (in_grid(gi, G)) {
                 const int4 raw = grid[gi];
                 const int4 cv = nice ? update_cell_xyz(raw, cx, cy, cz, wall_hi, prm)
@@ -599,10 +610,8 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
             Node3 n;
             node3_init(x[k], n);
             float v[3], B[3][3];
-            if (vol > 0 && tame[k]) {
-                const WinSinkG sink{lvel, win_index(w, ic[k][0] - 1, ic[k][1] - 1, ic[k][2] - 1),
-                                    w.dim[1], w.dim[2]};
-                g2p_gather(n, sink, v, B);
+            if (slot[k] >= 0) {
+                g2p_gather(n, WinSinkG{lvel, tile_l0(slot[k], ic[k])}, v, B);
             } else {
                 g2p_gather(n, GridSinkG{grid, G, Y, Z, {wall_hi[0], wall_hi[1], wall_hi[2]}, prm},
                            v, B);
@@ -647,7 +656,7 @@ inline unsigned walk_blocks(const BatchArgs &a) {
 template <int PPT>
 hipError_t launch_win(int which, const BatchArgs &a, hipStream_t st) {
     if (!a.nchunks) return hipSuccess;
-    const size_t lds = 16 * (size_t)a.wmax;
+    const size_t lds = 16 * (size_t)kWinCells * (size_t)a.max_slots;
     const unsigned nb = walk_blocks(a);
     switch (which) {
     case 1: k_p2g1_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
