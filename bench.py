@@ -81,6 +81,8 @@ def main() -> int:
     ap.add_argument("--particles", type=int, default=300_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-substeps", type=int, default=200)
+    ap.add_argument("--tuning", type=str, default="",
+                    help='JSON launch configuration, e.g. {"particles_per_thread": 1}')
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -95,6 +97,8 @@ def main() -> int:
     n = rec.shape[0]
     sim = MLSMPMSimulator(particles=rec, init_box_size=box)
     g = sim.grid_count
+    if args.tuning:
+        sim.set_tuning(**json.loads(args.tuning))
 
     # warmup on the same scene, then restart from the initial state
     for _ in range(args.warmup):
