@@ -12,8 +12,8 @@ namespace mpm {
 
 struct Tuning {
     int path = 0;          // 0: binned LDS-window kernels, 1: reference-shaped stage kernels
-    int ppt = 2;           // particles per thread: chunk = 256 * ppt particles
-    int wmax = 2160;       // LDS window budget in cells: wmax/216 tile windows (1..16)
+    int ppt = 1;           // particles per thread: chunk = 256 * ppt particles
+    int wmax = 864;        // LDS window budget in cells: wmax/216 tile windows (1..16)
     int sort_period = 1;   // re-bin every k substeps
     int grid_blocks = 0;   // persistent blocks per launch (0: one block per chunk)
     int xcd_remap = 1;     // contiguous chunk ranges per XCD

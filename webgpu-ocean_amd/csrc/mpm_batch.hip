@@ -393,7 +393,7 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const int ppts[3] = {1, 2, 4};
-    const int wmaxs[3] = {1080, 2160, 3456};  // 5, 10, 16 tile windows
+    const int wmaxs[3] = {432, 864, 1728};  // 2, 4, 8 tile windows
     const int periods[3] = {1, 2, 4};
     for (int pi = 0; pi < 3 && e == hipSuccess; ++pi)
         for (int wi = 0; wi < 3 && e == hipSuccess; ++wi)

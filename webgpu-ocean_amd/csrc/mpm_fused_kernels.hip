@@ -33,7 +33,9 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;
-constexpr float kTameLimit = 1048576.0f;  // |floor(x)| <= 2^20: exact int coordinates
+// Tame particles have |floor(x)| < 2040 cells, so their tiles pack into
+// tile_key and their cell coordinates are exact integers.
+constexpr float kTameLimit = 2040.0f;
 
 // ---------------------------------------------------------------------------
 // chunk walk: blocks stride over the chunk table; with xcd_remap, the chunks
@@ -159,24 +161,21 @@ __device__ __forceinline__ void tile_origin(uint32_t key, int o[3]) {
     o[2] = ((int)(key & 1023u) - kTileCoordLimit) * kTileEdge - 1;
 }
 
-// Registers the tiles of the block's tame particles; slot[k] = the tile's slot
-// or -1 (table full: the particle takes the global path).  Returns the number
-// of slots in use.  Must be called by every thread of the block.
+// Registers the tiles of the particles still to do (bit k of `todo`) in the
+// block's slot table; slot[k] = the tile's slot, or -1 when the table is full
+// (the particle waits for the next pass).  Returns the number of slots in use
+// (>= 1 whenever any thread has work).  Called by every thread of the block.
 template <int PPT>
-__device__ __forceinline__ int block_tiles(const int ic[PPT][3], const bool tame[PPT],
-                                           int max_slots, int slot[PPT], uint32_t *s_tiles) {
+__device__ __forceinline__ int block_tiles(const int ic[PPT][3], uint32_t todo, int max_slots,
+                                           int slot[PPT], uint32_t *s_tiles) {
     if (threadIdx.x < kMaxSlots) s_tiles[threadIdx.x] = kNoTile;
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         slot[k] = -1;
-        if (!tame[k]) continue;
-        const int tx = ic[k][0] >> kTileBits, ty = ic[k][1] >> kTileBits,
-                  tz = ic[k][2] >> kTileBits;
-        if (tx < -kTileCoordLimit || tx >= kTileCoordLimit || ty < -kTileCoordLimit ||
-            ty >= kTileCoordLimit || tz < -kTileCoordLimit || tz >= kTileCoordLimit)
-            continue;
-        const uint32_t key = tile_key(tx, ty, tz);
+        if (!(todo & (1u << k))) continue;
+        const uint32_t key = tile_key(ic[k][0] >> kTileBits, ic[k][1] >> kTileBits,
+                                      ic[k][2] >> kTileBits);
         // Slots fill as a prefix: a thread tries slot s only after seeing
         // slots 0..s-1 taken, and a taken slot is never released.
         for (int sI = 0; sI < max_slots; ++sI) {
@@ -229,7 +228,7 @@ __device__ __forceinline__ void load_chunk(const float *__restrict__ st, uint32_
             for (int a = 0; a < 3; ++a) {
                 x[k][a] = st[(PX + a) * stride + i];
                 float ci = floorf(x[k][a]);
-                t = t && (ci >= -kTameLimit) && (ci <= kTameLimit);
+                t = t && (ci >= -kTameLimit) && (ci < kTameLimit);
                 ic[k][a] = t ? (int)ci : 0;
             }
             tame[k] = t;
@@ -448,44 +447,55 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
         const uint32_t G = sd->G;
         const float mult = sd->prm.mult;
         int32_t *grid = (int32_t *)(a.grid + sd->goff);
-        float x[PPT][3];
+        float x[PPT][3], v[PPT][3], C[PPT][3][3];
         int ic[PPT][3];
         bool valid[PPT], tame[PPT];
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        int slot[PPT];
-        const int vol = kWinCells * block_tiles<PPT>(ic, tame, a.max_slots, slot, s_tiles);
-        for (int l = threadIdx.x; l < 4 * vol; l += kBlock) lds[l] = 0;
-        __syncthreads();
-
+        uint32_t todo = 0;
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            const uint32_t i = ch.begin + slot_of<PPT>(k);
-            float v[3], C[3][3];
-            load_vC(a.state, a.stride, i, v, C);
-            Node3 n;
-            node3_init(x[k], n);
-            if (slot[k] >= 0) {
-                p2g1_particle(n, v, C, mult, WinSink1{lds, vol, tile_l0(slot[k], ic[k])});
-            } else {
-                p2g1_particle(n, v, C, mult, GridSink1{grid, G, Y, Z});
+            load_vC(a.state, a.stride, ch.begin + slot_of<PPT>(k), v[k], C[k]);
+            if (tame[k]) todo |= 1u << k;
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            if (valid[k] && !tame[k]) {
+                Node3 n;
+                node3_init(x[k], n);
+                p2g1_particle(n, v[k], C[k], mult, GridSink1{grid, G, Y, Z});
             }
+        // one pass per group of at most max_slots tiles
+        while (__syncthreads_or(todo != 0)) {
+            int slot[PPT];
+            const int vol = kWinCells * block_tiles<PPT>(ic, todo, a.max_slots, slot, s_tiles);
+            for (int l = threadIdx.x; l < 4 * vol; l += kBlock) lds[l] = 0;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (slot[k] < 0) continue;
+                Node3 n;
+                node3_init(x[k], n);
+                p2g1_particle(n, v[k], C[k], mult, WinSink1{lds, vol, tile_l0(slot[k], ic[k])});
+                todo &= ~(1u << k);
+            }
+            __syncthreads();
+            for (int l = threadIdx.x; l < vol; l += kBlock) {
+                int cx, cy, cz;
+                slot_cell(s_tiles, l, cx, cy, cz);
+                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+                if (!in_grid(gi, G)) continue;
+                int32_t *cell = grid + 4 * (size_t)gi;
+                const int32_t m = lds[l], vx = lds[vol + l], vy = lds[2 * vol + l],
+                              vz = lds[3 * vol + l];
+                if (m) atomicAdd(cell + 3, m);
+                if (vx) atomicAdd(cell + 0, vx);
+                if (vy) atomicAdd(cell + 1, vy);
+                if (vz) atomicAdd(cell + 2, vz);
+            }
+            // the next pass (or chunk) reuses LDS and the slot table; the
+            // barrier at the loop head orders them
         }
-        __syncthreads();
-        for (int l = threadIdx.x; l < vol; l += kBlock) {
-            int cx, cy, cz;
-            slot_cell(s_tiles, l, cx, cy, cz);
-            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-            if (!in_grid(gi, G)) continue;
-            int32_t *cell = grid + 4 * (size_t)gi;
-            const int32_t m = lds[l], vx = lds[vol + l], vy = lds[2 * vol + l],
-                          vz = lds[3 * vol + l];
-            if (m) atomicAdd(cell + 3, m);
-            if (vx) atomicAdd(cell + 0, vx);
-            if (vy) atomicAdd(cell + 1, vy);
-            if (vz) atomicAdd(cell + 2, vz);
-        }
-        __syncthreads();  // LDS is reused by the next chunk
     }
 }
 
@@ -506,54 +516,63 @@ __global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
         const uint32_t G = sd->G;
         const Params prm = sd->prm;
         int32_t *grid = (int32_t *)(a.grid + sd->goff);
-        float x[PPT][3];
+        float x[PPT][3], C[PPT][3][3];
         int ic[PPT][3];
         bool valid[PPT], tame[PPT];
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        int slot[PPT];
-        const int vol = kWinCells * block_tiles<PPT>(ic, tame, a.max_slots, slot, s_tiles);
-        for (int l = threadIdx.x; l < vol; l += kBlock) {
-            int cx, cy, cz;
-            slot_cell(s_tiles, l, cx, cy, cz);
-            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-            const int32_t m = in_grid(gi, G) ? grid[4 * (size_t)gi + 3] : 0;
-            dmass[l] = decode(m, prm.mult);
-            lds[vol + l] = 0;
-            lds[2 * vol + l] = 0;
-            lds[3 * vol + l] = 0;
-        }
-        __syncthreads();
-
+        uint32_t todo = 0;
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
             const uint32_t i = ch.begin + slot_of<PPT>(k);
-            float C[3][3];
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                for (int r = 0; r < 3; ++r) C[cc][r] = a.state[(C00 + 3 * cc + r) * a.stride + i];
-            Node3 n;
-            node3_init(x[k], n);
-            if (slot[k] >= 0) {
-                p2g2_particle(n, C, prm, WinSink2{lds, vol, tile_l0(slot[k], ic[k])});
-            } else {
-                p2g2_particle(n, C, prm, GridSink2{grid, G, Y, Z, prm.mult});
+                for (int r = 0; r < 3; ++r) C[k][cc][r] = a.state[(C00 + 3 * cc + r) * a.stride + i];
+            if (tame[k]) todo |= 1u << k;
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; ++k)
+            if (valid[k] && !tame[k]) {
+                Node3 n;
+                node3_init(x[k], n);
+                p2g2_particle(n, C[k], prm, GridSink2{grid, G, Y, Z, prm.mult});
+            }
+        while (__syncthreads_or(todo != 0)) {
+            int slot[PPT];
+            const int vol = kWinCells * block_tiles<PPT>(ic, todo, a.max_slots, slot, s_tiles);
+            for (int l = threadIdx.x; l < vol; l += kBlock) {
+                int cx, cy, cz;
+                slot_cell(s_tiles, l, cx, cy, cz);
+                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+                const int32_t m = in_grid(gi, G) ? grid[4 * (size_t)gi + 3] : 0;
+                dmass[l] = decode(m, prm.mult);
+                lds[vol + l] = 0;
+                lds[2 * vol + l] = 0;
+                lds[3 * vol + l] = 0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (slot[k] < 0) continue;
+                Node3 n;
+                node3_init(x[k], n);
+                p2g2_particle(n, C[k], prm, WinSink2{lds, vol, tile_l0(slot[k], ic[k])});
+                todo &= ~(1u << k);
+            }
+            __syncthreads();
+            for (int l = threadIdx.x; l < vol; l += kBlock) {
+                int cx, cy, cz;
+                slot_cell(s_tiles, l, cx, cy, cz);
+                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+                if (!in_grid(gi, G)) continue;
+                int32_t *cell = grid + 4 * (size_t)gi;
+                const int32_t vx = lds[vol + l], vy = lds[2 * vol + l], vz = lds[3 * vol + l];
+                if (vx) atomicAdd(cell + 0, vx);
+                if (vy) atomicAdd(cell + 1, vy);
+                if (vz) atomicAdd(cell + 2, vz);
             }
         }
-        __syncthreads();
-        for (int l = threadIdx.x; l < vol; l += kBlock) {
-            int cx, cy, cz;
-            slot_cell(s_tiles, l, cx, cy, cz);
-            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-            if (!in_grid(gi, G)) continue;
-            int32_t *cell = grid + 4 * (size_t)gi;
-            const int32_t vx = lds[vol + l], vy = lds[2 * vol + l], vz = lds[3 * vol + l];
-            if (vx) atomicAdd(cell + 0, vx);
-            if (vy) atomicAdd(cell + 1, vy);
-            if (vz) atomicAdd(cell + 2, vz);
-        }
-        __syncthreads();
     }
 }
 
@@ -575,53 +594,21 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
         const uint32_t G = sd->G;
         const Params prm = sd->prm;
         const int4 *grid = a.grid + sd->goff;
+        const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
         float x[PPT][3];
         int ic[PPT][3];
         bool valid[PPT], tame[PPT];
         load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        int slot[PPT];
-        const int vol = kWinCells * block_tiles<PPT>(ic, tame, a.max_slots, slot, s_tiles);
-        for (int l = threadIdx.x; l < vol; l += kBlock) {
-            int cx, cy, cz;
-            slot_cell(s_tiles, l, cx, cy, cz);
-            const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-            float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            // A cell inside [0,X) x [0,Y) x [0,Z) whose index did not wrap
-            // decomposes to its own coordinates; others like updateGrid.wgsl:37-39.
-            const bool nice = cx >= 0 && (uint32_t)cy < (uint32_t)Y &&
-                              (uint32_t)cz < (uint32_t)Z &&
-                              (int64_t)cx * Y * Z + (int64_t)cy * Z + cz < (int64_t)G;
-            if (in_grid(gi, G)) {
-                const int4 raw = grid[gi];
-                const int4 cv = nice ? update_cell_xyz(raw, cx, cy, cz, wall_hi, prm)
-                                     : update_cell_id(raw, gi, Y, Z, wall_hi, prm);
-                o = make_float4(decode(cv.x, prm.mult), decode(cv.y, prm.mult),
-                                decode(cv.z, prm.mult), 0.0f);
-            }
-            lvel[l] = o;
-        }
-        __syncthreads();
-
-        const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
-#pragma unroll
-        for (int k = 0; k < PPT; ++k) {
-            if (!valid[k]) continue;
+        uint32_t todo = 0;
+        // gather, then the particle's tail (C, advection, walls) and stores
+        auto finish = [&](int k, float v[3], float B[3][3]) {
             const uint32_t i = ch.begin + slot_of<PPT>(k);
-            Node3 n;
-            node3_init(x[k], n);
-            float v[3], B[3][3];
-            if (slot[k] >= 0) {
-                g2p_gather(n, WinSinkG{lvel, tile_l0(slot[k], ic[k])}, v, B);
-            } else {
-                g2p_gather(n, GridSinkG{grid, G, Y, Z, {wall_hi[0], wall_hi[1], wall_hi[2]}, prm},
-                           v, B);
-            }
-            float xo[3], C[3][3];
-            g2p_finish(x[k], rb, prm.dt, v, B, xo, C);
+            float xo[3], Cn[3][3];
+            g2p_finish(x[k], rb, prm.dt, v, B, xo, Cn);
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-                for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * cc + r) * a.stride + i] = C[cc][r];
+                for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * cc + r) * a.stride + i] = Cn[cc][r];
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
                 a.state[(PX + ax) * a.stride + i] = xo[ax];
@@ -630,8 +617,55 @@ __global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
             const size_t o = a.id[i];
             a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
             a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
+        };
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            if (!valid[k]) continue;
+            if (tame[k]) {
+                todo |= 1u << k;
+                continue;
+            }
+            Node3 n;
+            node3_init(x[k], n);
+            float v[3], B[3][3];
+            g2p_gather(n, GridSinkG{grid, G, Y, Z, {wall_hi[0], wall_hi[1], wall_hi[2]}, prm}, v,
+                       B);
+            finish(k, v, B);
         }
-        __syncthreads();
+        while (__syncthreads_or(todo != 0)) {
+            int slot[PPT];
+            const int vol = kWinCells * block_tiles<PPT>(ic, todo, a.max_slots, slot, s_tiles);
+            for (int l = threadIdx.x; l < vol; l += kBlock) {
+                int cx, cy, cz;
+                slot_cell(s_tiles, l, cx, cy, cz);
+                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
+                float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                // A cell inside [0,X) x [0,Y) x [0,Z) whose index did not wrap
+                // decomposes to its own coordinates; others like updateGrid.wgsl:37-39.
+                const bool nice = cx >= 0 && (uint32_t)cy < (uint32_t)Y &&
+                                  (uint32_t)cz < (uint32_t)Z &&
+                                  (int64_t)cx * Y * Z + (int64_t)cy * Z + cz < (int64_t)G;
+                if (in_grid(gi, G)) {
+                    const int4 raw = grid[gi];
+                    const int4 cv = nice ? update_cell_xyz(raw, cx, cy, cz, wall_hi, prm)
+                                         : update_cell_id(raw, gi, Y, Z, wall_hi, prm);
+                    o = make_float4(decode(cv.x, prm.mult), decode(cv.y, prm.mult),
+                                    decode(cv.z, prm.mult), 0.0f);
+                }
+                lvel[l] = o;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (slot[k] < 0) continue;
+                Node3 n;
+                node3_init(x[k], n);
+                float v[3], B[3][3];
+                g2p_gather(n, WinSinkG{lvel, tile_l0(slot[k], ic[k])}, v, B);
+                finish(k, v, B);
+                todo &= ~(1u << k);
+            }
+        }
     }
 }
 
