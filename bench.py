@@ -48,6 +48,23 @@ def stage_bytes(name: str, n: int, g: int) -> int:
     return table[name]
 
 
+def calibration_copy(nbytes: int = 64 << 20) -> None:
+    """One device-to-device copy of a known size through the HIP runtime, so a
+    profiler's FETCH_SIZE / WRITE_SIZE for it calibrate the counters
+    (the copy reads and writes exactly nbytes)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
     """Oracle (CPU restatement of the reference WGSL) timed on the host cores."""
     import oracle
@@ -81,6 +98,8 @@ def main() -> int:
     ap.add_argument("--particles", type=int, default=300_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-substeps", type=int, default=200)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 64 MiB device copy (counter calibration under a profiler)")
     ap.add_argument("--tuning", type=str, default="",
                     help='JSON launch configuration, e.g. {"particles_per_thread": 1}')
     args = ap.parse_args()
@@ -92,6 +111,9 @@ def main() -> int:
         return 2
 
     from mlsmpm import MLSMPMSimulator, scenes
+
+    if args.calibrate:
+        calibration_copy()
 
     rec, box = scenes.config1(n=args.particles)
     n = rec.shape[0]
