@@ -171,6 +171,26 @@ int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in);
  * counterpart. */
 int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen);
 
+/* ---- Differentiable rollout (no reference counterpart: the reference has
+ * no gradients).  Runs `substeps` substeps from the current state, keeping a
+ * checkpoint every `checkpoint_every` substeps, evaluates
+ *     loss = |centre of mass - target|^2 + ke_weight * mean |v|^2
+ * on the final state, and back-propagates through a hand-written adjoint of
+ * every substep (recomputing each segment from its checkpoint).  Fixed-point
+ * encode/decode are treated as the identity.  Outputs (any may be NULL):
+ * d loss / d initial velocity and position (n x 3 floats, caller order),
+ * d loss / d stiffness and / d dynamic_viscosity.  Afterwards the simulator
+ * holds the rollout's final state, as after mpm_step(sim, substeps). */
+typedef struct {
+    uint32_t substeps;
+    uint32_t checkpoint_every;
+    float target[3];
+    float ke_weight;
+} mpm_adjoint_config;
+
+int mpm_backward(mpm_sim *sim, const mpm_adjoint_config *cfg, float *grad_v0, float *grad_x0,
+                 double *loss, float *grad_stiffness, float *grad_viscosity);
+
 /* ---- Batched scenes (no reference counterpart: the reference runs one scene).
  * Many independent scenes with their own particle counts, boxes and
  * parameters, packed into shared buffers and advanced by ONE launch per stage

@@ -74,6 +74,11 @@ def lib():
         L.oracle_pow5.restype = ctypes.c_float
         L.oracle_pow5.argtypes = [ctypes.c_float]
         L.oracle_weights.argtypes = [ctypes.c_float, F3]
+        D3 = ctypes.POINTER(ctypes.c_double)
+        L.oracle_rollout_loss_f64.restype = ctypes.c_double
+        L.oracle_rollout_loss_f64.argtypes = [P, P, u32, F3, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, u32, D3,
+                                              ctypes.c_double, P]
         _lib = L
     return _lib
 
@@ -159,3 +164,19 @@ def weights(x: float) -> np.ndarray:
     w = (ctypes.c_float * 3)()
     lib().oracle_weights(x, w)
     return np.array(list(w), dtype=np.float32)
+
+
+def rollout_loss_f64(particles, box, substeps, target, ke_weight=1e-3, v0=None,
+                     stiffness=3.0, viscosity=0.1, rest_density=4.0, dt=0.2,
+                     return_state=False):
+    """FP64 unquantized rollout (oracle_rollout_loss_f64): the loss whose
+    central differences check the HIP adjoint."""
+    rec = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 20)
+    n = rec.shape[0]
+    v = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64).reshape(n, 3)
+    tgt = (ctypes.c_double * 3)(*[float(t) for t in target])
+    out = np.zeros((n, 20), dtype=np.float32) if return_state else None
+    loss = lib().oracle_rollout_loss_f64(
+        _ptr(rec), None if v is None else _ptr(v), n, _f3(box), stiffness, viscosity,
+        rest_density, dt, substeps, tgt, ke_weight, None if out is None else _ptr(out))
+    return (loss, out) if return_state else loss

@@ -68,6 +68,15 @@ void oracle_substep_omp(oracle_particle *p, uint32_t n, oracle_cell *cells, uint
                         oracle_posvel *posvel, const float real_box[3], const float init_box[3],
                         const oracle_params *prm, int nthreads);
 
+/* FP64, unquantized rollout of `substeps` substeps (real box = init box);
+ * returns |com - target|^2 + ke_weight * mean|v|^2 of the final state.
+ * v0 (n x 3, may be NULL) overrides the records' velocities; final_state
+ * (may be NULL) receives the final particles as 80-B f32 records. */
+double oracle_rollout_loss_f64(const float *particles, const double *v0, uint32_t n,
+                               const float init_box[3], double stiffness, double viscosity,
+                               double rest_density, double dt, uint32_t substeps,
+                               const double target[3], double ke_weight, float *final_state);
+
 /* Scalar helpers exported so tests can pin them directly. */
 int32_t oracle_encode(float f, float mult);
 float   oracle_decode(int32_t i, float mult);

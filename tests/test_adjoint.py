@@ -1,0 +1,110 @@
+"""Differentiable rollout (mpm_backward): the hand-written adjoint against
+FP64 central finite differences of the unquantized model.
+
+The FP64 rollout in oracle/mpm_oracle_f64.c is itself pinned on the CPU
+against the FP32 oracle (same trajectory to fixed-point / rounding noise),
+and its central differences are checked to converge in h."""
+import numpy as np
+import pytest
+
+import oracle
+from mlsmpm import MLSMPMSimulator, scenes
+
+SUBSTEPS, CKPT, H = 64, 8, 1e-4
+
+
+def fd_scene():
+    return scenes.config4(n=1000, edge=16.0)
+
+
+# --- CPU: pin the FP64 forward and the FD reference -----------------------
+
+def test_f64_rollout_tracks_the_f32_oracle():
+    rec, box, target = fd_scene()
+    ref = oracle.Scene(rec, box)
+    ref.execute(4)
+    _, st64 = oracle.rollout_loss_f64(rec, box, 4, target, return_state=True)
+    assert np.max(np.abs(st64[:, 0:3] - ref.particles[:, 0:3])) < 1e-4
+    assert np.max(np.abs(st64[:, 4:7] - ref.particles[:, 4:7])) < 1e-3
+    # the loss definition: |com - target|^2 + w * mean|v|^2
+    loss = oracle.rollout_loss_f64(rec, box, 4, target, ke_weight=0.5)
+    p = st64.astype(np.float64)
+    want = np.sum((p[:, 0:3].mean(axis=0) - target) ** 2) + 0.5 * np.mean(np.sum(p[:, 4:7] ** 2, axis=1))
+    assert abs(loss - want) < 1e-5 * abs(want)
+
+
+def test_central_differences_converge_in_h():
+    rec, box, target = fd_scene()
+    v0 = rec[:, 4:7].astype(np.float64)
+    u = np.random.default_rng(1).normal(size=v0.shape)
+    d = []
+    for h in (1e-3, 1e-4, 1e-5):
+        lp = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, v0=v0 + h * u)
+        lm = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, v0=v0 - h * u)
+        d.append((lp - lm) / (2 * h))
+    assert abs(d[1] - d[2]) < 5e-4 * abs(d[2])   # h = 1e-4 is converged well inside 1e-3
+    assert abs(d[0] - d[2]) < 1e-2 * abs(d[2])
+
+
+# --- GPU: the adjoint ------------------------------------------------------
+
+def fd_dir(rec, box, target, u, h=H):
+    v0 = rec[:, 4:7].astype(np.float64)
+    lp = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, v0=v0 + h * u)
+    lm = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, v0=v0 - h * u)
+    return (lp - lm) / (2 * h)
+
+
+@pytest.mark.gpu
+def test_adjoint_matches_fp64_central_differences():
+    rec, box, target = fd_scene()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        g = sim.backward(SUBSTEPS, target, ke_weight=1e-3, checkpoint_every=CKPT)
+    loss64 = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target)
+    assert abs(g["loss"] - loss64) <= 1e-3 * abs(loss64)
+    rng = np.random.default_rng(2)
+    tol = 1e-3   # relative, FP32 adjoint on the quantized trajectory vs FP64 FD
+    for trial in range(3):
+        u = rng.normal(size=(rec.shape[0], 3))
+        fd = fd_dir(rec, box, target, u)
+        ad = float(np.sum(g["grad_v0"].astype(np.float64) * u))
+        assert abs(ad - fd) <= tol * abs(fd), f"direction {trial}: adjoint {ad} fd {fd}"
+    for name, kw, base in (("grad_stiffness", "stiffness", 3.0),
+                           ("grad_viscosity", "viscosity", 0.1)):
+        lp = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, **{kw: base + H})
+        lm = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, **{kw: base - H})
+        fd = (lp - lm) / (2 * H)
+        assert abs(g[name] - fd) <= tol * abs(fd), f"{name}: adjoint {g[name]} fd {fd}"
+    # the largest per-particle velocity gradients, component by component
+    flat = np.argsort(-np.abs(g["grad_v0"]).ravel())[:4]
+    for f in flat:
+        u = np.zeros(rec.shape[0] * 3)
+        u[f] = 1.0
+        fd = fd_dir(rec, box, target, u.reshape(-1, 3))
+        ad = float(g["grad_v0"].ravel()[f])
+        assert abs(ad - fd) <= 5 * tol * abs(fd), f"component {f}: adjoint {ad} fd {fd}"
+
+
+@pytest.mark.gpu
+def test_checkpoint_interval_does_not_change_gradients():
+    rec, box, target = fd_scene()
+    res = []
+    for k in (1, 8, 64):
+        with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+            res.append(sim.backward(SUBSTEPS, target, checkpoint_every=k))
+    for r in res[1:]:
+        assert r["loss"] == res[0]["loss"]
+        np.testing.assert_allclose(r["grad_v0"], res[0]["grad_v0"], rtol=1e-4, atol=1e-7)
+        assert abs(r["grad_stiffness"] - res[0]["grad_stiffness"]) <= 1e-4 * abs(res[0]["grad_stiffness"])
+
+
+@pytest.mark.gpu
+def test_backward_leaves_the_rollout_final_state():
+    rec, box, target = fd_scene()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as a, \
+            MLSMPMSimulator(particles=rec, init_box_size=box) as b:
+        a.backward(24, target, checkpoint_every=5)
+        b.execute(24)
+        pa, pb = a.read_particles(), b.read_particles()
+        assert np.array_equal(pa.view(np.int32), pb.view(np.int32))
+        assert np.array_equal(a.read_posvel().view(np.int32), b.read_posvel().view(np.int32))

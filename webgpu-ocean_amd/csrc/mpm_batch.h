@@ -29,6 +29,23 @@ struct SceneInit {
     Params prm;
 };
 
+// Differentiable rollout (mpm_backward): loss = |com - target|^2 +
+// ke_weight * mean |v|^2 after `substeps` substeps.
+struct AdjointConfig {
+    uint32_t substeps;
+    uint32_t checkpoint_every;
+    double target[3];
+    double ke_weight;
+};
+
+struct AdjointResult {
+    double loss = 0.0;
+    float *grad_v0 = nullptr;  // n x 3 (host, caller order) or null
+    float *grad_x0 = nullptr;  // n x 3 (host, caller order) or null
+    float grad_stiffness = 0.0f;
+    float grad_viscosity = 0.0f;
+};
+
 class Batch {
 public:
     Batch() = default;
@@ -63,6 +80,11 @@ public:
     // Times candidate tunings on a snapshot of the current state (restored
     // afterwards) and keeps the fastest.
     hipError_t autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *chosen);
+
+    // Forward rollout from the current state with checkpoints, then the
+    // adjoint back to the current state (single-scene batches).  Leaves the
+    // batch at the rollout's final state.
+    hipError_t backward(const AdjointConfig &cfg, hipStream_t st, AdjointResult &res);
 
 private:
     hipError_t ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys);

@@ -390,6 +390,29 @@ int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen) 
     return MPM_OK;
 }
 
+int mpm_backward(mpm_sim *sim, const mpm_adjoint_config *cfg, float *grad_v0, float *grad_x0,
+                 double *loss, float *grad_stiffness, float *grad_viscosity) {
+    if (!sim || !cfg) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (cfg->substeps == 0 || cfg->checkpoint_every == 0)
+        return fail(MPM_ERR_INVALID_ARGUMENT, "substeps and checkpoint_every must be >= 1");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(cfg->target[a]))
+            return fail(MPM_ERR_INVALID_ARGUMENT, "target must be finite");
+    if (!std::isfinite(cfg->ke_weight)) return fail(MPM_ERR_INVALID_ARGUMENT, "ke_weight must be finite");
+    int rc = select_device(sim->ctx);
+    if (rc) return rc;
+    mpm::AdjointConfig ac{cfg->substeps, cfg->checkpoint_every,
+                          {cfg->target[0], cfg->target[1], cfg->target[2]}, cfg->ke_weight};
+    mpm::AdjointResult res;
+    res.grad_v0 = grad_v0;
+    res.grad_x0 = grad_x0;
+    HIP_TRY(sim->ctx.batch.backward(ac, sim->ctx.stream, res));
+    if (loss) *loss = res.loss;
+    if (grad_stiffness) *grad_stiffness = res.grad_stiffness;
+    if (grad_viscosity) *grad_viscosity = res.grad_viscosity;
+    return MPM_OK;
+}
+
 // ---- batch of independent scenes ---------------------------------------------
 
 int mpm_batch_create(const mpm_scene *scenes, uint32_t count, mpm_batch **out) {

@@ -30,7 +30,7 @@ EXPORTED = (
     "mpm_synchronize", "mpm_read_posvel", "mpm_posvel_device_ptr",
     "mpm_read_particles", "mpm_write_particles", "mpm_read_grid", "mpm_write_grid",
     "mpm_get_counts", "mpm_time_steps", "mpm_time_stages", "mpm_get_tuning",
-    "mpm_set_tuning", "mpm_autotune",
+    "mpm_set_tuning", "mpm_autotune", "mpm_backward",
     "mpm_batch_create", "mpm_batch_destroy", "mpm_batch_step", "mpm_batch_set_box",
     "mpm_batch_synchronize", "mpm_batch_get_counts", "mpm_batch_read_posvel",
     "mpm_batch_read_particles", "mpm_batch_get_tuning", "mpm_batch_set_tuning",
@@ -62,6 +62,15 @@ class Params(ctypes.Structure):
         ("dynamic_viscosity", ctypes.c_float),
         ("dt", ctypes.c_float),
         ("fixed_point_multiplier", ctypes.c_float),
+    ]
+
+
+class AdjointConfig(ctypes.Structure):
+    _fields_ = [
+        ("substeps", ctypes.c_uint32),
+        ("checkpoint_every", ctypes.c_uint32),
+        ("target", ctypes.c_float * 3),
+        ("ke_weight", ctypes.c_float),
     ]
 
 
@@ -112,6 +121,8 @@ def load(path: str = LIB_PATH):
         "mpm_get_tuning": ([P, ctypes.POINTER(Tuning)], i32),
         "mpm_set_tuning": ([P, ctypes.POINTER(Tuning)], i32),
         "mpm_autotune": ([P, u32, ctypes.POINTER(Tuning)], i32),
+        "mpm_backward": ([P, ctypes.POINTER(AdjointConfig), P, P, ctypes.POINTER(ctypes.c_double),
+                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], i32),
         "mpm_batch_create": ([ctypes.POINTER(Scene), u32, ctypes.POINTER(P)], i32),
         "mpm_batch_destroy": ([P], i32),
         "mpm_batch_step": ([P, u32], i32),

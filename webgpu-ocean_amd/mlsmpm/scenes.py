@@ -156,3 +156,19 @@ def config3(seed: int = 3, scenes: int = 64, n_lo: int = 5_000, n_hi: int = 200_
                 "dynamic_viscosity": float(rng.uniform(0.02, 0.2))}
         out.append((records(pts), box, phys))
     return out
+
+
+def config4(seed: int = 4, n: int = 100_000, edge: float = 64.0, vel: float = 0.5):
+    """Differentiable scene: seeded dam-break block (spacing 0.55, lower-left
+    45% of x and z) with initial velocities ~ U(-vel, vel), and a seeded
+    target point for the centre-of-mass loss.  The FD-check instance is
+    config4(n=1000, edge=16)."""
+    rng = np.random.default_rng(seed)
+    box = [edge, edge, edge]
+    frac = 0.45 if edge >= 32 else 0.6
+    lo = (3.0, 3.0, 3.0)
+    hi = (3.0 + frac * (edge - 6), edge - 3.0, 3.0 + frac * (edge - 6))
+    pts = jittered_block(lo, hi, 0.55, n, rng, 0.1)
+    rec = records(pts, rng.uniform(-vel, vel, size=(n, 3)))
+    target = rng.uniform(0.3 * edge, 0.7 * edge, size=3).tolist()
+    return rec, box, target

@@ -170,6 +170,24 @@ class MLSMPMSimulator:
         t = _capi.Tuning(**cur)
         _capi.check(self._L.mpm_set_tuning(self._h, ctypes.byref(t)))
 
+    def backward(self, substeps: int, target, ke_weight: float = 1e-3,
+                 checkpoint_every: int = 8) -> dict:
+        """Differentiable rollout from the current state (mpm_backward):
+        loss = |com - target|^2 + ke_weight * mean|v|^2 after `substeps`.
+        Returns loss and d loss / d (initial v, initial x, stiffness,
+        viscosity); the simulator ends at the rollout's final state."""
+        n = self.num_particles
+        cfg = _capi.AdjointConfig(substeps, checkpoint_every,
+                                  (ctypes.c_float * 3)(*[float(t) for t in target]), ke_weight)
+        gv = np.zeros((n, 3), dtype=np.float32)
+        gx = np.zeros((n, 3), dtype=np.float32)
+        loss = ctypes.c_double()
+        gk, gmu = ctypes.c_float(), ctypes.c_float()
+        _capi.check(self._L.mpm_backward(self._h, ctypes.byref(cfg), self._data(gv), self._data(gx),
+                                         ctypes.byref(loss), ctypes.byref(gk), ctypes.byref(gmu)))
+        return {"loss": loss.value, "grad_v0": gv, "grad_x0": gx,
+                "grad_stiffness": gk.value, "grad_viscosity": gmu.value}
+
     def autotune(self, substeps_per_trial: int = 4) -> dict:
         t = _capi.Tuning()
         _capi.check(self._L.mpm_autotune(self._h, substeps_per_trial, ctypes.byref(t)))
