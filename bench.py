@@ -90,6 +90,82 @@ def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
     }
 
 
+def bench_batched(substeps: int = 20) -> dict:
+    """64 uneven scenes in one launch per stage (autotuned) vs the same scenes
+    advanced one after another by single simulators; results are identical
+    per scene (tests/test_batch_gpu.py)."""
+    from mlsmpm import MLSMPMBatch, MLSMPMSimulator, Params, scenes
+
+    sc = scenes.config3()
+    params = []
+    for _, _, phys in sc:
+        p = Params(3.0, 4.0, 0.1, 0.2, 1e7)
+        p.stiffness, p.dynamic_viscosity = phys["stiffness"], phys["dynamic_viscosity"]
+        params.append(p)
+    total = sum(r.shape[0] for r, _, _ in sc)
+    with MLSMPMBatch([(r, b, p) for (r, b, _), p in zip(sc, params)]) as bt:
+        tuning = bt.autotune(4)
+        bt.step(2)
+        bt.synchronize()
+        t0 = time.perf_counter()
+        bt.step(substeps)
+        bt.synchronize()
+        batched_s = time.perf_counter() - t0
+    sims = [MLSMPMSimulator(params=p, particles=r, init_box_size=b)
+            for (r, b, _), p in zip(sc, params)]
+    for sim in sims:
+        sim.execute(2)
+    for sim in sims:
+        sim.synchronize()
+    t0 = time.perf_counter()
+    for sim in sims:
+        sim.execute(substeps)
+    for sim in sims:
+        sim.synchronize()
+    seq_s = time.perf_counter() - t0
+    for sim in sims:
+        sim.close()
+    return {
+        "workload": "64 uneven scenes (5k-200k particles, edges 32-96), %d substeps" % substeps,
+        "particles": total,
+        "value": total * substeps / batched_s,
+        "sequential_value": total * substeps / seq_s,
+        "speedup": seq_s / batched_s,
+        "unit": "particle-substeps/s",
+        "tuning": tuning,
+    }
+
+
+def bench_adjoint(substeps: int = 64, every: int = 8) -> dict:
+    """100k-particle 64^3 differentiable rollout: backward (forward with
+    checkpoints + recomputation + adjoint) vs the plain forward rollout."""
+    from mlsmpm import MLSMPMSimulator, scenes
+
+    rec, box, target = scenes.config4()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        sim.execute(4)
+        sim.reset(rec.shape[0], box, particles=rec)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.execute(substeps)
+        sim.synchronize()
+        fwd = time.perf_counter() - t0
+        sim.reset(rec.shape[0], box, particles=rec)
+        sim.backward(4, target, checkpoint_every=2)   # warm the adjoint kernels
+        sim.reset(rec.shape[0], box, particles=rec)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        g = sim.backward(substeps, target, checkpoint_every=every)
+        bwd = time.perf_counter() - t0
+    return {
+        "workload": "100k particles, 64^3, %d substeps, checkpoint every %d" % (substeps, every),
+        "forward_ms": 1e3 * fwd,
+        "backward_ms": 1e3 * bwd,
+        "cost_ratio": bwd / fwd,
+        "loss": g["loss"],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +173,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=300_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched and adjoint comparisons")
     ap.add_argument("--stage-substeps", type=int, default=200)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 64 MiB device copy (counter calibration under a profiler)")
@@ -196,6 +274,9 @@ def main() -> int:
             "wall_us_per_substep": 1e6 * elapsed / substeps,
         },
     }
+    if not args.no_extras:
+        out["batched"] = bench_batched()
+        out["adjoint"] = bench_adjoint()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rec, box)
     print(json.dumps(out), flush=True)
