@@ -142,22 +142,19 @@ int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls
 
 /* Launch configuration of the production substep (no reference counterpart:
  * the reference fixes @workgroup_size(64) and one thread per item).
- *   path                 0 = binned LDS-window kernels (default),
+ *   path                 0 = binned tile-partial kernels (default),
  *                        1 = reference-shaped kernels (one launch per WGSL kernel;
  *                            single simulators only)
- *   particles_per_thread 1, 2 or 4: a P2G/G2P block covers 256 x this particles
- *   window_cells         LDS budget per block in cells; a block stages up to
- *                        window_cells/216 (1..16) tile windows of 6^3 cells
- *   sort_period          re-bin particles every k >= 1 substeps
- *   grid_blocks          persistent blocks per launch, 0 = one block per chunk
+ *   particles_per_thread 1, 2 or 4: a P2G/G2P block takes a chunk of at most
+ *                        256 x this particles of one 4^3-cell tile
+ *   grid_blocks          persistent blocks of the chunk kernels, 0 = 2048,
+ *                        at most 2^20
  *   xcd_remap            1 = each XCD walks a contiguous range of chunks
  * Results do not depend on these (integer adds commute; per-particle float
  * math is unchanged); only speed does. */
 typedef struct {
     int path;
     int particles_per_thread;
-    int window_cells;
-    int sort_period;
     int grid_blocks;
     int xcd_remap;
 } mpm_tuning;
@@ -165,8 +162,8 @@ typedef struct {
 int mpm_get_tuning(mpm_sim *sim, mpm_tuning *out);
 int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in);
 
-/* Autotuner: times candidate tunings (particles_per_thread x window_cells x
- * sort_period) for `substeps_per_trial` substeps each on a snapshot of the
+/* Autotuner: times candidate tunings (particles_per_thread x grid_blocks x
+ * xcd_remap) for `substeps_per_trial` substeps each on a snapshot of the
  * current state, restores the state and keeps the fastest.  No reference
  * counterpart. */
 int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen);

@@ -40,9 +40,8 @@ def oracle_scene(rec, box, phys):
     return oracle.Scene(rec, box, oracle.default_params(**phys))
 
 
-@pytest.mark.parametrize("tuning", [None, dict(particles_per_thread=1, sort_period=2,
-                                                grid_blocks=5, xcd_remap=0),
-                                    dict(particles_per_thread=4, window_cells=64)])
+@pytest.mark.parametrize("tuning", [None, dict(particles_per_thread=1, grid_blocks=5, xcd_remap=0),
+                                    dict(particles_per_thread=4, grid_blocks=64)])
 def test_batch_matches_oracle_per_scene(tuning):
     sc = small_batch()
     refs = [oracle_scene(r, b, p) for r, b, p in sc]

@@ -194,17 +194,20 @@ def test_errors_are_status_codes():
 
 
 TUNINGS = [
-    dict(path=0, particles_per_thread=1, window_cells=2048, sort_period=1),
-    dict(path=0, particles_per_thread=2, window_cells=2048, sort_period=3),
-    dict(path=0, particles_per_thread=4, window_cells=8192, sort_period=2),
-    dict(path=0, particles_per_thread=2, window_cells=40, sort_period=1),   # global fallback
-    dict(path=1, particles_per_thread=2, window_cells=2048, sort_period=1),
+    dict(path=0, particles_per_thread=1, grid_blocks=0, xcd_remap=1),
+    dict(path=0, particles_per_thread=2, grid_blocks=24, xcd_remap=1),
+    dict(path=0, particles_per_thread=4, grid_blocks=3, xcd_remap=0),
+    dict(path=0, particles_per_thread=1, grid_blocks=4096, xcd_remap=0),
+    dict(path=1, particles_per_thread=2, grid_blocks=0, xcd_remap=1),
 ]
+# integral init box: tame particles take the tile-partial path; a non-integral
+# one sends every particle down the global-atomic path
+BOXES = [[40.0, 36.0, 44.0], [40.5, 36.0, 43.25]]
 
 
+@pytest.mark.parametrize("box", BOXES, ids=["integral", "fractional"])
 @pytest.mark.parametrize("tun", TUNINGS, ids=lambda t: "-".join(str(v) for v in t.values()))
-def test_tunings_do_not_change_results(tun):
-    box = [40.0, 36.0, 44.0]
+def test_tunings_do_not_change_results(tun, box):
     rec = scenes.small_scene(20000, box, seed=21, velocity=1.5, affine=0.4)
     sim, ref = pair(rec, box)
     with sim:
@@ -227,7 +230,7 @@ def test_time_stages_reports_every_kernel_slot():
     rec, box = scenes.config0()
     with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
         t = sim.time_stages(8)
-        assert set(t) == {"bin", "clear_grid", "p2g_1", "p2g_2", "g2p_fused"}
+        assert set(t) == {"bin", "p2g_1", "reduce_1", "p2g_2", "reduce_2", "g2p"}
         assert all(ms > 0 for ms, calls in t.values())
         assert t["p2g_1"][1] == 8
         ms = sim.time_steps(4)

@@ -569,9 +569,7 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         };
         auto restore = [&](const char *src) -> hipError_t {
             TRY(hipMemcpyAsync(state_, src, sbytes, hipMemcpyDeviceToDevice, st));
-            TRY(hipMemcpyAsync(id_, src + sbytes, ibytes, hipMemcpyDeviceToDevice, st));
-            since_sort_ = 0;
-            return hipSuccess;
+            return hipMemcpyAsync(id_, src + sbytes, ibytes, hipMemcpyDeviceToDevice, st);
         };
         // forward rollout with checkpoints every K substeps
         for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {

@@ -11,11 +11,9 @@
 namespace mpm {
 
 struct Tuning {
-    int path = 0;          // 0: binned LDS-window kernels, 1: reference-shaped stage kernels
-    int ppt = 1;           // particles per thread: chunk = 256 * ppt particles
-    int wmax = 864;        // LDS window budget in cells: wmax/216 tile windows (1..16)
-    int sort_period = 1;   // re-bin every k substeps
-    int grid_blocks = 0;   // persistent blocks per launch (0: one block per chunk)
+    int path = 0;          // 0: binned tile-partial kernels, 1: reference-shaped stage kernels
+    int ppt = 1;           // particles per thread: tile chunks of <= 256 * ppt particles
+    int grid_blocks = 0;   // persistent blocks of the chunk kernels (0: 2048)
     int xcd_remap = 1;     // contiguous chunk ranges per XCD
 };
 
@@ -87,26 +85,31 @@ public:
     hipError_t backward(const AdjointConfig &cfg, hipStream_t st, AdjointResult &res);
 
 private:
-    hipError_t ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys);
+    hipError_t ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys,
+                               uint64_t tchunks);
     hipError_t upload_chunks();
     BatchArgs args() const;
-    hipError_t launch_substep(hipStream_t st, bool sort, hipEvent_t *ev);
+    BinBuffers bin_buffers() const;
+    hipError_t launch_substep(hipStream_t st, hipEvent_t *ev);
     void drop_graphs();
-    bool sort_due() const;
-    void after_substep(bool sort);
+    void after_substep();
 
     std::vector<SceneDesc> h_scenes_;
     std::vector<Chunk> h_chunks_;
     SceneDesc *d_scenes_ = nullptr;
     Chunk *d_chunks_ = nullptr;
-    uint32_t *d_tile_rank_ = nullptr;
-    size_t tile_rank_cap_ = 0;
+    uint32_t *d_tile_rank_ = nullptr;  // per scene: x-major tile -> Hilbert rank
+    uint32_t *d_tile_xmaj_ = nullptr;  // key tile -> x-major tile (kWildTile: wild)
+    uint32_t *d_nbr_ = nullptr;        // 27 neighbour key tiles per spatial tile
+    uint4 *d_tile_info_ = nullptr;     // (x, y, z, scene) per spatial tile
+    size_t tile_rank_cap_ = 0, tile_xmaj_cap_ = 0, tile_tab_cap_ = 0;
     size_t chunk_cap_ = 0;
 
     uint32_t total_n_ = 0, stride_ = 0;
     uint64_t total_cells_ = 0, total_keys_ = 0;
+    uint32_t total_tiles_ = 0;  // spatial tiles of all scenes
     uint32_t cap_n_ = 0;
-    uint64_t cap_cells_ = 0, cap_keys_ = 0;
+    uint64_t cap_cells_ = 0, cap_keys_ = 0, cap_tchunks_ = 0;
     uint32_t scene_cap_ = 0;
 
     float *state_ = nullptr, *spare_state_ = nullptr;
@@ -114,18 +117,22 @@ private:
     int4 *grid_ = nullptr;
     float4 *posvel_ = nullptr;
     float *staging_ = nullptr;  // 80-B records of every particle
-    uint32_t *key_ = nullptr, *rank_ = nullptr, *count_ = nullptr, *offset_ = nullptr,
-             *block_sums_ = nullptr;
+    float4 *vel_ = nullptr;     // decoded updated grid velocities
+    int4 *wild_ = nullptr;      // wild-particle accumulators, zero between substeps
+    int4 *partial_ = nullptr;   // kWinCells per tile chunk
+    uint32_t *key_ = nullptr, *rank_ = nullptr, *count_ = nullptr, *offset_ = nullptr;
+    uint2 *block_sums_ = nullptr;
+    TileChunk *tchunks_ = nullptr;
+    uint32_t *ntchunks_ = nullptr;
+    uint2 *tile_chunk_ = nullptr;  // key tile -> (first tile chunk, count)
 
     Tuning tun_{};
-    unsigned since_sort_ = 0;
 
     struct GraphEntry {
-        bool sort;
         float *state;
         hipGraphExec_t exec;
     };
-    GraphEntry graphs_[4] = {};
+    GraphEntry graphs_[2] = {};
     int ngraphs_ = 0;
 };
 

@@ -10,7 +10,9 @@
 namespace mpm {
 namespace {
 
-const char *const kFusedNames[5] = {"bin", "clear_grid", "p2g_1", "p2g_2", "g2p_fused"};
+constexpr int kFusedSlots = 6;
+const char *const kFusedNames[kFusedSlots] = {"bin",   "p2g_1",    "reduce_1",
+                                              "p2g_2", "reduce_2", "g2p"};
 const char *const kStageNames[6] = {"clear_grid", "p2g_1", "p2g_2",
                                     "update_grid", "g2p", "copy_position"};
 
@@ -50,15 +52,20 @@ Uniforms derive_uniforms(const float init_box[3], const float real_box[3]) {
     return u;
 }
 
-hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys) {
+hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys,
+                                  uint64_t tchunks) {
     const uint32_t cap = std::max<uint32_t>(64u, (total_n + 63u) & ~63u);
-    if (cap <= cap_n_ && cells <= cap_cells_ && keys <= cap_keys_) return hipSuccess;
+    if (cap <= cap_n_ && cells <= cap_cells_ && keys <= cap_keys_ && tchunks <= cap_tchunks_)
+        return hipSuccess;
     drop_graphs();
     free_ptr(state_);
     free_ptr(spare_state_);
     free_ptr(id_);
     free_ptr(spare_id_);
     free_ptr(grid_);
+    free_ptr(vel_);
+    free_ptr(wild_);
+    free_ptr(partial_);
     free_ptr(posvel_);
     free_ptr(staging_);
     free_ptr(key_);
@@ -66,25 +73,38 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     free_ptr(count_);
     free_ptr(offset_);
     free_ptr(block_sums_);
+    free_ptr(tchunks_);
+    free_ptr(ntchunks_);
+    free_ptr(tile_chunk_);
     cap_n_ = 0;
-    cap_cells_ = cap_keys_ = 0;
-    const size_t c = cap, g = std::max<uint64_t>(cells, 1), k = std::max<uint64_t>(keys, 1);
+    cap_cells_ = cap_keys_ = cap_tchunks_ = 0;
+    const size_t c = cap, g = std::max<uint64_t>(cells, 1), k = std::max<uint64_t>(keys, 1),
+                 tc = std::max<uint64_t>(tchunks, 1);
     TRY(hipMalloc(&state_, sizeof(float) * NCOMP * c));
     TRY(hipMalloc(&spare_state_, sizeof(float) * NCOMP * c));
     TRY(hipMalloc(&id_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&spare_id_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&grid_, sizeof(int4) * g));
+    TRY(hipMalloc(&vel_, sizeof(float4) * g));
+    TRY(hipMalloc(&wild_, sizeof(int4) * g));
+    TRY(hipMemset(wild_, 0, sizeof(int4) * g));
+    TRY(hipMalloc(&partial_, sizeof(int4) * kWinCells * tc));
     TRY(hipMalloc(&posvel_, 2 * sizeof(float4) * c));
     TRY(hipMalloc(&staging_, 80 * c));
     TRY(hipMalloc(&key_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&rank_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
     TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
-    TRY(hipMalloc(&block_sums_, sizeof(uint32_t) * (k / 4096 + 1)));
+    TRY(hipMalloc(&block_sums_, sizeof(uint2) * (k / 4096 + 1)));
+    TRY(hipMalloc(&tchunks_, sizeof(TileChunk) * tc));
+    TRY(hipMalloc(&ntchunks_, sizeof(uint32_t)));
+    TRY(hipMalloc(&tile_chunk_, sizeof(uint2) * (k / kTileCells)));
     TRY(hipMemset(count_, 0, sizeof(uint32_t) * k));
+    TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t)));
     cap_n_ = cap;
     cap_cells_ = g;
     cap_keys_ = k;
+    cap_tchunks_ = tc;
     return hipSuccess;
 }
 
@@ -92,7 +112,7 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
     TRY(hipStreamSynchronize(st));
     drop_graphs();
     h_scenes_.assign(count, SceneDesc{});
-    uint64_t poff = 0, goff = 0, koff = 0;
+    uint64_t poff = 0, goff = 0, koff = 0, soff = 0;
     for (uint32_t s = 0; s < count; ++s) {
         SceneDesc &d = h_scenes_[s];
         d.uni = derive_uniforms(sc[s].init_box, sc[s].init_box);
@@ -105,32 +125,64 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
                          (uint64_t)std::ceil(sc[s].init_box[2]));
         scene_key_geometry(d);
         d.koff = (uint32_t)koff;
+        d.toff = (uint32_t)(koff / kTileCells);
+        d.soff = (uint32_t)soff;
         d.rank_off = 0;  // set below
         poff += d.n;
         goff += d.G;
         koff += d.nkeys;
+        soff += d.ntiles;
     }
     total_n_ = (uint32_t)poff;
     total_cells_ = goff;
     total_keys_ = koff;
-    TRY(ensure_capacity(total_n_, total_cells_, total_keys_));
+    total_tiles_ = (uint32_t)soff;
+    // Tile chunks of one binning: sum over key tiles of ceil(count / 256) <=
+    // n / 256 + (non-empty key tiles).
+    const uint64_t key_tiles = koff / kTileCells;
+    const uint64_t max_tchunks = (poff + kBlock - 1) / kBlock + std::min<uint64_t>(poff, key_tiles);
+    TRY(ensure_capacity(total_n_, total_cells_, total_keys_, max_tchunks));
     stride_ = cap_n_;
     if (count > scene_cap_) {
         free_ptr(d_scenes_);
         TRY(hipMalloc(&d_scenes_, sizeof(SceneDesc) * count));
         scene_cap_ = count;
     }
-    std::vector<uint32_t> ranks;
+    std::vector<uint32_t> ranks, xmaj, nbr;
+    std::vector<uint4> info;
     for (uint32_t s = 0; s < count; ++s) {
-        h_scenes_[s].rank_off = (uint32_t)ranks.size();
-        append_hilbert_ranks(h_scenes_[s], ranks);
+        SceneDesc &d = h_scenes_[s];
+        d.rank_off = (uint32_t)ranks.size();
+        const size_t x0 = xmaj.size();
+        append_hilbert_ranks(d, ranks, xmaj);
+        append_tile_tables(d, s, ranks.data() + d.rank_off, xmaj.data() + x0, nbr, info);
+    }
+    if (info.size() > tile_tab_cap_) {
+        free_ptr(d_nbr_);
+        free_ptr(d_tile_info_);
+        TRY(hipMalloc(&d_nbr_, sizeof(uint32_t) * nbr.size()));
+        TRY(hipMalloc(&d_tile_info_, sizeof(uint4) * info.size()));
+        tile_tab_cap_ = info.size();
+    }
+    if (!info.empty()) {
+        TRY(hipMemcpy(d_nbr_, nbr.data(), sizeof(uint32_t) * nbr.size(), hipMemcpyHostToDevice));
+        TRY(hipMemcpy(d_tile_info_, info.data(), sizeof(uint4) * info.size(),
+                      hipMemcpyHostToDevice));
     }
     if (ranks.size() > tile_rank_cap_) {
         free_ptr(d_tile_rank_);
         TRY(hipMalloc(&d_tile_rank_, sizeof(uint32_t) * ranks.size()));
         tile_rank_cap_ = ranks.size();
     }
-    TRY(hipMemcpy(d_tile_rank_, ranks.data(), sizeof(uint32_t) * ranks.size(),
+    if (xmaj.size() > tile_xmaj_cap_) {
+        free_ptr(d_tile_xmaj_);
+        TRY(hipMalloc(&d_tile_xmaj_, sizeof(uint32_t) * xmaj.size()));
+        tile_xmaj_cap_ = xmaj.size();
+    }
+    if (!ranks.empty())
+        TRY(hipMemcpy(d_tile_rank_, ranks.data(), sizeof(uint32_t) * ranks.size(),
+                      hipMemcpyHostToDevice));
+    TRY(hipMemcpy(d_tile_xmaj_, xmaj.data(), sizeof(uint32_t) * xmaj.size(),
                   hipMemcpyHostToDevice));
     TRY(hipMemcpy(d_scenes_, h_scenes_.data(), sizeof(SceneDesc) * count,
                   hipMemcpyHostToDevice));
@@ -144,12 +196,11 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
     all.n = total_n_;
     TRY(launch_unpack_particles(staging_, all, st));
     TRY(upload_chunks());
-    since_sort_ = 0;
     return hipStreamSynchronize(st);
 }
 
 hipError_t Batch::upload_chunks() {
-    const uint32_t per = (uint32_t)kBlock * (uint32_t)tun_.ppt;
+    const uint32_t per = (uint32_t)kBlock;
     h_chunks_.clear();
     for (uint32_t s = 0; s < h_scenes_.size(); ++s) {
         const SceneDesc &d = h_scenes_[s];
@@ -189,19 +240,45 @@ DeviceScene Batch::stage_view() const {
 BatchArgs Batch::args() const {
     BatchArgs a{};
     a.scenes = d_scenes_;
+    a.nscenes = (uint32_t)h_scenes_.size();
     a.tile_rank = d_tile_rank_;
+    a.tile_xmaj = d_tile_xmaj_;
     a.chunks = d_chunks_;
     a.nchunks = (uint32_t)h_chunks_.size();
+    a.tchunks = tchunks_;
+    a.ntchunks = ntchunks_;
+    a.tile_chunk = tile_chunk_;
+    a.stiles = total_tiles_;
+    a.nbr = d_nbr_;
+    a.tile_info = d_tile_info_;
     a.state = state_;
     a.id = id_;
     a.stride = stride_;
     a.grid = grid_;
+    a.vel = vel_;
+    a.wild = wild_;
+    a.partial = partial_;
     a.posvel = posvel_;
     a.ppt = tun_.ppt;
-    a.max_slots = std::min(16, std::max(1, tun_.wmax / 216));
     a.grid_blocks = tun_.grid_blocks;
     a.xcd_remap = tun_.xcd_remap;
     return a;
+}
+
+BinBuffers Batch::bin_buffers() const {
+    BinBuffers b{};
+    b.key = key_;
+    b.rank = rank_;
+    b.count = count_;
+    b.offset = offset_;
+    b.block_sums = block_sums_;
+    b.nkeys = (uint32_t)total_keys_;
+    b.state_out = spare_state_;
+    b.id_out = spare_id_;
+    b.tchunks = tchunks_;
+    b.ntchunks = ntchunks_;
+    b.tile_chunk = tile_chunk_;
+    return b;
 }
 
 hipError_t Batch::set_box(uint32_t s, const float real_box[3], hipStream_t st) {
@@ -210,20 +287,16 @@ hipError_t Batch::set_box(uint32_t s, const float real_box[3], hipStream_t st) {
     return launch_set_scene_uniforms(d_scenes_ + s, d.uni, st);
 }
 
-bool Batch::sort_due() const {
-    return tun_.path == 0 && total_n_ > 0 && (since_sort_ % (unsigned)tun_.sort_period) == 0;
-}
-
-void Batch::after_substep(bool sort) {
-    if (sort) {
+// The binned path re-bins every substep: the tile chunks are only valid for
+// the positions they were built from.
+void Batch::after_substep() {
+    if (tun_.path == 0) {
         std::swap(state_, spare_state_);
         std::swap(id_, spare_id_);
-        since_sort_ = 0;
     }
-    ++since_sort_;
 }
 
-hipError_t Batch::launch_substep(hipStream_t st, bool sort, hipEvent_t *ev) {
+hipError_t Batch::launch_substep(hipStream_t st, hipEvent_t *ev) {
     int k = 0;
     auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
     if (tun_.path == 1) {
@@ -236,34 +309,31 @@ hipError_t Batch::launch_substep(hipStream_t st, bool sort, hipEvent_t *ev) {
     }
     BatchArgs a = args();
     TRY(mark());
-    if (sort) {
-        BinBuffers b{key_, rank_, count_, offset_, block_sums_, (uint32_t)total_keys_,
-                     spare_state_, spare_id_};
-        TRY(launch_bin(a, b, st));
-        a.state = spare_state_;
-        a.id = spare_id_;
-    }
-    TRY(mark());
-    TRY(launch_clear(grid_, (uint32_t)total_cells_, st));
+    TRY(launch_bin(a, bin_buffers(), st));
+    a.state = spare_state_;
+    a.id = spare_id_;
     TRY(mark());
     TRY(launch_p2g1(a, st));
     TRY(mark());
+    TRY(launch_reduce1(a, st));
+    TRY(mark());
     TRY(launch_p2g2(a, st));
+    TRY(mark());
+    TRY(launch_reduce2(a, st));
     TRY(mark());
     TRY(launch_g2p(a, st));
     return mark();
 }
 
 hipError_t Batch::substep(hipStream_t st) {
-    const bool sort = sort_due();
     GraphEntry *g = nullptr;
     for (int i = 0; i < ngraphs_; ++i)
-        if (graphs_[i].sort == sort && graphs_[i].state == state_) g = &graphs_[i];
+        if (graphs_[i].state == state_) g = &graphs_[i];
     if (!g) {
-        if (ngraphs_ == 4) drop_graphs();
+        if (ngraphs_ == 2) drop_graphs();
         hipGraph_t graph = nullptr;
         TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_substep(st, sort, nullptr);
+        hipError_t le = launch_substep(st, nullptr);
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (le != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -274,20 +344,20 @@ hipError_t Batch::substep(hipStream_t st) {
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return e;
-        graphs_[ngraphs_] = GraphEntry{sort, state_, exec};
+        graphs_[ngraphs_] = GraphEntry{state_, exec};
         g = &graphs_[ngraphs_++];
     }
     TRY(hipGraphLaunch(g->exec, st));
-    after_substep(sort);
+    after_substep();
     return hipSuccess;
 }
 
 hipError_t Batch::time_stages(hipStream_t st, unsigned substeps, float *ms, int *calls,
                               const char **names, int capacity, int *count) {
     const bool fused = tun_.path == 0;
-    const int nslots = fused ? 5 : 6;
+    const int nslots = fused ? kFusedSlots : 6;
     if (capacity < nslots) return hipErrorInvalidValue;
-    hipEvent_t ev[7];
+    hipEvent_t ev[kFusedSlots + 1];
     for (int i = 0; i <= nslots; ++i) TRY(hipEventCreate(&ev[i]));
     for (int i = 0; i < nslots; ++i) {
         ms[i] = 0.0f;
@@ -296,17 +366,15 @@ hipError_t Batch::time_stages(hipStream_t st, unsigned substeps, float *ms, int 
     }
     hipError_t e = hipSuccess;
     for (unsigned k = 0; k < substeps && e == hipSuccess; ++k) {
-        const bool sort = sort_due();
-        e = launch_substep(st, sort, ev);
+        e = launch_substep(st, ev);
         if (e == hipSuccess) e = hipEventSynchronize(ev[nslots]);
         for (int i = 0; i < nslots && e == hipSuccess; ++i) {
-            if (fused && i == 0 && !sort) continue;
             float t = 0.0f;
             e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
             ms[i] += t;
             if (calls) calls[i] += 1;
         }
-        after_substep(sort);
+        after_substep();
     }
     for (int i = 0; i <= nslots; ++i) (void)hipEventDestroy(ev[i]);
     *count = nslots;
@@ -338,7 +406,6 @@ hipError_t Batch::write_particles(uint32_t s, const void *src, uint32_t n, hipSt
         TRY(hipMemcpyAsync(staging_ + 20 * (size_t)h_scenes_[s].poff, src, 80 * (size_t)n,
                            hipMemcpyHostToDevice, st));
     TRY(launch_unpack_particles(staging_, all, st));
-    since_sort_ = 0;
     return hipStreamSynchronize(st);
 }
 
@@ -359,14 +426,12 @@ hipError_t Batch::write_grid(uint32_t s, const int32_t *src, uint64_t ncells, hi
 hipError_t Batch::set_tuning(const Tuning &t) {
     if (t.path < 0 || t.path > 1) return hipErrorInvalidValue;
     if (t.ppt != 1 && t.ppt != 2 && t.ppt != 4) return hipErrorInvalidValue;
-    if (t.wmax < 27 || t.wmax > 8192) return hipErrorInvalidValue;
-    if (t.sort_period < 1 || t.grid_blocks < 0 || t.xcd_remap < 0 || t.xcd_remap > 1)
+    if (t.grid_blocks < 0 || t.grid_blocks > (1 << 20) || t.xcd_remap < 0 || t.xcd_remap > 1)
         return hipErrorInvalidValue;
     if (t.path == 1 && h_scenes_.size() > 1) return hipErrorInvalidValue;
     drop_graphs();
-    const bool rechunk = t.ppt != tun_.ppt;
     tun_ = t;
-    return rechunk ? upload_chunks() : hipSuccess;
+    return hipSuccess;
 }
 
 hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *chosen) {
@@ -385,7 +450,6 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     };
     const Tuning orig = tun_;
     float *const orig_state = state_;
-    const unsigned orig_since = since_sort_;
     hipError_t e = save();
     Tuning best = orig;
     float best_ms = 1e30f;
@@ -393,16 +457,16 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     const int ppts[3] = {1, 2, 4};
-    const int wmaxs[3] = {432, 864, 1728};  // 2, 4, 8 tile windows
-    const int periods[3] = {1, 2, 4};
+    const int blocks[3] = {1024, 2048, 4096};
+    const int remaps[2] = {1, 0};
     for (int pi = 0; pi < 3 && e == hipSuccess; ++pi)
-        for (int wi = 0; wi < 3 && e == hipSuccess; ++wi)
-            for (int si = 0; si < 3 && e == hipSuccess; ++si) {
+        for (int bi = 0; bi < 3 && e == hipSuccess; ++bi)
+            for (int ri = 0; ri < 2 && e == hipSuccess; ++ri) {
                 Tuning t = orig;
                 t.path = 0;
                 t.ppt = ppts[pi];
-                t.wmax = wmaxs[wi];
-                t.sort_period = periods[si];
+                t.grid_blocks = blocks[bi];
+                t.xcd_remap = remaps[ri];
                 if ((e = set_tuning(t)) != hipSuccess) break;
                 // restore the snapshot into the current buffers
                 if (state_ != orig_state) {
@@ -411,18 +475,15 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
                 }
                 e = hipMemcpyAsync(state_, snap, sb, hipMemcpyDeviceToDevice, st);
                 if (e == hipSuccess) e = hipMemcpyAsync(id_, snap + sb, ib, hipMemcpyDeviceToDevice, st);
-                since_sort_ = 0;
                 // eager launches: graph capture per candidate would dominate
                 for (unsigned k = 0; k < 2 && e == hipSuccess; ++k) {
-                    const bool sort = sort_due();
-                    e = launch_substep(st, sort, nullptr);
-                    after_substep(sort);
+                    e = launch_substep(st, nullptr);
+                    after_substep();
                 }
                 if (e == hipSuccess) e = hipEventRecord(e0, st);
                 for (unsigned k = 0; k < substeps_per_trial && e == hipSuccess; ++k) {
-                    const bool sort = sort_due();
-                    e = launch_substep(st, sort, nullptr);
-                    after_substep(sort);
+                    e = launch_substep(st, nullptr);
+                    after_substep();
                 }
                 if (e == hipSuccess) e = hipEventRecord(e1, st);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -447,7 +508,6 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     (void)hipFree(snap);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    since_sort_ = orig_since;
     if (e == hipSuccess) e = r;
     if (e != hipSuccess) {
         (void)set_tuning(orig);
@@ -470,6 +530,9 @@ void Batch::release() {
     free_ptr(id_);
     free_ptr(spare_id_);
     free_ptr(grid_);
+    free_ptr(vel_);
+    free_ptr(wild_);
+    free_ptr(partial_);
     free_ptr(posvel_);
     free_ptr(staging_);
     free_ptr(key_);
@@ -477,12 +540,18 @@ void Batch::release() {
     free_ptr(count_);
     free_ptr(offset_);
     free_ptr(block_sums_);
+    free_ptr(tchunks_);
+    free_ptr(ntchunks_);
+    free_ptr(tile_chunk_);
     free_ptr(d_scenes_);
     free_ptr(d_chunks_);
     free_ptr(d_tile_rank_);
-    tile_rank_cap_ = 0;
+    free_ptr(d_tile_xmaj_);
+    free_ptr(d_nbr_);
+    free_ptr(d_tile_info_);
+    tile_rank_cap_ = tile_xmaj_cap_ = tile_tab_cap_ = 0;
     cap_n_ = 0;
-    cap_cells_ = cap_keys_ = 0;
+    cap_cells_ = cap_keys_ = cap_tchunks_ = 0;
     scene_cap_ = 0;
     chunk_cap_ = 0;
     h_scenes_.clear();

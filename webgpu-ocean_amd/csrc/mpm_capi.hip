@@ -119,8 +119,6 @@ mpm::Params to_params(const mpm_params &p) {
 void to_c_tuning(const mpm::Tuning &t, mpm_tuning *out) {
     out->path = t.path;
     out->particles_per_thread = t.ppt;
-    out->window_cells = t.wmax;
-    out->sort_period = t.sort_period;
     out->grid_blocks = t.grid_blocks;
     out->xcd_remap = t.xcd_remap;
 }
@@ -129,15 +127,12 @@ int from_c_tuning(const mpm_tuning *in, mpm::Batch &b) {
     mpm::Tuning t;
     t.path = in->path;
     t.ppt = in->particles_per_thread;
-    t.wmax = in->window_cells;
-    t.sort_period = in->sort_period;
     t.grid_blocks = in->grid_blocks;
     t.xcd_remap = in->xcd_remap;
     if (b.set_tuning(t) != hipSuccess)
         return fail(MPM_ERR_INVALID_ARGUMENT,
                     "tuning out of range: path 0|1 (1 only for one scene), "
-                    "particles_per_thread 1|2|4, window_cells 27..8192, sort_period >= 1, "
-                    "grid_blocks >= 0, xcd_remap 0|1");
+                    "particles_per_thread 1|2|4, grid_blocks 0..2^20, xcd_remap 0|1");
     return MPM_OK;
 }
 

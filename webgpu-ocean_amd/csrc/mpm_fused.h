@@ -2,8 +2,12 @@
 //
 // The kernels serve a *batch* of independent scenes (one scene is a batch of
 // one): particle slots, grid cells and bin keys of all scenes are packed into
-// single buffers, and every stage is ONE launch over a table of particle
-// chunks, each chunk lying inside one scene.
+// shared buffers, and every stage is one launch over all scenes.
+//
+// Substep: bin (counting sort of particles by tile-major cell key; the scan
+// also emits the list of tile chunks) -> p2g_1 (LDS tile windows -> partial
+// windows) -> reduce_1 (per tile: sum the partials covering it; writes every
+// cell) -> p2g_2 -> reduce_2 (+ updateGrid -> decoded velocities) -> g2p.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -14,11 +18,18 @@
 
 namespace mpm {
 
-// Bin keys are tile-major: tiles of (2^kTileBits)^3 cells ordered along a 3D
-// Hilbert curve (consecutive tiles share a face, so a run of particles in key
-// order covers a compact box), cells x-major inside a tile.
+// Tiles are (2^kTileBits)^3 cells, ordered along a 3D Hilbert curve per scene;
+// bin key = koff + (tile rank << 6 | x-major cell within the tile).  Each
+// scene has one extra key tile (rank = ntiles) collecting the particles whose
+// stencil leaves the grid ("wild" particles).
 constexpr int kTileBits = 2;
 constexpr int kBlock = 256;
+constexpr int kTileEdge = 1 << kTileBits;                      // 4
+constexpr int kTileCells = kTileEdge * kTileEdge * kTileEdge;  // 64
+constexpr int kWinEdge = kTileEdge + 2;                        // 6
+constexpr int kWinCells = kWinEdge * kWinEdge * kWinEdge;      // 216
+constexpr uint32_t kWildTile = 0xffffffffu;
+constexpr uint32_t kNoTile = 0xffffffffu;  // neighbour outside the grid
 
 // Per-scene constants, device resident.
 struct SceneDesc {
@@ -26,53 +37,86 @@ struct SceneDesc {
     Params prm;          // WGSL override constants of this scene
     uint32_t poff, n;    // particle slots [poff, poff + n)
     uint32_t goff, G;    // grid cells [goff, goff + G)
-    uint32_t koff;       // first bin key of the scene
-    int32_t kdim[3];     // ceil(init_box): cells per axis used for keys
-    uint32_t knt[3];     // key tiles per axis
-    uint32_t nkeys;      // knt0*knt1*knt2 * 2^(3*kTileBits)
-    uint32_t rank_off;   // first entry of the scene's tile -> Hilbert rank table
+    uint32_t koff;       // first bin key of the scene (multiple of 64)
+    uint32_t toff;       // first key tile (koff / 64)
+    uint32_t soff;       // first spatial tile in the reduce launch
+    uint32_t ntiles;     // spatial tiles knt0*knt1*knt2 (the wild tile is extra)
+    int32_t kdim[3];     // ceil(init_box): cells per axis
+    uint32_t knt[3];     // tiles per axis
+    uint32_t nkeys;      // (ntiles + 1) * 64
+    uint32_t rank_off;   // first entry of the scene's x-major tile -> rank table
+    uint32_t fast;       // 1: integer box (cells map to distinct indices), 0: all wild
 };
 
-// A run of consecutive particle slots of one scene, handled by one block.
+// A run of consecutive particle slots of one scene (static partition used by
+// the binning kernels).
 struct Chunk {
     uint32_t scene, begin, count, pad;
 };
 
+// A run of at most 256*ppt particles of one tile (emitted by the scan):
+// x = scene, y = x-major tile index or kWildTile, z = first slot, w = count.
+using TileChunk = uint4;
+
 struct BatchArgs {
     const SceneDesc *scenes;
-    const uint32_t *tile_rank;  // per scene: x-major tile index -> Hilbert rank
-    const Chunk *chunks;
+    uint32_t nscenes;
+    const uint32_t *tile_rank;   // rank_off + x-major tile -> Hilbert rank
+    const uint32_t *tile_xmaj;   // key tile (toff + rank) -> x-major tile (kWildTile: wild)
+    const Chunk *chunks;         // static slot chunks
     uint32_t nchunks;
-    float *state;        // SoA particle state, NCOMP x stride, bin order
-    uint32_t *id;        // slot -> global caller index (poff + local index)
+    const TileChunk *tchunks;    // tile chunks of the current binning
+    const uint32_t *ntchunks;    // their number (device)
+    const uint2 *tile_chunk;     // key tile -> (first tile chunk, count)
+    uint32_t stiles;             // spatial tiles of all scenes (reduce launch)
+    const uint32_t *nbr;         // 27 per spatial tile: key tile of each neighbour or kNoTile
+    const uint4 *tile_info;      // per spatial tile: (tile x, y, z, scene)
+    float *state;                // SoA particle state, NCOMP x stride, bin order
+    uint32_t *id;                // slot -> global caller index (poff + local index)
     uint32_t stride;
-    int4 *grid;          // all scenes' cells
-    float4 *posvel;      // 2 float4 per global caller index
-    int ppt;             // particles per thread: chunk = kBlock * ppt
-    int max_slots;       // tile windows per block (1..16), 216 cells x 16 B each
-    int grid_blocks;     // persistent blocks per launch (0: one per chunk)
-    int xcd_remap;       // give each XCD a contiguous range of chunks
+    int4 *grid;                  // all scenes' cells: raw P2G accumulators (vx,vy,vz,mass)
+    float4 *vel;                 // decoded updated velocities (written by reduce_2)
+    int4 *wild;                  // global-atomic accumulators of wild particles;
+                                 // all zero between substeps
+    int4 *partial;               // kWinCells entries per tile chunk
+    float4 *posvel;              // 2 float4 per global caller index
+    int ppt;                     // particles per thread: chunk <= kBlock * ppt
+    int grid_blocks;             // persistent blocks of the chunk kernels (0: default)
+    int xcd_remap;               // contiguous chunk ranges per XCD
 };
 
 struct BinBuffers {
     uint32_t *key, *rank;      // per slot
     uint32_t *count, *offset;  // per key; count is zero on entry and exit
-    uint32_t *block_sums;      // ceil(nkeys / 4096)
+    uint2 *block_sums;         // ceil(nkeys / 4096): (particles, tile chunks)
     uint32_t nkeys;            // all scenes
     float *state_out;
     uint32_t *id_out;
+    TileChunk *tchunks;
+    uint32_t *ntchunks;
+    uint2 *tile_chunk;
 };
 
-// Derives key geometry of one scene from its init box.
+// Derives key/tile geometry of one scene from its init box.
 void scene_key_geometry(SceneDesc &d);
 
-// Appends the scene's tile -> Hilbert-rank table (knt0*knt1*knt2 entries).
-void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks);
+// Appends the scene's x-major tile -> Hilbert-rank table (ntiles entries) and
+// its inverse (ntiles + 1 entries, the last = kWildTile).
+void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks,
+                          std::vector<uint32_t> &xmaj);
+
+// Appends the reduce tables of scene `s` (spatial tiles in Hilbert order):
+// 27 neighbour key tiles per tile and the tile's coordinates.  `ranks` and
+// `xmaj` are the scene's own tables from append_hilbert_ranks.
+void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
+                        const uint32_t *xmaj, std::vector<uint32_t> &nbr,
+                        std::vector<uint4> &info);
 
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
-hipError_t launch_clear(int4 *grid, uint32_t cells, hipStream_t st);
 hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st);
+hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st);
+hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st);
 hipError_t launch_g2p(const BatchArgs &a, hipStream_t st);
 // Writes uniforms of one scene (stream-ordered, like queue.writeBuffer).
 hipError_t launch_set_scene_uniforms(SceneDesc *d, const Uniforms &u, hipStream_t st);

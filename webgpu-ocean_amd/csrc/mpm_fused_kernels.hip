@@ -1,23 +1,28 @@
 // mpm_fused_kernels.hip -- the production substep for gfx950.
 //
-//   bin (counting sort by tile-major cell key, every `sort_period` substeps)
-//   -> clear -> p2g_1 -> p2g_2 -> g2p (+updateGrid +copyPosition)
+//   bin -> p2g_1 -> reduce_1 -> p2g_2 -> reduce_2 -> g2p
 //
-// Each stage is one launch over a table of particle chunks covering every
-// scene of the batch; blocks walk the table persistently.  Particles live in
-// SoA storage permuted into bin order; id[] maps a storage slot to the
-// caller's particle index, so readout order never changes.  A block stages
-// the 6^3-cell windows (4^3 tile + halo) of the tiles its chunk occupies in
-// LDS:
-//   p2g_1 / p2g_2: i32 fixed-point ds_add into the windows, then one global
-//                  atomic per non-zero window cell and component;
-//   g2p:           window cells loaded once with updateGrid applied on load
-//                  (a pure per-cell function), 27-node gathers from LDS.
+// Particles live in SoA storage permuted into bin order (tile-major cell keys,
+// tiles along a Hilbert curve); id[] maps a storage slot to the caller's
+// particle index, so readout order never changes.  The binning scan also
+// emits "tile chunks": runs of at most 256*ppt particles of ONE 4^3-cell
+// tile.  A P2G block accumulates its chunk into the tile's 6^3-cell LDS
+// window (tile + one-cell halo) with i32 fixed-point LDS adds, then stores the
+// whole window with coalesced plain stores as a partial.  The reduce kernels
+// give every grid cell exactly one writer: one wave per tile sums, for each of
+// its 64 cells, the partials of the chunks of the 27 surrounding tiles whose
+// windows cover it, then writes the cell, so there is no clear pass and no
+// global atomic.  reduce_2 also applies updateGrid and stores the decoded
+// velocities that g2p stages in LDS for its 27-node gathers.
+//
 // Integer adds commute, and every float expression is evaluated per particle
 // exactly as in the reference-shaped stage kernels, so results are identical
-// to them.  Particles whose floor(x) is non-finite or far outside any grid,
-// and particles whose tile finds no free window slot, take the global-memory
-// path of the stage kernels instead.
+// to them.  Particles whose stencil leaves [0, X) x [0, Y) x [0, Z) (where the
+// i32 cell index wraps onto other cells), non-finite positions, and every
+// particle of a scene whose init box is not integral are "wild": they are
+// binned to a separate key tile and add into a separate grid-sized
+// accumulator with global atomics; the reduce kernels fold it into the cells
+// and clear it again, only for scenes that have wild particles.
 #include <algorithm>
 #include <cmath>
 #include <utility>
@@ -33,37 +38,10 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;
-// Tame particles have |floor(x)| < 2040 cells, so their tiles pack into
-// tile_key and their cell coordinates are exact integers.
-constexpr float kTameLimit = 2040.0f;
+constexpr unsigned kDefaultWalkBlocks = 2048;
 
 // ---------------------------------------------------------------------------
-// chunk walk: blocks stride over the chunk table; with xcd_remap, the chunks
-// are dealt so that the blocks of one XCD (block id mod 8) take a contiguous
-// range, keeping neighbouring windows in the same L2.
-
-struct ChunkWalk {
-    uint32_t t, end, step, per;
-    bool remap;
-};
-
-__device__ __forceinline__ ChunkWalk walk_begin(const BatchArgs &a) {
-    ChunkWalk w;
-    w.remap = a.xcd_remap != 0;
-    w.per = (a.nchunks + kXcds - 1) / kXcds;
-    w.end = w.remap ? w.per * kXcds : a.nchunks;
-    w.t = blockIdx.x;
-    w.step = gridDim.x;
-    return w;
-}
-
-__device__ __forceinline__ uint32_t walk_chunk(const ChunkWalk &w, uint32_t nchunks) {
-    uint32_t c = w.remap ? (w.t % kXcds) * w.per + w.t / kXcds : w.t;
-    return c < nchunks ? c : 0xffffffffu;
-}
-
-// ---------------------------------------------------------------------------
-// small helpers
+// helpers
 
 // Slot (within its chunk) of a thread's k-th particle.  Lanes of a wave are
 // 4*PPT slots apart, so in cell-sorted storage they mostly touch different
@@ -71,17 +49,6 @@ __device__ __forceinline__ uint32_t walk_chunk(const ChunkWalk &w, uint32_t nchu
 template <int PPT>
 __device__ __forceinline__ uint32_t slot_of(int k) {
     return (threadIdx.x & 63) * (4 * PPT) + (threadIdx.x >> 6) * PPT + k;
-}
-
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-    return v;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
@@ -112,244 +79,124 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
     return wbase + inc - v;
 }
 
-__device__ __forceinline__ uint32_t block_sum(uint32_t v) {
-    uint32_t tot;
-    (void)block_excl_scan(v, &tot);
-    return tot;
+// Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
+// (mls-mpm/p2g_1.wgsl:53-56).
+__device__ __forceinline__ uint32_t cell_linear(int32_t x, int32_t y, int32_t z, int32_t Y,
+                                                int32_t Z) {
+    return (uint32_t)x * (uint32_t)Y * (uint32_t)Z + (uint32_t)y * (uint32_t)Z + (uint32_t)z;
 }
 
-__device__ __forceinline__ void load_vC(const float *__restrict__ st, uint32_t stride,
-                                        uint32_t i, float v[3], float C[3][3]) {
+__device__ __forceinline__ int4 add4(int4 a, int4 b) {
+    return make_int4((int32_t)((uint32_t)a.x + (uint32_t)b.x),
+                     (int32_t)((uint32_t)a.y + (uint32_t)b.y),
+                     (int32_t)((uint32_t)a.z + (uint32_t)b.z),
+                     (int32_t)((uint32_t)a.w + (uint32_t)b.w));
+}
+
+// Tile coordinates of an x-major tile index.
+__device__ __forceinline__ void tile_coords(uint32_t xm, const SceneDesc *sd, int t[3]) {
+    const uint32_t n12 = sd->knt[1] * sd->knt[2];
+    t[0] = (int)(xm / n12);
+    const uint32_t r = xm - (uint32_t)t[0] * n12;
+    t[1] = (int)(r / sd->knt[2]);
+    t[2] = (int)(r - (uint32_t)t[1] * sd->knt[2]);
+}
+
+// Scene owning element `v` of a per-scene offset field (binary search; the
+// scenes are packed in increasing offset order).
+template <typename F>
+__device__ __forceinline__ uint32_t find_scene(const SceneDesc *sc, uint32_t nscenes, uint32_t v,
+                                               F off) {
+    uint32_t lo = 0, hi = nscenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off(sc[mid]) <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Persistent walk over the device-side tile-chunk list.  With xcd_remap the
+// blocks of one XCD (block id mod 8) take a contiguous range of chunks, so
+// neighbouring tiles' windows meet in the same L2.
+struct TileWalk {
+    uint32_t c, end, step;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
+    const uint32_t n = *a.ntchunks;
+    TileWalk w;
+    if (a.xcd_remap && (gridDim.x % kXcds) == 0) {
+        const uint32_t per = (n + kXcds - 1) / kXcds, x = blockIdx.x % kXcds;
+        w.c = x * per + blockIdx.x / kXcds;
+        w.end = min(n, (x + 1) * per);
+        w.step = gridDim.x / kXcds;
+    } else {
+        w.c = blockIdx.x;
+        w.end = n;
+        w.step = gridDim.x;
+    }
+    return w;
+}
+
+__device__ __forceinline__ bool in_dims(int x, int y, int z, const SceneDesc *sd) {
+    return (uint32_t)x < (uint32_t)sd->kdim[0] && (uint32_t)y < (uint32_t)sd->kdim[1] &&
+           (uint32_t)z < (uint32_t)sd->kdim[2];
+}
+
+// Grid cell of window cell l (0..215) of the tile with coordinates t, or -1
+// outside the scene's cells.
+__device__ __forceinline__ int64_t win_cell_index(int l, const int t[3], const SceneDesc *sd) {
+    const int x = t[0] * kTileEdge - 1 + l / (kWinEdge * kWinEdge);
+    const int y = t[1] * kTileEdge - 1 + (l / kWinEdge) % kWinEdge;
+    const int z = t[2] * kTileEdge - 1 + l % kWinEdge;
+    if (!in_dims(x, y, z, sd)) return -1;
+    return (int64_t)sd->goff + cell_linear(x, y, z, sd->uni.Y, sd->uni.Z);
+}
+
+// LDS cell of node (0,0,0) of a particle with base cell `x` in its tile's
+// window (the window starts one cell before the tile).
+__device__ __forceinline__ int win_l0(const float x[3]) {
+    const int m = kTileEdge - 1;
+    const int cx = (int)floorf(x[0]), cy = (int)floorf(x[1]), cz = (int)floorf(x[2]);
+    return ((cx & m) * kWinEdge + (cy & m)) * kWinEdge + (cz & m);
+}
+
+__device__ __forceinline__ void load_x(const float *__restrict__ st, uint32_t stride, uint32_t i,
+                                       float x[3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) x[a] = st[(PX + a) * stride + i];
+}
+
+__device__ __forceinline__ void load_v(const float *__restrict__ st, uint32_t stride, uint32_t i,
+                                       float v[3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) v[a] = st[(VX + a) * stride + i];
+}
+
+__device__ __forceinline__ void load_C(const float *__restrict__ st, uint32_t stride, uint32_t i,
+                                       float C[3][3]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int r = 0; r < 3; ++r) C[c][r] = st[(C00 + 3 * c + r) * stride + i];
 }
 
-// Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
-// (mls-mpm/p2g_1.wgsl:53-56).
-__device__ __forceinline__ int32_t cell_linear(int32_t x, int32_t y, int32_t z, int32_t Y,
-                                               int32_t Z) {
-    return (int32_t)((uint32_t)x * (uint32_t)Y * (uint32_t)Z + (uint32_t)y * (uint32_t)Z +
-                     (uint32_t)z);
-}
-
-// ---------------------------------------------------------------------------
-// tile windows: each block registers the 4^3-cell tiles its particles occupy
-// in a small LDS table; tile slot s owns a fixed 6^3-cell window (the tile
-// plus a one-cell halo) at LDS cells [s*216, (s+1)*216).  A particle's
-// 27-node stencil always lies inside its own tile's window.
-
-constexpr int kTileEdge = 1 << kTileBits;            // 4
-constexpr int kWinEdge = kTileEdge + 2;              // 6
-constexpr int kWinCells = kWinEdge * kWinEdge * kWinEdge;  // 216
-constexpr int kMaxSlots = 16;
-constexpr uint32_t kNoTile = 0xffffffffu;
-constexpr int kTileCoordLimit = 512;                 // packed tile coords: 10 bits each
-
-__device__ __forceinline__ uint32_t tile_key(int tx, int ty, int tz) {
-    return ((uint32_t)(tx + kTileCoordLimit) << 20) | ((uint32_t)(ty + kTileCoordLimit) << 10) |
-           (uint32_t)(tz + kTileCoordLimit);
-}
-
-__device__ __forceinline__ void tile_origin(uint32_t key, int o[3]) {
-    // first cell of the slot's window: tile origin minus the halo
-    o[0] = ((int)(key >> 20) - kTileCoordLimit) * kTileEdge - 1;
-    o[1] = ((int)((key >> 10) & 1023u) - kTileCoordLimit) * kTileEdge - 1;
-    o[2] = ((int)(key & 1023u) - kTileCoordLimit) * kTileEdge - 1;
-}
-
-// Registers the tiles of the particles still to do (bit k of `todo`) in the
-// block's slot table; slot[k] = the tile's slot, or -1 when the table is full
-// (the particle waits for the next pass).  Returns the number of slots in use
-// (>= 1 whenever any thread has work).  Called by every thread of the block.
-template <int PPT>
-__device__ __forceinline__ int block_tiles(const int ic[PPT][3], uint32_t todo, int max_slots,
-                                           int slot[PPT], uint32_t *s_tiles) {
-    if (threadIdx.x < kMaxSlots) s_tiles[threadIdx.x] = kNoTile;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        slot[k] = -1;
-        if (!(todo & (1u << k))) continue;
-        const uint32_t key = tile_key(ic[k][0] >> kTileBits, ic[k][1] >> kTileBits,
-                                      ic[k][2] >> kTileBits);
-        // Slots fill as a prefix: a thread tries slot s only after seeing
-        // slots 0..s-1 taken, and a taken slot is never released.
-        for (int sI = 0; sI < max_slots; ++sI) {
-            uint32_t cur = s_tiles[sI];
-            if (cur == kNoTile) cur = atomicCAS(&s_tiles[sI], kNoTile, key);
-            if (cur == kNoTile || cur == key) {
-                slot[k] = sI;
-                break;
-            }
-        }
-    }
-    __syncthreads();
-    int used = 0;
-    for (int sI = 0; sI < max_slots; ++sI) used += s_tiles[sI] != kNoTile;
-    return used;
-}
-
-// LDS cell of node (gx,gy,gz) = l0 + (gx*6 + gy)*6 + gz for a particle whose
-// base cell is `ic` in tile slot `sl`.
-__device__ __forceinline__ int tile_l0(int sl, const int ic[3]) {
-    const int m = kTileEdge - 1;
-    return sl * kWinCells + ((ic[0] & m) * kWinEdge + (ic[1] & m)) * kWinEdge + (ic[2] & m);
-}
-
-// Cell coordinates of LDS cell l of the slot table.
-__device__ __forceinline__ void slot_cell(const uint32_t *s_tiles, int l, int &x, int &y,
-                                          int &z) {
-    const int sl = l / kWinCells, r = l - sl * kWinCells;
-    int o[3];
-    tile_origin(s_tiles[sl], o);
-    x = o[0] + r / (kWinEdge * kWinEdge);
-    y = o[1] + (r / kWinEdge) % kWinEdge;
-    z = o[2] + r % kWinEdge;
-}
-
-// Loads a chunk's positions, integer base cells and "tame" flags.
-template <int PPT>
-__device__ __forceinline__ void load_chunk(const float *__restrict__ st, uint32_t stride,
-                                           const Chunk &ch, float x[PPT][3], int ic[PPT][3],
-                                           bool valid[PPT], bool tame[PPT]) {
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
-        const uint32_t j = slot_of<PPT>(k);
-        const uint32_t i = ch.begin + j;
-        valid[k] = j < ch.count;
-        tame[k] = false;
-        if (valid[k]) {
-            bool t = true;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                x[k][a] = st[(PX + a) * stride + i];
-                float ci = floorf(x[k][a]);
-                t = t && (ci >= -kTameLimit) && (ci < kTameLimit);
-                ic[k][a] = t ? (int)ci : 0;
-            }
-            tame[k] = t;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// binning sort
-
-__global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a, BinBuffers b) {
-    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
-        const uint32_t c = walk_chunk(cw, a.nchunks);
-        if (c == 0xffffffffu) continue;
-        const Chunk ch = a.chunks[c];
-        const SceneDesc *sd = a.scenes + ch.scene;
-        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
-            const uint32_t i = ch.begin + j;
-            int cc[3];
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const float f = floorf(a.state[(PX + ax) * a.stride + i]);
-                const int dim = sd->kdim[ax];
-                cc[ax] = (f >= 0.0f) ? ((f < (float)dim) ? (int)f : dim - 1) : 0;  // NaN -> 0
-            }
-            const uint32_t tile = a.tile_rank[sd->rank_off +
-                                              ((uint32_t)(cc[0] >> kTileBits) * sd->knt[1] +
-                                               (uint32_t)(cc[1] >> kTileBits)) *
-                                                  sd->knt[2] +
-                                              (uint32_t)(cc[2] >> kTileBits)];
-            constexpr int m = (1 << kTileBits) - 1;
-            const uint32_t local = ((uint32_t)(cc[0] & m) << (2 * kTileBits)) |
-                                   ((uint32_t)(cc[1] & m) << kTileBits) | (uint32_t)(cc[2] & m);
-            const uint32_t k = sd->koff + ((tile << (3 * kTileBits)) | local);
-            b.key[i] = k;
-            b.rank[i] = atomicAdd(&b.count[k], 1u);
-        }
-    }
-}
-
-constexpr int kScanItems = 16;
-constexpr int kScanTile = kBlock * kScanItems;
-
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ count,
-                                                        uint32_t nkeys,
-                                                        uint32_t *__restrict__ block_sums) {
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j)
-        if (base + j < nkeys) s += count[base + j];
-    uint32_t tot = block_sum(s);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *__restrict__ block_sums,
-                                                     uint32_t nblocks) {
-    uint32_t carry = 0;
-    for (uint32_t b0 = 0; b0 < nblocks; b0 += kBlock) {
-        uint32_t i = b0 + threadIdx.x;
-        uint32_t v = i < nblocks ? block_sums[i] : 0;
-        uint32_t tot;
-        uint32_t ex = block_excl_scan(v, &tot);
-        if (i < nblocks) block_sums[i] = carry + ex;
-        carry += tot;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ count,
-                                                      uint32_t *__restrict__ offset,
-                                                      uint32_t nkeys,
-                                                      const uint32_t *__restrict__ block_sums) {
-    uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        v[j] = (base + j < nkeys) ? count[base + j] : 0;
-        s += v[j];
-    }
-    uint32_t tot;
-    uint32_t run = block_sums[blockIdx.x] + block_excl_scan(s, &tot);
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j)
-        if (base + j < nkeys) {
-            offset[base + j] = run;
-            run += v[j];
-            count[base + j] = 0;  // ready for the next binning pass
-        }
-}
-
-// Keys of scene s lie in [koff_s, koff_s + nkeys_s) and scenes are packed in
-// slot order, so the scattered slot stays inside the scene's slot range.
-__global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers b) {
-    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
-        const uint32_t c = walk_chunk(cw, a.nchunks);
-        if (c == 0xffffffffu) continue;
-        const Chunk ch = a.chunks[c];
-        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
-            const uint32_t i = ch.begin + j;
-            const uint32_t d = b.offset[b.key[i]] + b.rank[i];
-#pragma unroll
-            for (int k = 0; k < NCOMP; ++k) b.state_out[k * a.stride + d] = a.state[k * a.stride + i];
-            b.id_out[d] = a.id[i];
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // sinks: where a particle's 27 node contributions go / come from
 
-struct WinSink1 {  // p2g_1 into the LDS windows: [mass | vx | vy | vz] x vol
+struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x 216
     int *lds;
-    int vol, l0;
+    int l0;
     __device__ __forceinline__ void add4(const Node3 &, int gx, int gy, int gz, int32_t m,
                                          int32_t x, int32_t y, int32_t z) const {
         const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
         atomicAdd(&lds[l], m);
-        atomicAdd(&lds[vol + l], x);
-        atomicAdd(&lds[2 * vol + l], y);
-        atomicAdd(&lds[3 * vol + l], z);
+        atomicAdd(&lds[kWinCells + l], x);
+        atomicAdd(&lds[2 * kWinCells + l], y);
+        atomicAdd(&lds[3 * kWinCells + l], z);
     }
 };
 
@@ -369,23 +216,24 @@ struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
     }
 };
 
-struct WinSink2 {  // p2g_2: decoded masses [0, vol), momentum ints [vol, 4 vol)
+struct WinSink2 {  // p2g_2: decoded masses [0, 216), momentum ints [216, 864)
     int *lds;
-    int vol, l0;
+    int l0;
     __device__ __forceinline__ float mass(const Node3 &, int gx, int gy, int gz) const {
         return reinterpret_cast<const float *>(lds)[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
     }
     __device__ __forceinline__ void add3(const Node3 &, int gx, int gy, int gz, int32_t x,
                                          int32_t y, int32_t z) const {
         const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
-        atomicAdd(&lds[vol + l], x);
-        atomicAdd(&lds[2 * vol + l], y);
-        atomicAdd(&lds[3 * vol + l], z);
+        atomicAdd(&lds[kWinCells + l], x);
+        atomicAdd(&lds[2 * kWinCells + l], y);
+        atomicAdd(&lds[3 * kWinCells + l], z);
     }
 };
 
-struct GridSink2 {
-    int32_t *grid;
+struct GridSink2 {  // masses from the completed grid, momentum to the wild accumulator
+    const int32_t *grid;
+    int32_t *acc;
     uint32_t G;
     int32_t Y, Z;
     float mult;
@@ -397,7 +245,7 @@ struct GridSink2 {
                                          int32_t y, int32_t z) const {
         const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
         if (!in_grid(idx, G)) return;
-        int32_t *cell = grid + 4 * (size_t)idx;
+        int32_t *cell = acc + 4 * (size_t)idx;
         atomicAdd(cell + 0, x);
         atomicAdd(cell + 1, y);
         atomicAdd(cell + 2, z);
@@ -415,87 +263,352 @@ struct WinSinkG {  // g2p: updated, decoded velocities staged in LDS
     }
 };
 
-struct GridSinkG {  // g2p from the raw accumulators, updateGrid applied per node
-    const int4 *grid;
+struct GridSinkG {  // g2p straight from the decoded velocities
+    const float4 *cells;
     uint32_t G;
     int32_t Y, Z;
-    int32_t wall_hi[3];
-    Params prm;
     __device__ __forceinline__ void vel(const Node3 &n, int gx, int gy, int gz, float cv[3]) const {
         const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
-        int4 c = make_int4(0, 0, 0, 0);
-        if (in_grid(idx, G)) c = update_cell_id(grid[idx], idx, Y, Z, wall_hi, prm);
-        cv[0] = decode(c.x, prm.mult);
-        cv[1] = decode(c.y, prm.mult);
-        cv[2] = decode(c.z, prm.mult);
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (in_grid(idx, G)) o = cells[idx];
+        cv[0] = o.x;
+        cv[1] = o.y;
+        cv[2] = o.z;
     }
 };
 
 // ---------------------------------------------------------------------------
-// p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65)
+// binning: key per particle, counting sort, tile-chunk list
 
-template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
-    extern __shared__ int lds[];  // [mass | vx | vy | vz] x vol
-    __shared__ uint32_t s_tiles[kMaxSlots];
-    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
-        const uint32_t c = walk_chunk(cw, a.nchunks);
-        if (c == 0xffffffffu) continue;
+__global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a, BinBuffers b) {
+    for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         const Chunk ch = a.chunks[c];
         const SceneDesc *sd = a.scenes + ch.scene;
-        const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
-        const uint32_t G = sd->G;
+        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
+            const uint32_t i = ch.begin + j;
+            float x[3];
+            load_x(a.state, a.stride, i, x);
+            // tame: whole 3x3x3 stencil inside [0, kdim) of an integral box
+            // (NaN fails every comparison and is wild)
+            bool tame = sd->fast != 0;
+            int cc[3];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const float f = floorf(x[ax]);
+                tame = tame && f >= 1.0f && f <= (float)(sd->kdim[ax] - 2);
+                cc[ax] = tame ? (int)f : 0;
+            }
+            uint32_t k;
+            if (tame) {
+                const uint32_t tile =
+                    a.tile_rank[sd->rank_off +
+                                ((uint32_t)(cc[0] >> kTileBits) * sd->knt[1] +
+                                 (uint32_t)(cc[1] >> kTileBits)) *
+                                    sd->knt[2] +
+                                (uint32_t)(cc[2] >> kTileBits)];
+                constexpr int m = kTileEdge - 1;
+                const uint32_t local = ((uint32_t)(cc[0] & m) << (2 * kTileBits)) |
+                                       ((uint32_t)(cc[1] & m) << kTileBits) | (uint32_t)(cc[2] & m);
+                k = sd->koff + ((tile << (3 * kTileBits)) | local);
+            } else {
+                k = sd->koff + (sd->ntiles << (3 * kTileBits));  // the scene's wild tile
+            }
+            b.key[i] = k;
+            b.rank[i] = atomicAdd(&b.count[k], 1u);
+        }
+    }
+}
+
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;  // 4096 keys = 64 key tiles
+
+// Sum of v over the 4 consecutive lanes that share one 64-key tile.
+__device__ __forceinline__ uint32_t quad_sum(uint32_t v) {
+    v += __shfl_xor(v, 1, kWave);
+    v += __shfl_xor(v, 2, kWave);
+    return v;
+}
+
+// Per block: particles and tile chunks (4 threads cover one 64-key tile).
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ count,
+                                                        uint32_t nkeys, uint32_t chunk_cap,
+                                                        uint2 *__restrict__ block_sums) {
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < nkeys) s += count[base + j];
+    const uint32_t t = quad_sum(s);
+    const uint32_t nch = (threadIdx.x & 3) == 0 ? (t + chunk_cap - 1) / chunk_cap : 0u;
+    uint32_t tot_s, tot_c;
+    (void)block_excl_scan(s, &tot_s);
+    (void)block_excl_scan(nch, &tot_c);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = make_uint2(tot_s, tot_c);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_top(uint2 *__restrict__ block_sums,
+                                                     uint32_t nblocks,
+                                                     uint32_t *__restrict__ ntchunks) {
+    uint32_t carry_s = 0, carry_c = 0;
+    for (uint32_t b0 = 0; b0 < nblocks; b0 += kBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint2 v = i < nblocks ? block_sums[i] : make_uint2(0u, 0u);
+        uint32_t ts, tc;
+        const uint32_t es = block_excl_scan(v.x, &ts);
+        const uint32_t ec = block_excl_scan(v.y, &tc);
+        if (i < nblocks) block_sums[i] = make_uint2(carry_s + es, carry_c + ec);
+        carry_s += ts;
+        carry_c += tc;
+    }
+    if (threadIdx.x == 0) *ntchunks = carry_c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(BatchArgs a, BinBuffers b,
+                                                      uint32_t chunk_cap) {
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < b.nkeys) ? b.count[base + j] : 0;
+        s += v[j];
+    }
+    const uint32_t t = quad_sum(s);
+    const bool leader = (threadIdx.x & 3) == 0 && base < b.nkeys;
+    const uint32_t nch = leader ? (t + chunk_cap - 1) / chunk_cap : 0u;
+    const uint2 bs = b.block_sums[blockIdx.x];
+    uint32_t tot;
+    uint32_t run = bs.x + block_excl_scan(s, &tot);
+    const uint32_t cbase = bs.y + block_excl_scan(nch, &tot);
+    const uint32_t tile_start = run;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j)
+        if (base + j < b.nkeys) {
+            b.offset[base + j] = run;
+            run += v[j];
+            b.count[base + j] = 0;  // ready for the next binning pass
+        }
+    if (leader) {
+        const uint32_t ktile = base / kTileCells;  // key tile across all scenes
+        b.tile_chunk[ktile] = make_uint2(cbase, nch);
+        if (nch) {
+            const uint32_t sc = find_scene(a.scenes, a.nscenes, ktile,
+                                           [](const SceneDesc &d) { return d.toff; });
+            const uint32_t xm = a.tile_xmaj[ktile];
+            for (uint32_t j = 0; j < nch; ++j)
+                b.tchunks[cbase + j] = make_uint4(sc, xm, tile_start + j * chunk_cap,
+                                                  min(chunk_cap, t - j * chunk_cap));
+        }
+    }
+}
+
+// Keys of scene s lie in [koff_s, koff_s + nkeys_s) and scenes are packed in
+// slot order, so the scattered slot stays inside the scene's slot range.
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers b) {
+    for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
+        const Chunk ch = a.chunks[c];
+        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
+            const uint32_t i = ch.begin + j;
+            const uint32_t d = b.offset[b.key[i]] + b.rank[i];
+#pragma unroll
+            for (int k = 0; k < NCOMP; ++k) b.state_out[k * a.stride + d] = a.state[k * a.stride + i];
+            b.id_out[d] = a.id[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65) into tile partials
+
+template <int PPT>
+__global__ __launch_bounds__(kBlock) void k_p2g1(BatchArgs a) {
+    __shared__ int lds[4 * kWinCells];  // [mass | vx | vy | vz]
+    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = a.tchunks[w.c];
+        const SceneDesc *sd = a.scenes + ch.x;
         const float mult = sd->prm.mult;
-        int32_t *grid = (int32_t *)(a.grid + sd->goff);
+        if (ch.y == kWildTile) {
+            const GridSink1 sink{(int32_t *)(a.wild + sd->goff), sd->G, sd->uni.Y, sd->uni.Z};
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
+                float x[3], v[3], C[3][3];
+                load_x(a.state, a.stride, ch.z + j, x);
+                load_v(a.state, a.stride, ch.z + j, v);
+                load_C(a.state, a.stride, ch.z + j, C);
+                Node3 n;
+                node3_init(x, n);
+                p2g1_particle(n, v, C, mult, sink);
+            }
+            continue;
+        }
         float x[PPT][3], v[PPT][3], C[PPT][3][3];
-        int ic[PPT][3];
-        bool valid[PPT], tame[PPT];
-        load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        uint32_t todo = 0;
+        bool valid[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const uint32_t j = slot_of<PPT>(k);
+            valid[k] = j < ch.w;
+            if (valid[k]) {
+                load_x(a.state, a.stride, ch.z + j, x[k]);
+                load_v(a.state, a.stride, ch.z + j, v[k]);
+                load_C(a.state, a.stride, ch.z + j, C[k]);
+            }
+        }
+        for (int l = threadIdx.x; l < 4 * kWinCells; l += kBlock) lds[l] = 0;
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            load_vC(a.state, a.stride, ch.begin + slot_of<PPT>(k), v[k], C[k]);
-            if (tame[k]) todo |= 1u << k;
+            Node3 n;
+            node3_init(x[k], n);
+            p2g1_particle(n, v[k], C[k], mult, WinSink1{lds, win_l0(x[k])});
         }
+        __syncthreads();
+        int4 *out = a.partial + (size_t)w.c * kWinCells;
+        for (int l = threadIdx.x; l < kWinCells; l += kBlock)
+            out[l] = make_int4(lds[kWinCells + l], lds[2 * kWinCells + l], lds[3 * kWinCells + l],
+                               lds[l]);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reduce: one wave per spatial tile (tiles in Hilbert order).  Lanes 0..26
+// fetch the chunk ranges of the 27 neighbour tiles; then each lane owns one
+// of the tile's 64 cells, which lies in the window of its own tile plus, per
+// axis, the neighbour below (local coordinate 0) or above (3): at most 8
+// windows.  The partial loads of all those windows are issued together, one
+// round per chunk of the fullest neighbour.
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
-        for (int k = 0; k < PPT; ++k)
-            if (valid[k] && !tame[k]) {
-                Node3 n;
-                node3_init(x[k], n);
-                p2g1_particle(n, v[k], C[k], mult, GridSink1{grid, G, Y, Z});
-            }
-        // one pass per group of at most max_slots tiles
-        while (__syncthreads_or(todo != 0)) {
-            int slot[PPT];
-            const int vol = kWinCells * block_tiles<PPT>(ic, todo, a.max_slots, slot, s_tiles);
-            for (int l = threadIdx.x; l < 4 * vol; l += kBlock) lds[l] = 0;
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                if (slot[k] < 0) continue;
-                Node3 n;
-                node3_init(x[k], n);
-                p2g1_particle(n, v[k], C[k], mult, WinSink1{lds, vol, tile_l0(slot[k], ic[k])});
-                todo &= ~(1u << k);
-            }
-            __syncthreads();
-            for (int l = threadIdx.x; l < vol; l += kBlock) {
-                int cx, cy, cz;
-                slot_cell(s_tiles, l, cx, cy, cz);
-                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-                if (!in_grid(gi, G)) continue;
-                int32_t *cell = grid + 4 * (size_t)gi;
-                const int32_t m = lds[l], vx = lds[vol + l], vy = lds[2 * vol + l],
-                              vz = lds[3 * vol + l];
-                if (m) atomicAdd(cell + 3, m);
-                if (vx) atomicAdd(cell + 0, vx);
-                if (vy) atomicAdd(cell + 1, vy);
-                if (vz) atomicAdd(cell + 2, vz);
-            }
-            // the next pass (or chunk) reuses LDS and the slot table; the
-            // barrier at the loop head orders them
+    for (int o = 1; o < kWave; o <<= 1) v = max(v, (uint32_t)__shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Spatial tile of this block: with xcd_remap the blocks of one XCD take a
+// contiguous range of tiles, so neighbouring partials meet in one L2.  The
+// grid is a multiple of 8 blocks; tiles >= stiles are padding.
+__device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
+    if (a.xcd_remap) {
+        const uint32_t per = gridDim.x / kXcds;
+        return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    }
+    return blockIdx.x;
+}
+
+struct CellSum {
+    int4 acc;         // sum of the partials covering the lane's cell
+    uint32_t rounds;  // chunks of the fullest neighbour (wave-uniform)
+};
+
+__device__ __forceinline__ CellSum sum_partials(const BatchArgs &a, uint32_t b, const int l[3],
+                                                bool cell_ok) {
+    const int lane = threadIdx.x;
+    uint32_t nfirst = 0, ncnt = 0;
+    if (lane < 27) {
+        const uint32_t kt = a.nbr[(size_t)b * 27 + lane];
+        if (kt != kNoTile) {
+            const uint2 tc = a.tile_chunk[kt];
+            nfirst = tc.x;
+            ncnt = tc.y;
         }
+    }
+    int ext[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) ext[ax] = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
+    uint32_t base[8], cnt[8], mc = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        bool valid = cell_ok;
+        int d[3];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const bool bit = (s >> ax) & 1;
+            d[ax] = bit ? ext[ax] : 0;
+            valid = valid && !(bit && ext[ax] == 0);
+        }
+        const int q = (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1);
+        const uint32_t f = __shfl(nfirst, q, kWave), c = __shfl(ncnt, q, kWave);
+        const int widx = ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) *
+                             kWinEdge +
+                         (l[2] + 1 - kTileEdge * d[2]);
+        base[s] = f * (uint32_t)kWinCells + (uint32_t)widx;
+        cnt[s] = valid ? c : 0u;
+        mc = max(mc, cnt[s]);
+    }
+    CellSum r;
+    r.acc = make_int4(0, 0, 0, 0);
+    r.rounds = wave_max(mc);
+    for (uint32_t j = 0; j < r.rounds; ++j) {
+        int4 v[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            v[s] = j < cnt[s] ? a.partial[(size_t)base[s] + (size_t)j * kWinCells]
+                              : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) r.acc = add4(r.acc, v[s]);
+    }
+    return r;
+}
+
+// Folds the wild accumulator of cell gi into v and clears it.
+__device__ __forceinline__ void take_wild(const BatchArgs &a, uint64_t gi, int4 &v, bool mass) {
+    const int4 w = a.wild[gi];
+    v = add4(v, make_int4(w.x, w.y, w.z, mass ? w.w : 0));
+    a.wild[gi] = make_int4(0, 0, 0, 0);
+}
+
+template <int PHASE>
+__global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
+    const uint32_t b = reduce_tile(a);
+    if (b >= a.stiles) return;
+    const uint4 ti = a.tile_info[b];
+    const SceneDesc *sd = a.scenes + ti.w;
+    const bool has_wild = a.tile_chunk[sd->toff + sd->ntiles].y != 0;
+    const Params prm = sd->prm;
+    const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
+    if (!sd->fast) {  // every particle is wild: cells in linear order
+        const uint32_t li = (b - sd->soff) * kTileCells + threadIdx.x;
+        if (li >= sd->G) return;
+        const uint64_t gi = (uint64_t)sd->goff + li;
+        if (PHASE == 1) {
+            int4 g = make_int4(0, 0, 0, 0);
+            if (has_wild) take_wild(a, gi, g, true);
+            a.grid[gi] = g;
+        } else {
+            int4 g = a.grid[gi];
+            if (has_wild) take_wild(a, gi, g, false);
+            a.grid[gi] = g;
+            const int4 u = update_cell_id(g, (int32_t)li, sd->uni.Y, sd->uni.Z, wall_hi, prm);
+            a.vel[gi] = make_float4(decode(u.x, prm.mult), decode(u.y, prm.mult),
+                                    decode(u.z, prm.mult), 0.0f);
+        }
+        return;
+    }
+    const int l[3] = {(int)threadIdx.x >> 4, ((int)threadIdx.x >> 2) & 3, (int)threadIdx.x & 3};
+    const int cx = (int)ti.x * kTileEdge + l[0], cy = (int)ti.y * kTileEdge + l[1],
+              cz = (int)ti.z * kTileEdge + l[2];
+    const bool cell_ok = in_dims(cx, cy, cz, sd);
+    const uint64_t gi = (uint64_t)sd->goff + cell_linear(cx, cy, cz, sd->uni.Y, sd->uni.Z);
+    if (PHASE == 1) {
+        CellSum cs = sum_partials(a, b, l, cell_ok);
+        if (!cell_ok) return;
+        if (has_wild) take_wild(a, gi, cs.acc, true);
+        a.grid[gi] = cs.acc;
+    } else {
+        int4 g = make_int4(0, 0, 0, 0);
+        if (cell_ok) g = a.grid[gi];
+        const CellSum cs = sum_partials(a, b, l, cell_ok);
+        if (!cell_ok) return;
+        if (cs.rounds == 0 && !has_wild) {  // no particle nearby: the cell stays empty
+            a.vel[gi] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return;
+        }
+        g = add4(g, make_int4(cs.acc.x, cs.acc.y, cs.acc.z, 0));
+        if (has_wild) take_wild(a, gi, g, false);
+        a.grid[gi] = g;
+        const int4 u = update_cell_xyz(g, cx, cy, cz, wall_hi, prm);
+        a.vel[gi] = make_float4(decode(u.x, prm.mult), decode(u.y, prm.mult),
+                                decode(u.z, prm.mult), 0.0f);
     }
 }
 
@@ -503,176 +616,131 @@ __global__ __launch_bounds__(kBlock) void k_p2g1_win(BatchArgs a) {
 // p2g_2: density, pressure, viscous stress momentum (mls-mpm/p2g_2.wgsl:30-94)
 
 template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_p2g2_win(BatchArgs a) {
-    extern __shared__ int lds[];  // [decoded mass (f32) | vx | vy | vz] x vol
-    __shared__ uint32_t s_tiles[kMaxSlots];
-    float *dmass = reinterpret_cast<float *>(lds);
-    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
-        const uint32_t c = walk_chunk(cw, a.nchunks);
-        if (c == 0xffffffffu) continue;
-        const Chunk ch = a.chunks[c];
-        const SceneDesc *sd = a.scenes + ch.scene;
-        const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
-        const uint32_t G = sd->G;
+__global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
+    __shared__ int lds[4 * kWinCells];  // [decoded mass (f32) | vx | vy | vz]
+    float *dm = reinterpret_cast<float *>(lds);
+    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = a.tchunks[w.c];
+        const SceneDesc *sd = a.scenes + ch.x;
         const Params prm = sd->prm;
-        int32_t *grid = (int32_t *)(a.grid + sd->goff);
+        if (ch.y == kWildTile) {
+            const GridSink2 sink{(const int32_t *)(a.grid + sd->goff),
+                                 (int32_t *)(a.wild + sd->goff), sd->G, sd->uni.Y, sd->uni.Z,
+                                 prm.mult};
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
+                float x[3], C[3][3];
+                load_x(a.state, a.stride, ch.z + j, x);
+                load_C(a.state, a.stride, ch.z + j, C);
+                Node3 n;
+                node3_init(x, n);
+                p2g2_particle(n, C, prm, sink);
+            }
+            continue;
+        }
         float x[PPT][3], C[PPT][3][3];
-        int ic[PPT][3];
-        bool valid[PPT], tame[PPT];
-        load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        uint32_t todo = 0;
+        bool valid[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const uint32_t j = slot_of<PPT>(k);
+            valid[k] = j < ch.w;
+            if (valid[k]) {
+                load_x(a.state, a.stride, ch.z + j, x[k]);
+                load_C(a.state, a.stride, ch.z + j, C[k]);
+            }
+        }
+        int t[3];
+        tile_coords(ch.y, sd, t);
+        for (int l = threadIdx.x; l < kWinCells; l += kBlock) {
+            const int64_t gi = win_cell_index(l, t, sd);
+            dm[l] = gi >= 0 ? decode(a.grid[gi].w, prm.mult) : 0.0f;
+            lds[kWinCells + l] = 0;
+            lds[2 * kWinCells + l] = 0;
+            lds[3 * kWinCells + l] = 0;
+        }
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            const uint32_t i = ch.begin + slot_of<PPT>(k);
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) C[k][cc][r] = a.state[(C00 + 3 * cc + r) * a.stride + i];
-            if (tame[k]) todo |= 1u << k;
+            Node3 n;
+            node3_init(x[k], n);
+            p2g2_particle(n, C[k], prm, WinSink2{lds, win_l0(x[k])});
         }
-#pragma unroll
-        for (int k = 0; k < PPT; ++k)
-            if (valid[k] && !tame[k]) {
-                Node3 n;
-                node3_init(x[k], n);
-                p2g2_particle(n, C[k], prm, GridSink2{grid, G, Y, Z, prm.mult});
-            }
-        while (__syncthreads_or(todo != 0)) {
-            int slot[PPT];
-            const int vol = kWinCells * block_tiles<PPT>(ic, todo, a.max_slots, slot, s_tiles);
-            for (int l = threadIdx.x; l < vol; l += kBlock) {
-                int cx, cy, cz;
-                slot_cell(s_tiles, l, cx, cy, cz);
-                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-                const int32_t m = in_grid(gi, G) ? grid[4 * (size_t)gi + 3] : 0;
-                dmass[l] = decode(m, prm.mult);
-                lds[vol + l] = 0;
-                lds[2 * vol + l] = 0;
-                lds[3 * vol + l] = 0;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                if (slot[k] < 0) continue;
-                Node3 n;
-                node3_init(x[k], n);
-                p2g2_particle(n, C[k], prm, WinSink2{lds, vol, tile_l0(slot[k], ic[k])});
-                todo &= ~(1u << k);
-            }
-            __syncthreads();
-            for (int l = threadIdx.x; l < vol; l += kBlock) {
-                int cx, cy, cz;
-                slot_cell(s_tiles, l, cx, cy, cz);
-                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-                if (!in_grid(gi, G)) continue;
-                int32_t *cell = grid + 4 * (size_t)gi;
-                const int32_t vx = lds[vol + l], vy = lds[2 * vol + l], vz = lds[3 * vol + l];
-                if (vx) atomicAdd(cell + 0, vx);
-                if (vy) atomicAdd(cell + 1, vy);
-                if (vz) atomicAdd(cell + 2, vz);
-            }
-        }
+        __syncthreads();
+        int4 *out = a.partial + (size_t)w.c * kWinCells;
+        for (int l = threadIdx.x; l < kWinCells; l += kBlock)
+            out[l] = make_int4(lds[kWinCells + l], lds[2 * kWinCells + l], lds[3 * kWinCells + l], 0);
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// g2p with updateGrid applied on window load and copyPosition fused
-// (mls-mpm/updateGrid.wgsl:23-46, g2p.wgsl:26-92, copyPosition.wgsl:15-21)
+// g2p with copyPosition fused (mls-mpm/g2p.wgsl:26-92, copyPosition.wgsl:15-21)
+
+__device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, const float xo[3],
+                                          const float v[3], const float Cn[3][3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * c + r) * a.stride + i] = Cn[c][r];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        a.state[(PX + ax) * a.stride + i] = xo[ax];
+        a.state[(VX + ax) * a.stride + i] = v[ax];
+    }
+    const size_t o = a.id[i];
+    a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
+    a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
+}
 
 template <int PPT>
-__global__ __launch_bounds__(kBlock) void k_g2p_win(BatchArgs a) {
-    extern __shared__ float4 lvel[];  // decoded updated velocity x vol
-    __shared__ uint32_t s_tiles[kMaxSlots];
-    for (ChunkWalk cw = walk_begin(a); cw.t < cw.end; cw.t += cw.step) {
-        const uint32_t c = walk_chunk(cw, a.nchunks);
-        if (c == 0xffffffffu) continue;
-        const Chunk ch = a.chunks[c];
-        const SceneDesc *sd = a.scenes + ch.scene;
-        const int32_t Y = sd->uni.Y, Z = sd->uni.Z;
-        const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
-        const uint32_t G = sd->G;
-        const Params prm = sd->prm;
-        const int4 *grid = a.grid + sd->goff;
+__global__ __launch_bounds__(kBlock) void k_g2p(BatchArgs a) {
+    __shared__ float4 lvel[kWinCells];
+    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = a.tchunks[w.c];
+        const SceneDesc *sd = a.scenes + ch.x;
+        const float dt = sd->prm.dt;
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
-        float x[PPT][3];
-        int ic[PPT][3];
-        bool valid[PPT], tame[PPT];
-        load_chunk<PPT>(a.state, a.stride, ch, x, ic, valid, tame);
-        uint32_t todo = 0;
-        // gather, then the particle's tail (C, advection, walls) and stores
-        auto finish = [&](int k, float v[3], float B[3][3]) {
-            const uint32_t i = ch.begin + slot_of<PPT>(k);
-            float xo[3], Cn[3][3];
-            g2p_finish(x[k], rb, prm.dt, v, B, xo, Cn);
-#pragma unroll
-            for (int cc = 0; cc < 3; ++cc)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * cc + r) * a.stride + i] = Cn[cc][r];
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                a.state[(PX + ax) * a.stride + i] = xo[ax];
-                a.state[(VX + ax) * a.stride + i] = v[ax];
+        if (ch.y == kWildTile) {
+            const GridSinkG sink{a.vel + sd->goff, sd->G, sd->uni.Y, sd->uni.Z};
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
+                const uint32_t i = ch.z + j;
+                float x[3], v[3], B[3][3], xo[3], Cn[3][3];
+                load_x(a.state, a.stride, i, x);
+                Node3 n;
+                node3_init(x, n);
+                g2p_gather(n, sink, v, B);
+                g2p_finish(x, rb, dt, v, B, xo, Cn);
+                g2p_store(a, i, xo, v, Cn);
             }
-            const size_t o = a.id[i];
-            a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
-            a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
-        };
+            continue;
+        }
+        float x[PPT][3];
+        bool valid[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const uint32_t j = slot_of<PPT>(k);
+            valid[k] = j < ch.w;
+            if (valid[k]) load_x(a.state, a.stride, ch.z + j, x[k]);
+        }
+        int t[3];
+        tile_coords(ch.y, sd, t);
+        for (int l = threadIdx.x; l < kWinCells; l += kBlock) {
+            const int64_t gi = win_cell_index(l, t, sd);
+            lvel[l] = gi >= 0 ? a.vel[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
-            if (tame[k]) {
-                todo |= 1u << k;
-                continue;
-            }
+            float v[3], B[3][3], xo[3], Cn[3][3];
             Node3 n;
             node3_init(x[k], n);
-            float v[3], B[3][3];
-            g2p_gather(n, GridSinkG{grid, G, Y, Z, {wall_hi[0], wall_hi[1], wall_hi[2]}, prm}, v,
-                       B);
-            finish(k, v, B);
+            g2p_gather(n, WinSinkG{lvel, win_l0(x[k])}, v, B);
+            g2p_finish(x[k], rb, dt, v, B, xo, Cn);
+            g2p_store(a, ch.z + slot_of<PPT>(k), xo, v, Cn);
         }
-        while (__syncthreads_or(todo != 0)) {
-            int slot[PPT];
-            const int vol = kWinCells * block_tiles<PPT>(ic, todo, a.max_slots, slot, s_tiles);
-            for (int l = threadIdx.x; l < vol; l += kBlock) {
-                int cx, cy, cz;
-                slot_cell(s_tiles, l, cx, cy, cz);
-                const int32_t gi = cell_linear(cx, cy, cz, Y, Z);
-                float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                // A cell inside [0,X) x [0,Y) x [0,Z) whose index did not wrap
-                // decomposes to its own coordinates; others like updateGrid.wgsl:37-39.
-                const bool nice = cx >= 0 && (uint32_t)cy < (uint32_t)Y &&
-                                  (uint32_t)cz < (uint32_t)Z &&
-                                  (int64_t)cx * Y * Z + (int64_t)cy * Z + cz < (int64_t)G;
-                if (in_grid(gi, G)) {
-                    const int4 raw = grid[gi];
-                    const int4 cv = nice ? update_cell_xyz(raw, cx, cy, cz, wall_hi, prm)
-                                         : update_cell_id(raw, gi, Y, Z, wall_hi, prm);
-                    o = make_float4(decode(cv.x, prm.mult), decode(cv.y, prm.mult),
-                                    decode(cv.z, prm.mult), 0.0f);
-                }
-                lvel[l] = o;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < PPT; ++k) {
-                if (slot[k] < 0) continue;
-                Node3 n;
-                node3_init(x[k], n);
-                float v[3], B[3][3];
-                g2p_gather(n, WinSinkG{lvel, tile_l0(slot[k], ic[k])}, v, B);
-                finish(k, v, B);
-                todo &= ~(1u << k);
-            }
-        }
+        __syncthreads();
     }
-}
-
-__global__ __launch_bounds__(kBlock) void k_clear(int4 *__restrict__ grid, uint32_t cells) {
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cells; i += stride)
-        grid[i] = make_int4(0, 0, 0, 0);
 }
 
 __global__ void k_set_scene_uniforms(SceneDesc *d, Uniforms u) { d->uni = u; }
@@ -682,46 +750,32 @@ inline unsigned blocks_for(uint64_t count, uint64_t per) {
 }
 
 inline unsigned walk_blocks(const BatchArgs &a) {
-    unsigned nb = a.nchunks;
-    if (a.grid_blocks > 0 && (unsigned)a.grid_blocks < nb) nb = (unsigned)a.grid_blocks;
-    return nb ? nb : 1;
+    const unsigned nb = a.grid_blocks > 0 ? (unsigned)a.grid_blocks : kDefaultWalkBlocks;
+    return ((nb + kXcds - 1) / kXcds) * kXcds;
 }
 
 template <int PPT>
-hipError_t launch_win(int which, const BatchArgs &a, hipStream_t st) {
-    if (!a.nchunks) return hipSuccess;
-    const size_t lds = 16 * (size_t)kWinCells * (size_t)a.max_slots;
+hipError_t launch_ppt(int which, const BatchArgs &a, hipStream_t st) {
     const unsigned nb = walk_blocks(a);
     switch (which) {
-    case 1: k_p2g1_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
-    case 2: k_p2g2_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
-    case 3: k_g2p_win<PPT><<<nb, kBlock, lds, st>>>(a); break;
+    case 1: k_p2g1<PPT><<<nb, kBlock, 0, st>>>(a); break;
+    case 2: k_p2g2<PPT><<<nb, kBlock, 0, st>>>(a); break;
+    case 3: k_g2p<PPT><<<nb, kBlock, 0, st>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_win_ppt(int which, const BatchArgs &a, hipStream_t st) {
+hipError_t launch_by_ppt(int which, const BatchArgs &a, hipStream_t st) {
     switch (a.ppt) {
-    case 1: return launch_win<1>(which, a, st);
-    case 2: return launch_win<2>(which, a, st);
-    case 4: return launch_win<4>(which, a, st);
+    case 1: return launch_ppt<1>(which, a, st);
+    case 2: return launch_ppt<2>(which, a, st);
+    case 4: return launch_ppt<4>(which, a, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
-
-void scene_key_geometry(SceneDesc &d) {
-    uint64_t nk = 1;
-    for (int a = 0; a < 3; ++a) {
-        int32_t dim = (int32_t)std::ceil(d.uni.init_box[a]);
-        d.kdim[a] = dim < 1 ? 1 : dim;
-        d.knt[a] = (uint32_t)((d.kdim[a] + (1 << kTileBits) - 1) >> kTileBits);
-        nk *= d.knt[a];
-    }
-    d.nkeys = (uint32_t)(nk << (3 * kTileBits));
-}
 
 // 3D Hilbert index of (x, y, z) in a 2^bits cube: J. Skilling, "Programming
 // the Hilbert curve", AIP Conf. Proc. 707, 381 (2004) -- axes to transposed
@@ -753,7 +807,8 @@ static uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
     return h;
 }
 
-void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks) {
+void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks,
+                          std::vector<uint32_t> &xmaj) {
     const uint32_t n0 = d.knt[0], n1 = d.knt[1], n2 = d.knt[2];
     const uint32_t nt = n0 * n1 * n2;
     int bits = 0;
@@ -768,32 +823,77 @@ void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks) {
     std::sort(order.begin(), order.end());
     const size_t base = ranks.size();
     ranks.resize(base + nt);
-    for (uint32_t r = 0; r < nt; ++r) ranks[base + order[r].second] = r;
+    for (uint32_t r = 0; r < nt; ++r) {
+        ranks[base + order[r].second] = r;
+        xmaj.push_back(order[r].second);
+    }
+    xmaj.push_back(kWildTile);
+}
+
+void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
+                        const uint32_t *xmaj, std::vector<uint32_t> &nbr,
+                        std::vector<uint4> &info) {
+    const uint32_t n1 = d.knt[1], n2 = d.knt[2];
+    for (uint32_t r = 0; r < d.ntiles; ++r) {
+        const uint32_t xm = xmaj[r];
+        const uint32_t t[3] = {xm / (n1 * n2), (xm / n2) % n1, xm % n2};
+        info.push_back(make_uint4(t[0], t[1], t[2], s));
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int64_t nx = (int64_t)t[0] + dx, ny = (int64_t)t[1] + dy,
+                                  nz = (int64_t)t[2] + dz;
+                    if (nx < 0 || ny < 0 || nz < 0 || nx >= d.knt[0] || ny >= n1 || nz >= n2) {
+                        nbr.push_back(kNoTile);
+                        continue;
+                    }
+                    nbr.push_back(d.toff + ranks[((uint32_t)nx * n1 + (uint32_t)ny) * n2 +
+                                                 (uint32_t)nz]);
+                }
+    }
+}
+
+void scene_key_geometry(SceneDesc &d) {
+    uint64_t nt = 1;
+    bool integral = true;
+    for (int a = 0; a < 3; ++a) {
+        const float b = d.uni.init_box[a];
+        const int32_t dim = (int32_t)std::ceil(b);
+        d.kdim[a] = dim < 1 ? 1 : dim;
+        d.knt[a] = (uint32_t)((d.kdim[a] + kTileEdge - 1) >> kTileBits);
+        nt *= d.knt[a];
+        integral = integral && std::floor(b) == b;
+    }
+    d.ntiles = (uint32_t)nt;
+    d.nkeys = (uint32_t)((nt + 1) << (3 * kTileBits));
+    d.fast = integral ? 1u : 0u;
 }
 
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
-    if (!a.nchunks) return hipSuccess;
-    const unsigned nb = walk_blocks(a);
-    k_bin_count<<<nb, kBlock, 0, st>>>(a, b);
+    const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
+    const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
+    if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a, b);
     const unsigned nsb = blocks_for(b.nkeys, kScanTile);
-    k_scan_reduce<<<nsb, kBlock, 0, st>>>(b.count, b.nkeys, b.block_sums);
-    k_scan_top<<<1, kBlock, 0, st>>>(b.block_sums, nsb);
-    k_scan_down<<<nsb, kBlock, 0, st>>>(b.count, b.offset, b.nkeys, b.block_sums);
-    k_bin_scatter<<<nb, kBlock, 0, st>>>(a, b);
+    k_scan_reduce<<<nsb, kBlock, 0, st>>>(b.count, b.nkeys, cap, b.block_sums);
+    k_scan_top<<<1, kBlock, 0, st>>>(b.block_sums, nsb, b.ntchunks);
+    k_scan_down<<<nsb, kBlock, 0, st>>>(a, b, cap);
+    if (a.nchunks) k_bin_scatter<<<nbin, kBlock, 0, st>>>(a, b);
     return hipGetLastError();
 }
 
-hipError_t launch_clear(int4 *grid, uint32_t cells, hipStream_t st) {
-    if (!cells) return hipSuccess;
-    unsigned nb = blocks_for(cells, kBlock);
-    if (nb > 8192) nb = 8192;
-    k_clear<<<nb, kBlock, 0, st>>>(grid, cells);
+hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(1, a, st); }
+hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(2, a, st); }
+hipError_t launch_g2p(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(3, a, st); }
+
+hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st) {
+    if (a.stiles) k_reduce<1><<<(a.stiles + kXcds - 1) / kXcds * kXcds, kWave, 0, st>>>(a);
     return hipGetLastError();
 }
 
-hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st) { return launch_win_ppt(1, a, st); }
-hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st) { return launch_win_ppt(2, a, st); }
-hipError_t launch_g2p(const BatchArgs &a, hipStream_t st) { return launch_win_ppt(3, a, st); }
+hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st) {
+    if (a.stiles) k_reduce<2><<<(a.stiles + kXcds - 1) / kXcds * kXcds, kWave, 0, st>>>(a);
+    return hipGetLastError();
+}
 
 hipError_t launch_set_scene_uniforms(SceneDesc *d, const Uniforms &u, hipStream_t st) {
     k_set_scene_uniforms<<<1, 1, 0, st>>>(d, u);

@@ -48,8 +48,6 @@ class Tuning(ctypes.Structure):
     _fields_ = [
         ("path", ctypes.c_int),
         ("particles_per_thread", ctypes.c_int),
-        ("window_cells", ctypes.c_int),
-        ("sort_period", ctypes.c_int),
         ("grid_blocks", ctypes.c_int),
         ("xcd_remap", ctypes.c_int),
     ]
