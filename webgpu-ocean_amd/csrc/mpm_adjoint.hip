@@ -569,6 +569,7 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         };
         auto restore = [&](const char *src) -> hipError_t {
             TRY(hipMemcpyAsync(state_, src, sbytes, hipMemcpyDeviceToDevice, st));
+            keys_ready_ = false;
             return hipMemcpyAsync(id_, src + sbytes, ibytes, hipMemcpyDeviceToDevice, st);
         };
         // forward rollout with checkpoints every K substeps

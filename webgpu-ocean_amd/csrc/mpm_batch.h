@@ -72,6 +72,8 @@ public:
 
     // View of scene 0 for the reference-shaped stage kernels.
     DeviceScene stage_view() const;
+    // Call after changing particle state outside substep().
+    void invalidate_keys() { keys_ready_ = false; }
 
     const Tuning &tuning() const { return tun_; }
     hipError_t set_tuning(const Tuning &t);
@@ -120,19 +122,28 @@ private:
     float4 *vel_ = nullptr;     // decoded updated grid velocities
     int4 *wild_ = nullptr;      // wild-particle accumulators, zero between substeps
     int4 *partial_ = nullptr;   // kWinCells per tile chunk
-    uint32_t *key_ = nullptr, *rank_ = nullptr, *count_ = nullptr, *offset_ = nullptr;
-    uint2 *block_sums_ = nullptr;
+    uint32_t *key_ = nullptr, *rank_ = nullptr, *src_ = nullptr, *count_ = nullptr,
+             *offset_ = nullptr;
+    uint32_t *mig_ = nullptr;  // 27 per tile chunk
+    uint2 *seg_ = nullptr, *sup_ = nullptr;  // scan totals, two parities
+    size_t seg_cap_ = 0, sup_cap_ = 0;
+    float *state_buf0_ = nullptr;  // parity 0 = this buffer is the current state
     TileChunk *tchunks_ = nullptr;
     uint32_t *ntchunks_ = nullptr;
     uint2 *tile_chunk_ = nullptr;  // key tile -> (first tile chunk, count)
 
     Tuning tun_{};
 
+    // True when key_/rank_/count_ hold the binning counts of state_ (left by
+    // the last g2p); false after anything else changed the particles.
+    bool keys_ready_ = false;
+
     struct GraphEntry {
         float *state;
+        bool keys_ready;
         hipGraphExec_t exec;
     };
-    GraphEntry graphs_[2] = {};
+    GraphEntry graphs_[4] = {};
     int ngraphs_ = 0;
 };
 

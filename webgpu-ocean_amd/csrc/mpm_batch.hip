@@ -70,9 +70,12 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     free_ptr(staging_);
     free_ptr(key_);
     free_ptr(rank_);
+    free_ptr(src_);
+    free_ptr(mig_);
     free_ptr(count_);
     free_ptr(offset_);
-    free_ptr(block_sums_);
+    free_ptr(seg_);
+    free_ptr(sup_);
     free_ptr(tchunks_);
     free_ptr(ntchunks_);
     free_ptr(tile_chunk_);
@@ -81,6 +84,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     const size_t c = cap, g = std::max<uint64_t>(cells, 1), k = std::max<uint64_t>(keys, 1),
                  tc = std::max<uint64_t>(tchunks, 1);
     TRY(hipMalloc(&state_, sizeof(float) * NCOMP * c));
+    state_buf0_ = state_;
     TRY(hipMalloc(&spare_state_, sizeof(float) * NCOMP * c));
     TRY(hipMalloc(&id_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&spare_id_, sizeof(uint32_t) * c));
@@ -93,12 +97,20 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMalloc(&staging_, 80 * c));
     TRY(hipMalloc(&key_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&rank_, sizeof(uint32_t) * c));
+    TRY(hipMalloc(&src_, sizeof(uint32_t) * c));
+    TRY(hipMalloc(&mig_, sizeof(uint32_t) * 27 * tc));
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
     TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
-    TRY(hipMalloc(&block_sums_, sizeof(uint2) * (k / 4096 + 1)));
+    const size_t nseg = (k + 63) / 64, nsup = (nseg + 63) / 64;
+    TRY(hipMalloc(&seg_, sizeof(uint2) * 2 * nseg));  // two substep parities
+    TRY(hipMalloc(&sup_, sizeof(uint2) * 2 * nsup));
+    TRY(hipMemset(seg_, 0, sizeof(uint2) * 2 * nseg));
+    TRY(hipMemset(sup_, 0, sizeof(uint2) * 2 * nsup));
+    seg_cap_ = nseg;
+    sup_cap_ = nsup;
     TRY(hipMalloc(&tchunks_, sizeof(TileChunk) * tc));
     TRY(hipMalloc(&ntchunks_, sizeof(uint32_t)));
-    TRY(hipMalloc(&tile_chunk_, sizeof(uint2) * (k / kTileCells)));
+    TRY(hipMalloc(&tile_chunk_, sizeof(uint2) * k));
     TRY(hipMemset(count_, 0, sizeof(uint32_t) * k));
     TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t)));
     cap_n_ = cap;
@@ -125,7 +137,7 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
                          (uint64_t)std::ceil(sc[s].init_box[2]));
         scene_key_geometry(d);
         d.koff = (uint32_t)koff;
-        d.toff = (uint32_t)(koff / kTileCells);
+        d.toff = (uint32_t)koff;
         d.soff = (uint32_t)soff;
         d.rank_off = 0;  // set below
         poff += d.n;
@@ -137,10 +149,9 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
     total_cells_ = goff;
     total_keys_ = koff;
     total_tiles_ = (uint32_t)soff;
-    // Tile chunks of one binning: sum over key tiles of ceil(count / 256) <=
-    // n / 256 + (non-empty key tiles).
-    const uint64_t key_tiles = koff / kTileCells;
-    const uint64_t max_tchunks = (poff + kBlock - 1) / kBlock + std::min<uint64_t>(poff, key_tiles);
+    // Tile chunks of one binning: sum over keys of ceil(count / 256) <=
+    // n / 256 + (non-empty keys).
+    const uint64_t max_tchunks = (poff + kBlock - 1) / kBlock + std::min<uint64_t>(poff, koff);
     TRY(ensure_capacity(total_n_, total_cells_, total_keys_, max_tchunks));
     stride_ = cap_n_;
     if (count > scene_cap_) {
@@ -195,6 +206,7 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
     DeviceScene all = stage_view();
     all.n = total_n_;
     TRY(launch_unpack_particles(staging_, all, st));
+    keys_ready_ = false;
     TRY(upload_chunks());
     return hipStreamSynchronize(st);
 }
@@ -247,6 +259,7 @@ BatchArgs Batch::args() const {
     a.nchunks = (uint32_t)h_chunks_.size();
     a.tchunks = tchunks_;
     a.ntchunks = ntchunks_;
+    a.tchunk_cap = (uint32_t)cap_tchunks_;
     a.tile_chunk = tile_chunk_;
     a.stiles = total_tiles_;
     a.nbr = d_nbr_;
@@ -257,6 +270,12 @@ BatchArgs Batch::args() const {
     a.grid = grid_;
     a.vel = vel_;
     a.wild = wild_;
+    a.key = key_;
+    a.rank = rank_;
+    a.src = src_;
+    a.mig = mig_;
+    a.count = count_;
+    a.count_next = 1;
     a.partial = partial_;
     a.posvel = posvel_;
     a.ppt = tun_.ppt;
@@ -267,15 +286,21 @@ BatchArgs Batch::args() const {
 
 BinBuffers Batch::bin_buffers() const {
     BinBuffers b{};
-    b.key = key_;
-    b.rank = rank_;
     b.count = count_;
     b.offset = offset_;
-    b.block_sums = block_sums_;
+    // totals of this substep's parity (state buffer 0 or 1), the other one next
+    const size_t par = state_ == state_buf0_ ? 0 : 1;
+    b.nseg = (uint32_t)((total_keys_ + 63) / 64);
+    b.nsup = (b.nseg + 63) / 64;
+    b.seg = seg_ + par * seg_cap_;
+    b.seg_next = seg_ + (1 - par) * seg_cap_;
+    b.sup = sup_ + par * sup_cap_;
+    b.sup_next = sup_ + (1 - par) * sup_cap_;
     b.nkeys = (uint32_t)total_keys_;
     b.state_out = spare_state_;
     b.id_out = spare_id_;
     b.tchunks = tchunks_;
+    b.tchunk_cap = (uint32_t)cap_tchunks_;
     b.ntchunks = ntchunks_;
     b.tile_chunk = tile_chunk_;
     return b;
@@ -288,11 +313,15 @@ hipError_t Batch::set_box(uint32_t s, const float real_box[3], hipStream_t st) {
 }
 
 // The binned path re-bins every substep: the tile chunks are only valid for
-// the positions they were built from.
+// the positions they were built from.  Its g2p leaves the keys and counts of
+// the new state for the next binning.
 void Batch::after_substep() {
     if (tun_.path == 0) {
         std::swap(state_, spare_state_);
         std::swap(id_, spare_id_);
+        keys_ready_ = total_n_ > 0;
+    } else {
+        keys_ready_ = false;
     }
 }
 
@@ -309,7 +338,7 @@ hipError_t Batch::launch_substep(hipStream_t st, hipEvent_t *ev) {
     }
     BatchArgs a = args();
     TRY(mark());
-    TRY(launch_bin(a, bin_buffers(), st));
+    TRY(launch_bin(a, bin_buffers(), !keys_ready_, st));
     a.state = spare_state_;
     a.id = spare_id_;
     TRY(mark());
@@ -328,9 +357,9 @@ hipError_t Batch::launch_substep(hipStream_t st, hipEvent_t *ev) {
 hipError_t Batch::substep(hipStream_t st) {
     GraphEntry *g = nullptr;
     for (int i = 0; i < ngraphs_; ++i)
-        if (graphs_[i].state == state_) g = &graphs_[i];
+        if (graphs_[i].state == state_ && graphs_[i].keys_ready == keys_ready_) g = &graphs_[i];
     if (!g) {
-        if (ngraphs_ == 2) drop_graphs();
+        if (ngraphs_ == 4) drop_graphs();
         hipGraph_t graph = nullptr;
         TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t le = launch_substep(st, nullptr);
@@ -344,7 +373,7 @@ hipError_t Batch::substep(hipStream_t st) {
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return e;
-        graphs_[ngraphs_] = GraphEntry{state_, exec};
+        graphs_[ngraphs_] = GraphEntry{state_, keys_ready_, exec};
         g = &graphs_[ngraphs_++];
     }
     TRY(hipGraphLaunch(g->exec, st));
@@ -406,6 +435,7 @@ hipError_t Batch::write_particles(uint32_t s, const void *src, uint32_t n, hipSt
         TRY(hipMemcpyAsync(staging_ + 20 * (size_t)h_scenes_[s].poff, src, 80 * (size_t)n,
                            hipMemcpyHostToDevice, st));
     TRY(launch_unpack_particles(staging_, all, st));
+    keys_ready_ = false;
     return hipStreamSynchronize(st);
 }
 
@@ -431,6 +461,7 @@ hipError_t Batch::set_tuning(const Tuning &t) {
     if (t.path == 1 && h_scenes_.size() > 1) return hipErrorInvalidValue;
     drop_graphs();
     tun_ = t;
+    keys_ready_ = false;  // counts were taken for the previous chunk size
     return hipSuccess;
 }
 
@@ -475,6 +506,7 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
                 }
                 e = hipMemcpyAsync(state_, snap, sb, hipMemcpyDeviceToDevice, st);
                 if (e == hipSuccess) e = hipMemcpyAsync(id_, snap + sb, ib, hipMemcpyDeviceToDevice, st);
+                keys_ready_ = false;
                 // eager launches: graph capture per candidate would dominate
                 for (unsigned k = 0; k < 2 && e == hipSuccess; ++k) {
                     e = launch_substep(st, nullptr);
@@ -501,6 +533,7 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     }
     hipError_t r = hipMemcpyAsync(state_, snap, sb, hipMemcpyDeviceToDevice, st);
     if (r == hipSuccess) r = hipMemcpyAsync(id_, snap + sb, ib, hipMemcpyDeviceToDevice, st);
+    keys_ready_ = false;
     if (r == hipSuccess) r = hipMemcpyAsync(grid_, snap + sb + ib, gb, hipMemcpyDeviceToDevice, st);
     if (r == hipSuccess)
         r = hipMemcpyAsync(posvel_, snap + sb + ib + gb, pb, hipMemcpyDeviceToDevice, st);
@@ -537,9 +570,12 @@ void Batch::release() {
     free_ptr(staging_);
     free_ptr(key_);
     free_ptr(rank_);
+    free_ptr(src_);
+    free_ptr(mig_);
     free_ptr(count_);
     free_ptr(offset_);
-    free_ptr(block_sums_);
+    free_ptr(seg_);
+    free_ptr(sup_);
     free_ptr(tchunks_);
     free_ptr(ntchunks_);
     free_ptr(tile_chunk_);

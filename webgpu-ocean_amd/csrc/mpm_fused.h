@@ -4,10 +4,11 @@
 // one): particle slots, grid cells and bin keys of all scenes are packed into
 // shared buffers, and every stage is one launch over all scenes.
 //
-// Substep: bin (counting sort of particles by tile-major cell key; the scan
-// also emits the list of tile chunks) -> p2g_1 (LDS tile windows -> partial
-// windows) -> reduce_1 (per tile: sum the partials covering it; writes every
-// cell) -> p2g_2 -> reduce_2 (+ updateGrid -> decoded velocities) -> g2p.
+// Substep: bin (counting sort of particles by tile key; the scan also emits
+// the list of tile chunks) -> p2g_1 (LDS tile windows -> partial windows) ->
+// reduce_1 (per tile: sum the partials covering it; writes every cell) ->
+// p2g_2 -> reduce_2 (+ updateGrid -> decoded velocities) -> g2p (+ the
+// counts of the next binning).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,9 +20,8 @@
 namespace mpm {
 
 // Tiles are (2^kTileBits)^3 cells, ordered along a 3D Hilbert curve per scene;
-// bin key = koff + (tile rank << 6 | x-major cell within the tile).  Each
-// scene has one extra key tile (rank = ntiles) collecting the particles whose
-// stencil leaves the grid ("wild" particles).
+// bin key = koff + tile rank.  Each scene has one extra key (rank = ntiles)
+// collecting the particles whose stencil leaves the grid ("wild" particles).
 constexpr int kTileBits = 2;
 constexpr int kBlock = 256;
 constexpr int kTileEdge = 1 << kTileBits;                      // 4
@@ -37,13 +37,13 @@ struct SceneDesc {
     Params prm;          // WGSL override constants of this scene
     uint32_t poff, n;    // particle slots [poff, poff + n)
     uint32_t goff, G;    // grid cells [goff, goff + G)
-    uint32_t koff;       // first bin key of the scene (multiple of 64)
-    uint32_t toff;       // first key tile (koff / 64)
+    uint32_t koff;       // first bin key of the scene
+    uint32_t toff;       // = koff (keys are tiles)
     uint32_t soff;       // first spatial tile in the reduce launch
     uint32_t ntiles;     // spatial tiles knt0*knt1*knt2 (the wild tile is extra)
     int32_t kdim[3];     // ceil(init_box): cells per axis
     uint32_t knt[3];     // tiles per axis
-    uint32_t nkeys;      // (ntiles + 1) * 64
+    uint32_t nkeys;      // ntiles + 1
     uint32_t rank_off;   // first entry of the scene's x-major tile -> rank table
     uint32_t fast;       // 1: integer box (cells map to distinct indices), 0: all wild
 };
@@ -62,14 +62,15 @@ struct BatchArgs {
     const SceneDesc *scenes;
     uint32_t nscenes;
     const uint32_t *tile_rank;   // rank_off + x-major tile -> Hilbert rank
-    const uint32_t *tile_xmaj;   // key tile (toff + rank) -> x-major tile (kWildTile: wild)
+    const uint32_t *tile_xmaj;   // key (koff + rank) -> x-major tile (kWildTile: wild)
     const Chunk *chunks;         // static slot chunks
     uint32_t nchunks;
     const TileChunk *tchunks;    // tile chunks of the current binning
     const uint32_t *ntchunks;    // their number (device)
-    const uint2 *tile_chunk;     // key tile -> (first tile chunk, count)
+    uint32_t tchunk_cap;         // capacity of tchunks, partial and mig (in chunks)
+    const uint2 *tile_chunk;     // key -> (first tile chunk, count)
     uint32_t stiles;             // spatial tiles of all scenes (reduce launch)
-    const uint32_t *nbr;         // 27 per spatial tile: key tile of each neighbour or kNoTile
+    const uint32_t *nbr;         // 27 per spatial tile: key of each neighbour or kNoTile
     const uint4 *tile_info;      // per spatial tile: (tile x, y, z, scene)
     float *state;                // SoA particle state, NCOMP x stride, bin order
     uint32_t *id;                // slot -> global caller index (poff + local index)
@@ -80,19 +81,28 @@ struct BatchArgs {
                                  // all zero between substeps
     int4 *partial;               // kWinCells entries per tile chunk
     float4 *posvel;              // 2 float4 per global caller index
+    uint32_t *key, *rank;        // per slot: bin key and rank within the key (or within
+                                 // its source chunk and direction, see src)
+    uint32_t *src;               // per slot: chunk * 27 + direction, or final-rank marker
+    uint32_t *mig;               // 27 per tile chunk: particles moving each direction,
+                                 // then their offset within the destination tile
+    uint32_t *count;             // per key: particles (zero after each scan)
+    int count_next;              // g2p also counts the keys of its output
     int ppt;                     // particles per thread: chunk <= kBlock * ppt
     int grid_blocks;             // persistent blocks of the chunk kernels (0: default)
     int xcd_remap;               // contiguous chunk ranges per XCD
 };
 
 struct BinBuffers {
-    uint32_t *key, *rank;      // per slot
-    uint32_t *count, *offset;  // per key; count is zero on entry and exit
-    uint2 *block_sums;         // ceil(nkeys / 4096): (particles, tile chunks)
+    uint32_t *count, *offset;  // per key; the scan clears count
+    uint2 *seg, *sup;          // totals per 64 keys / per 4096 keys, this substep
+    uint2 *seg_next, *sup_next;  // the next substep's (the scan clears sup_next)
+    uint32_t nseg, nsup;
     uint32_t nkeys;            // all scenes
     float *state_out;
     uint32_t *id_out;
     TileChunk *tchunks;
+    uint32_t tchunk_cap;
     uint32_t *ntchunks;
     uint2 *tile_chunk;
 };
@@ -112,7 +122,10 @@ void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
                         const uint32_t *xmaj, std::vector<uint32_t> &nbr,
                         std::vector<uint4> &info);
 
-hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
+// Counting sort of a.state into b.state_out.  With `count`, the keys and
+// counts are taken first (after clearing count); otherwise the previous
+// g2p's counts are used.
+hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st);
 hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st);

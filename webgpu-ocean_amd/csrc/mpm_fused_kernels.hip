@@ -43,12 +43,18 @@ constexpr unsigned kDefaultWalkBlocks = 2048;
 // ---------------------------------------------------------------------------
 // helpers
 
-// Slot (within its chunk) of a thread's k-th particle.  Lanes of a wave are
-// 4*PPT slots apart, so in cell-sorted storage they mostly touch different
-// cells and their LDS adds rarely hit the same address in one instruction.
-template <int PPT>
-__device__ __forceinline__ uint32_t slot_of(int k) {
-    return (threadIdx.x & 63) * (4 * PPT) + (threadIdx.x >> 6) * PPT + k;
+// A chunk of `count` particles is ceil(count / 64) wave tasks; task t of the
+// block's waves is k * 4 + wave for the thread's k-th particle.
+__device__ __forceinline__ uint32_t wave_tasks(uint32_t count) { return (count + 63) >> 6; }
+
+__device__ __forceinline__ uint32_t task_of(int k) { return (uint32_t)k * 4 + (threadIdx.x >> 6); }
+
+// P2G slot of lane l in task t: the lanes of a task are `tasks` slots apart,
+// so in cell-sorted storage one wave spans the whole chunk and its LDS adds
+// rarely hit the same cell in one instruction, while every task but the
+// last runs full.
+__device__ __forceinline__ uint32_t p2g_slot(uint32_t t, uint32_t tasks) {
+    return t + tasks * (threadIdx.x & 63);
 }
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
@@ -60,23 +66,32 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
     return v;
 }
 
-// Exclusive scan over the block; *total receives the block sum.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
-    __shared__ uint32_t s_w[kBlock / kWave];
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    uint32_t inc = wave_incl_scan(v, lane);
-    if (lane == kWave - 1) s_w[w] = inc;
-    __syncthreads();
-    uint32_t wbase = 0, tot = 0;
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
-    for (int i = 0; i < kBlock / kWave; ++i) {
-        uint32_t s = s_w[i];
-        if (i < w) wbase += s;
-        tot += s;
+    for (int o = 1; o < kWave; o <<= 1) v = max(v, (uint32_t)__shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Blocks of a one-wave-per-tile launch: the spatial tiles padded to a
+// multiple of 8 (tiles >= stiles are padding).
+__device__ __forceinline__ uint32_t tile_blocks(const BatchArgs &a) {
+    return (a.stiles + kXcds - 1) / kXcds * kXcds;
+}
+
+// Spatial tile of this block: with xcd_remap the blocks of one XCD take a
+// contiguous range of tiles, so neighbouring partials meet in one L2.
+__device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
+    if (a.xcd_remap) {
+        const uint32_t per = tile_blocks(a) / kXcds;
+        return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     }
-    __syncthreads();
-    *total = tot;
-    return wbase + inc - v;
+    return blockIdx.x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) v += (uint32_t)__shfl_xor(v, o, kWave);
+    return v;
 }
 
 // Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
@@ -126,7 +141,7 @@ struct TileWalk {
 };
 
 __device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
-    const uint32_t n = *a.ntchunks;
+    const uint32_t n = min(*a.ntchunks, a.tchunk_cap);
     TileWalk w;
     if (a.xcd_remap && (gridDim.x % kXcds) == 0) {
         const uint32_t per = (n + kXcds - 1) / kXcds, x = blockIdx.x % kXcds;
@@ -278,143 +293,247 @@ struct GridSinkG {  // g2p straight from the decoded velocities
 };
 
 // ---------------------------------------------------------------------------
-// binning: key per particle, counting sort, tile-chunk list
+// binning: one key per (scene, tile) in Hilbert order plus one wild key per
+// scene; counting sort; tile-chunk list.
+//
+// The counts come from the previous substep's g2p, without global atomics in
+// the common case: a g2p block counts, in LDS, how many of its chunk's
+// particles moved to each of the 27 tiles around the chunk's tile ("mig",
+// 27 per chunk) and gives each particle its rank within its (chunk,
+// direction).  k_bin_gather then visits every tile and turns the counts of
+// the chunks that can move into it into running offsets.  Particles that
+// jump further than one tile, or become wild, take a rank from a global
+// atomic on their key instead (rank "final").  After the state was replaced
+// from outside, k_bin_count ranks every particle that way.
 
-__global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a, BinBuffers b) {
+constexpr uint32_t kDirs = 27;              // near directions (dx+1)*9 + (dy+1)*3 + dz+1
+constexpr uint32_t kDirWild = kDirs;        // became wild: block-level global atomic
+constexpr uint32_t kDirFar = kDirs + 1;     // moved further: per-particle global atomic
+constexpr uint32_t kRankFinal = 0xffffffffu;
+
+// Bin key of a particle at x: its tile's key, or the scene's wild key when
+// the 3x3x3 stencil is not inside [0, kdim) of an integral box (NaN fails
+// every comparison).  *dir receives the direction from tile `from` (kDirFar
+// when further away or when `from` is not a tile, kDirWild when wild).
+__device__ __forceinline__ uint32_t bin_key(const BatchArgs &a, const SceneDesc *sd,
+                                            const float x[3], const int from[3], uint32_t *dir) {
+    bool tame = sd->fast != 0;
+    int t[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float f = floorf(x[ax]);
+        tame = tame && f >= 1.0f && f <= (float)(sd->kdim[ax] - 2);
+        t[ax] = tame ? (int32_t)f >> kTileBits : 0;
+    }
+    if (!tame) {
+        *dir = kDirWild;
+        return sd->koff + sd->ntiles;
+    }
+    const int dx = t[0] - from[0], dy = t[1] - from[1], dz = t[2] - from[2];
+    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+    *dir = near ? (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) : kDirFar;
+    return sd->koff + a.tile_rank[sd->rank_off +
+                                  ((uint32_t)t[0] * sd->knt[1] + (uint32_t)t[1]) * sd->knt[2] +
+                                  (uint32_t)t[2]];
+}
+
+// Counts one particle per active lane into count[key] and returns the lane's
+// rank among its key's particles.  Every lane of the wave must call this.
+// Runs of consecutive active lanes with equal keys (the common case: storage
+// is already in bin order) share one atomic.
+__device__ __forceinline__ uint32_t count_rank(uint32_t *count, uint32_t key, bool active) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t prev_key = __shfl_up(key, 1, kWave);
+    const int prev_active = __shfl_up(active ? 1 : 0, 1, kWave);
+    const bool head = lane == 0 || !active || !prev_active || prev_key != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes <= this
+    const uint64_t above = heads & ~le;
+    const uint32_t end = above ? (uint32_t)__ffsll((unsigned long long)above) - 1 : (uint32_t)kWave;
+    uint32_t base = 0;
+    if (head && active) base = atomicAdd(&count[key], end - (uint32_t)lane);
+    const int my_head = 63 - __clzll((unsigned long long)(heads & le));  // lane 0 is a head
+    return __shfl(base, my_head, kWave) + (uint32_t)(lane - my_head);
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a) {
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         const Chunk ch = a.chunks[c];
         const SceneDesc *sd = a.scenes + ch.scene;
-        for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
-            const uint32_t i = ch.begin + j;
+        const uint32_t j = threadIdx.x;  // chunks hold <= kBlock slots
+        const bool active = j < ch.count;
+        uint32_t key = 0;
+        if (active) {
             float x[3];
-            load_x(a.state, a.stride, i, x);
-            // tame: whole 3x3x3 stencil inside [0, kdim) of an integral box
-            // (NaN fails every comparison and is wild)
-            bool tame = sd->fast != 0;
-            int cc[3];
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const float f = floorf(x[ax]);
-                tame = tame && f >= 1.0f && f <= (float)(sd->kdim[ax] - 2);
-                cc[ax] = tame ? (int)f : 0;
-            }
-            uint32_t k;
-            if (tame) {
-                const uint32_t tile =
-                    a.tile_rank[sd->rank_off +
-                                ((uint32_t)(cc[0] >> kTileBits) * sd->knt[1] +
-                                 (uint32_t)(cc[1] >> kTileBits)) *
-                                    sd->knt[2] +
-                                (uint32_t)(cc[2] >> kTileBits)];
-                constexpr int m = kTileEdge - 1;
-                const uint32_t local = ((uint32_t)(cc[0] & m) << (2 * kTileBits)) |
-                                       ((uint32_t)(cc[1] & m) << kTileBits) | (uint32_t)(cc[2] & m);
-                k = sd->koff + ((tile << (3 * kTileBits)) | local);
-            } else {
-                k = sd->koff + (sd->ntiles << (3 * kTileBits));  // the scene's wild tile
-            }
-            b.key[i] = k;
-            b.rank[i] = atomicAdd(&b.count[k], 1u);
+            const int nowhere[3] = {-4, -4, -4};
+            uint32_t dir;
+            load_x(a.state, a.stride, ch.begin + j, x);
+            key = bin_key(a, sd, x, nowhere, &dir);
+        }
+        const uint32_t r = count_rank(a.count, key, active);
+        if (active) {
+            a.key[ch.begin + j] = key;
+            a.rank[ch.begin + j] = r;
+            a.src[ch.begin + j] = kRankFinal;
         }
     }
 }
 
-constexpr int kScanItems = 16;
-constexpr int kScanTile = kBlock * kScanItems;  // 4096 keys = 64 key tiles
-
-// Sum of v over the 4 consecutive lanes that share one 64-key tile.
-__device__ __forceinline__ uint32_t quad_sum(uint32_t v) {
-    v += __shfl_xor(v, 1, kWave);
-    v += __shfl_xor(v, 2, kWave);
-    return v;
-}
-
-// Per block: particles and tile chunks (4 threads cover one 64-key tile).
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ count,
-                                                        uint32_t nkeys, uint32_t chunk_cap,
-                                                        uint2 *__restrict__ block_sums) {
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t s = 0;
+// Exclusive scan over a block of NT threads; *total receives the block sum.
+template <int NT>
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t s_w[NT / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const uint32_t inc = wave_incl_scan(v, lane);
+    if (lane == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j)
-        if (base + j < nkeys) s += count[base + j];
-    const uint32_t t = quad_sum(s);
-    const uint32_t nch = (threadIdx.x & 3) == 0 ? (t + chunk_cap - 1) / chunk_cap : 0u;
-    uint32_t tot_s, tot_c;
-    (void)block_excl_scan(s, &tot_s);
-    (void)block_excl_scan(nch, &tot_c);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = make_uint2(tot_s, tot_c);
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_top(uint2 *__restrict__ block_sums,
-                                                     uint32_t nblocks,
-                                                     uint32_t *__restrict__ ntchunks) {
-    uint32_t carry_s = 0, carry_c = 0;
-    for (uint32_t b0 = 0; b0 < nblocks; b0 += kBlock) {
-        const uint32_t i = b0 + threadIdx.x;
-        const uint2 v = i < nblocks ? block_sums[i] : make_uint2(0u, 0u);
-        uint32_t ts, tc;
-        const uint32_t es = block_excl_scan(v.x, &ts);
-        const uint32_t ec = block_excl_scan(v.y, &tc);
-        if (i < nblocks) block_sums[i] = make_uint2(carry_s + es, carry_c + ec);
-        carry_s += ts;
-        carry_c += tc;
+    for (int i = 0; i < NT / kWave; ++i) {
+        const uint32_t s = s_w[i];
+        if (i < w) wbase += s;
+        tot += s;
     }
-    if (threadIdx.x == 0) *ntchunks = carry_c;
+    __syncthreads();
+    *total = tot;
+    return wbase + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_down(BatchArgs a, BinBuffers b,
-                                                      uint32_t chunk_cap) {
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j) {
-        v[j] = (base + j < b.nkeys) ? b.count[base + j] : 0;
-        s += v[j];
-    }
-    const uint32_t t = quad_sum(s);
-    const bool leader = (threadIdx.x & 3) == 0 && base < b.nkeys;
-    const uint32_t nch = leader ? (t + chunk_cap - 1) / chunk_cap : 0u;
-    const uint2 bs = b.block_sums[blockIdx.x];
-    uint32_t tot;
-    uint32_t run = bs.x + block_excl_scan(s, &tot);
-    const uint32_t cbase = bs.y + block_excl_scan(nch, &tot);
-    const uint32_t tile_start = run;
-#pragma unroll
-    for (int j = 0; j < kScanItems; ++j)
-        if (base + j < b.nkeys) {
-            b.offset[base + j] = run;
-            run += v[j];
-            b.count[base + j] = 0;  // ready for the next binning pass
+__device__ __forceinline__ uint32_t chunks_of(uint32_t count, uint32_t cap) {
+    return (count + cap - 1) / cap;
+}
+
+// The scan runs one wave per segment of 64 keys.  Its prefix comes from
+// per-segment totals of (particles, tile chunks) and their sums per
+// super-segment of 64 segments.  The super-segment sums are accumulated with
+// atomics and double-buffered by substep parity: a scan clears the other
+// parity's, which the next substep fills.
+constexpr uint32_t kSegBits = 6;
+constexpr uint32_t kSupBits = 12;
+
+// One wave per segment, once every count is final.
+__global__ __launch_bounds__(kWave) void k_seg_sums(BinBuffers b, uint32_t cap) {
+    const uint32_t sg = blockIdx.x;
+    const uint32_t k = (sg << kSegBits) + threadIdx.x;
+    const uint32_t c = k < b.nkeys ? b.count[k] : 0u;
+    const uint32_t ts = wave_sum(c), tc = wave_sum(chunks_of(c, cap));
+    if (threadIdx.x == 0) {
+        b.seg[sg] = make_uint2(ts, tc);
+        if (ts) {
+            atomicAdd(&b.sup[sg >> (kSupBits - kSegBits)].x, ts);
+            atomicAdd(&b.sup[sg >> (kSupBits - kSegBits)].y, tc);
         }
-    if (leader) {
-        const uint32_t ktile = base / kTileCells;  // key tile across all scenes
-        b.tile_chunk[ktile] = make_uint2(cbase, nch);
+    }
+}
+
+// One wave per segment of 64 keys: key offsets, the keys' tile-chunk ranges
+// and the tile chunks themselves; clears the counts for the next pass.
+__global__ __launch_bounds__(kWave) void k_scan_keys(BatchArgs a, BinBuffers b, uint32_t cap) {
+    const uint32_t sg = blockIdx.x, sp = sg >> (kSupBits - kSegBits);
+    const int lane = threadIdx.x;
+    const uint32_t k = (sg << kSegBits) + (uint32_t)lane;
+    const bool in = k < b.nkeys;
+    const uint32_t c = in ? b.count[k] : 0u;
+    const uint32_t xm = in ? a.tile_xmaj[k] : 0u;
+    uint32_t ps = 0, pc = 0;  // totals of all earlier segments
+    for (uint32_t i0 = 0; i0 < sp; i0 += kWave) {
+        const uint32_t i = i0 + (uint32_t)lane;
+        if (i < sp) {
+            const uint2 v = b.sup[i];
+            ps += v.x;
+            pc += v.y;
+        }
+    }
+    {
+        const uint32_t i = (sp << (kSupBits - kSegBits)) + (uint32_t)lane;
+        if (i < sg) {
+            const uint2 v = b.seg[i];
+            ps += v.x;
+            pc += v.y;
+        }
+    }
+    ps = wave_sum(ps);
+    pc = wave_sum(pc);
+    const uint32_t nch = chunks_of(c, cap);
+    const uint32_t off = ps + wave_incl_scan(c, lane) - c;
+    const uint32_t cb = pc + wave_incl_scan(nch, lane) - nch;
+    if (in) {
+        b.offset[k] = off;
+        b.count[k] = 0;  // ready for the next counting pass
+        b.tile_chunk[k] = make_uint2(cb, nch);
         if (nch) {
-            const uint32_t sc = find_scene(a.scenes, a.nscenes, ktile,
-                                           [](const SceneDesc &d) { return d.toff; });
-            const uint32_t xm = a.tile_xmaj[ktile];
-            for (uint32_t j = 0; j < nch; ++j)
-                b.tchunks[cbase + j] = make_uint4(sc, xm, tile_start + j * chunk_cap,
-                                                  min(chunk_cap, t - j * chunk_cap));
+            const uint32_t sc =
+                find_scene(a.scenes, a.nscenes, k, [](const SceneDesc &d) { return d.koff; });
+            for (uint32_t i = 0; i < nch; ++i)
+                if (cb + i < b.tchunk_cap)
+                    b.tchunks[cb + i] = make_uint4(sc, xm, off + i * cap, min(cap, c - i * cap));
         }
+        if (k == b.nkeys - 1) *b.ntchunks = cb + nch;
     }
+    if (lane == 0 && (sg & ((1u << (kSupBits - kSegBits)) - 1)) == 0)
+        b.sup_next[sp] = make_uint2(0u, 0u);
 }
 
 // Keys of scene s lie in [koff_s, koff_s + nkeys_s) and scenes are packed in
-// slot order, so the scattered slot stays inside the scene's slot range.
+// slot order, so the scattered slot stays inside the scene's slot range (the
+// check only guards the buffers against inconsistent ranks).
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers b) {
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         const Chunk ch = a.chunks[c];
+        const SceneDesc *sd = a.scenes + ch.scene;
+        const uint32_t lo = sd->poff, hi = sd->poff + sd->n;
         for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
             const uint32_t i = ch.begin + j;
-            const uint32_t d = b.offset[b.key[i]] + b.rank[i];
+            const uint32_t src = a.src[i];
+            uint32_t r = a.rank[i];
+            if (src != kRankFinal) r += a.mig[src];
+            const uint32_t d = b.offset[a.key[i]] + r;
+            if (d < lo || d >= hi) continue;
 #pragma unroll
             for (int k = 0; k < NCOMP; ++k) b.state_out[k * a.stride + d] = a.state[k * a.stride + i];
             b.id_out[d] = a.id[i];
         }
     }
+}
+
+// One wave per tile T: the chunks of the 27 tiles around T (previous binning)
+// that sent particles into T, in a fixed order; their counts in mig become
+// running offsets within T, after the particles that reached T through
+// global atomics (already in count[T]).  Each (chunk, direction) entry
+// belongs to exactly one destination tile.
+__global__ __launch_bounds__(kWave) void k_bin_gather(BatchArgs a) {
+    const uint32_t b = reduce_tile(a);
+    if (b >= a.stiles) return;
+    const uint4 ti = a.tile_info[b];
+    const SceneDesc *sd = a.scenes + ti.w;
+    if (!sd->fast) return;  // every particle is wild: ranks are final
+    const int lane = threadIdx.x;
+    uint32_t first = 0, cnt = 0, dir = 0;
+    if (lane < (int)kDirs) {
+        const uint32_t s = a.nbr[(size_t)b * kDirs + lane];  // source tile T + d
+        if (s != kNoTile) {
+            const uint2 tc = a.tile_chunk[s];
+            first = tc.x;
+            cnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
+        }
+        dir = kDirs - 1 - lane;  // from the source, T lies at -d
+    }
+    const uint32_t self = a.nbr[(size_t)b * kDirs + 13];
+    uint32_t running = a.count[self];
+    const uint32_t rounds = wave_max(cnt);
+    for (uint32_t j = 0; j < rounds; ++j) {
+        const bool has = j < cnt;
+        const size_t e = (size_t)(first + j) * kDirs + dir;
+        const uint32_t v = has ? a.mig[e] : 0u;
+        const uint32_t inc = wave_incl_scan(v, lane);
+        if (has) a.mig[e] = running + inc - v;
+        running += __shfl(inc, kWave - 1, kWave);
+    }
+    if (lane == 0) a.count[self] = running;
 }
 
 // ---------------------------------------------------------------------------
@@ -442,10 +561,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g1(BatchArgs a) {
         }
         float x[PPT][3], v[PPT][3], C[PPT][3][3];
         bool valid[PPT];
+        const uint32_t tasks = wave_tasks(ch.w);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const uint32_t j = slot_of<PPT>(k);
-            valid[k] = j < ch.w;
+            const uint32_t t = task_of(k), j = p2g_slot(t, tasks);
+            valid[k] = t < tasks && j < ch.w;
             if (valid[k]) {
                 load_x(a.state, a.stride, ch.z + j, x[k]);
                 load_v(a.state, a.stride, ch.z + j, v[k]);
@@ -478,23 +598,6 @@ __global__ __launch_bounds__(kBlock) void k_p2g1(BatchArgs a) {
 // windows.  The partial loads of all those windows are issued together, one
 // round per chunk of the fullest neighbour.
 
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) v = max(v, (uint32_t)__shfl_xor(v, o, kWave));
-    return v;
-}
-
-// Spatial tile of this block: with xcd_remap the blocks of one XCD take a
-// contiguous range of tiles, so neighbouring partials meet in one L2.  The
-// grid is a multiple of 8 blocks; tiles >= stiles are padding.
-__device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
-    if (a.xcd_remap) {
-        const uint32_t per = gridDim.x / kXcds;
-        return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-    }
-    return blockIdx.x;
-}
-
 struct CellSum {
     int4 acc;         // sum of the partials covering the lane's cell
     uint32_t rounds;  // chunks of the fullest neighbour (wave-uniform)
@@ -509,7 +612,7 @@ __device__ __forceinline__ CellSum sum_partials(const BatchArgs &a, uint32_t b, 
         if (kt != kNoTile) {
             const uint2 tc = a.tile_chunk[kt];
             nfirst = tc.x;
-            ncnt = tc.y;
+            ncnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
         }
     }
     int ext[3];
@@ -639,10 +742,11 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
         }
         float x[PPT][3], C[PPT][3][3];
         bool valid[PPT];
+        const uint32_t tasks = wave_tasks(ch.w);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const uint32_t j = slot_of<PPT>(k);
-            valid[k] = j < ch.w;
+            const uint32_t t = task_of(k), j = p2g_slot(t, tasks);
+            valid[k] = t < tasks && j < ch.w;
             if (valid[k]) {
                 load_x(a.state, a.stride, ch.z + j, x[k]);
                 load_C(a.state, a.stride, ch.z + j, C[k]);
@@ -692,33 +796,76 @@ __device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, const 
     a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
 }
 
+// Ranks slot i for the next binning (every lane of the wave calls this).
+// Near moves rank within (chunk c, direction) through the block's LDS
+// counters; wild ones too, finished after the chunk (wild_rank); far moves
+// take a final rank from a global atomic on their key.
+__device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, uint32_t c,
+                                          uint32_t i, uint32_t key, uint32_t dir, bool active) {
+    const bool local = active && dir <= kDirWild;
+    const uint32_t lr = count_rank(lcnt, dir, local);
+    const uint32_t fr = count_rank(a.count, key, active && dir == kDirFar);
+    if (active) {
+        a.key[i] = key;
+        a.rank[i] = local ? lr : fr;
+        a.src[i] = dir < kDirs ? c * kDirs + dir : kRankFinal;
+    }
+}
+
+// After a chunk: publish its near counts, reserve the wild particles' ranks
+// with one atomic, and finish their ranks.  lcnt[kDirWild + 1] carries the
+// reserved base between the two barriers.
+__device__ __forceinline__ void finish_chunk_counts(const BatchArgs &a, const SceneDesc *sd,
+                                                    uint32_t *lcnt, uint32_t c) {
+    __syncthreads();
+    if (threadIdx.x < kDirs) a.mig[(size_t)c * kDirs + threadIdx.x] = lcnt[threadIdx.x];
+    if (threadIdx.x == kDirWild) {
+        const uint32_t nw = lcnt[kDirWild];
+        lcnt[kDirWild + 1] = nw ? atomicAdd(&a.count[sd->koff + sd->ntiles], nw) : 0u;
+    }
+    __syncthreads();
+}
+
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_g2p(BatchArgs a) {
     __shared__ float4 lvel[kWinCells];
+    __shared__ uint32_t lcnt[kDirWild + 2];  // per-direction counts, wild base
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
         const SceneDesc *sd = a.scenes + ch.x;
         const float dt = sd->prm.dt;
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
+        if (threadIdx.x < kDirWild + 1) lcnt[threadIdx.x] = 0;
         if (ch.y == kWildTile) {
+            // every move from a wild chunk is "far": no tile to count from
             const GridSinkG sink{a.vel + sd->goff, sd->G, sd->uni.Y, sd->uni.Z};
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
-                const uint32_t i = ch.z + j;
-                float x[3], v[3], B[3][3], xo[3], Cn[3][3];
-                load_x(a.state, a.stride, i, x);
-                Node3 n;
-                node3_init(x, n);
-                g2p_gather(n, sink, v, B);
-                g2p_finish(x, rb, dt, v, B, xo, Cn);
-                g2p_store(a, i, xo, v, Cn);
+            const int nowhere[3] = {-4, -4, -4};
+            for (uint32_t j0 = 0; j0 < ch.w; j0 += kBlock) {  // uniform trip count
+                const uint32_t i = ch.z + j0 + threadIdx.x;
+                const bool active = j0 + threadIdx.x < ch.w;
+                uint32_t key = 0, dir = kDirFar;
+                if (active) {
+                    float x[3], v[3], B[3][3], xo[3], Cn[3][3];
+                    load_x(a.state, a.stride, i, x);
+                    Node3 n;
+                    node3_init(x, n);
+                    g2p_gather(n, sink, v, B);
+                    g2p_finish(x, rb, dt, v, B, xo, Cn);
+                    g2p_store(a, i, xo, v, Cn);
+                    key = bin_key(a, sd, xo, nowhere, &dir);
+                    dir = kDirFar;
+                }
+                if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, active);
             }
+            __syncthreads();
             continue;
         }
         float x[PPT][3];
         bool valid[PPT];
+        // contiguous slots per task: coalesced state stores
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const uint32_t j = slot_of<PPT>(k);
+            const uint32_t j = task_of(k) * 64 + (threadIdx.x & 63);
             valid[k] = j < ch.w;
             if (valid[k]) load_x(a.state, a.stride, ch.z + j, x[k]);
         }
@@ -729,15 +876,31 @@ __global__ __launch_bounds__(kBlock) void k_g2p(BatchArgs a) {
             lvel[l] = gi >= 0 ? a.vel[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         __syncthreads();
+        bool wild[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            if (!valid[k]) continue;
-            float v[3], B[3][3], xo[3], Cn[3][3];
-            Node3 n;
-            node3_init(x[k], n);
-            g2p_gather(n, WinSinkG{lvel, win_l0(x[k])}, v, B);
-            g2p_finish(x[k], rb, dt, v, B, xo, Cn);
-            g2p_store(a, ch.z + slot_of<PPT>(k), xo, v, Cn);
+            const uint32_t i = ch.z + task_of(k) * 64 + (threadIdx.x & 63);
+            uint32_t key = 0, dir = kDirFar;
+            if (valid[k]) {
+                float v[3], B[3][3], xo[3], Cn[3][3];
+                Node3 n;
+                node3_init(x[k], n);
+                g2p_gather(n, WinSinkG{lvel, win_l0(x[k])}, v, B);
+                g2p_finish(x[k], rb, dt, v, B, xo, Cn);
+                g2p_store(a, i, xo, v, Cn);
+                key = bin_key(a, sd, xo, t, &dir);
+            }
+            wild[k] = valid[k] && dir == kDirWild;
+            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, valid[k]);
+        }
+        if (a.count_next) {
+            finish_chunk_counts(a, sd, lcnt, w.c);
+#pragma unroll
+            for (int k = 0; k < PPT; ++k)
+                if (wild[k]) {
+                    const uint32_t i = ch.z + task_of(k) * 64 + (threadIdx.x & 63);
+                    a.rank[i] += lcnt[kDirWild + 1];
+                }
         }
         __syncthreads();
     }
@@ -865,18 +1028,25 @@ void scene_key_geometry(SceneDesc &d) {
         integral = integral && std::floor(b) == b;
     }
     d.ntiles = (uint32_t)nt;
-    d.nkeys = (uint32_t)((nt + 1) << (3 * kTileBits));
+    d.nkeys = (uint32_t)(nt + 1);  // one key per tile plus the wild key
     d.fast = integral ? 1u : 0u;
 }
 
-hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
+hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st) {
     const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
     const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
-    if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a, b);
-    const unsigned nsb = blocks_for(b.nkeys, kScanTile);
-    k_scan_reduce<<<nsb, kBlock, 0, st>>>(b.count, b.nkeys, cap, b.block_sums);
-    k_scan_top<<<1, kBlock, 0, st>>>(b.block_sums, nsb, b.ntchunks);
-    k_scan_down<<<nsb, kBlock, 0, st>>>(a, b, cap);
+    const unsigned nseg = blocks_for(b.nkeys, 1u << kSegBits);
+    const unsigned ntb = (a.stiles + kXcds - 1) / kXcds * kXcds;
+    if (count) {
+        const unsigned zn = std::max(1u, std::min(blocks_for(b.nkeys, kBlock), 1024u));
+        k_zero_u32<<<zn, kBlock, 0, st>>>(b.count, b.nkeys);
+        k_zero_u32<<<1, kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(b.sup), 2 * b.nsup);
+        if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a);
+    } else if (a.stiles) {
+        k_bin_gather<<<ntb, kWave, 0, st>>>(a);
+    }
+    k_seg_sums<<<nseg, kWave, 0, st>>>(b, cap);
+    k_scan_keys<<<nseg, kWave, 0, st>>>(a, b, cap);
     if (a.nchunks) k_bin_scatter<<<nbin, kBlock, 0, st>>>(a, b);
     return hipGetLastError();
 }
