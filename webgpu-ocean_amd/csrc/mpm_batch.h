@@ -72,8 +72,10 @@ public:
 
     // View of scene 0 for the reference-shaped stage kernels.
     DeviceScene stage_view() const;
-    // Call after changing particle state outside substep().
-    void invalidate_keys() { keys_ready_ = false; }
+    // Call after changing particles or grid cells outside substep() (stage
+    // kernels, copies): the next substep re-counts the keys and rewrites
+    // every cell.
+    hipError_t external_change(hipStream_t st);
 
     const Tuning &tuning() const { return tun_; }
     hipError_t set_tuning(const Tuning &t);
@@ -94,7 +96,7 @@ private:
     BinBuffers bin_buffers() const;
     hipError_t launch_substep(hipStream_t st, hipEvent_t *ev);
     void drop_graphs();
-    void after_substep();
+    void after_substep(hipStream_t st);
 
     std::vector<SceneDesc> h_scenes_;
     std::vector<Chunk> h_chunks_;
@@ -104,6 +106,7 @@ private:
     uint32_t *d_tile_xmaj_ = nullptr;  // key tile -> x-major tile (kWildTile: wild)
     uint32_t *d_nbr_ = nullptr;        // 27 neighbour key tiles per spatial tile
     uint4 *d_tile_info_ = nullptr;     // (x, y, z, scene) per spatial tile
+    uint8_t *d_dirty_ = nullptr;       // per spatial tile: cells may be non-zero
     size_t tile_rank_cap_ = 0, tile_xmaj_cap_ = 0, tile_tab_cap_ = 0;
     size_t chunk_cap_ = 0;
 

@@ -101,7 +101,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMalloc(&mig_, sizeof(uint32_t) * 27 * tc));
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
     TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
-    const size_t nseg = (k + 63) / 64, nsup = (nseg + 63) / 64;
+    const size_t nseg = (k + 15) / 16, nsup = (k + 4095) / 4096;
     TRY(hipMalloc(&seg_, sizeof(uint2) * 2 * nseg));  // two substep parities
     TRY(hipMalloc(&sup_, sizeof(uint2) * 2 * nsup));
     TRY(hipMemset(seg_, 0, sizeof(uint2) * 2 * nseg));
@@ -171,10 +171,13 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
     if (info.size() > tile_tab_cap_) {
         free_ptr(d_nbr_);
         free_ptr(d_tile_info_);
+        free_ptr(d_dirty_);
         TRY(hipMalloc(&d_nbr_, sizeof(uint32_t) * nbr.size()));
         TRY(hipMalloc(&d_tile_info_, sizeof(uint4) * info.size()));
+        TRY(hipMalloc(&d_dirty_, info.size()));
         tile_tab_cap_ = info.size();
     }
+    if (!info.empty()) TRY(hipMemset(d_dirty_, 1, info.size()));
     if (!info.empty()) {
         TRY(hipMemcpy(d_nbr_, nbr.data(), sizeof(uint32_t) * nbr.size(), hipMemcpyHostToDevice));
         TRY(hipMemcpy(d_tile_info_, info.data(), sizeof(uint4) * info.size(),
@@ -264,6 +267,7 @@ BatchArgs Batch::args() const {
     a.stiles = total_tiles_;
     a.nbr = d_nbr_;
     a.tile_info = d_tile_info_;
+    a.dirty = d_dirty_;
     a.state = state_;
     a.id = id_;
     a.stride = stride_;
@@ -290,8 +294,8 @@ BinBuffers Batch::bin_buffers() const {
     b.offset = offset_;
     // totals of this substep's parity (state buffer 0 or 1), the other one next
     const size_t par = state_ == state_buf0_ ? 0 : 1;
-    b.nseg = (uint32_t)((total_keys_ + 63) / 64);
-    b.nsup = (b.nseg + 63) / 64;
+    b.nseg = (uint32_t)((total_keys_ + 15) / 16);
+    b.nsup = (uint32_t)((total_keys_ + 4095) / 4096);
     b.seg = seg_ + par * seg_cap_;
     b.seg_next = seg_ + (1 - par) * seg_cap_;
     b.sup = sup_ + par * sup_cap_;
@@ -314,15 +318,22 @@ hipError_t Batch::set_box(uint32_t s, const float real_box[3], hipStream_t st) {
 
 // The binned path re-bins every substep: the tile chunks are only valid for
 // the positions they were built from.  Its g2p leaves the keys and counts of
-// the new state for the next binning.
-void Batch::after_substep() {
+// the new state for the next binning.  The reference-shaped kernels change
+// particles and cells behind the binned path's back.
+void Batch::after_substep(hipStream_t st) {
     if (tun_.path == 0) {
         std::swap(state_, spare_state_);
         std::swap(id_, spare_id_);
         keys_ready_ = total_n_ > 0;
     } else {
-        keys_ready_ = false;
+        (void)external_change(st);
     }
+}
+
+hipError_t Batch::external_change(hipStream_t st) {
+    keys_ready_ = false;
+    if (total_tiles_ == 0) return hipSuccess;
+    return hipMemsetAsync(d_dirty_, 1, total_tiles_, st);
 }
 
 hipError_t Batch::launch_substep(hipStream_t st, hipEvent_t *ev) {
@@ -377,7 +388,7 @@ hipError_t Batch::substep(hipStream_t st) {
         g = &graphs_[ngraphs_++];
     }
     TRY(hipGraphLaunch(g->exec, st));
-    after_substep();
+    after_substep(st);
     return hipSuccess;
 }
 
@@ -403,7 +414,7 @@ hipError_t Batch::time_stages(hipStream_t st, unsigned substeps, float *ms, int 
             ms[i] += t;
             if (calls) calls[i] += 1;
         }
-        after_substep();
+        after_substep(st);
     }
     for (int i = 0; i <= nslots; ++i) (void)hipEventDestroy(ev[i]);
     *count = nslots;
@@ -450,6 +461,7 @@ hipError_t Batch::write_grid(uint32_t s, const int32_t *src, uint64_t ncells, hi
     if (ncells)
         TRY(hipMemcpyAsync(grid_ + h_scenes_[s].goff, src, 16 * (size_t)ncells,
                            hipMemcpyHostToDevice, st));
+    TRY(external_change(st));
     return hipStreamSynchronize(st);
 }
 
@@ -510,12 +522,12 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
                 // eager launches: graph capture per candidate would dominate
                 for (unsigned k = 0; k < 2 && e == hipSuccess; ++k) {
                     e = launch_substep(st, nullptr);
-                    after_substep();
+                    after_substep(st);
                 }
                 if (e == hipSuccess) e = hipEventRecord(e0, st);
                 for (unsigned k = 0; k < substeps_per_trial && e == hipSuccess; ++k) {
                     e = launch_substep(st, nullptr);
-                    after_substep();
+                    after_substep(st);
                 }
                 if (e == hipSuccess) e = hipEventRecord(e1, st);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -537,6 +549,7 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     if (r == hipSuccess) r = hipMemcpyAsync(grid_, snap + sb + ib, gb, hipMemcpyDeviceToDevice, st);
     if (r == hipSuccess)
         r = hipMemcpyAsync(posvel_, snap + sb + ib + gb, pb, hipMemcpyDeviceToDevice, st);
+    if (r == hipSuccess) r = external_change(st);
     if (r == hipSuccess) r = hipStreamSynchronize(st);
     (void)hipFree(snap);
     if (e0) (void)hipEventDestroy(e0);
@@ -585,6 +598,7 @@ void Batch::release() {
     free_ptr(d_tile_xmaj_);
     free_ptr(d_nbr_);
     free_ptr(d_tile_info_);
+    free_ptr(d_dirty_);
     tile_rank_cap_ = tile_xmaj_cap_ = tile_tab_cap_ = 0;
     cap_n_ = 0;
     cap_cells_ = cap_keys_ = cap_tchunks_ = 0;

@@ -275,7 +275,7 @@ int mpm_run_stage(mpm_sim *sim, int stage) {
     int rc = select_device(sim->ctx);
     if (rc) return rc;
     HIP_TRY(mpm::launch_stage(stage, sim->ctx.batch.stage_view(), sim->ctx.stream));
-    sim->ctx.batch.invalidate_keys();
+    HIP_TRY(sim->ctx.batch.external_change(sim->ctx.stream));
     return MPM_OK;
 }
 

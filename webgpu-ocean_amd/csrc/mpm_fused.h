@@ -72,6 +72,7 @@ struct BatchArgs {
     uint32_t stiles;             // spatial tiles of all scenes (reduce launch)
     const uint32_t *nbr;         // 27 per spatial tile: key of each neighbour or kNoTile
     const uint4 *tile_info;      // per spatial tile: (tile x, y, z, scene)
+    uint8_t *dirty;              // per spatial tile: cells or velocities may be non-zero
     float *state;                // SoA particle state, NCOMP x stride, bin order
     uint32_t *id;                // slot -> global caller index (poff + local index)
     uint32_t stride;

@@ -407,39 +407,26 @@ __device__ __forceinline__ uint32_t chunks_of(uint32_t count, uint32_t cap) {
     return (count + cap - 1) / cap;
 }
 
-// The scan runs one wave per segment of 64 keys.  Its prefix comes from
-// per-segment totals of (particles, tile chunks) and their sums per
-// super-segment of 64 segments.  The super-segment sums are accumulated with
-// atomics and double-buffered by substep parity: a scan clears the other
-// parity's, which the next substep fills.
-constexpr uint32_t kSegBits = 6;
+// The scan runs one wave per 64 keys.  Its prefix comes from totals of
+// (particles, tile chunks) per segment of 16 keys, written by the gather
+// kernel (one block per segment), and their sums per super-segment of 4096
+// keys, accumulated with atomics and double-buffered by substep parity: a
+// scan clears the other parity's, which the next substep fills.
+constexpr uint32_t kSegBits = 4;
+constexpr uint32_t kSegKeys = 1u << kSegBits;
 constexpr uint32_t kSupBits = 12;
+constexpr uint32_t kSegPerSup = 1u << (kSupBits - kSegBits);
 
-// One wave per segment, once every count is final.
-__global__ __launch_bounds__(kWave) void k_seg_sums(BinBuffers b, uint32_t cap) {
-    const uint32_t sg = blockIdx.x;
-    const uint32_t k = (sg << kSegBits) + threadIdx.x;
-    const uint32_t c = k < b.nkeys ? b.count[k] : 0u;
-    const uint32_t ts = wave_sum(c), tc = wave_sum(chunks_of(c, cap));
-    if (threadIdx.x == 0) {
-        b.seg[sg] = make_uint2(ts, tc);
-        if (ts) {
-            atomicAdd(&b.sup[sg >> (kSupBits - kSegBits)].x, ts);
-            atomicAdd(&b.sup[sg >> (kSupBits - kSegBits)].y, tc);
-        }
-    }
-}
-
-// One wave per segment of 64 keys: key offsets, the keys' tile-chunk ranges
-// and the tile chunks themselves; clears the counts for the next pass.
+// One wave per 64 keys: key offsets, the keys' tile-chunk ranges and the
+// tile chunks themselves; clears the counts for the next pass.
 __global__ __launch_bounds__(kWave) void k_scan_keys(BatchArgs a, BinBuffers b, uint32_t cap) {
-    const uint32_t sg = blockIdx.x, sp = sg >> (kSupBits - kSegBits);
     const int lane = threadIdx.x;
-    const uint32_t k = (sg << kSegBits) + (uint32_t)lane;
+    const uint32_t k0 = blockIdx.x * kWave, k = k0 + (uint32_t)lane;
+    const uint32_t sp = k0 >> kSupBits, sg = k0 >> kSegBits;  // first segment of the wave
     const bool in = k < b.nkeys;
     const uint32_t c = in ? b.count[k] : 0u;
     const uint32_t xm = in ? a.tile_xmaj[k] : 0u;
-    uint32_t ps = 0, pc = 0;  // totals of all earlier segments
+    uint32_t ps = 0, pc = 0;  // totals of all earlier keys
     for (uint32_t i0 = 0; i0 < sp; i0 += kWave) {
         const uint32_t i = i0 + (uint32_t)lane;
         if (i < sp) {
@@ -448,8 +435,8 @@ __global__ __launch_bounds__(kWave) void k_scan_keys(BatchArgs a, BinBuffers b, 
             pc += v.y;
         }
     }
-    {
-        const uint32_t i = (sp << (kSupBits - kSegBits)) + (uint32_t)lane;
+    for (uint32_t i0 = sp * kSegPerSup; i0 < sg; i0 += kWave) {
+        const uint32_t i = i0 + (uint32_t)lane;
         if (i < sg) {
             const uint2 v = b.seg[i];
             ps += v.x;
@@ -474,8 +461,7 @@ __global__ __launch_bounds__(kWave) void k_scan_keys(BatchArgs a, BinBuffers b, 
         }
         if (k == b.nkeys - 1) *b.ntchunks = cb + nch;
     }
-    if (lane == 0 && (sg & ((1u << (kSupBits - kSegBits)) - 1)) == 0)
-        b.sup_next[sp] = make_uint2(0u, 0u);
+    if (lane == 0 && (k0 & ((1u << kSupBits) - 1)) == 0) b.sup_next[sp] = make_uint2(0u, 0u);
 }
 
 // Keys of scene s lie in [koff_s, koff_s + nkeys_s) and scenes are packed in
@@ -500,40 +486,74 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers 
     }
 }
 
-// One wave per tile T: the chunks of the 27 tiles around T (previous binning)
-// that sent particles into T, in a fixed order; their counts in mig become
+// One block of 16 waves per segment of 16 keys, one wave per key.  For a
+// tile key T: the chunks of the 27 tiles around T (previous binning) that
+// sent particles into T, in a fixed order; their counts in mig become
 // running offsets within T, after the particles that reached T through
 // global atomics (already in count[T]).  Each (chunk, direction) entry
-// belongs to exactly one destination tile.
-__global__ __launch_bounds__(kWave) void k_bin_gather(BatchArgs a) {
-    const uint32_t b = reduce_tile(a);
-    if (b >= a.stiles) return;
-    const uint4 ti = a.tile_info[b];
-    const SceneDesc *sd = a.scenes + ti.w;
-    if (!sd->fast) return;  // every particle is wild: ranks are final
-    const int lane = threadIdx.x;
-    uint32_t first = 0, cnt = 0, dir = 0;
-    if (lane < (int)kDirs) {
-        const uint32_t s = a.nbr[(size_t)b * kDirs + lane];  // source tile T + d
-        if (s != kNoTile) {
-            const uint2 tc = a.tile_chunk[s];
-            first = tc.x;
-            cnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
+// belongs to exactly one destination tile.  With `final_counts` (after
+// k_bin_count) or for wild keys the count is already complete.  The block
+// then writes the segment's totals for the scan.
+__global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, BinBuffers bb,
+                                                                 uint32_t cap, int final_counts) {
+    __shared__ uint32_t s_cnt[kSegKeys], s_nch[kSegKeys];
+    const uint32_t nseg = (bb.nkeys + kSegKeys - 1) / kSegKeys;
+    uint32_t sg = blockIdx.x;
+    if (a.xcd_remap) {  // contiguous segments per XCD: neighbouring tiles share an L2
+        const uint32_t per = gridDim.x / kXcds;
+        sg = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    }
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const uint32_t k = sg * kSegKeys + (uint32_t)w;
+    uint32_t total = 0;
+    if (sg < nseg && k < bb.nkeys) {
+        const uint32_t scn =
+            find_scene(a.scenes, a.nscenes, k, [](const SceneDesc &d) { return d.koff; });
+        const SceneDesc *sd = a.scenes + scn;
+        const uint32_t rank = k - sd->koff;
+        total = bb.count[k];
+        if (!final_counts && sd->fast && rank < sd->ntiles) {
+            const uint32_t b = sd->soff + rank;  // spatial tile of key k
+            uint32_t first = 0, cnt = 0, dir = 0;
+            if (lane < (int)kDirs) {
+                const uint32_t s = a.nbr[(size_t)b * kDirs + lane];  // source tile T + d
+                if (s != kNoTile) {
+                    const uint2 tc = a.tile_chunk[s];
+                    first = tc.x;
+                    cnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
+                }
+                dir = kDirs - 1 - lane;  // from the source, T lies at -d
+            }
+            const uint32_t rounds = wave_max(cnt);
+            for (uint32_t j = 0; j < rounds; ++j) {
+                const bool has = j < cnt;
+                const size_t e = (size_t)(first + j) * kDirs + dir;
+                const uint32_t v = has ? a.mig[e] : 0u;
+                const uint32_t inc = wave_incl_scan(v, lane);
+                if (has) a.mig[e] = total + inc - v;
+                total += __shfl(inc, kWave - 1, kWave);
+            }
+            if (lane == 0) bb.count[k] = total;
         }
-        dir = kDirs - 1 - lane;  // from the source, T lies at -d
     }
-    const uint32_t self = a.nbr[(size_t)b * kDirs + 13];
-    uint32_t running = a.count[self];
-    const uint32_t rounds = wave_max(cnt);
-    for (uint32_t j = 0; j < rounds; ++j) {
-        const bool has = j < cnt;
-        const size_t e = (size_t)(first + j) * kDirs + dir;
-        const uint32_t v = has ? a.mig[e] : 0u;
-        const uint32_t inc = wave_incl_scan(v, lane);
-        if (has) a.mig[e] = running + inc - v;
-        running += __shfl(inc, kWave - 1, kWave);
+    if (lane == 0) {
+        s_cnt[w] = total;
+        s_nch[w] = chunks_of(total, cap);
     }
-    if (lane == 0) a.count[self] = running;
+    __syncthreads();
+    if (threadIdx.x == 0 && sg < nseg) {
+        uint32_t ts = 0, tc = 0;
+#pragma unroll
+        for (int i = 0; i < (int)kSegKeys; ++i) {
+            ts += s_cnt[i];
+            tc += s_nch[i];
+        }
+        bb.seg[sg] = make_uint2(ts, tc);
+        if (ts) {
+            atomicAdd(&bb.sup[sg / kSegPerSup].x, ts);
+            atomicAdd(&bb.sup[sg / kSegPerSup].y, tc);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -598,13 +618,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g1(BatchArgs a) {
 // windows.  The partial loads of all those windows are issued together, one
 // round per chunk of the fullest neighbour.
 
-struct CellSum {
-    int4 acc;         // sum of the partials covering the lane's cell
+// The partial windows covering one lane's cell: up to 8 (neighbour, window
+// index) slots and the chunk count of each.
+struct CellPlan {
+    uint32_t base[8], cnt[8];
     uint32_t rounds;  // chunks of the fullest neighbour (wave-uniform)
 };
 
-__device__ __forceinline__ CellSum sum_partials(const BatchArgs &a, uint32_t b, const int l[3],
-                                                bool cell_ok) {
+__device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b, const int l[3],
+                                                  bool cell_ok) {
     const int lane = threadIdx.x;
     uint32_t nfirst = 0, ncnt = 0;
     if (lane < 27) {
@@ -618,7 +640,8 @@ __device__ __forceinline__ CellSum sum_partials(const BatchArgs &a, uint32_t b, 
     int ext[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) ext[ax] = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
-    uint32_t base[8], cnt[8], mc = 0;
+    CellPlan p;
+    uint32_t mc = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         bool valid = cell_ok;
@@ -634,23 +657,27 @@ __device__ __forceinline__ CellSum sum_partials(const BatchArgs &a, uint32_t b, 
         const int widx = ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) *
                              kWinEdge +
                          (l[2] + 1 - kTileEdge * d[2]);
-        base[s] = f * (uint32_t)kWinCells + (uint32_t)widx;
-        cnt[s] = valid ? c : 0u;
-        mc = max(mc, cnt[s]);
+        p.base[s] = f * (uint32_t)kWinCells + (uint32_t)widx;
+        p.cnt[s] = valid ? c : 0u;
+        mc = max(mc, p.cnt[s]);
     }
-    CellSum r;
-    r.acc = make_int4(0, 0, 0, 0);
-    r.rounds = wave_max(mc);
-    for (uint32_t j = 0; j < r.rounds; ++j) {
+    p.rounds = wave_max(mc);
+    return p;
+}
+
+// Sum of the planned partials: all slots' loads of one round in flight at once.
+__device__ __forceinline__ int4 sum_partials(const BatchArgs &a, const CellPlan &p) {
+    int4 acc = make_int4(0, 0, 0, 0);
+    for (uint32_t j = 0; j < p.rounds; ++j) {
         int4 v[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s)
-            v[s] = j < cnt[s] ? a.partial[(size_t)base[s] + (size_t)j * kWinCells]
-                              : make_int4(0, 0, 0, 0);
+            v[s] = j < p.cnt[s] ? a.partial[(size_t)p.base[s] + (size_t)j * kWinCells]
+                                : make_int4(0, 0, 0, 0);
 #pragma unroll
-        for (int s = 0; s < 8; ++s) r.acc = add4(r.acc, v[s]);
+        for (int s = 0; s < 8; ++s) acc = add4(acc, v[s]);
     }
-    return r;
+    return acc;
 }
 
 // Folds the wild accumulator of cell gi into v and clears it.
@@ -692,21 +719,29 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
               cz = (int)ti.z * kTileEdge + l[2];
     const bool cell_ok = in_dims(cx, cy, cz, sd);
     const uint64_t gi = (uint64_t)sd->goff + cell_linear(cx, cy, cz, sd->uni.Y, sd->uni.Z);
+    const bool dirty = a.dirty[b] != 0;
+    const CellPlan plan = plan_partials(a, b, l, cell_ok);
+    // No particle near the tile: its cells are empty.  If they were already
+    // empty after the last substep (not dirty), nothing needs writing.
+    const bool empty = plan.rounds == 0 && !has_wild;
     if (PHASE == 1) {
-        CellSum cs = sum_partials(a, b, l, cell_ok);
+        if (empty && !dirty) return;
+        int4 acc = sum_partials(a, plan);
         if (!cell_ok) return;
-        if (has_wild) take_wild(a, gi, cs.acc, true);
-        a.grid[gi] = cs.acc;
+        if (has_wild) take_wild(a, gi, acc, true);
+        a.grid[gi] = acc;
     } else {
-        int4 g = make_int4(0, 0, 0, 0);
-        if (cell_ok) g = a.grid[gi];
-        const CellSum cs = sum_partials(a, b, l, cell_ok);
-        if (!cell_ok) return;
-        if (cs.rounds == 0 && !has_wild) {  // no particle nearby: the cell stays empty
-            a.vel[gi] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (empty) {
+            if (dirty && cell_ok) a.vel[gi] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (threadIdx.x == 0 && dirty) a.dirty[b] = 0;
             return;
         }
-        g = add4(g, make_int4(cs.acc.x, cs.acc.y, cs.acc.z, 0));
+        int4 g = make_int4(0, 0, 0, 0);
+        if (cell_ok) g = a.grid[gi];
+        const int4 acc = sum_partials(a, plan);
+        if (threadIdx.x == 0 && !dirty) a.dirty[b] = 1;
+        if (!cell_ok) return;
+        g = add4(g, make_int4(acc.x, acc.y, acc.z, 0));
         if (has_wild) take_wild(a, gi, g, false);
         a.grid[gi] = g;
         const int4 u = update_cell_xyz(g, cx, cy, cz, wall_hi, prm);
@@ -1035,18 +1070,16 @@ void scene_key_geometry(SceneDesc &d) {
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st) {
     const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
     const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
-    const unsigned nseg = blocks_for(b.nkeys, 1u << kSegBits);
-    const unsigned ntb = (a.stiles + kXcds - 1) / kXcds * kXcds;
+    const unsigned nseg = blocks_for(b.nkeys, kSegKeys);
     if (count) {
         const unsigned zn = std::max(1u, std::min(blocks_for(b.nkeys, kBlock), 1024u));
         k_zero_u32<<<zn, kBlock, 0, st>>>(b.count, b.nkeys);
         k_zero_u32<<<1, kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(b.sup), 2 * b.nsup);
         if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a);
-    } else if (a.stiles) {
-        k_bin_gather<<<ntb, kWave, 0, st>>>(a);
     }
-    k_seg_sums<<<nseg, kWave, 0, st>>>(b, cap);
-    k_scan_keys<<<nseg, kWave, 0, st>>>(a, b, cap);
+    k_bin_gather<<<(nseg + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, 0, st>>>(a, b, cap,
+                                                                             count ? 1 : 0);
+    k_scan_keys<<<blocks_for(b.nkeys, kWave), kWave, 0, st>>>(a, b, cap);
     if (a.nchunks) k_bin_scatter<<<nbin, kBlock, 0, st>>>(a, b);
     return hipGetLastError();
 }
