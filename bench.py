@@ -120,16 +120,15 @@ def bench_batched(substeps: int = 20) -> dict:
         bt.step(substeps)
         bt.synchronize()
         batched_s = time.perf_counter() - t0
+    # sequential plain launches: each scene advanced alone, one after another
     sims = [MLSMPMSimulator(params=p, particles=r, init_box_size=b)
             for (r, b, _), p in zip(sc, params)]
     for sim in sims:
         sim.execute(2)
-    for sim in sims:
         sim.synchronize()
     t0 = time.perf_counter()
     for sim in sims:
         sim.execute(substeps)
-    for sim in sims:
         sim.synchronize()
     seq_s = time.perf_counter() - t0
     for sim in sims:

@@ -133,50 +133,71 @@ __global__ __launch_bounds__(kAdjBlock) void k_grid_lin(const int4 *__restrict__
     lin[id] = L;
 }
 
-// ---- G2P adjoint: particle adjoints -> grid velocity adjoint (float scatter)
+// ---- per-particle adjoint bodies ----------------------------------------------
+// Written once over an accessor A that is either the chunk's LDS tile window
+// (every node inside the grid) or the global grid (wild particles: nodes
+// outside [0, G) are skipped, like the forward).
 
-__global__ __launch_bounds__(kAdjBlock) void k_adj_g2p(
-    const float *__restrict__ st, const uint32_t *__restrict__ id, uint32_t stride, uint32_t n,
-    const GridLin *__restrict__ lin, uint32_t G, const Uniforms *__restrict__ u, Params prm,
-    const float *__restrict__ adj_in, float *__restrict__ adj_out, uint32_t astride,
-    float *__restrict__ gvel) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t o = id[i];
+struct AdjArgs {
+    const SceneDesc *sd;           // the scene (device)
+    const float *st;               // taped binned state of the substep
+    const uint32_t *id;
+    uint32_t stride;
+    const TileChunk *tchunks;      // taped tile chunks of the substep
+    const uint32_t *ntchunks;
+    const uint2 *tile_chunk;
+    uint32_t tchunk_cap;
+    const uint32_t *nbr;
+    const uint4 *tile_info;
+    uint32_t stiles;
+    const GridLin *lin;
+    uint32_t G;
+    float *gvel, *gmv, *gm;        // per cell: 3, 3, 1 floats
+    float4 *partial;               // kWinCells per tile chunk
+    const float *adj_in;
+    float *adj_out;
+    uint32_t astride;
+    float *gtheta;
+    Params prm;
+    Uniforms uni;
+    int xcd_remap;
+};
+
+template <class A>
+__device__ __forceinline__ void adj_g2p_particle(const AdjArgs &q, uint32_t i, const A &acc) {
+    const uint32_t o = q.id[i];
     float x[3], vold[3], C[3][3];
-    load_state(st, stride, i, x, vold, C);
+    load_state(q.st, q.stride, i, x, vold, C);
     StencilD s;
     stencil_d(x, s);
-    const int32_t Y = u->Y, Z = u->Z;
-    // forward G2P recomputation
-    float v[3] = {0.0f, 0.0f, 0.0f};
+    float v[3] = {0.0f, 0.0f, 0.0f};  // forward G2P recomputation
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
         for (int gy = 0; gy < 3; ++gy)
 #pragma unroll
             for (int gz = 0; gz < 3; ++gz) {
-                const int32_t idx = sidx(s, gx, gy, gz, Y, Z);
-                if (!in_grid(idx, G)) continue;
+                if (!acc.ok(s, gx, gy, gz)) continue;
                 const float w = (s.w[gx][0] * s.w[gy][1]) * s.w[gz][2];
-                const GridLin &L = lin[idx];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) v[r] = v[r] + L.vg[r] * w;
+                const float3 vg = acc.vg(s, gx, gy, gz);
+                v[0] = v[0] + vg.x * w;
+                v[1] = v[1] + vg.y * w;
+                v[2] = v[2] + vg.z * w;
             }
-    const float dt = prm.dt, k = 3.0f, ws = 0.3f;
+    const float dt = q.prm.dt, k = 3.0f, ws = 0.3f;
     float gv[3], gxp[3], B_[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) B_[c][r] = 4.0f * adj_in[(AC + 3 * c + r) * astride + o];
+        for (int r = 0; r < 3; ++r) B_[c][r] = 4.0f * q.adj_in[(AC + 3 * c + r) * q.astride + o];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float rb = u->real_box[a];
+        const float rb = q.uni.real_box[a];
         const float x1 = x[a] + v[a] * dt;
         const float xc = fminf(fmaxf(x1, 1.0f), rb - 2.0f);
         const float xn = xc + (v[a] * dt) * k;
-        const float gvo = adj_in[(AV + a) * astride + o];
-        const float gxo = adj_in[(AX + a) * astride + o];
+        const float gvo = q.adj_in[(AV + a) * q.astride + o];
+        const float gxo = q.adj_in[(AX + a) * q.astride + o];
         float act = 0.0f;
         if (xn < 3.0f) act += 1.0f;
         if (xn > rb - 4.0f) act += 1.0f;
@@ -193,182 +214,396 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p(
         for (int gy = 0; gy < 3; ++gy)
 #pragma unroll
             for (int gz = 0; gz < 3; ++gz) {
-                const int32_t idx = sidx(s, gx, gy, gz, Y, Z);
-                if (!in_grid(idx, G)) continue;
+                if (!acc.ok(s, gx, gy, gz)) continue;
                 float w, dw[3];
                 weight_grad(s, gx, gy, gz, w, dw);
                 const float d[3] = {s.dist[0][gx], s.dist[1][gy], s.dist[2][gz]};
-                const GridLin &L = lin[idx];
+                const float3 vg3 = acc.vg(s, gx, gy, gz);
+                const float vg[3] = {vg3.x, vg3.y, vg3.z};
                 float coef = 0.0f;  // d(scalar)/dw
+                float add[3];
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
                     float bd = 0.0f;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) bd += B_[c][r] * d[c];
                     const float g = gv[r] + bd;  // adjoint of v_i[r] per unit weight
-                    unsafeAtomicAdd(&gvel[3 * (size_t)idx + r], w * g);
-                    coef += L.vg[r] * g;
+                    add[r] = w * g;
+                    coef += vg[r] * g;
                 }
+                acc.add_gvel(s, gx, gy, gz, add);
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     float bv = 0.0f;  // d dist[a] / dx[a] = -1
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) bv += B_[a][r] * L.vg[r];
+                    for (int r = 0; r < 3; ++r) bv += B_[a][r] * vg[r];
                     gxp[a] += dw[a] * coef - w * bv;
                 }
             }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) adj_out[(AX + a) * astride + o] = gxp[a];
+    for (int a = 0; a < 3; ++a) q.adj_out[(AX + a) * q.astride + o] = gxp[a];
 }
 
-// ---- grid update adjoint -----------------------------------------------------
+template <class A>
+__device__ __forceinline__ void adj_p2g2_particle(const AdjArgs &q, uint32_t i, const A &acc,
+                                                  float &gk, float &gmu) {
+    const uint32_t o = q.id[i];
+    const Params prm = q.prm;
+    float x[3], v[3], C[3][3];
+    load_state(q.st, q.stride, i, x, v, C);
+    StencilD s;
+    stencil_d(x, s);
+    float rho = 0.0f;
+#pragma unroll
+    for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+        for (int gy = 0; gy < 3; ++gy)
+#pragma unroll
+            for (int gz = 0; gz < 3; ++gz) {
+                if (!acc.ok(s, gx, gy, gz)) continue;
+                rho = rho + acc.m(s, gx, gy, gz) * ((s.w[gx][0] * s.w[gy][1]) * s.w[gz][2]);
+            }
+    const float vol = 1.0f / rho;
+    const float r = rho / prm.rest_density;
+    const float r4 = (r * r) * (r * r);
+    const float pw = prm.stiffness * (r4 * r - 1.0f);
+    const bool pos = pw > 0.0f;
+    const float P = pos ? pw : 0.0f;
+    float S[3][3], M[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+            S[c][rr] = (c == rr ? -P : 0.0f) + prm.viscosity * (C[c][rr] + C[rr][c]);
+            M[c][rr] = -4.0f * vol * prm.dt * S[c][rr];
+        }
+    // gather: M adjoint and the weight / dist terms
+    float gM[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    float gxp[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+        for (int gy = 0; gy < 3; ++gy)
+#pragma unroll
+            for (int gz = 0; gz < 3; ++gz) {
+                if (!acc.ok(s, gx, gy, gz)) continue;
+                float w, dw[3];
+                weight_grad(s, gx, gy, gz, w, dw);
+                const float d[3] = {s.dist[0][gx], s.dist[1][gy], s.dist[2][gz]};
+                const float3 g3 = acc.gmv3(s, gx, gy, gz);
+                const float g[3] = {g3.x, g3.y, g3.z};
+                float coef = 0.0f;
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr) {
+                    const float md = M[0][rr] * d[0] + M[1][rr] * d[1] + M[2][rr] * d[2];
+                    coef += g[rr] * md;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) gM[c][rr] += w * g[rr] * d[c];
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float mg = 0.0f;
+#pragma unroll
+                    for (int rr = 0; rr < 3; ++rr) mg += M[a][rr] * g[rr];
+                    gxp[a] += dw[a] * coef - w * mg;
+                }
+            }
+    // M = -4 dt vol S
+    float gS[3][3], gvol = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+            gS[c][rr] = -4.0f * vol * prm.dt * gM[c][rr];
+            gvol += -4.0f * prm.dt * S[c][rr] * gM[c][rr];
+        }
+    float gC[3][3];
+    const float gP = -(gS[0][0] + gS[1][1] + gS[2][2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+            gC[c][rr] = prm.viscosity * (gS[c][rr] + gS[rr][c]);
+            gmu += gS[c][rr] * (C[c][rr] + C[rr][c]);
+        }
+    float grho = -gvol / (rho * rho);
+    if (pos) {
+        grho += gP * prm.stiffness * 5.0f * r4 / prm.rest_density;
+        gk += gP * (r4 * r - 1.0f);
+    }
+    // density = sum m_i w: scatter to the masses, weight terms to x
+#pragma unroll
+    for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+        for (int gy = 0; gy < 3; ++gy)
+#pragma unroll
+            for (int gz = 0; gz < 3; ++gz) {
+                if (!acc.ok(s, gx, gy, gz)) continue;
+                float w, dw[3];
+                weight_grad(s, gx, gy, gz, w, dw);
+                acc.add_gm(s, gx, gy, gz, grho * w);
+                const float m = acc.m(s, gx, gy, gz);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) gxp[a] += grho * m * dw[a];
+            }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q.adj_out[(AX + a) * q.astride + o] += gxp[a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) q.adj_out[(AC + 3 * c + rr) * q.astride + o] = gC[c][rr];
+}
 
-__global__ __launch_bounds__(kAdjBlock) void k_adj_update(const GridLin *__restrict__ lin,
-                                                          uint32_t G, float *__restrict__ gvel,
-                                                          float *__restrict__ gmv,
-                                                          float *__restrict__ gm) {
-    const uint32_t id = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (id >= G) return;
-    const GridLin L = lin[id];
-    float g[3];
+template <class A>
+__device__ __forceinline__ void adj_p2g1_particle(const AdjArgs &q, uint32_t i, const A &acc) {
+    const uint32_t o = q.id[i];
+    float x[3], v[3], C[3][3];
+    load_state(q.st, q.stride, i, x, v, C);
+    StencilD s;
+    stencil_d(x, s);
+    float gvp[3] = {0.0f, 0.0f, 0.0f}, gxp[3] = {0.0f, 0.0f, 0.0f};
+    float gC[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+        for (int gy = 0; gy < 3; ++gy)
+#pragma unroll
+            for (int gz = 0; gz < 3; ++gz) {
+                if (!acc.ok(s, gx, gy, gz)) continue;
+                float w, dw[3];
+                weight_grad(s, gx, gy, gz, w, dw);
+                const float d[3] = {s.dist[0][gx], s.dist[1][gy], s.dist[2][gz]};
+                const float3 g3 = acc.gmv3(s, gx, gy, gz);
+                const float g[3] = {g3.x, g3.y, g3.z};
+                float coef = acc.gm1(s, gx, gy, gz);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float qd = C[0][r] * d[0] + C[1][r] * d[1] + C[2][r] * d[2];
+                    coef += g[r] * (v[r] + qd);
+                    gvp[r] += w * g[r];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) gC[c][r] += w * g[r] * d[c];
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float cg = 0.0f;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) cg += C[a][r] * g[r];
+                    gxp[a] += dw[a] * coef - w * cg;
+                }
+            }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        g[a] = (L.flags & (1u << a)) ? gvel[3 * (size_t)id + a] : 0.0f;
-        gvel[3 * (size_t)id + a] = 0.0f;  // ready for the next substep
+        q.adj_out[(AX + a) * q.astride + o] += gxp[a];
+        q.adj_out[(AV + a) * q.astride + o] = gvp[a];
     }
-    float m_adj = 0.0f;
-    if (L.flags & 8u) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            gmv[3 * (size_t)id + a] = g[a] / L.m;
-            m_adj -= g[a] * L.vpre[a] / L.m;
-        }
-    } else {
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) gmv[3 * (size_t)id + a] = g[a];
-    }
-    gm[id] = m_adj;
+        for (int r = 0; r < 3; ++r) q.adj_out[(AC + 3 * c + r) * q.astride + o] += gC[c][r];
 }
 
-// ---- P2G-2 adjoint: gather of the momentum adjoint, scatter of the density's
+// ---- accessors ----------------------------------------------------------------
 
-__global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2(
-    const float *__restrict__ st, const uint32_t *__restrict__ id, uint32_t stride, uint32_t n,
-    const GridLin *__restrict__ lin, uint32_t G, const Uniforms *__restrict__ u, Params prm,
-    const float *__restrict__ gmv, float *__restrict__ gm, float *__restrict__ adj_out,
-    uint32_t astride, float *__restrict__ gtheta) {
-    __shared__ float s_red[2][kAdjBlock / 64];
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    float gk = 0.0f, gmu = 0.0f;
-    if (i < n) {
-        const uint32_t o = id[i];
-        float x[3], v[3], C[3][3];
-        load_state(st, stride, i, x, v, C);
-        StencilD s;
-        stencil_d(x, s);
-        const int32_t Y = u->Y, Z = u->Z;
-        float rho = 0.0f;
-#pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    const int32_t idx = sidx(s, gx, gy, gz, Y, Z);
-                    if (!in_grid(idx, G)) continue;
-                    rho = rho + lin[idx].m * ((s.w[gx][0] * s.w[gy][1]) * s.w[gz][2]);
-                }
-        const float vol = 1.0f / rho;
-        const float r = rho / prm.rest_density;
-        const float r4 = (r * r) * (r * r);
-        const float pw = prm.stiffness * (r4 * r - 1.0f);
-        const bool pos = pw > 0.0f;
-        const float P = pos ? pw : 0.0f;
-        float S[3][3], M[3][3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-                S[c][rr] = (c == rr ? -P : 0.0f) + prm.viscosity * (C[c][rr] + C[rr][c]);
-                M[c][rr] = -4.0f * vol * prm.dt * S[c][rr];
-            }
-        // gather: M adjoint and the weight / dist terms
-        float gM[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-        float gxp[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    const int32_t idx = sidx(s, gx, gy, gz, Y, Z);
-                    if (!in_grid(idx, G)) continue;
-                    float w, dw[3];
-                    weight_grad(s, gx, gy, gz, w, dw);
-                    const float d[3] = {s.dist[0][gx], s.dist[1][gy], s.dist[2][gz]};
-                    const float g[3] = {gmv[3 * (size_t)idx], gmv[3 * (size_t)idx + 1],
-                                        gmv[3 * (size_t)idx + 2]};
-                    float coef = 0.0f;
-#pragma unroll
-                    for (int rr = 0; rr < 3; ++rr) {
-                        const float md = M[0][rr] * d[0] + M[1][rr] * d[1] + M[2][rr] * d[2];
-                        coef += g[rr] * md;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) gM[c][rr] += w * g[rr] * d[c];
-                    }
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        float mg = 0.0f;
-#pragma unroll
-                        for (int rr = 0; rr < 3; ++rr) mg += M[a][rr] * g[rr];
-                        gxp[a] += dw[a] * coef - w * mg;
-                    }
-                }
-        // M = -4 dt vol S
-        float gS[3][3], gvol = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-                gS[c][rr] = -4.0f * vol * prm.dt * gM[c][rr];
-                gvol += -4.0f * prm.dt * S[c][rr] * gM[c][rr];
-            }
-        float gC[3][3];
-        const float gP = -(gS[0][0] + gS[1][1] + gS[2][2]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) {
-                gC[c][rr] = prm.viscosity * (gS[c][rr] + gS[rr][c]);
-                gmu += gS[c][rr] * (C[c][rr] + C[rr][c]);
-            }
-        float grho = -gvol / (rho * rho);
-        if (pos) {
-            grho += gP * prm.stiffness * 5.0f * r4 / prm.rest_density;
-            gk += gP * (r4 * r - 1.0f);
+struct GlobalAcc {  // wild particles: the whole grid, float atomics
+    const GridLin *lin;
+    float *gvel, *gmv, *gm;
+    uint32_t G;
+    int32_t Y, Z;
+    __device__ __forceinline__ explicit GlobalAcc(const AdjArgs &q)
+        : lin(q.lin), gvel(q.gvel), gmv(q.gmv), gm(q.gm), G(q.G), Y(q.uni.Y), Z(q.uni.Z) {}
+    __device__ __forceinline__ int32_t idx(const StencilD &s, int gx, int gy, int gz) const {
+        return sidx(s, gx, gy, gz, Y, Z);
+    }
+    __device__ __forceinline__ bool ok(const StencilD &s, int gx, int gy, int gz) const {
+        return in_grid(idx(s, gx, gy, gz), G);
+    }
+    __device__ __forceinline__ float3 vg(const StencilD &s, int gx, int gy, int gz) const {
+        const GridLin &L = lin[idx(s, gx, gy, gz)];
+        return make_float3(L.vg[0], L.vg[1], L.vg[2]);
+    }
+    __device__ __forceinline__ float m(const StencilD &s, int gx, int gy, int gz) const {
+        return lin[idx(s, gx, gy, gz)].m;
+    }
+    __device__ __forceinline__ float3 gmv3(const StencilD &s, int gx, int gy, int gz) const {
+        const size_t i = 3 * (size_t)idx(s, gx, gy, gz);
+        return make_float3(gmv[i], gmv[i + 1], gmv[i + 2]);
+    }
+    __device__ __forceinline__ float gm1(const StencilD &s, int gx, int gy, int gz) const {
+        return gm[idx(s, gx, gy, gz)];
+    }
+    __device__ __forceinline__ void add_gvel(const StencilD &s, int gx, int gy, int gz,
+                                             const float a[3]) const {
+        const size_t i = 3 * (size_t)idx(s, gx, gy, gz);
+        unsafeAtomicAdd(&gvel[i], a[0]);
+        unsafeAtomicAdd(&gvel[i + 1], a[1]);
+        unsafeAtomicAdd(&gvel[i + 2], a[2]);
+    }
+    __device__ __forceinline__ void add_gm(const StencilD &s, int gx, int gy, int gz,
+                                           float a) const {
+        unsafeAtomicAdd(&gm[idx(s, gx, gy, gz)], a);
+    }
+};
+
+// The chunk's 6^3 tile window in LDS: w4 holds per-cell inputs (vg or gmv in
+// xyz, mass or gm in w), acc per-cell float accumulators (x, y, z).
+struct WinAcc {
+    const float4 *w4;
+    float *acc;  // 3 x kWinCells
+    int l0;
+    __device__ __forceinline__ int l(int gx, int gy, int gz) const {
+        return l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
+    }
+    __device__ __forceinline__ bool ok(const StencilD &, int, int, int) const { return true; }
+    __device__ __forceinline__ float3 vg(const StencilD &, int gx, int gy, int gz) const {
+        const float4 c = w4[l(gx, gy, gz)];
+        return make_float3(c.x, c.y, c.z);
+    }
+    __device__ __forceinline__ float3 gmv3(const StencilD &s, int gx, int gy, int gz) const {
+        return vg(s, gx, gy, gz);
+    }
+    __device__ __forceinline__ float m(const StencilD &, int gx, int gy, int gz) const {
+        return w4[l(gx, gy, gz)].w;
+    }
+    __device__ __forceinline__ float gm1(const StencilD &s, int gx, int gy, int gz) const {
+        return m(s, gx, gy, gz);
+    }
+    __device__ __forceinline__ void add_gvel(const StencilD &, int gx, int gy, int gz,
+                                             const float a[3]) const {
+        const int c = l(gx, gy, gz);
+        atomicAdd(&acc[c], a[0]);
+        atomicAdd(&acc[kWinCells + c], a[1]);
+        atomicAdd(&acc[2 * kWinCells + c], a[2]);
+    }
+    __device__ __forceinline__ void add_gm(const StencilD &, int gx, int gy, int gz,
+                                           float a) const {
+        atomicAdd(&acc[l(gx, gy, gz)], a);
+    }
+};
+
+__device__ __forceinline__ int win_l0_of(const float *st, uint32_t stride, uint32_t i) {
+    const int m = kTileEdge - 1;
+    const int cx = (int)floorf(st[(PX + 0) * stride + i]);
+    const int cy = (int)floorf(st[(PX + 1) * stride + i]);
+    const int cz = (int)floorf(st[(PX + 2) * stride + i]);
+    return ((cx & m) * kWinEdge + (cy & m)) * kWinEdge + (cz & m);
+}
+
+// Grid cell of window cell l of tile t, or -1 outside the scene.
+__device__ __forceinline__ int64_t adj_win_cell(const AdjArgs &q, int l, const int t[3]) {
+    const SceneDesc *sd = q.sd;
+    const int x = t[0] * kTileEdge - 1 + l / (kWinEdge * kWinEdge);
+    const int y = t[1] * kTileEdge - 1 + (l / kWinEdge) % kWinEdge;
+    const int z = t[2] * kTileEdge - 1 + l % kWinEdge;
+    if ((uint32_t)x >= (uint32_t)sd->kdim[0] || (uint32_t)y >= (uint32_t)sd->kdim[1] ||
+        (uint32_t)z >= (uint32_t)sd->kdim[2])
+        return -1;
+    return (int64_t)((uint32_t)x * (uint32_t)q.uni.Y * (uint32_t)q.uni.Z +
+                     (uint32_t)y * (uint32_t)q.uni.Z + (uint32_t)z);
+}
+
+__device__ __forceinline__ void adj_tile_coords(const AdjArgs &q, uint32_t xm, int t[3]) {
+    const uint32_t n1 = q.sd->knt[1], n2 = q.sd->knt[2];
+    t[0] = (int)(xm / (n1 * n2));
+    t[1] = (int)((xm / n2) % n1);
+    t[2] = (int)(xm % n2);
+}
+
+// Persistent walk over the taped tile chunks (XCD-contiguous ranges).
+struct AdjWalk {
+    uint32_t c, end, step;
+};
+__device__ __forceinline__ AdjWalk adj_walk(const AdjArgs &q) {
+    const uint32_t n = min(*q.ntchunks, q.tchunk_cap);
+    AdjWalk w;
+    if (q.xcd_remap && (gridDim.x % 8) == 0) {
+        const uint32_t per = (n + 7) / 8, x = blockIdx.x % 8;
+        w.c = x * per + blockIdx.x / 8;
+        w.end = min(n, (x + 1) * per);
+        w.step = gridDim.x / 8;
+    } else {
+        w.c = blockIdx.x;
+        w.end = n;
+        w.step = gridDim.x;
+    }
+    return w;
+}
+
+// ---- chunk kernels ---------------------------------------------------------------
+
+// G2P adjoint: window of vg in, window of grid-velocity adjoints out (partial).
+__global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
+    __shared__ float4 s_w4[kWinCells];
+    __shared__ float s_acc[3 * kWinCells];
+    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = q.tchunks[w.c];
+        if (ch.y == kWildTile) {
+            const GlobalAcc acc(q);
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
+                adj_g2p_particle(q, ch.z + j, acc);
+            continue;
         }
-        // density = sum m_i w: scatter to the masses, weight terms to x
-#pragma unroll
-        for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-            for (int gy = 0; gy < 3; ++gy)
-#pragma unroll
-                for (int gz = 0; gz < 3; ++gz) {
-                    const int32_t idx = sidx(s, gx, gy, gz, Y, Z);
-                    if (!in_grid(idx, G)) continue;
-                    float w, dw[3];
-                    weight_grad(s, gx, gy, gz, w, dw);
-                    unsafeAtomicAdd(&gm[idx], grho * w);
-                    const float m = lin[idx].m;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) gxp[a] += grho * m * dw[a];
-                }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) adj_out[(AX + a) * astride + o] += gxp[a];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int rr = 0; rr < 3; ++rr) adj_out[(AC + 3 * c + rr) * astride + o] = gC[c][rr];
+        int t[3];
+        adj_tile_coords(q, ch.y, t);
+        for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
+            const int64_t gi = adj_win_cell(q, l, t);
+            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gi >= 0) {
+                const GridLin &L = q.lin[gi];
+                c = make_float4(L.vg[0], L.vg[1], L.vg[2], 0.0f);
+            }
+            s_w4[l] = c;
+            s_acc[l] = s_acc[kWinCells + l] = s_acc[2 * kWinCells + l] = 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+            const uint32_t i = ch.z + j;
+            const WinAcc acc{s_w4, s_acc, win_l0_of(q.st, q.stride, i)};
+            adj_g2p_particle(q, i, acc);
+        }
+        __syncthreads();
+        float4 *out = q.partial + (size_t)w.c * kWinCells;
+        for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock)
+            out[l] = make_float4(s_acc[l], s_acc[kWinCells + l], s_acc[2 * kWinCells + l], 0.0f);
+        __syncthreads();
+    }
+}
+
+// P2G-2 adjoint: window of (gmv, mass) in, window of mass adjoints out.
+__global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
+    __shared__ float4 s_w4[kWinCells];
+    __shared__ float s_acc[kWinCells];
+    __shared__ float s_red[2][kAdjBlock / 64];
+    float gk = 0.0f, gmu = 0.0f;
+    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = q.tchunks[w.c];
+        if (ch.y == kWildTile) {
+            const GlobalAcc acc(q);
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
+                adj_p2g2_particle(q, ch.z + j, acc, gk, gmu);
+            continue;
+        }
+        int t[3];
+        adj_tile_coords(q, ch.y, t);
+        for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
+            const int64_t gi = adj_win_cell(q, l, t);
+            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gi >= 0)
+                c = make_float4(q.gmv[3 * gi], q.gmv[3 * gi + 1], q.gmv[3 * gi + 2], q.lin[gi].m);
+            s_w4[l] = c;
+            s_acc[l] = 0.0f;
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+            const uint32_t i = ch.z + j;
+            const WinAcc acc{s_w4, s_acc, win_l0_of(q.st, q.stride, i)};
+            adj_p2g2_particle(q, i, acc, gk, gmu);
+        }
+        __syncthreads();
+        float4 *out = q.partial + (size_t)w.c * kWinCells;
+        for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock)
+            out[l] = make_float4(s_acc[l], 0.0f, 0.0f, 0.0f);
+        __syncthreads();
     }
     // block reduction of the scalar parameter adjoints
 #pragma unroll
@@ -387,66 +622,149 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2(
             a += s_red[0][wv];
             b += s_red[1][wv];
         }
-        unsafeAtomicAdd(&gtheta[0], a);
-        unsafeAtomicAdd(&gtheta[1], b);
+        if (a != 0.0f) unsafeAtomicAdd(&q.gtheta[0], a);
+        if (b != 0.0f) unsafeAtomicAdd(&q.gtheta[1], b);
     }
 }
 
-// ---- P2G-1 adjoint: gather of mass and momentum adjoints ---------------------
+// P2G-1 adjoint: window of (gmv, gm) in, particle adjoints out.
+__global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
+    __shared__ float4 s_w4[kWinCells];
+    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = q.tchunks[w.c];
+        if (ch.y == kWildTile) {
+            const GlobalAcc acc(q);
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
+                adj_p2g1_particle(q, ch.z + j, acc);
+            continue;
+        }
+        int t[3];
+        adj_tile_coords(q, ch.y, t);
+        for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
+            const int64_t gi = adj_win_cell(q, l, t);
+            float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (gi >= 0)
+                c = make_float4(q.gmv[3 * gi], q.gmv[3 * gi + 1], q.gmv[3 * gi + 2], q.gm[gi]);
+            s_w4[l] = c;
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+            const uint32_t i = ch.z + j;
+            const WinAcc acc{s_w4, nullptr, win_l0_of(q.st, q.stride, i)};
+            adj_p2g1_particle(q, i, acc);
+        }
+        __syncthreads();
+    }
+}
 
-__global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1(
-    const float *__restrict__ st, const uint32_t *__restrict__ id, uint32_t stride, uint32_t n,
-    uint32_t G, const Uniforms *__restrict__ u, const float *__restrict__ gmv,
-    const float *__restrict__ gm, float *__restrict__ adj_out, uint32_t astride) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t o = id[i];
-    float x[3], v[3], C[3][3];
-    load_state(st, stride, i, x, v, C);
-    StencilD s;
-    stencil_d(x, s);
-    const int32_t Y = u->Y, Z = u->Z;
-    float gvp[3] = {0.0f, 0.0f, 0.0f}, gxp[3] = {0.0f, 0.0f, 0.0f};
-    float gC[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+// Sum of the partial windows covering each of a tile's 64 cells (one wave
+// per tile; the same cover as the forward reduce).
+__device__ __forceinline__ float4 adj_sum_partials(const AdjArgs &q, uint32_t b, const int l[3],
+                                                   bool cell_ok) {
+    const int lane = threadIdx.x & 63;
+    uint32_t nfirst = 0, ncnt = 0;
+    if (lane < 27) {
+        const uint32_t kt = q.nbr[(size_t)b * 27 + lane];
+        if (kt != kNoTile) {
+            const uint2 tc = q.tile_chunk[kt];
+            nfirst = tc.x;
+            ncnt = tc.x <= q.tchunk_cap && tc.y <= q.tchunk_cap - tc.x ? tc.y : 0u;
+        }
+    }
+    int ext[3];
 #pragma unroll
-    for (int gx = 0; gx < 3; ++gx)
+    for (int ax = 0; ax < 3; ++ax) ext[ax] = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-        for (int gy = 0; gy < 3; ++gy)
+    for (int s = 0; s < 8; ++s) {
+        bool valid = cell_ok;
+        int d[3];
 #pragma unroll
-            for (int gz = 0; gz < 3; ++gz) {
-                const int32_t idx = sidx(s, gx, gy, gz, Y, Z);
-                if (!in_grid(idx, G)) continue;
-                float w, dw[3];
-                weight_grad(s, gx, gy, gz, w, dw);
-                const float d[3] = {s.dist[0][gx], s.dist[1][gy], s.dist[2][gz]};
-                const float g[3] = {gmv[3 * (size_t)idx], gmv[3 * (size_t)idx + 1],
-                                    gmv[3 * (size_t)idx + 2]};
-                float coef = gm[idx];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const float q = C[0][r] * d[0] + C[1][r] * d[1] + C[2][r] * d[2];
-                    coef += g[r] * (v[r] + q);
-                    gvp[r] += w * g[r];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) gC[c][r] += w * g[r] * d[c];
-                }
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    float cg = 0.0f;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) cg += C[a][r] * g[r];
-                    gxp[a] += dw[a] * coef - w * cg;
-                }
-            }
+        for (int ax = 0; ax < 3; ++ax) {
+            const bool bit = (s >> ax) & 1;
+            d[ax] = bit ? ext[ax] : 0;
+            valid = valid && !(bit && ext[ax] == 0);
+        }
+        const int qq = (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1);
+        const uint32_t f = __shfl(nfirst, qq, 64), c = __shfl(ncnt, qq, 64);
+        const int widx = ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) *
+                             kWinEdge +
+                         (l[2] + 1 - kTileEdge * d[2]);
+        if (!valid) continue;
+        for (uint32_t j = 0; j < c; ++j) {
+            const float4 v = q.partial[(size_t)(f + j) * kWinCells + widx];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+        }
+    }
+    return acc;
+}
+
+// Grid-update adjoint per tile: grid-velocity adjoint = global scatter (wild
+// particles) + the G2P partials, then through the update to (gmv, gm).
+__global__ __launch_bounds__(64) void k_adj_update_t(AdjArgs q) {
+    const uint32_t b = blockIdx.x;
+    if (b >= q.stiles) return;
+    const SceneDesc *sd = q.sd;
+    int l[3] = {(int)threadIdx.x >> 4, ((int)threadIdx.x >> 2) & 3, (int)threadIdx.x & 3};
+    int64_t gi = -1;
+    float4 part = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (sd->fast) {
+        const uint4 ti = q.tile_info[b];
+        const int c[3] = {(int)ti.x * kTileEdge + l[0], (int)ti.y * kTileEdge + l[1],
+                          (int)ti.z * kTileEdge + l[2]};
+        const bool ok = (uint32_t)c[0] < (uint32_t)sd->kdim[0] &&
+                        (uint32_t)c[1] < (uint32_t)sd->kdim[1] &&
+                        (uint32_t)c[2] < (uint32_t)sd->kdim[2];
+        part = adj_sum_partials(q, b, l, ok);
+        if (ok)
+            gi = (int64_t)((uint32_t)c[0] * (uint32_t)q.uni.Y * (uint32_t)q.uni.Z +
+                           (uint32_t)c[1] * (uint32_t)q.uni.Z + (uint32_t)c[2]);
+    } else {
+        const uint32_t li = (b - sd->soff) * kTileCells + threadIdx.x;
+        if (li < q.G) gi = li;
+    }
+    if (gi < 0) return;
+    const GridLin L = q.lin[gi];
+    const float gsum[3] = {q.gvel[3 * gi] + part.x, q.gvel[3 * gi + 1] + part.y,
+                           q.gvel[3 * gi + 2] + part.z};
+    float g[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        adj_out[(AX + a) * astride + o] += gxp[a];
-        adj_out[(AV + a) * astride + o] = gvp[a];
+        g[a] = (L.flags & (1u << a)) ? gsum[a] : 0.0f;
+        q.gvel[3 * gi + a] = 0.0f;  // ready for the next substep
     }
+    float m_adj = 0.0f;
+    if (L.flags & 8u) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            q.gmv[3 * gi + a] = g[a] / L.m;
+            m_adj -= g[a] * L.vpre[a] / L.m;
+        }
+    } else {
 #pragma unroll
-        for (int r = 0; r < 3; ++r) adj_out[(AC + 3 * c + r) * astride + o] += gC[c][r];
+        for (int a = 0; a < 3; ++a) q.gmv[3 * gi + a] = g[a];
+    }
+    q.gm[gi] = m_adj;
+}
+
+// Adds the P2G-2 partials (mass adjoints) into gm, per tile.
+__global__ __launch_bounds__(64) void k_adj_reduce_gm_t(AdjArgs q) {
+    const uint32_t b = blockIdx.x;
+    if (b >= q.stiles || !q.sd->fast) return;
+    const SceneDesc *sd = q.sd;
+    const int l[3] = {(int)threadIdx.x >> 4, ((int)threadIdx.x >> 2) & 3, (int)threadIdx.x & 3};
+    const uint4 ti = q.tile_info[b];
+    const int c[3] = {(int)ti.x * kTileEdge + l[0], (int)ti.y * kTileEdge + l[1],
+                      (int)ti.z * kTileEdge + l[2]};
+    const bool ok = (uint32_t)c[0] < (uint32_t)sd->kdim[0] &&
+                    (uint32_t)c[1] < (uint32_t)sd->kdim[1] && (uint32_t)c[2] < (uint32_t)sd->kdim[2];
+    const float4 part = adj_sum_partials(q, b, l, ok);
+    if (!ok) return;
+    const uint64_t gi = (uint32_t)c[0] * (uint32_t)q.uni.Y * (uint32_t)q.uni.Z +
+                        (uint32_t)c[1] * (uint32_t)q.uni.Z + (uint32_t)c[2];
+    if (part.x != 0.0f) q.gm[gi] += part.x;
 }
 
 // ---- loss and its gradient at the final state ---------------------------------
@@ -511,6 +829,40 @@ inline unsigned nblocks(uint64_t count) {
 
 }  // namespace
 
+// Byte offsets of one substep's tape record: the binned state and ids the
+// substep's kernels read, its tile chunks, and its raw grid after P2G.
+struct TapeLayout {
+    size_t state, id, grid, tchunks, tile_chunk, ntchunks, bytes;
+};
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+hipError_t Batch::substep_taped(hipStream_t st, char *rec, const void *layout) {
+    const TapeLayout &L = *static_cast<const TapeLayout *>(layout);
+    BatchArgs a = args();
+    TRY(launch_bin(a, bin_buffers(), !keys_ready_, st));
+    a.state = spare_state_;
+    a.id = spare_id_;
+    TRY(hipMemcpyAsync(rec + L.state, spare_state_, sizeof(float) * NCOMP * stride_,
+                       hipMemcpyDeviceToDevice, st));
+    TRY(hipMemcpyAsync(rec + L.id, spare_id_, sizeof(uint32_t) * stride_, hipMemcpyDeviceToDevice,
+                       st));
+    TRY(hipMemcpyAsync(rec + L.tchunks, tchunks_, sizeof(TileChunk) * cap_tchunks_,
+                       hipMemcpyDeviceToDevice, st));
+    TRY(hipMemcpyAsync(rec + L.tile_chunk, tile_chunk_, sizeof(uint2) * total_keys_,
+                       hipMemcpyDeviceToDevice, st));
+    TRY(hipMemcpyAsync(rec + L.ntchunks, ntchunks_, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    TRY(launch_p2g1(a, st));
+    TRY(launch_reduce1(a, st));
+    TRY(launch_p2g2(a, st));
+    TRY(launch_reduce2(a, st));
+    TRY(launch_g2p(a, st));
+    TRY(hipMemcpyAsync(rec + L.grid, grid_ + h_scenes_[0].goff,
+                       sizeof(int4) * (size_t)h_scenes_[0].G, hipMemcpyDeviceToDevice, st));
+    after_substep(st);
+    return hipSuccess;
+}
+
 hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResult &res) {
     if (h_scenes_.size() != 1 || cfg.substeps == 0 || cfg.checkpoint_every == 0)
         return hipErrorInvalidValue;
@@ -518,7 +870,14 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
     const uint32_t n = d.n, G = d.G, T = cfg.substeps, K = cfg.checkpoint_every;
     const uint32_t nseg = (T + K - 1) / K;
     const size_t sbytes = sizeof(float) * NCOMP * stride_, ibytes = sizeof(uint32_t) * stride_;
-    const size_t gbytes = sizeof(int4) * (size_t)G;
+    TapeLayout L{};
+    L.state = 0;
+    L.id = align256(L.state + sbytes);
+    L.grid = align256(L.id + ibytes);
+    L.tchunks = align256(L.grid + sizeof(int4) * (size_t)G);
+    L.tile_chunk = align256(L.tchunks + sizeof(TileChunk) * cap_tchunks_);
+    L.ntchunks = align256(L.tile_chunk + sizeof(uint2) * total_keys_);
+    L.bytes = align256(L.ntchunks + sizeof(uint32_t));
     const Tuning saved_tuning = tun_;
     if (tun_.path != 0) {
         Tuning t = tun_;
@@ -526,8 +885,7 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         TRY(set_tuning(t));
     }
     // Device buffers: checkpoints (state + id) per segment start, the segment
-    // tape (state, id, raw P2G grid per substep), grid linearisation,
-    // adjoints.
+    // tape, grid linearisation, adjoints.
     char *ckpt = nullptr, *tape = nullptr;
     GridLin *lin = nullptr;
     float *adjA = nullptr, *adjB = nullptr, *gvel = nullptr, *gmv = nullptr, *gm = nullptr,
@@ -546,20 +904,21 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         if (lacc) (void)hipFree(lacc);
     };
     hipError_t e = hipSuccess;
-    const size_t per_ck = sbytes + ibytes, per_tape = sbytes + ibytes + gbytes;
+    const size_t per_ck = sbytes + ibytes;
     const uint32_t astride = std::max<uint32_t>(n, 1);
+    const size_t g1 = std::max<uint32_t>(G, 1);
     do {
         if ((e = hipMalloc(&ckpt, per_ck * (nseg + 1))) != hipSuccess) break;
-        if ((e = hipMalloc(&tape, per_tape * K)) != hipSuccess) break;
-        if ((e = hipMalloc(&lin, sizeof(GridLin) * std::max<uint32_t>(G, 1))) != hipSuccess) break;
+        if ((e = hipMalloc(&tape, L.bytes * K)) != hipSuccess) break;
+        if ((e = hipMalloc(&lin, sizeof(GridLin) * g1)) != hipSuccess) break;
         if ((e = hipMalloc(&adjA, sizeof(float) * ANCOMP * astride)) != hipSuccess) break;
         if ((e = hipMalloc(&adjB, sizeof(float) * ANCOMP * astride)) != hipSuccess) break;
-        if ((e = hipMalloc(&gvel, sizeof(float) * 3 * std::max<uint32_t>(G, 1))) != hipSuccess) break;
-        if ((e = hipMalloc(&gmv, sizeof(float) * 3 * std::max<uint32_t>(G, 1))) != hipSuccess) break;
-        if ((e = hipMalloc(&gm, sizeof(float) * std::max<uint32_t>(G, 1))) != hipSuccess) break;
+        if ((e = hipMalloc(&gvel, sizeof(float) * 3 * g1)) != hipSuccess) break;
+        if ((e = hipMalloc(&gmv, sizeof(float) * 3 * g1)) != hipSuccess) break;
+        if ((e = hipMalloc(&gm, sizeof(float) * g1)) != hipSuccess) break;
         if ((e = hipMalloc(&gtheta, sizeof(float) * 2)) != hipSuccess) break;
         if ((e = hipMalloc(&lacc, sizeof(double) * 4)) != hipSuccess) break;
-        if ((e = hipMemsetAsync(gvel, 0, sizeof(float) * 3 * G, st)) != hipSuccess) break;
+        if ((e = hipMemsetAsync(gvel, 0, sizeof(float) * 3 * g1, st)) != hipSuccess) break;
         if ((e = hipMemsetAsync(gtheta, 0, sizeof(float) * 2, st)) != hipSuccess) break;
         if ((e = hipMemsetAsync(lacc, 0, sizeof(double) * 4, st)) != hipSuccess) break;
 
@@ -606,39 +965,49 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         }
         // backward over segments, recomputing each segment's tape
         float *adj_in = adjA, *adj_out = adjB;
-        const Uniforms *uni = reinterpret_cast<const Uniforms *>(
+        const Uniforms *uni_d = reinterpret_cast<const Uniforms *>(
             reinterpret_cast<const char *>(d_scenes_) + offsetof(SceneDesc, uni));
+        AdjArgs q{};
+        q.sd = d_scenes_;
+        q.stride = stride_;
+        q.tchunk_cap = (uint32_t)cap_tchunks_;
+        q.nbr = d_nbr_;
+        q.tile_info = d_tile_info_;
+        q.stiles = total_tiles_;
+        q.lin = lin;
+        q.G = G;
+        q.gvel = gvel;
+        q.gmv = gmv;
+        q.gm = gm;
+        q.partial = reinterpret_cast<float4 *>(partial_);
+        q.astride = astride;
+        q.gtheta = gtheta;
+        q.prm = d.prm;
+        q.uni = d.uni;
+        q.xcd_remap = tun_.xcd_remap;
+        const unsigned nchunk_blocks = 2048;
         for (int32_t sgi = (int32_t)nseg - 1; sgi >= 0 && e == hipSuccess; --sgi) {
             const uint32_t t0 = (uint32_t)sgi * K, t1 = std::min(T, t0 + K);
             if ((e = restore(ckpt + per_ck * sgi)) != hipSuccess) break;
-            for (uint32_t t = t0; t < t1 && e == hipSuccess; ++t) {
-                char *tp = tape + per_tape * (t - t0);
-                if ((e = save(tp)) != hipSuccess) break;
-                if ((e = substep(st)) != hipSuccess) break;
-                e = hipMemcpyAsync(tp + sbytes + ibytes, grid_ + d.goff, gbytes,
-                                   hipMemcpyDeviceToDevice, st);
-            }
+            for (uint32_t t = t0; t < t1 && e == hipSuccess; ++t)
+                e = substep_taped(st, tape + L.bytes * (t - t0), &L);
             for (int32_t t = (int32_t)t1 - 1; t >= (int32_t)t0 && e == hipSuccess; --t) {
-                const char *tp = tape + per_tape * (t - t0);
-                const float *tst = reinterpret_cast<const float *>(tp);
-                const uint32_t *tid = reinterpret_cast<const uint32_t *>(tp + sbytes);
-                const int4 *tgrid = reinterpret_cast<const int4 *>(tp + sbytes + ibytes);
-                if (G) {
-                    k_grid_lin<<<nblocks(G), kAdjBlock, 0, st>>>(tgrid, G, uni, d.prm, lin);
-                }
-                if (n) {
-                    k_adj_g2p<<<nblocks(n), kAdjBlock, 0, st>>>(tst, tid, stride_, n, lin, G, uni,
-                                                                 d.prm, adj_in, adj_out, astride,
-                                                                 gvel);
-                }
-                if (G) k_adj_update<<<nblocks(G), kAdjBlock, 0, st>>>(lin, G, gvel, gmv, gm);
-                if (n) {
-                    k_adj_p2g2<<<nblocks(n), kAdjBlock, 0, st>>>(tst, tid, stride_, n, lin, G, uni,
-                                                                  d.prm, gmv, gm, adj_out, astride,
-                                                                  gtheta);
-                    k_adj_p2g1<<<nblocks(n), kAdjBlock, 0, st>>>(tst, tid, stride_, n, G, uni, gmv,
-                                                                  gm, adj_out, astride);
-                }
+                const char *rec = tape + L.bytes * (t - t0);
+                q.st = reinterpret_cast<const float *>(rec + L.state);
+                q.id = reinterpret_cast<const uint32_t *>(rec + L.id);
+                q.tchunks = reinterpret_cast<const TileChunk *>(rec + L.tchunks);
+                q.tile_chunk = reinterpret_cast<const uint2 *>(rec + L.tile_chunk);
+                q.ntchunks = reinterpret_cast<const uint32_t *>(rec + L.ntchunks);
+                q.adj_in = adj_in;
+                q.adj_out = adj_out;
+                if (G)
+                    k_grid_lin<<<nblocks(G), kAdjBlock, 0, st>>>(
+                        reinterpret_cast<const int4 *>(rec + L.grid), G, uni_d, d.prm, lin);
+                if (n) k_adj_g2p_t<<<nchunk_blocks, kAdjBlock, 0, st>>>(q);
+                if (total_tiles_) k_adj_update_t<<<total_tiles_, 64, 0, st>>>(q);
+                if (n) k_adj_p2g2_t<<<nchunk_blocks, kAdjBlock, 0, st>>>(q);
+                if (total_tiles_) k_adj_reduce_gm_t<<<total_tiles_, 64, 0, st>>>(q);
+                if (n) k_adj_p2g1_t<<<nchunk_blocks, kAdjBlock, 0, st>>>(q);
                 e = hipGetLastError();
                 std::swap(adj_in, adj_out);
             }

@@ -95,6 +95,9 @@ private:
     BatchArgs args() const;
     BinBuffers bin_buffers() const;
     hipError_t launch_substep(hipStream_t st, hipEvent_t *ev);
+    // One path-0 substep that also records its binned state, tile chunks and
+    // raw grid at `rec` (layout private to the adjoint).
+    hipError_t substep_taped(hipStream_t st, char *rec, const void *layout);
     void drop_graphs();
     void after_substep(hipStream_t st);
 
