@@ -134,6 +134,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_grid_lin(const int4 *__restrict__
 }
 
 // ---- per-particle adjoint bodies ----------------------------------------------
+// The adjoint is compared with finite differences, not bit for bit, so its
+// own arithmetic may contract multiply-adds (the forward recomputation and
+// the weights above keep the forward's rounding).
+#pragma clang fp contract(fast)
+
 // Written once over an accessor A that is either the chunk's LDS tile window
 // (every node inside the grid) or the global grid (wild particles: nodes
 // outside [0, G) are skipped, like the forward).
@@ -163,13 +168,23 @@ struct AdjArgs {
     int xcd_remap;
 };
 
+// G2P adjoint of one particle, in two phases: (a) the forward recomputation
+// and the particle-side adjoints, (b) the reverse gather (scatter onto the
+// grid).  Between them the chunk kernel bounds the scatter values to pick a
+// fixed-point scale.
+struct G2PAdj {
+    StencilD s;
+    float gv[3], gxp[3], B_[3][3];
+    uint32_t o;
+};
+
 template <class A>
-__device__ __forceinline__ void adj_g2p_particle(const AdjArgs &q, uint32_t i, const A &acc) {
-    const uint32_t o = q.id[i];
+__device__ __forceinline__ void adj_g2p_a(const AdjArgs &q, uint32_t i, const A &acc, G2PAdj &p) {
+    p.o = q.id[i];
     float x[3], vold[3], C[3][3];
     load_state(q.st, q.stride, i, x, vold, C);
-    StencilD s;
-    stencil_d(x, s);
+    stencil_d(x, p.s);
+    const StencilD &s = p.s;
     float v[3] = {0.0f, 0.0f, 0.0f};  // forward G2P recomputation
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
@@ -185,29 +200,43 @@ __device__ __forceinline__ void adj_g2p_particle(const AdjArgs &q, uint32_t i, c
                 v[2] = v[2] + vg.z * w;
             }
     const float dt = q.prm.dt, k = 3.0f, ws = 0.3f;
-    float gv[3], gxp[3], B_[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) B_[c][r] = 4.0f * q.adj_in[(AC + 3 * c + r) * q.astride + o];
+        for (int r = 0; r < 3; ++r)
+            p.B_[c][r] = 4.0f * q.adj_in[(AC + 3 * c + r) * q.astride + p.o];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float rb = q.uni.real_box[a];
         const float x1 = x[a] + v[a] * dt;
         const float xc = fminf(fmaxf(x1, 1.0f), rb - 2.0f);
         const float xn = xc + (v[a] * dt) * k;
-        const float gvo = q.adj_in[(AV + a) * q.astride + o];
-        const float gxo = q.adj_in[(AX + a) * q.astride + o];
+        const float gvo = q.adj_in[(AV + a) * q.astride + p.o];
+        const float gxo = q.adj_in[(AX + a) * q.astride + p.o];
         float act = 0.0f;
         if (xn < 3.0f) act += 1.0f;
         if (xn > rb - 4.0f) act += 1.0f;
         const float gxn = -ws * gvo * act;
         const float gxc = gxo + gxn;
         const float gx1 = (x1 > 1.0f && x1 < rb - 2.0f) ? gxc : 0.0f;
-        gv[a] = gvo + gxn * dt * k + gx1 * dt;
-        gxp[a] = gx1;
+        p.gv[a] = gvo + gxn * dt * k + gx1 * dt;
+        p.gxp[a] = gx1;
     }
-    // reverse gather: scatter to the grid and accumulate the weight terms
+}
+
+// Bound on |w g_r| over the particle's nodes: w <= 1, |dist| <= 1.5.
+__device__ __forceinline__ float adj_g2p_bound(const G2PAdj &p) {
+    float b = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        b = fmaxf(b, fabsf(p.gv[r]) +
+                         1.5f * (fabsf(p.B_[0][r]) + fabsf(p.B_[1][r]) + fabsf(p.B_[2][r])));
+    return b;
+}
+
+template <class A>
+__device__ __forceinline__ void adj_g2p_b(const AdjArgs &q, const A &acc, G2PAdj &p) {
+    const StencilD &s = p.s;
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
@@ -226,8 +255,8 @@ __device__ __forceinline__ void adj_g2p_particle(const AdjArgs &q, uint32_t i, c
                 for (int r = 0; r < 3; ++r) {
                     float bd = 0.0f;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) bd += B_[c][r] * d[c];
-                    const float g = gv[r] + bd;  // adjoint of v_i[r] per unit weight
+                    for (int c = 0; c < 3; ++c) bd += p.B_[c][r] * d[c];
+                    const float g = p.gv[r] + bd;  // adjoint of v_i[r] per unit weight
                     add[r] = w * g;
                     coef += vg[r] * g;
                 }
@@ -236,23 +265,31 @@ __device__ __forceinline__ void adj_g2p_particle(const AdjArgs &q, uint32_t i, c
                 for (int a = 0; a < 3; ++a) {
                     float bv = 0.0f;  // d dist[a] / dx[a] = -1
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) bv += B_[a][r] * vg[r];
-                    gxp[a] += dw[a] * coef - w * bv;
+                    for (int r = 0; r < 3; ++r) bv += p.B_[a][r] * vg[r];
+                    p.gxp[a] += dw[a] * coef - w * bv;
                 }
             }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) q.adj_out[(AX + a) * q.astride + o] = gxp[a];
+    for (int a = 0; a < 3; ++a) q.adj_out[(AX + a) * q.astride + p.o] = p.gxp[a];
 }
 
+// P2G-2 adjoint of one particle: (a) everything up to the density adjoint
+// grho (stores the C adjoint), (b) the density scatter onto the masses.
+struct P2G2Adj {
+    StencilD s;
+    float grho, gxp[3];
+    uint32_t o;
+};
+
 template <class A>
-__device__ __forceinline__ void adj_p2g2_particle(const AdjArgs &q, uint32_t i, const A &acc,
-                                                  float &gk, float &gmu) {
-    const uint32_t o = q.id[i];
+__device__ __forceinline__ void adj_p2g2_a(const AdjArgs &q, uint32_t i, const A &acc, P2G2Adj &p,
+                                           float &gk, float &gmu) {
+    p.o = q.id[i];
     const Params prm = q.prm;
     float x[3], v[3], C[3][3];
     load_state(q.st, q.stride, i, x, v, C);
-    StencilD s;
-    stencil_d(x, s);
+    stencil_d(x, p.s);
+    const StencilD &s = p.s;
     float rho = 0.0f;
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
@@ -279,7 +316,7 @@ __device__ __forceinline__ void adj_p2g2_particle(const AdjArgs &q, uint32_t i, 
         }
     // gather: M adjoint and the weight / dist terms
     float gM[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-    float gxp[3] = {0.0f, 0.0f, 0.0f};
+    p.gxp[0] = p.gxp[1] = p.gxp[2] = 0.0f;
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
@@ -305,7 +342,7 @@ __device__ __forceinline__ void adj_p2g2_particle(const AdjArgs &q, uint32_t i, 
                     float mg = 0.0f;
 #pragma unroll
                     for (int rr = 0; rr < 3; ++rr) mg += M[a][rr] * g[rr];
-                    gxp[a] += dw[a] * coef - w * mg;
+                    p.gxp[a] += dw[a] * coef - w * mg;
                 }
             }
     // M = -4 dt vol S
@@ -317,21 +354,25 @@ __device__ __forceinline__ void adj_p2g2_particle(const AdjArgs &q, uint32_t i, 
             gS[c][rr] = -4.0f * vol * prm.dt * gM[c][rr];
             gvol += -4.0f * prm.dt * S[c][rr] * gM[c][rr];
         }
-    float gC[3][3];
     const float gP = -(gS[0][0] + gS[1][1] + gS[2][2]);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int rr = 0; rr < 3; ++rr) {
-            gC[c][rr] = prm.viscosity * (gS[c][rr] + gS[rr][c]);
+            q.adj_out[(AC + 3 * c + rr) * q.astride + p.o] = prm.viscosity * (gS[c][rr] + gS[rr][c]);
             gmu += gS[c][rr] * (C[c][rr] + C[rr][c]);
         }
-    float grho = -gvol / (rho * rho);
+    p.grho = -gvol / (rho * rho);
     if (pos) {
-        grho += gP * prm.stiffness * 5.0f * r4 / prm.rest_density;
+        p.grho += gP * prm.stiffness * 5.0f * r4 / prm.rest_density;
         gk += gP * (r4 * r - 1.0f);
     }
-    // density = sum m_i w: scatter to the masses, weight terms to x
+}
+
+// density = sum m_i w: scatter to the masses, weight terms to x
+template <class A>
+__device__ __forceinline__ void adj_p2g2_b(const AdjArgs &q, const A &acc, P2G2Adj &p) {
+    const StencilD &s = p.s;
 #pragma unroll
     for (int gx = 0; gx < 3; ++gx)
 #pragma unroll
@@ -341,17 +382,13 @@ __device__ __forceinline__ void adj_p2g2_particle(const AdjArgs &q, uint32_t i, 
                 if (!acc.ok(s, gx, gy, gz)) continue;
                 float w, dw[3];
                 weight_grad(s, gx, gy, gz, w, dw);
-                acc.add_gm(s, gx, gy, gz, grho * w);
+                acc.add_gm(s, gx, gy, gz, p.grho * w);
                 const float m = acc.m(s, gx, gy, gz);
 #pragma unroll
-                for (int a = 0; a < 3; ++a) gxp[a] += grho * m * dw[a];
+                for (int a = 0; a < 3; ++a) p.gxp[a] += p.grho * m * dw[a];
             }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) q.adj_out[(AX + a) * q.astride + o] += gxp[a];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int rr = 0; rr < 3; ++rr) q.adj_out[(AC + 3 * c + rr) * q.astride + o] = gC[c][rr];
+    for (int a = 0; a < 3; ++a) q.adj_out[(AX + a) * q.astride + p.o] += p.gxp[a];
 }
 
 template <class A>
@@ -446,11 +483,14 @@ struct GlobalAcc {  // wild particles: the whole grid, float atomics
 };
 
 // The chunk's 6^3 tile window in LDS: w4 holds per-cell inputs (vg or gmv in
-// xyz, mass or gm in w), acc per-cell float accumulators (x, y, z).
+// xyz, mass or gm in w), acc per-cell accumulators (x, y, z) in i32 fixed
+// point at `scale` (LDS float atomics are far slower than integer ones).
 struct WinAcc {
     const float4 *w4;
-    float *acc;  // 3 x kWinCells
+    int *acc;  // 3 x kWinCells
     int l0;
+    float scale;
+    __device__ __forceinline__ int fx(float a) const { return (int)rintf(a * scale); }
     __device__ __forceinline__ int l(int gx, int gy, int gz) const {
         return l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
     }
@@ -471,15 +511,35 @@ struct WinAcc {
     __device__ __forceinline__ void add_gvel(const StencilD &, int gx, int gy, int gz,
                                              const float a[3]) const {
         const int c = l(gx, gy, gz);
-        atomicAdd(&acc[c], a[0]);
-        atomicAdd(&acc[kWinCells + c], a[1]);
-        atomicAdd(&acc[2 * kWinCells + c], a[2]);
+        atomicAdd(&acc[c], fx(a[0]));
+        atomicAdd(&acc[kWinCells + c], fx(a[1]));
+        atomicAdd(&acc[2 * kWinCells + c], fx(a[2]));
     }
     __device__ __forceinline__ void add_gm(const StencilD &, int gx, int gy, int gz,
                                            float a) const {
-        atomicAdd(&acc[l(gx, gy, gz)], a);
+        atomicAdd(&acc[l(gx, gy, gz)], fx(a));
     }
 };
+
+// Fixed-point scale for one round of at most kAdjBlock particles whose
+// scatter values are bounded by `bound`: each i32 cell sum stays below 2^30
+// and the rounding error of a value is at most 2^-23 of the bound.
+__device__ __forceinline__ float round_scale(float bound) {
+    if (!(bound > 0.0f) || !isfinite(bound)) return 1.0f;
+    int e;
+    (void)frexpf(bound, &e);  // bound < 2^e
+    return ldexpf(1.0f, min(126, max(-126, 22 - e)));
+}
+
+// Block-wide maximum of non-negative values (their bit patterns order like
+// unsigned integers); *slot must be zero before the first call.
+__device__ __forceinline__ float block_max(float v, uint32_t *slot) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(slot, __float_as_uint(v));
+    __syncthreads();
+    return __uint_as_float(*slot);
+}
 
 __device__ __forceinline__ int win_l0_of(const float *st, uint32_t stride, uint32_t i) {
     const int m = kTileEdge - 1;
@@ -531,16 +591,24 @@ __device__ __forceinline__ AdjWalk adj_walk(const AdjArgs &q) {
 
 // ---- chunk kernels ---------------------------------------------------------------
 
-// G2P adjoint: window of vg in, window of grid-velocity adjoints out (partial).
+// G2P adjoint: window of vg in, window of grid-velocity adjoints out
+// (partial).  Rounds of up to 256 particles, one per thread; each round
+// accumulates in i32 fixed point at its own scale and is folded into the
+// float window by the cell owners.
 __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
     __shared__ float4 s_w4[kWinCells];
-    __shared__ float s_acc[3 * kWinCells];
+    __shared__ int s_acc[3 * kWinCells];
+    __shared__ float s_sum[3 * kWinCells];
+    __shared__ uint32_t s_max;
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
         if (ch.y == kWildTile) {
             const GlobalAcc acc(q);
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
-                adj_g2p_particle(q, ch.z + j, acc);
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+                G2PAdj p;
+                adj_g2p_a(q, ch.z + j, acc, p);
+                adj_g2p_b(q, acc, p);
+            }
             continue;
         }
         int t[3];
@@ -553,34 +621,60 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
                 c = make_float4(L.vg[0], L.vg[1], L.vg[2], 0.0f);
             }
             s_w4[l] = c;
-            s_acc[l] = s_acc[kWinCells + l] = s_acc[2 * kWinCells + l] = 0.0f;
         }
-        __syncthreads();
-        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
-            const uint32_t i = ch.z + j;
-            const WinAcc acc{s_w4, s_acc, win_l0_of(q.st, q.stride, i)};
-            adj_g2p_particle(q, i, acc);
+        for (int l = threadIdx.x; l < 3 * kWinCells; l += kAdjBlock) {
+            s_acc[l] = 0;
+            s_sum[l] = 0.0f;
         }
+        if (threadIdx.x == 0) s_max = 0u;
         __syncthreads();
+        for (uint32_t r0 = 0; r0 < ch.w; r0 += kAdjBlock) {
+            // wave tasks of 64 lanes spread over the round: in cell-sorted
+            // storage the lanes of one LDS add rarely share a cell
+            const uint32_t cnt = min((uint32_t)kAdjBlock, ch.w - r0);
+            const uint32_t tasks = (cnt + 63) >> 6, task = threadIdx.x >> 6;
+            const uint32_t j = r0 + task + tasks * (threadIdx.x & 63);
+            const bool active = task < tasks && j < r0 + cnt;
+            const uint32_t i = ch.z + (active ? j : r0);
+            const WinAcc pre{s_w4, s_acc, win_l0_of(q.st, q.stride, i), 1.0f};
+            G2PAdj p;
+            if (active) adj_g2p_a(q, i, pre, p);
+            const float scale = round_scale(block_max(active ? adj_g2p_bound(p) : 0.0f, &s_max));
+            if (active) adj_g2p_b(q, WinAcc{s_w4, s_acc, pre.l0, scale}, p);
+            __syncthreads();
+            const float inv = 1.0f / scale;
+            for (int l = threadIdx.x; l < 3 * kWinCells; l += kAdjBlock) {
+                s_sum[l] += (float)s_acc[l] * inv;
+                s_acc[l] = 0;
+            }
+            if (threadIdx.x == 0) s_max = 0u;
+            __syncthreads();
+        }
         float4 *out = q.partial + (size_t)w.c * kWinCells;
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock)
-            out[l] = make_float4(s_acc[l], s_acc[kWinCells + l], s_acc[2 * kWinCells + l], 0.0f);
+            out[l] = make_float4(s_sum[l], s_sum[kWinCells + l], s_sum[2 * kWinCells + l], 0.0f);
         __syncthreads();
     }
 }
 
-// P2G-2 adjoint: window of (gmv, mass) in, window of mass adjoints out.
+// P2G-2 adjoint: window of (gmv, mass) in, window of mass adjoints out
+// (fixed-point rounds as in k_adj_g2p_t).
 __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
     __shared__ float4 s_w4[kWinCells];
-    __shared__ float s_acc[kWinCells];
+    __shared__ int s_acc[kWinCells];
+    __shared__ float s_sum[kWinCells];
+    __shared__ uint32_t s_max;
     __shared__ float s_red[2][kAdjBlock / 64];
     float gk = 0.0f, gmu = 0.0f;
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
         if (ch.y == kWildTile) {
             const GlobalAcc acc(q);
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
-                adj_p2g2_particle(q, ch.z + j, acc, gk, gmu);
+            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+                P2G2Adj p;
+                adj_p2g2_a(q, ch.z + j, acc, p, gk, gmu);
+                adj_p2g2_b(q, acc, p);
+            }
             continue;
         }
         int t[3];
@@ -591,18 +685,34 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
             if (gi >= 0)
                 c = make_float4(q.gmv[3 * gi], q.gmv[3 * gi + 1], q.gmv[3 * gi + 2], q.lin[gi].m);
             s_w4[l] = c;
-            s_acc[l] = 0.0f;
+            s_acc[l] = 0;
+            s_sum[l] = 0.0f;
         }
+        if (threadIdx.x == 0) s_max = 0u;
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
-            const uint32_t i = ch.z + j;
-            const WinAcc acc{s_w4, s_acc, win_l0_of(q.st, q.stride, i)};
-            adj_p2g2_particle(q, i, acc, gk, gmu);
+        for (uint32_t r0 = 0; r0 < ch.w; r0 += kAdjBlock) {
+            const uint32_t cnt = min((uint32_t)kAdjBlock, ch.w - r0);
+            const uint32_t tasks = (cnt + 63) >> 6, task = threadIdx.x >> 6;
+            const uint32_t j = r0 + task + tasks * (threadIdx.x & 63);
+            const bool active = task < tasks && j < r0 + cnt;
+            const uint32_t i = ch.z + (active ? j : r0);
+            const WinAcc pre{s_w4, s_acc, win_l0_of(q.st, q.stride, i), 1.0f};
+            P2G2Adj p;
+            if (active) adj_p2g2_a(q, i, pre, p, gk, gmu);
+            const float scale = round_scale(block_max(active ? fabsf(p.grho) : 0.0f, &s_max));
+            if (active) adj_p2g2_b(q, WinAcc{s_w4, s_acc, pre.l0, scale}, p);
+            __syncthreads();
+            const float inv = 1.0f / scale;
+            for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
+                s_sum[l] += (float)s_acc[l] * inv;
+                s_acc[l] = 0;
+            }
+            if (threadIdx.x == 0) s_max = 0u;
+            __syncthreads();
         }
-        __syncthreads();
         float4 *out = q.partial + (size_t)w.c * kWinCells;
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock)
-            out[l] = make_float4(s_acc[l], 0.0f, 0.0f, 0.0f);
+            out[l] = make_float4(s_sum[l], 0.0f, 0.0f, 0.0f);
         __syncthreads();
     }
     // block reduction of the scalar parameter adjoints
@@ -648,9 +758,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
             s_w4[l] = c;
         }
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+        // wave tasks of 64 lanes spread over the chunk: in cell-sorted storage
+        // the lanes of one LDS add rarely share a cell
+        const uint32_t tasks = (ch.w + 63) >> 6;
+        for (uint32_t t = threadIdx.x >> 6; t < tasks; t += kAdjBlock / 64) {
+            const uint32_t j = t + tasks * (threadIdx.x & 63);
+            if (j >= ch.w) continue;
             const uint32_t i = ch.z + j;
-            const WinAcc acc{s_w4, nullptr, win_l0_of(q.st, q.stride, i)};
+            const WinAcc acc{s_w4, nullptr, win_l0_of(q.st, q.stride, i), 1.0f};
             adj_p2g1_particle(q, i, acc);
         }
         __syncthreads();
@@ -837,21 +952,41 @@ struct TapeLayout {
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Several device-to-device copies in one launch (sizes in 4-byte words).
+struct CopySegs {
+    const uint32_t *src[5];
+    uint32_t *dst[5];
+    uint64_t words[5];
+    int count;
+};
+
+__global__ __launch_bounds__(256) void k_copy_segs(CopySegs c) {
+    for (int s = 0; s < c.count; ++s)
+        for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < c.words[s];
+             i += (uint64_t)gridDim.x * 256)
+            c.dst[s][i] = c.src[s][i];
+}
+
 hipError_t Batch::substep_taped(hipStream_t st, char *rec, const void *layout) {
     const TapeLayout &L = *static_cast<const TapeLayout *>(layout);
     BatchArgs a = args();
     TRY(launch_bin(a, bin_buffers(), !keys_ready_, st));
     a.state = spare_state_;
     a.id = spare_id_;
-    TRY(hipMemcpyAsync(rec + L.state, spare_state_, sizeof(float) * NCOMP * stride_,
-                       hipMemcpyDeviceToDevice, st));
-    TRY(hipMemcpyAsync(rec + L.id, spare_id_, sizeof(uint32_t) * stride_, hipMemcpyDeviceToDevice,
-                       st));
-    TRY(hipMemcpyAsync(rec + L.tchunks, tchunks_, sizeof(TileChunk) * cap_tchunks_,
-                       hipMemcpyDeviceToDevice, st));
-    TRY(hipMemcpyAsync(rec + L.tile_chunk, tile_chunk_, sizeof(uint2) * total_keys_,
-                       hipMemcpyDeviceToDevice, st));
-    TRY(hipMemcpyAsync(rec + L.ntchunks, ntchunks_, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    CopySegs cs{};
+    auto seg = [&](size_t at, const void *src, size_t bytes) {
+        cs.src[cs.count] = static_cast<const uint32_t *>(src);
+        cs.dst[cs.count] = reinterpret_cast<uint32_t *>(rec + at);
+        cs.words[cs.count] = bytes / 4;
+        ++cs.count;
+    };
+    seg(L.state, spare_state_, sizeof(float) * NCOMP * stride_);
+    seg(L.id, spare_id_, sizeof(uint32_t) * stride_);
+    seg(L.tchunks, tchunks_, sizeof(TileChunk) * cap_tchunks_);
+    seg(L.tile_chunk, tile_chunk_, sizeof(uint2) * total_keys_);
+    seg(L.ntchunks, ntchunks_, sizeof(uint32_t));
+    k_copy_segs<<<1024, 256, 0, st>>>(cs);
+    TRY(hipGetLastError());
     TRY(launch_p2g1(a, st));
     TRY(launch_reduce1(a, st));
     TRY(launch_p2g2(a, st));
@@ -884,40 +1019,40 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         t.path = 0;
         TRY(set_tuning(t));
     }
-    // Device buffers: checkpoints (state + id) per segment start, the segment
-    // tape, grid linearisation, adjoints.
-    char *ckpt = nullptr, *tape = nullptr;
-    GridLin *lin = nullptr;
-    float *adjA = nullptr, *adjB = nullptr, *gvel = nullptr, *gmv = nullptr, *gm = nullptr,
-          *gtheta = nullptr;
-    double *lacc = nullptr;
-    auto cleanup = [&]() {
-        if (ckpt) (void)hipFree(ckpt);
-        if (tape) (void)hipFree(tape);
-        if (lin) (void)hipFree(lin);
-        if (adjA) (void)hipFree(adjA);
-        if (adjB) (void)hipFree(adjB);
-        if (gvel) (void)hipFree(gvel);
-        if (gmv) (void)hipFree(gmv);
-        if (gm) (void)hipFree(gm);
-        if (gtheta) (void)hipFree(gtheta);
-        if (lacc) (void)hipFree(lacc);
-    };
+    // Device workspace (kept between calls): checkpoints (state + id) per
+    // segment start, the segment tape, grid linearisation, adjoints.
     hipError_t e = hipSuccess;
     const size_t per_ck = sbytes + ibytes;
     const uint32_t astride = std::max<uint32_t>(n, 1);
     const size_t g1 = std::max<uint32_t>(G, 1);
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t at = off;
+        off = align256(off + bytes);
+        return at;
+    };
+    const size_t o_ckpt = carve(per_ck * (nseg + 1)), o_tape = carve(L.bytes * K),
+                 o_lin = carve(sizeof(GridLin) * g1), o_adjA = carve(sizeof(float) * ANCOMP * astride),
+                 o_adjB = carve(sizeof(float) * ANCOMP * astride), o_gvel = carve(sizeof(float) * 3 * g1),
+                 o_gmv = carve(sizeof(float) * 3 * g1), o_gm = carve(sizeof(float) * g1),
+                 o_gtheta = carve(sizeof(float) * 2), o_lacc = carve(sizeof(double) * 4);
+    if (off > adj_ws_cap_) {
+        if (adj_ws_) (void)hipFree(adj_ws_);
+        adj_ws_ = nullptr;
+        adj_ws_cap_ = 0;
+        TRY(hipMalloc(&adj_ws_, off));
+        adj_ws_cap_ = off;
+    }
+    char *const ckpt = adj_ws_ + o_ckpt, *const tape = adj_ws_ + o_tape;
+    GridLin *const lin = reinterpret_cast<GridLin *>(adj_ws_ + o_lin);
+    float *const adjA = reinterpret_cast<float *>(adj_ws_ + o_adjA);
+    float *const adjB = reinterpret_cast<float *>(adj_ws_ + o_adjB);
+    float *const gvel = reinterpret_cast<float *>(adj_ws_ + o_gvel);
+    float *const gmv = reinterpret_cast<float *>(adj_ws_ + o_gmv);
+    float *const gm = reinterpret_cast<float *>(adj_ws_ + o_gm);
+    float *const gtheta = reinterpret_cast<float *>(adj_ws_ + o_gtheta);
+    double *const lacc = reinterpret_cast<double *>(adj_ws_ + o_lacc);
     do {
-        if ((e = hipMalloc(&ckpt, per_ck * (nseg + 1))) != hipSuccess) break;
-        if ((e = hipMalloc(&tape, L.bytes * K)) != hipSuccess) break;
-        if ((e = hipMalloc(&lin, sizeof(GridLin) * g1)) != hipSuccess) break;
-        if ((e = hipMalloc(&adjA, sizeof(float) * ANCOMP * astride)) != hipSuccess) break;
-        if ((e = hipMalloc(&adjB, sizeof(float) * ANCOMP * astride)) != hipSuccess) break;
-        if ((e = hipMalloc(&gvel, sizeof(float) * 3 * g1)) != hipSuccess) break;
-        if ((e = hipMalloc(&gmv, sizeof(float) * 3 * g1)) != hipSuccess) break;
-        if ((e = hipMalloc(&gm, sizeof(float) * g1)) != hipSuccess) break;
-        if ((e = hipMalloc(&gtheta, sizeof(float) * 2)) != hipSuccess) break;
-        if ((e = hipMalloc(&lacc, sizeof(double) * 4)) != hipSuccess) break;
         if ((e = hipMemsetAsync(gvel, 0, sizeof(float) * 3 * g1, st)) != hipSuccess) break;
         if ((e = hipMemsetAsync(gtheta, 0, sizeof(float) * 2, st)) != hipSuccess) break;
         if ((e = hipMemsetAsync(lacc, 0, sizeof(double) * 4, st)) != hipSuccess) break;
@@ -1046,7 +1181,6 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         if (e == hipSuccess) e = launch_stage(5, stage_view(), st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     } while (false);
-    cleanup();
     if (saved_tuning.path != tun_.path) (void)set_tuning(saved_tuning);
     return e;
 }

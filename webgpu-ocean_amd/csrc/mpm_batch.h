@@ -139,6 +139,8 @@ private:
     uint2 *tile_chunk_ = nullptr;  // key tile -> (first tile chunk, count)
 
     Tuning tun_{};
+    char *adj_ws_ = nullptr;  // backward() workspace, kept between calls
+    size_t adj_ws_cap_ = 0;
 
     // True when key_/rank_/count_ hold the binning counts of state_ (left by
     // the last g2p); false after anything else changed the particles.

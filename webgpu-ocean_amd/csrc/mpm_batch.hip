@@ -571,6 +571,8 @@ void Batch::drop_graphs() {
 
 void Batch::release() {
     drop_graphs();
+    free_ptr(adj_ws_);
+    adj_ws_cap_ = 0;
     free_ptr(state_);
     free_ptr(spare_state_);
     free_ptr(id_);
