@@ -57,21 +57,121 @@ def stage_bytes(name: str, n: int, g: int, binned_path: bool = True) -> int:
     return (binned if binned_path else table)[name]
 
 
-def calibration_copy(nbytes: int = 64 << 20) -> None:
+CALIBRATION_BYTES = 64 << 20
+
+
+def calibration_copy(nbytes: int = CALIBRATION_BYTES) -> None:
     """One device-to-device copy of a known size through the HIP runtime, so a
-    profiler's FETCH_SIZE / WRITE_SIZE for it calibrate the counters
-    (the copy reads and writes exactly nbytes)."""
+    profiler's FETCH_SIZE / WRITE_SIZE for it calibrate the counters (the copy
+    reads and writes exactly nbytes).  A 512 MiB fill between writing the
+    source and copying it evicts the source from the L2 caches."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    a, b, ev = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    evict = 512 << 20
     if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
-            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(ev), ctypes.c_size_t(evict)):
         raise RuntimeError("hipMalloc failed")
     hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemset(ev, 2, ctypes.c_size_t(evict))
+    hip.hipDeviceSynchronize()
     hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
     hip.hipDeviceSynchronize()
     hip.hipFree(a)
     hip.hipFree(b)
+    hip.hipFree(ev)
+
+
+def _short_kernel(name: str) -> str:
+    n = name.replace("mpm::(anonymous namespace)::", "").replace("mpm::", "")
+    return n.split("(")[0].replace("void ", "").strip()
+
+
+def summarize_profiles(trace_dir: str, fetch_dir: str = "", write_dir: str = "") -> str:
+    """Table of a rocprofv3 --kernel-trace --stats run (csv output), with the
+    per-dispatch FETCH_SIZE / WRITE_SIZE of two --pmc runs of the same
+    command with --calibrate, scaled by what the counters read for the
+    calibration copy (its true size is CALIBRATION_BYTES each way)."""
+    import csv
+    import glob
+
+    def one(pattern):
+        hits = glob.glob(os.path.join(pattern, "**", "*"), recursive=True)
+        return hits
+
+    stats = [f for f in one(trace_dir) if f.endswith("kernel_stats.csv")]
+    if not stats:
+        raise SystemExit(f"no kernel_stats.csv under {trace_dir}")
+    rows = list(csv.DictReader(open(stats[0])))
+
+    def counters(d, cname):
+        per, calib = {}, None
+        files = [f for f in one(d) if f.endswith("counter_collection.csv")] if d else []
+        if not files:
+            return per, calib
+        acc = {}
+        for r in csv.DictReader(open(files[0])):
+            if r["Counter_Name"] != cname:
+                continue
+            k = _short_kernel(r["Kernel_Name"])
+            acc.setdefault(k, []).append(float(r["Counter_Value"]))
+        for k, v in acc.items():
+            per[k] = sum(v) / len(v)
+        copies = [max(v) for k, v in acc.items() if "copyBuffer" in k]
+        if copies:
+            calib = max(copies) * 1024.0 / CALIBRATION_BYTES
+        return per, calib
+
+    fetch, fcal = counters(fetch_dir, "FETCH_SIZE")
+    write, wcal = counters(write_dir, "WRITE_SIZE")
+    out = []
+    out.append(f"calibration: 64 MiB copy read FETCH_SIZE x{fcal:.3f} and WRITE_SIZE x{wcal:.3f} "
+               "of its true size" if fcal and wcal else "calibration: none")
+    out.append(f"{'kernel':28s} {'calls':>6s} {'avg_us':>8s} {'pct':>6s} {'FETCH_KiB':>10s} "
+               f"{'WRITE_KiB':>10s} {'HBM_bytes':>12s}")
+    for r in rows:
+        k = _short_kernel(r["Name"])
+        if k.startswith("__amd"):
+            continue
+        f, w = fetch.get(k), write.get(k)
+        hbm = ""
+        if f is not None and w is not None and fcal and wcal:
+            hbm = f"{(f / fcal + w / wcal) * 1024:.4g}"
+        out.append(f"{k[:28]:28s} {r['Calls']:>6s} {float(r['AverageNs']) / 1e3:8.2f} "
+                   f"{float(r['Percentage']):6.2f} {'' if f is None else f'{f:.0f}':>10s} "
+                   f"{'' if w is None else f'{w:.0f}':>10s} {hbm:>12s}")
+        if hbm:
+            out.append(f"TRAFFIC {k} {hbm}")
+    return "\n".join(out)
+
+
+DEFAULT_TUNING = {"path": 0, "particles_per_thread": 1, "grid_blocks": 0, "xcd_remap": 1}
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "substep_kernels_dambreak_300k.txt")
+
+
+# kernel behind each production time_stages slot (particles-per-thread 1)
+SLOT_KERNEL = {"p2g_1": "k_p2g1<1>", "reduce_1": "k_reduce<1>", "p2g_2": "k_p2g2<1>",
+               "reduce_2": "k_reduce<2>", "g2p": "k_g2p<1>"}
+
+
+def measured_traffic(slot: str):
+    """HBM bytes per launch of the kernel behind `slot`, from the committed
+    profile summary of the default 300k dam-break run, or None."""
+    kernel = SLOT_KERNEL.get(slot)
+    if kernel is None:
+        return None
+    try:
+        for line in open(TRAFFIC_FILE):
+            t = line.split()
+            if len(t) == 3 and t[0] == "TRAFFIC" and t[1] == kernel:
+                return float(t[2])
+    except OSError:
+        return None
+    return None
+    return None
 
 
 def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
@@ -186,9 +286,16 @@ def main() -> int:
     ap.add_argument("--stage-substeps", type=int, default=200)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 64 MiB device copy (counter calibration under a profiler)")
+    ap.add_argument("--summarize", nargs="+", metavar="DIR",
+                    help="print the table of a rocprofv3 kernel-trace run DIR "
+                         "[FETCH_SIZE-run DIR, WRITE_SIZE-run DIR] and exit")
     ap.add_argument("--tuning", type=str, default="",
                     help='JSON launch configuration, e.g. {"particles_per_thread": 1}')
     args = ap.parse_args()
+    if args.summarize:
+        d = list(args.summarize) + ["", ""]
+        print(summarize_profiles(d[0], d[1], d[2]))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
@@ -268,7 +375,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom) if tuning == DEFAULT_TUNING else None,
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_us": per_launch_us[dom],
         },
