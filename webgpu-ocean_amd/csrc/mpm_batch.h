@@ -110,6 +110,9 @@ private:
     uint32_t *d_nbr_ = nullptr;        // 27 neighbour key tiles per spatial tile
     uint4 *d_tile_info_ = nullptr;     // (x, y, z, scene) per spatial tile
     uint8_t *d_dirty_ = nullptr;       // per spatial tile: cells may be non-zero
+    uint8_t *d_near_ = nullptr;        // per spatial tile, two substep parities
+    uint32_t *d_key_scene_ = nullptr;  // per key
+    size_t key_scene_cap_ = 0;
     size_t tile_rank_cap_ = 0, tile_xmaj_cap_ = 0, tile_tab_cap_ = 0;
     size_t chunk_cap_ = 0;
 

@@ -172,12 +172,29 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
         free_ptr(d_nbr_);
         free_ptr(d_tile_info_);
         free_ptr(d_dirty_);
+        free_ptr(d_near_);
         TRY(hipMalloc(&d_nbr_, sizeof(uint32_t) * nbr.size()));
         TRY(hipMalloc(&d_tile_info_, sizeof(uint4) * info.size()));
         TRY(hipMalloc(&d_dirty_, info.size()));
+        TRY(hipMalloc(&d_near_, 2 * info.size()));
         tile_tab_cap_ = info.size();
     }
-    if (!info.empty()) TRY(hipMemset(d_dirty_, 1, info.size()));
+    if (!info.empty()) {
+        TRY(hipMemset(d_dirty_, 1, info.size()));
+        TRY(hipMemset(d_near_, 0, 2 * tile_tab_cap_));
+    }
+    // per key: its scene (keys of scene s are [koff_s, koff_s + nkeys_s))
+    std::vector<uint32_t> key_scene(xmaj.size());
+    for (uint32_t s = 0; s < count; ++s)
+        for (uint32_t k = 0; k < h_scenes_[s].nkeys; ++k) key_scene[h_scenes_[s].koff + k] = s;
+    if (key_scene.size() > key_scene_cap_) {
+        free_ptr(d_key_scene_);
+        TRY(hipMalloc(&d_key_scene_, sizeof(uint32_t) * key_scene.size()));
+        key_scene_cap_ = key_scene.size();
+    }
+    TRY(hipMemcpy(d_key_scene_, key_scene.data(), sizeof(uint32_t) * key_scene.size(),
+                  hipMemcpyHostToDevice));
+
     if (!info.empty()) {
         TRY(hipMemcpy(d_nbr_, nbr.data(), sizeof(uint32_t) * nbr.size(), hipMemcpyHostToDevice));
         TRY(hipMemcpy(d_tile_info_, info.data(), sizeof(uint4) * info.size(),
@@ -268,6 +285,12 @@ BatchArgs Batch::args() const {
     a.nbr = d_nbr_;
     a.tile_info = d_tile_info_;
     a.dirty = d_dirty_;
+    {
+        const size_t par = state_ == state_buf0_ ? 0 : 1;  // substep parity
+        a.near = d_near_ + par * tile_tab_cap_;
+        a.near_prev = d_near_ + (1 - par) * tile_tab_cap_;
+    }
+    a.key_scene = d_key_scene_;
     a.state = state_;
     a.id = id_;
     a.stride = stride_;
@@ -601,6 +624,9 @@ void Batch::release() {
     free_ptr(d_nbr_);
     free_ptr(d_tile_info_);
     free_ptr(d_dirty_);
+    free_ptr(d_near_);
+    free_ptr(d_key_scene_);
+    key_scene_cap_ = 0;
     tile_rank_cap_ = tile_xmaj_cap_ = tile_tab_cap_ = 0;
     cap_n_ = 0;
     cap_cells_ = cap_keys_ = cap_tchunks_ = 0;

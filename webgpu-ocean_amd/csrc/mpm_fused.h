@@ -73,6 +73,10 @@ struct BatchArgs {
     const uint32_t *nbr;         // 27 per spatial tile: key of each neighbour or kNoTile
     const uint4 *tile_info;      // per spatial tile: (tile x, y, z, scene)
     uint8_t *dirty;              // per spatial tile: cells or velocities may be non-zero
+    uint8_t *near;               // per spatial tile: particles within reach (this binning),
+                                 // or a scene with wild particles / a non-integral box
+    uint8_t *near_prev;          // the same for the previous binning
+    const uint32_t *key_scene;   // per key: its scene
     float *state;                // SoA particle state, NCOMP x stride, bin order
     uint32_t *id;                // slot -> global caller index (poff + local index)
     uint32_t stride;
@@ -95,7 +99,7 @@ struct BatchArgs {
 };
 
 struct BinBuffers {
-    uint32_t *count, *offset;  // per key; the scan clears count
+    uint32_t *count, *offset;  // per key; reduce_2 clears count
     uint2 *seg, *sup;          // totals per 64 keys / per 4096 keys, this substep
     uint2 *seg_next, *sup_next;  // the next substep's (the scan clears sup_next)
     uint32_t nseg, nsup;

@@ -133,6 +133,11 @@ __device__ __forceinline__ uint32_t find_scene(const SceneDesc *sc, uint32_t nsc
     return lo;
 }
 
+// Scene of bin key k (no lookup for a batch of one).
+__device__ __forceinline__ uint32_t key_scene_of(const BatchArgs &a, uint32_t k) {
+    return a.nscenes == 1 ? 0u : a.key_scene[k];
+}
+
 // Persistent walk over the device-side tile-chunk list.  With xcd_remap the
 // blocks of one XCD (block id mod 8) take a contiguous range of chunks, so
 // neighbouring tiles' windows meet in the same L2.
@@ -417,15 +422,43 @@ constexpr uint32_t kSegKeys = 1u << kSegBits;
 constexpr uint32_t kSupBits = 12;
 constexpr uint32_t kSegPerSup = 1u << (kSupBits - kSegBits);
 
-// One wave per 64 keys: key offsets, the keys' tile-chunk ranges and the
-// tile chunks themselves; clears the counts for the next pass.
-__global__ __launch_bounds__(kWave) void k_scan_keys(BatchArgs a, BinBuffers b, uint32_t cap) {
-    const int lane = threadIdx.x;
+// One block of 4 waves per 64 keys.  All waves first check whether a
+// particle is within reach of each key's tile (its own, or a quarter of its
+// 27 neighbours' counts each); then the first wave computes the key offsets,
+// the keys' tile-chunk ranges and the tile chunks themselves, and clears the
+// wild keys' counts for the next pass.
+constexpr int kScanWaves = 4;
+
+__global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, BinBuffers b,
+                                                                  uint32_t cap) {
+    __shared__ uint32_t s_any[kWave];
+    const int lane = threadIdx.x & (kWave - 1), part = threadIdx.x / kWave;
     const uint32_t k0 = blockIdx.x * kWave, k = k0 + (uint32_t)lane;
-    const uint32_t sp = k0 >> kSupBits, sg = k0 >> kSegBits;  // first segment of the wave
     const bool in = k < b.nkeys;
     const uint32_t c = in ? b.count[k] : 0u;
     const uint32_t xm = in ? a.tile_xmaj[k] : 0u;
+    if (part == 0) s_any[lane] = 0u;
+    __syncthreads();
+    if (in && xm != kWildTile && c == 0) {
+        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
+        if (sd->fast) {
+            const uint32_t *nb = a.nbr + (size_t)(sd->soff + (k - sd->koff)) * kDirs;
+            uint32_t nk[(kDirs + kScanWaves - 1) / kScanWaves];
+#pragma unroll
+            for (int j = 0; j < (int)((kDirs + kScanWaves - 1) / kScanWaves); ++j) {
+                const uint32_t d = (uint32_t)(part + kScanWaves * j);
+                nk[j] = d < kDirs ? nb[d] : kNoTile;
+            }
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < (int)((kDirs + kScanWaves - 1) / kScanWaves); ++j)
+                any = any || (nk[j] != kNoTile && b.count[nk[j]] != 0);
+            if (any) s_any[lane] = 1u;
+        }
+    }
+    __syncthreads();
+    if (part != 0) return;
+    const uint32_t sp = k0 >> kSupBits, sg = k0 >> kSegBits;  // first segment of the block
     uint32_t ps = 0, pc = 0;  // totals of all earlier keys
     for (uint32_t i0 = 0; i0 < sp; i0 += kWave) {
         const uint32_t i = i0 + (uint32_t)lane;
@@ -450,16 +483,22 @@ __global__ __launch_bounds__(kWave) void k_scan_keys(BatchArgs a, BinBuffers b, 
     const uint32_t cb = pc + wave_incl_scan(nch, lane) - nch;
     if (in) {
         b.offset[k] = off;
-        b.count[k] = 0;  // ready for the next counting pass
         b.tile_chunk[k] = make_uint2(cb, nch);
         if (nch) {
-            const uint32_t sc =
-                find_scene(a.scenes, a.nscenes, k, [](const SceneDesc &d) { return d.koff; });
+            const uint32_t sc = key_scene_of(a, k);
             for (uint32_t i = 0; i < nch; ++i)
                 if (cb + i < b.tchunk_cap)
                     b.tchunks[cb + i] = make_uint4(sc, xm, off + i * cap, min(cap, c - i * cap));
         }
         if (k == b.nkeys - 1) *b.ntchunks = cb + nch;
+    }
+    // Every tile key writes its reach flag; a scene with wild particles (or
+    // a non-integral box) needs every tile visited.  Counts stay readable
+    // until reduce_2 clears them.
+    if (in && xm != kWildTile) {
+        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
+        const bool wild = !sd->fast || b.count[sd->koff + sd->ntiles] != 0;
+        a.near[sd->soff + (k - sd->koff)] = (c != 0 || s_any[lane] != 0 || wild) ? 1 : 0;
     }
     if (lane == 0 && (k0 & ((1u << kSupBits) - 1)) == 0) b.sup_next[sp] = make_uint2(0u, 0u);
 }
@@ -507,12 +546,12 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, Bi
     const uint32_t k = sg * kSegKeys + (uint32_t)w;
     uint32_t total = 0;
     if (sg < nseg && k < bb.nkeys) {
-        const uint32_t scn =
-            find_scene(a.scenes, a.nscenes, k, [](const SceneDesc &d) { return d.koff; });
-        const SceneDesc *sd = a.scenes + scn;
+        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
         const uint32_t rank = k - sd->koff;
         total = bb.count[k];
-        if (!final_counts && sd->fast && rank < sd->ntiles) {
+        // a tile no particle came near last substep has no source chunk
+        const bool sources = rank < sd->ntiles && a.near_prev[sd->soff + rank] != 0;
+        if (!final_counts && sd->fast && rank < sd->ntiles && sources) {
             const uint32_t b = sd->soff + rank;  // spatial tile of key k
             uint32_t first = 0, cnt = 0, dir = 0;
             if (lane < (int)kDirs) {
@@ -692,7 +731,15 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
     const uint32_t b = reduce_tile(a);
     if (b >= a.stiles) return;
     const uint4 ti = a.tile_info[b];
+    const bool dirty = a.dirty[b] != 0;
+    // no particle near, nothing left to clear, no wild particle in the scene:
+    // the tile's cells are empty and already written as such
+    if (!a.near[b] && !dirty) return;  // (its count is zero too)
     const SceneDesc *sd = a.scenes + ti.w;
+    if (PHASE == 2 && threadIdx.x == 0) {  // ready for the next counting pass
+        a.count[sd->koff + (b - sd->soff)] = 0;
+        if (b == sd->soff) a.count[sd->koff + sd->ntiles] = 0;  // the scene's wild key
+    }
     const bool has_wild = a.tile_chunk[sd->toff + sd->ntiles].y != 0;
     const Params prm = sd->prm;
     const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
@@ -719,7 +766,6 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
               cz = (int)ti.z * kTileEdge + l[2];
     const bool cell_ok = in_dims(cx, cy, cz, sd);
     const uint64_t gi = (uint64_t)sd->goff + cell_linear(cx, cy, cz, sd->uni.Y, sd->uni.Z);
-    const bool dirty = a.dirty[b] != 0;
     const CellPlan plan = plan_partials(a, b, l, cell_ok);
     // No particle near the tile: its cells are empty.  If they were already
     // empty after the last substep (not dirty), nothing needs writing.
@@ -1079,7 +1125,7 @@ hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipSt
     }
     k_bin_gather<<<(nseg + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, 0, st>>>(a, b, cap,
                                                                              count ? 1 : 0);
-    k_scan_keys<<<blocks_for(b.nkeys, kWave), kWave, 0, st>>>(a, b, cap);
+    k_scan_keys<<<blocks_for(b.nkeys, kWave), kScanWaves * kWave, 0, st>>>(a, b, cap);
     if (a.nchunks) k_bin_scatter<<<nbin, kBlock, 0, st>>>(a, b);
     return hipGetLastError();
 }
