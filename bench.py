@@ -146,7 +146,7 @@ def summarize_profiles(trace_dir: str, fetch_dir: str = "", write_dir: str = "")
     return "\n".join(out)
 
 
-DEFAULT_TUNING = {"path": 0, "particles_per_thread": 1, "grid_blocks": 0, "xcd_remap": 1}
+DEFAULT_TUNING = {"path": 0, "particles_per_thread": 2, "grid_blocks": 0, "xcd_remap": 1}
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "substep_kernels_dambreak_300k.txt")

@@ -147,7 +147,7 @@ int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls
  *                            single simulators only)
  *   particles_per_thread 1, 2 or 4: a P2G/G2P block takes a chunk of at most
  *                        256 x this particles of one 4^3-cell tile
- *   grid_blocks          persistent blocks of the chunk kernels, 0 = 2048,
+ *   grid_blocks          persistent blocks of the chunk kernels, 0 = 4096,
  *                        at most 2^20
  *   xcd_remap            1 = each XCD walks a contiguous range of chunks
  * Results do not depend on these (integer adds commute; per-particle float

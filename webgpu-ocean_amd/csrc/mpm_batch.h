@@ -12,8 +12,8 @@ namespace mpm {
 
 struct Tuning {
     int path = 0;          // 0: binned tile-partial kernels, 1: reference-shaped stage kernels
-    int ppt = 1;           // particles per thread: tile chunks of <= 256 * ppt particles
-    int grid_blocks = 0;   // persistent blocks of the chunk kernels (0: 2048)
+    int ppt = 2;           // particles per thread: tile chunks of <= 256 * ppt particles
+    int grid_blocks = 0;   // persistent blocks of the chunk kernels (0: 4096)
     int xcd_remap = 1;     // contiguous chunk ranges per XCD
 };
 
@@ -134,6 +134,7 @@ private:
     uint32_t *key_ = nullptr, *rank_ = nullptr, *src_ = nullptr, *count_ = nullptr,
              *offset_ = nullptr;
     uint32_t *mig_ = nullptr;  // 27 per tile chunk
+    uint16_t *order_ = nullptr;  // per slot: P2G level order within its chunk
     uint2 *seg_ = nullptr, *sup_ = nullptr;  // scan totals, two parities
     size_t seg_cap_ = 0, sup_cap_ = 0;
     float *state_buf0_ = nullptr;  // parity 0 = this buffer is the current state

@@ -72,6 +72,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     free_ptr(rank_);
     free_ptr(src_);
     free_ptr(mig_);
+    free_ptr(order_);
     free_ptr(count_);
     free_ptr(offset_);
     free_ptr(seg_);
@@ -99,6 +100,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMalloc(&rank_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&src_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&mig_, sizeof(uint32_t) * 27 * tc));
+    TRY(hipMalloc(&order_, sizeof(uint16_t) * c));
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
     TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
     const size_t nseg = (k + 15) / 16, nsup = (k + 4095) / 4096;
@@ -301,6 +303,7 @@ BatchArgs Batch::args() const {
     a.rank = rank_;
     a.src = src_;
     a.mig = mig_;
+    a.order = order_;
     a.count = count_;
     a.count_next = 1;
     a.partial = partial_;
@@ -610,6 +613,7 @@ void Batch::release() {
     free_ptr(rank_);
     free_ptr(src_);
     free_ptr(mig_);
+    free_ptr(order_);
     free_ptr(count_);
     free_ptr(offset_);
     free_ptr(seg_);

@@ -38,24 +38,14 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kXcds = 8;
-constexpr unsigned kDefaultWalkBlocks = 2048;
+constexpr unsigned kDefaultWalkBlocks = 4096;
 
 // ---------------------------------------------------------------------------
 // helpers
 
 // A chunk of `count` particles is ceil(count / 64) wave tasks; task t of the
 // block's waves is k * 4 + wave for the thread's k-th particle.
-__device__ __forceinline__ uint32_t wave_tasks(uint32_t count) { return (count + 63) >> 6; }
-
 __device__ __forceinline__ uint32_t task_of(int k) { return (uint32_t)k * 4 + (threadIdx.x >> 6); }
-
-// P2G slot of lane l in task t: the lanes of a task are `tasks` slots apart,
-// so in cell-sorted storage one wave spans the whole chunk and its LDS adds
-// rarely hit the same cell in one instruction, while every task but the
-// last runs full.
-__device__ __forceinline__ uint32_t p2g_slot(uint32_t t, uint32_t tasks) {
-    return t + tasks * (threadIdx.x & 63);
-}
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 #pragma unroll
@@ -210,8 +200,9 @@ __device__ __forceinline__ void load_C(const float *__restrict__ st, uint32_t st
 struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x 216
     int *lds;
     int l0;
-    __device__ __forceinline__ void add4(const Node3 &, int gx, int gy, int gz, int32_t m,
-                                         int32_t x, int32_t y, int32_t z) const {
+    __device__ __forceinline__ void add4(const Node3 &, const AxisNode &, const AxisNode &, int gx,
+                                         int gy, int gz, int32_t m, int32_t x, int32_t y,
+                                         int32_t z) const {
         const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
         atomicAdd(&lds[l], m);
         atomicAdd(&lds[kWinCells + l], x);
@@ -224,9 +215,10 @@ struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
     int32_t *grid;
     uint32_t G;
     int32_t Y, Z;
-    __device__ __forceinline__ void add4(const Node3 &n, int gx, int gy, int gz, int32_t m,
-                                         int32_t x, int32_t y, int32_t z) const {
-        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+    __device__ __forceinline__ void add4(const Node3 &n, const AxisNode &ax, const AxisNode &ay,
+                                         int, int, int gz, int32_t m, int32_t x, int32_t y,
+                                         int32_t z) const {
+        const int32_t idx = node_index(n, ax, ay, gz, Y, Z);
         if (!in_grid(idx, G)) return;
         int32_t *cell = grid + 4 * (size_t)idx;
         atomicAdd(cell + 3, m);
@@ -239,11 +231,12 @@ struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
 struct WinSink2 {  // p2g_2: decoded masses [0, 216), momentum ints [216, 864)
     int *lds;
     int l0;
-    __device__ __forceinline__ float mass(const Node3 &, int gx, int gy, int gz) const {
+    __device__ __forceinline__ float mass(const Node3 &, const AxisNode &, const AxisNode &, int gx,
+                                          int gy, int gz) const {
         return reinterpret_cast<const float *>(lds)[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
     }
-    __device__ __forceinline__ void add3(const Node3 &, int gx, int gy, int gz, int32_t x,
-                                         int32_t y, int32_t z) const {
+    __device__ __forceinline__ void add3(const Node3 &, const AxisNode &, const AxisNode &, int gx,
+                                         int gy, int gz, int32_t x, int32_t y, int32_t z) const {
         const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
         atomicAdd(&lds[kWinCells + l], x);
         atomicAdd(&lds[2 * kWinCells + l], y);
@@ -257,13 +250,15 @@ struct GridSink2 {  // masses from the completed grid, momentum to the wild accu
     uint32_t G;
     int32_t Y, Z;
     float mult;
-    __device__ __forceinline__ float mass(const Node3 &n, int gx, int gy, int gz) const {
-        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+    __device__ __forceinline__ float mass(const Node3 &n, const AxisNode &ax, const AxisNode &ay,
+                                          int, int, int gz) const {
+        const int32_t idx = node_index(n, ax, ay, gz, Y, Z);
         return decode(in_grid(idx, G) ? grid[4 * (size_t)idx + 3] : 0, mult);
     }
-    __device__ __forceinline__ void add3(const Node3 &n, int gx, int gy, int gz, int32_t x,
-                                         int32_t y, int32_t z) const {
-        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+    __device__ __forceinline__ void add3(const Node3 &n, const AxisNode &ax, const AxisNode &ay,
+                                         int, int, int gz, int32_t x, int32_t y,
+                                         int32_t z) const {
+        const int32_t idx = node_index(n, ax, ay, gz, Y, Z);
         if (!in_grid(idx, G)) return;
         int32_t *cell = acc + 4 * (size_t)idx;
         atomicAdd(cell + 0, x);
@@ -275,7 +270,8 @@ struct GridSink2 {  // masses from the completed grid, momentum to the wild accu
 struct WinSinkG {  // g2p: updated, decoded velocities staged in LDS
     const float4 *lvel;
     int l0;
-    __device__ __forceinline__ void vel(const Node3 &, int gx, int gy, int gz, float cv[3]) const {
+    __device__ __forceinline__ void vel(const Node3 &, const AxisNode &, const AxisNode &, int gx,
+                                        int gy, int gz, float cv[3]) const {
         const float4 o = lvel[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
         cv[0] = o.x;
         cv[1] = o.y;
@@ -287,8 +283,9 @@ struct GridSinkG {  // g2p straight from the decoded velocities
     const float4 *cells;
     uint32_t G;
     int32_t Y, Z;
-    __device__ __forceinline__ void vel(const Node3 &n, int gx, int gy, int gz, float cv[3]) const {
-        const int32_t idx = node3_index(n, gx, gy, gz, Y, Z);
+    __device__ __forceinline__ void vel(const Node3 &n, const AxisNode &ax, const AxisNode &ay, int,
+                                        int, int gz, float cv[3]) const {
+        const int32_t idx = node_index(n, ax, ay, gz, Y, Z);
         float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (in_grid(idx, G)) o = cells[idx];
         cv[0] = o.x;
@@ -596,43 +593,99 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, Bi
 }
 
 // ---------------------------------------------------------------------------
+// Level order of a window chunk for the P2G adds.  Level k holds the k-th
+// particle of every cell of the tile that has more than k (cells in index
+// order), so the 64 lanes of a wave, which take 64 consecutive positions,
+// mostly hold 64 different cells: their ds_add_u32 to one stencil node then
+// rarely meet on one LDS address (same-address atomics of one instruction
+// serialise).  The position of the k-th particle of cell c is
+// base[k] + #{cells c' < c holding more than k}.  Integer adds commute, so
+// the order changes no result.
+
+template <int PPT>
+struct LevelOrder {
+    uint32_t cnt[kTileCells];      // particles per cell of the tile
+    uint32_t base[kBlock * PPT];   // first position of each level
+    uint64_t mask[kBlock * PPT];   // cells present in each level
+    uint16_t perm[kBlock * PPT];   // position -> slot within the chunk
+};
+
+// Fills o.perm for window chunk ch; every thread of the block calls it.
+template <int PPT>
+__device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk &ch,
+                                            LevelOrder<PPT> &o) {
+    const int lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x < kTileCells) o.cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t cell[PPT], lev[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
+        cell[k] = lev[k] = 0;
+        if (j < ch.w) {
+            const uint32_t i = ch.z + j;
+            const int m = kTileEdge - 1;
+            const int cx = (int)floorf(a.state[PX * a.stride + i]) & m;
+            const int cy = (int)floorf(a.state[(PX + 1) * a.stride + i]) & m;
+            const int cz = (int)floorf(a.state[(PX + 2) * a.stride + i]) & m;
+            cell[k] = (uint32_t)((cx * kTileEdge + cy) * kTileEdge + cz);
+            lev[k] = atomicAdd(&o.cnt[cell[k]], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // one wave: lane = cell
+        const uint32_t c = o.cnt[lane];
+        const uint32_t levels = wave_max(c);
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < levels; ++k) {
+            const uint64_t m = __ballot(c > k);
+            if (lane == 0) {
+                o.mask[k] = m;
+                o.base[k] = run;
+            }
+            run += (uint32_t)__popcll(m);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
+        if (j < ch.w) {
+            const uint64_t below = (1ull << cell[k]) - 1ull;
+            o.perm[o.base[lev[k]] + (uint32_t)__popcll(o.mask[lev[k]] & below)] = (uint16_t)j;
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
 // p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65) into tile partials
 
 template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_p2g1(BatchArgs a) {
     __shared__ int lds[4 * kWinCells];  // [mass | vx | vy | vz]
+    __shared__ LevelOrder<PPT> ord;
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
+        if (ch.y == kWildTile) continue;  // second loop
         const SceneDesc *sd = a.scenes + ch.x;
         const float mult = sd->prm.mult;
-        if (ch.y == kWildTile) {
-            const GridSink1 sink{(int32_t *)(a.wild + sd->goff), sd->G, sd->uni.Y, sd->uni.Z};
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
-                float x[3], v[3], C[3][3];
-                load_x(a.state, a.stride, ch.z + j, x);
-                load_v(a.state, a.stride, ch.z + j, v);
-                load_C(a.state, a.stride, ch.z + j, C);
-                Node3 n;
-                node3_init(x, n);
-                p2g1_particle(n, v, C, mult, sink);
-            }
-            continue;
-        }
+        for (int l = threadIdx.x; l < 4 * kWinCells; l += kBlock) lds[l] = 0;
+        level_order<PPT>(a, ch, ord);  // its barriers also publish the cleared window
         float x[PPT][3], v[PPT][3], C[PPT][3][3];
         bool valid[PPT];
-        const uint32_t tasks = wave_tasks(ch.w);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const uint32_t t = task_of(k), j = p2g_slot(t, tasks);
-            valid[k] = t < tasks && j < ch.w;
+            const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
+            valid[k] = pos < ch.w;
             if (valid[k]) {
+                const uint32_t j = ord.perm[pos];
+                a.order[ch.z + pos] = (uint16_t)j;  // p2g_2 takes the same order
                 load_x(a.state, a.stride, ch.z + j, x[k]);
                 load_v(a.state, a.stride, ch.z + j, v[k]);
                 load_C(a.state, a.stride, ch.z + j, C[k]);
             }
         }
-        for (int l = threadIdx.x; l < 4 * kWinCells; l += kBlock) lds[l] = 0;
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
@@ -646,6 +699,23 @@ __global__ __launch_bounds__(kBlock) void k_p2g1(BatchArgs a) {
             out[l] = make_int4(lds[kWinCells + l], lds[2 * kWinCells + l], lds[3 * kWinCells + l],
                                lds[l]);
         __syncthreads();
+    }
+    // Wild chunks (rare): global atomics into the wild accumulator.  A loop of
+    // their own keeps their registers out of the window loop's budget.
+    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = a.tchunks[w.c];
+        if (ch.y != kWildTile) continue;
+        const SceneDesc *sd = a.scenes + ch.x;
+        const GridSink1 sink{(int32_t *)(a.wild + sd->goff), sd->G, sd->uni.Y, sd->uni.Z};
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
+            float x[3], v[3], C[3][3];
+            load_x(a.state, a.stride, ch.z + j, x);
+            load_v(a.state, a.stride, ch.z + j, v);
+            load_C(a.state, a.stride, ch.z + j, C);
+            Node3 n;
+            node3_init(x, n);
+            p2g1_particle<true>(n, v, C, sd->prm.mult, sink);
+        }
     }
 }
 
@@ -805,30 +875,17 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
     float *dm = reinterpret_cast<float *>(lds);
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
+        if (ch.y == kWildTile) continue;  // second loop
         const SceneDesc *sd = a.scenes + ch.x;
         const Params prm = sd->prm;
-        if (ch.y == kWildTile) {
-            const GridSink2 sink{(const int32_t *)(a.grid + sd->goff),
-                                 (int32_t *)(a.wild + sd->goff), sd->G, sd->uni.Y, sd->uni.Z,
-                                 prm.mult};
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
-                float x[3], C[3][3];
-                load_x(a.state, a.stride, ch.z + j, x);
-                load_C(a.state, a.stride, ch.z + j, C);
-                Node3 n;
-                node3_init(x, n);
-                p2g2_particle(n, C, prm, sink);
-            }
-            continue;
-        }
         float x[PPT][3], C[PPT][3][3];
         bool valid[PPT];
-        const uint32_t tasks = wave_tasks(ch.w);
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
-            const uint32_t t = task_of(k), j = p2g_slot(t, tasks);
-            valid[k] = t < tasks && j < ch.w;
-            if (valid[k]) {
+            const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
+            valid[k] = pos < ch.w;
+            if (valid[k]) {  // the level order p2g_1 recorded
+                const uint32_t j = a.order[ch.z + pos];
                 load_x(a.state, a.stride, ch.z + j, x[k]);
                 load_C(a.state, a.stride, ch.z + j, C[k]);
             }
@@ -855,6 +912,23 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
         for (int l = threadIdx.x; l < kWinCells; l += kBlock)
             out[l] = make_int4(lds[kWinCells + l], lds[2 * kWinCells + l], lds[3 * kWinCells + l], 0);
         __syncthreads();
+    }
+    // Wild chunks (rare), in a loop of their own like k_p2g1's.
+    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = a.tchunks[w.c];
+        if (ch.y != kWildTile) continue;
+        const SceneDesc *sd = a.scenes + ch.x;
+        const Params prm = sd->prm;
+        const GridSink2 sink{(const int32_t *)(a.grid + sd->goff), (int32_t *)(a.wild + sd->goff),
+                             sd->G, sd->uni.Y, sd->uni.Z, prm.mult};
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
+            float x[3], C[3][3];
+            load_x(a.state, a.stride, ch.z + j, x);
+            load_C(a.state, a.stride, ch.z + j, C);
+            Node3 n;
+            node3_init(x, n);
+            p2g2_particle<true>(n, C, prm, sink);
+        }
     }
 }
 
@@ -885,7 +959,9 @@ __device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, ui
                                           uint32_t i, uint32_t key, uint32_t dir, bool active) {
     const bool local = active && dir <= kDirWild;
     const uint32_t lr = count_rank(lcnt, dir, local);
-    const uint32_t fr = count_rank(a.count, key, active && dir == kDirFar);
+    uint32_t fr = 0;
+    if (__ballot(active && dir == kDirFar))  // rare: a particle left its tile's neighbourhood
+        fr = count_rank(a.count, key, active && dir == kDirFar);
     if (active) {
         a.key[i] = key;
         a.rank[i] = local ? lr : fr;
@@ -913,34 +989,11 @@ __global__ __launch_bounds__(kBlock) void k_g2p(BatchArgs a) {
     __shared__ uint32_t lcnt[kDirWild + 2];  // per-direction counts, wild base
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
+        if (ch.y == kWildTile) continue;  // second loop
         const SceneDesc *sd = a.scenes + ch.x;
         const float dt = sd->prm.dt;
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
         if (threadIdx.x < kDirWild + 1) lcnt[threadIdx.x] = 0;
-        if (ch.y == kWildTile) {
-            // every move from a wild chunk is "far": no tile to count from
-            const GridSinkG sink{a.vel + sd->goff, sd->G, sd->uni.Y, sd->uni.Z};
-            const int nowhere[3] = {-4, -4, -4};
-            for (uint32_t j0 = 0; j0 < ch.w; j0 += kBlock) {  // uniform trip count
-                const uint32_t i = ch.z + j0 + threadIdx.x;
-                const bool active = j0 + threadIdx.x < ch.w;
-                uint32_t key = 0, dir = kDirFar;
-                if (active) {
-                    float x[3], v[3], B[3][3], xo[3], Cn[3][3];
-                    load_x(a.state, a.stride, i, x);
-                    Node3 n;
-                    node3_init(x, n);
-                    g2p_gather(n, sink, v, B);
-                    g2p_finish(x, rb, dt, v, B, xo, Cn);
-                    g2p_store(a, i, xo, v, Cn);
-                    key = bin_key(a, sd, xo, nowhere, &dir);
-                    dir = kDirFar;
-                }
-                if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, active);
-            }
-            __syncthreads();
-            continue;
-        }
         float x[PPT][3];
         bool valid[PPT];
         // contiguous slots per task: coalesced state stores
@@ -984,6 +1037,35 @@ __global__ __launch_bounds__(kBlock) void k_g2p(BatchArgs a) {
                 }
         }
         __syncthreads();
+    }
+    // Wild chunks (rare), in a loop of their own like k_p2g1's.  Every move
+    // from a wild chunk is "far" (no tile to count from): final ranks from
+    // global atomics, no LDS counts.
+    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = a.tchunks[w.c];
+        if (ch.y != kWildTile) continue;
+        const SceneDesc *sd = a.scenes + ch.x;
+        const float dt = sd->prm.dt;
+        const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
+        const GridSinkG sink{a.vel + sd->goff, sd->G, sd->uni.Y, sd->uni.Z};
+        const int nowhere[3] = {-4, -4, -4};
+        for (uint32_t j0 = 0; j0 < ch.w; j0 += kBlock) {  // uniform trip count
+            const uint32_t i = ch.z + j0 + threadIdx.x;
+            const bool active = j0 + threadIdx.x < ch.w;
+            uint32_t key = 0, dir = kDirFar;
+            if (active) {
+                float x[3], v[3], B[3][3], xo[3], Cn[3][3];
+                load_x(a.state, a.stride, i, x);
+                Node3 n;
+                node3_init(x, n);
+                g2p_gather<true>(n, sink, v, B);
+                g2p_finish(x, rb, dt, v, B, xo, Cn);
+                g2p_store(a, i, xo, v, Cn);
+                key = bin_key(a, sd, xo, nowhere, &dir);
+                dir = kDirFar;
+            }
+            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, active);
+        }
     }
 }
 

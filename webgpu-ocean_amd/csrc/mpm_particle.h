@@ -36,61 +36,92 @@ __device__ __forceinline__ void node3_init(const float x[3], Node3 &n) {
     }
 }
 
-__device__ __forceinline__ int32_t node3_index(const Node3 &n, int gx, int gy, int gz,
-                                               int32_t Y, int32_t Z) {
-    return (int32_t)((uint32_t)n.ic[0][gx] * (uint32_t)Y * (uint32_t)Z +
-                     (uint32_t)n.ic[1][gy] * (uint32_t)Z + (uint32_t)n.ic[2][gz]);
+// One stencil node along one axis: weight, cell_dist, integer cell coordinate.
+struct AxisNode {
+    float w, dist;
+    int32_t ic;
+};
+
+__device__ __forceinline__ AxisNode axis_node(const Node3 &n, int axis, int g) {
+    return AxisNode{n.w[g][axis], n.dist[axis][g], n.ic[axis][g]};
+}
+
+// Linear cell index of node (ax, ay, gz) (p2g_1.wgsl:53-56).
+__device__ __forceinline__ int32_t node_index(const Node3 &n, const AxisNode &ax,
+                                              const AxisNode &ay, int gz, int32_t Y, int32_t Z) {
+    return (int32_t)((uint32_t)ax.ic * (uint32_t)Y * (uint32_t)Z + (uint32_t)ay.ic * (uint32_t)Z +
+                     (uint32_t)n.ic[2][gz]);
+}
+
+// Visits the 9 (x, y) rows of the stencil, x-major like the WGSL loops,
+// calling row(ax, ay, gx, gy); each row then walks its 3 z nodes.  kRolled
+// keeps both loops rolled, the node values rotating through registers rather
+// than being indexed: far fewer live registers, for the rare global-memory
+// paths that share a kernel with the LDS-window path.  Both forms perform the
+// same operations in the same order.
+template <bool kRolled, class Row>
+__device__ __forceinline__ void for_rows(const Node3 &n, Row &&row) {
+    if constexpr (kRolled) {
+        AxisNode x0 = axis_node(n, 0, 0), x1 = axis_node(n, 0, 1), x2 = axis_node(n, 0, 2);
+#pragma unroll 1
+        for (int gx = 0; gx < 3; ++gx) {
+            AxisNode y0 = axis_node(n, 1, 0), y1 = axis_node(n, 1, 1), y2 = axis_node(n, 1, 2);
+#pragma unroll 1
+            for (int gy = 0; gy < 3; ++gy) {
+                row(x0, y0, gx, gy);
+                y0 = y1;
+                y1 = y2;
+            }
+            x0 = x1;
+            x1 = x2;
+        }
+    } else {
+#pragma unroll
+        for (int gx = 0; gx < 3; ++gx)
+#pragma unroll
+            for (int gy = 0; gy < 3; ++gy) row(axis_node(n, 0, gx), axis_node(n, 1, gy), gx, gy);
+    }
 }
 
 // ---- p2g_1 (mls-mpm/p2g_1.wgsl:36-63) ---------------------------------------
 
-template <class Sink>
+template <bool kRolled = false, class Sink>
 __device__ __forceinline__ void p2g1_particle(const Node3 &n, const float v[3],
                                               const float C[3][3], float mult, const Sink &sink) {
-    float P0[3][3], P1[3][3], P2[3][3];  // P_c[g][r] = C[c][r] * dist[c][g]
+    float P2[3][3];  // P2[gz][r] = C[2][r] * dist[2][gz]
 #pragma unroll
     for (int g = 0; g < 3; ++g)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            P0[g][r] = C[0][r] * n.dist[0][g];
-            P1[g][r] = C[1][r] * n.dist[1][g];
-            P2[g][r] = C[2][r] * n.dist[2][g];
+        for (int r = 0; r < 3; ++r) P2[g][r] = C[2][r] * n.dist[2][g];
+    for_rows<kRolled>(n, [&](const AxisNode &ax, const AxisNode &ay, int gx, int gy) {
+        const float wxy = ax.w * ay.w;
+        float PQ[3];  // C[0][r] * dist_x + C[1][r] * dist_y
+#pragma unroll
+        for (int r = 0; r < 3; ++r) PQ[r] = C[0][r] * ax.dist + C[1][r] * ay.dist;
+#pragma unroll
+        for (int gz = 0; gz < 3; ++gz) {
+            const float weight = wxy * n.w[gz][2];
+            float vel[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) vel[r] = weight * (v[r] + (PQ[r] + P2[gz][r]));
+            sink.add4(n, ax, ay, gx, gy, gz, encode(weight, mult), encode(vel[0], mult),
+                      encode(vel[1], mult), encode(vel[2], mult));
         }
-#pragma unroll
-    for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-        for (int gy = 0; gy < 3; ++gy) {
-            const float wxy = n.w[gx][0] * n.w[gy][1];
-            float PQ[3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) PQ[r] = P0[gx][r] + P1[gy][r];
-#pragma unroll
-            for (int gz = 0; gz < 3; ++gz) {
-                const float weight = wxy * n.w[gz][2];
-                float vel[3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) vel[r] = weight * (v[r] + (PQ[r] + P2[gz][r]));
-                sink.add4(n, gx, gy, gz, encode(weight, mult), encode(vel[0], mult),
-                          encode(vel[1], mult), encode(vel[2], mult));
-            }
-        }
+    });
 }
 
 // ---- p2g_2 (mls-mpm/p2g_2.wgsl:42-92) ---------------------------------------
 
-template <class Sink>
+template <bool kRolled = false, class Sink>
 __device__ __forceinline__ void p2g2_particle(const Node3 &n, const float C[3][3],
                                               const Params &prm, const Sink &sink) {
     float density = 0.0f;
+    for_rows<kRolled>(n, [&](const AxisNode &ax, const AxisNode &ay, int gx, int gy) {
+        const float wxy = ax.w * ay.w;
 #pragma unroll
-    for (int gx = 0; gx < 3; ++gx)
-#pragma unroll
-        for (int gy = 0; gy < 3; ++gy) {
-            const float wxy = n.w[gx][0] * n.w[gy][1];
-#pragma unroll
-            for (int gz = 0; gz < 3; ++gz)
-                density = density + sink.mass(n, gx, gy, gz) * (wxy * n.w[gz][2]);
-        }
+        for (int gz = 0; gz < 3; ++gz)
+            density = density + sink.mass(n, ax, ay, gx, gy, gz) * (wxy * n.w[gz][2]);
+    });
     const float volume = 1.0f / density;
     const float r = density / prm.rest_density;
     const float r2 = r * r;
@@ -107,29 +138,25 @@ __device__ __forceinline__ void p2g2_particle(const Node3 &n, const float C[3][3
             stress = stress + prm.viscosity * strain;
             M[c][rr] = ((-volume * 4.0f) * stress) * prm.dt;
         }
+    for_rows<kRolled>(n, [&](const AxisNode &ax, const AxisNode &ay, int gx, int gy) {
+        const float wxy = ax.w * ay.w;
 #pragma unroll
-    for (int gx = 0; gx < 3; ++gx)
+        for (int gz = 0; gz < 3; ++gz) {
+            const float weight = wxy * n.w[gz][2];
+            float mom[3];
 #pragma unroll
-        for (int gy = 0; gy < 3; ++gy) {
-            const float wxy = n.w[gx][0] * n.w[gy][1];
-#pragma unroll
-            for (int gz = 0; gz < 3; ++gz) {
-                const float weight = wxy * n.w[gz][2];
-                float mom[3];
-#pragma unroll
-                for (int rr = 0; rr < 3; ++rr)
-                    mom[rr] = ((M[0][rr] * weight) * n.dist[0][gx] +
-                               (M[1][rr] * weight) * n.dist[1][gy]) +
-                              (M[2][rr] * weight) * n.dist[2][gz];
-                sink.add3(n, gx, gy, gz, encode(mom[0], prm.mult), encode(mom[1], prm.mult),
-                          encode(mom[2], prm.mult));
-            }
+            for (int rr = 0; rr < 3; ++rr)
+                mom[rr] = ((M[0][rr] * weight) * ax.dist + (M[1][rr] * weight) * ay.dist) +
+                          (M[2][rr] * weight) * n.dist[2][gz];
+            sink.add3(n, ax, ay, gx, gy, gz, encode(mom[0], prm.mult), encode(mom[1], prm.mult),
+                      encode(mom[2], prm.mult));
         }
+    });
 }
 
 // ---- g2p gather (mls-mpm/g2p.wgsl:29-72) -------------------------------------
 
-template <class Sink>
+template <bool kRolled = false, class Sink>
 __device__ __forceinline__ void g2p_gather(const Node3 &n, const Sink &sink, float v[3],
                                            float B[3][3]) {
 #pragma unroll
@@ -138,26 +165,23 @@ __device__ __forceinline__ void g2p_gather(const Node3 &n, const Sink &sink, flo
 #pragma unroll
         for (int c = 0; c < 3; ++c) B[c][r] = 0.0f;
     }
+    for_rows<kRolled>(n, [&](const AxisNode &ax, const AxisNode &ay, int gx, int gy) {
+        const float wxy = ax.w * ay.w;
 #pragma unroll
-    for (int gx = 0; gx < 3; ++gx)
+        for (int gz = 0; gz < 3; ++gz) {
+            const float weight = wxy * n.w[gz][2];
+            float cv[3];
+            sink.vel(n, ax, ay, gx, gy, gz, cv);
+            const float wv[3] = {cv[0] * weight, cv[1] * weight, cv[2] * weight};
+            const float d[3] = {ax.dist, ay.dist, n.dist[2][gz]};
 #pragma unroll
-        for (int gy = 0; gy < 3; ++gy) {
-            const float wxy = n.w[gx][0] * n.w[gy][1];
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int gz = 0; gz < 3; ++gz) {
-                const float weight = wxy * n.w[gz][2];
-                float cv[3];
-                sink.vel(n, gx, gy, gz, cv);
-                const float wv[3] = {cv[0] * weight, cv[1] * weight, cv[2] * weight};
-                const float d[3] = {n.dist[0][gx], n.dist[1][gy], n.dist[2][gz]};
+                for (int r = 0; r < 3; ++r) B[c][r] = B[c][r] + wv[r] * d[c];
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) B[c][r] = B[c][r] + wv[r] * d[c];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) v[r] = v[r] + wv[r];
-            }
+            for (int r = 0; r < 3; ++r) v[r] = v[r] + wv[r];
         }
+    });
 }
 
 // g2p tail: C, advection, clamp, wall penalty (mls-mpm/g2p.wgsl:72-90).
