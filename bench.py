@@ -47,7 +47,9 @@ def stage_bytes(name: str, n: int, g: int, binned_path: bool = True) -> int:
         "copy_position": 48 * n,                  # read x, v; write PosVel (24 useful)
     }
     binned = {
-        "bin": (12 + 2 * 64) * n + 16 * n,        # read x; permute state + id; key, rank
+        "bin_gather": 8 * (g // 64),              # per tile: count in, running offset out
+        "bin_scan": 16 * (g // 64),               # per tile: offset, chunk range
+        "bin_scatter": 2 * 64 * n + 12 * n,       # permute state + id; read key, rank, src
         "p2g_1": 60 * n,                          # read x, v, C (cells written by reduce_1)
         "reduce_1": 16 * g,                       # write cells
         "p2g_2": 48 * n + 4 * g,                  # read x, C; read masses
@@ -146,15 +148,44 @@ def summarize_profiles(trace_dir: str, fetch_dir: str = "", write_dir: str = "")
     return "\n".join(out)
 
 
+def counter_table(dirs) -> str:
+    """Per-kernel averages (per dispatch) of every counter of one or more
+    rocprofv3 --pmc runs (csv output), one column per counter."""
+    import csv
+    import glob
+
+    acc = {}
+    names = []
+    for d in dirs:
+        files = [f for f in glob.glob(os.path.join(d, "**", "*"), recursive=True)
+                 if f.endswith("counter_collection.csv")]
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        for r in csv.DictReader(open(files[0])):
+            c = r["Counter_Name"]
+            if c not in names:
+                names.append(c)
+            acc.setdefault(_short_kernel(r["Kernel_Name"]), {}).setdefault(c, []).append(
+                float(r["Counter_Value"]))
+    out = [f"{'kernel':24s}" + "".join(f"{c[:18]:>19s}" for c in names)]
+    for k, per in acc.items():
+        if k.startswith("__amd") or "copyBuffer" in k or "fill" in k.lower():
+            continue
+        out.append(f"{k[:24]:24s}" + "".join(
+            f"{(sum(per[c]) / len(per[c]) if c in per else float('nan')):19.4g}" for c in names))
+    return "\n".join(out)
+
+
 DEFAULT_TUNING = {"path": 0, "particles_per_thread": 2, "grid_blocks": 0, "xcd_remap": 1}
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "substep_kernels_dambreak_300k.txt")
 
 
-# kernel behind each production time_stages slot (particles-per-thread 1)
-SLOT_KERNEL = {"p2g_1": "k_p2g1<1>", "reduce_1": "k_reduce<1>", "p2g_2": "k_p2g2<1>",
-               "reduce_2": "k_reduce<2>", "g2p": "k_g2p<1>"}
+# kernel behind each production time_stages slot (default tuning: 2 particles per thread)
+SLOT_KERNEL = {"bin_gather": "k_bin_gather", "bin_scan": "k_scan_keys",
+               "bin_scatter": "k_bin_scatter", "p2g_1": "k_p2g1<2>", "reduce_1": "k_reduce<1>",
+               "p2g_2": "k_p2g2<2>", "reduce_2": "k_reduce<2>", "g2p": "k_g2p<2>"}
 
 
 def measured_traffic(slot: str):
@@ -289,12 +320,18 @@ def main() -> int:
     ap.add_argument("--summarize", nargs="+", metavar="DIR",
                     help="print the table of a rocprofv3 kernel-trace run DIR "
                          "[FETCH_SIZE-run DIR, WRITE_SIZE-run DIR] and exit")
+    ap.add_argument("--counters", nargs="+", metavar="DIR",
+                    help="print per-kernel averages of the counters of rocprofv3 --pmc runs "
+                         "DIR... and exit")
     ap.add_argument("--tuning", type=str, default="",
                     help='JSON launch configuration, e.g. {"particles_per_thread": 1}')
     args = ap.parse_args()
     if args.summarize:
         d = list(args.summarize) + ["", ""]
         print(summarize_profiles(d[0], d[1], d[2]))
+        return 0
+    if args.counters:
+        print(counter_table(args.counters))
         return 0
 
     if args.gpus != 1:

@@ -131,10 +131,12 @@ int mpm_get_counts(mpm_sim *sim, uint32_t *num_particles, uint64_t *grid_count);
 
 /* Timing hooks (HIP events on the handle's stream, after a synchronize).
  * mpm_time_steps: runs `frames` x `substeps_per_frame` substeps; *ms = total.
- * mpm_time_stages: runs `substeps` substeps eagerly with events around every
- * kernel group of the production path; stage_ms[i] = summed duration of slot
- * i, calls[i] (may be NULL) = its launches, names[i] (may be NULL) = a static
- * name; *count = slots used.  No reference counterpart (the reference only
+ * mpm_time_stages: runs `substeps` substeps eagerly, timing every kernel of
+ * the production path with events bound to the kernel's own start and end
+ * (hipExtLaunchKernel), and every stage of the reference-shaped path with
+ * events around it; stage_ms[i] = summed duration of slot i, calls[i] (may be
+ * NULL) = its launches, names[i] (may be NULL) = a static name; *count = slots
+ * used.  No reference counterpart (the reference only
  * times JS encoding, main.ts:209,278-279). */
 int mpm_time_steps(mpm_sim *sim, uint32_t frames, uint32_t substeps_per_frame, float *ms);
 int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls,

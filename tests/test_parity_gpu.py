@@ -230,7 +230,8 @@ def test_time_stages_reports_every_kernel_slot():
     rec, box = scenes.config0()
     with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
         t = sim.time_stages(8)
-        assert set(t) == {"bin", "p2g_1", "reduce_1", "p2g_2", "reduce_2", "g2p"}
+        assert set(t) == {"bin_gather", "bin_scan", "bin_scatter", "p2g_1", "reduce_1", "p2g_2",
+                          "reduce_2", "g2p"}
         assert all(ms > 0 for ms, calls in t.values())
         assert t["p2g_1"][1] == 8
         ms = sim.time_steps(4)

@@ -94,7 +94,13 @@ private:
     hipError_t upload_chunks();
     BatchArgs args() const;
     BinBuffers bin_buffers() const;
-    hipError_t launch_substep(hipStream_t st, hipEvent_t *ev);
+    // Per-slot events of time_stages (slots in launch order).
+    struct SlotTimes {
+        static constexpr int kMax = 8;
+        hipEvent_t start[kMax], stop[kMax];
+        bool used[kMax];
+    };
+    hipError_t launch_substep(hipStream_t st, SlotTimes *t);
     // One path-0 substep that also records its binned state, tile chunks and
     // raw grid at `rec` (layout private to the adjoint).
     hipError_t substep_taped(hipStream_t st, char *rec, const void *layout);

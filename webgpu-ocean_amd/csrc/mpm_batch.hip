@@ -10,9 +10,10 @@
 namespace mpm {
 namespace {
 
-constexpr int kFusedSlots = 6;
-const char *const kFusedNames[kFusedSlots] = {"bin",   "p2g_1",    "reduce_1",
-                                              "p2g_2", "reduce_2", "g2p"};
+constexpr int kFusedSlots = 8;
+const char *const kFusedNames[kFusedSlots] = {"bin_gather", "bin_scan", "bin_scatter",
+                                              "p2g_1",      "reduce_1", "p2g_2",
+                                              "reduce_2",   "g2p"};
 const char *const kStageNames[6] = {"clear_grid", "p2g_1", "p2g_2",
                                     "update_grid", "g2p", "copy_position"};
 
@@ -362,33 +363,68 @@ hipError_t Batch::external_change(hipStream_t st) {
     return hipMemsetAsync(d_dirty_, 1, total_tiles_, st);
 }
 
-hipError_t Batch::launch_substep(hipStream_t st, hipEvent_t *ev) {
+// With `t`, each slot of the production path arms kernel-exact events for its
+// kernel (hipExtLaunchKernel timestamps); the reference-shaped path records
+// events around each stage.
+hipError_t Batch::launch_substep(hipStream_t st, SlotTimes *t) {
     int k = 0;
-    auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
+    if (!t) (void)disarm_kernel_timer();
+    auto before = [&]() -> hipError_t {
+        if (!t) return hipSuccess;
+        if (tun_.path == 1) return hipEventRecord(t->start[k], st);
+        arm_kernel_timer(t->start[k], t->stop[k]);
+        return hipSuccess;
+    };
+    auto after = [&]() -> hipError_t {
+        if (!t) return hipSuccess;
+        hipError_t e = hipSuccess;
+        if (tun_.path == 1) {
+            e = hipEventRecord(t->stop[k], st);
+            t->used[k] = true;
+        } else {
+            t->used[k] = !disarm_kernel_timer();  // false: the slot launched nothing
+        }
+        ++k;
+        return e;
+    };
     if (tun_.path == 1) {
         const DeviceScene v = stage_view();
         for (int stage = 0; stage < 6; ++stage) {
-            TRY(mark());
+            TRY(before());
             TRY(launch_stage(stage, v, st));
+            TRY(after());
         }
-        return mark();
+        return hipSuccess;
     }
     BatchArgs a = args();
-    TRY(mark());
-    TRY(launch_bin(a, bin_buffers(), !keys_ready_, st));
+    const BinBuffers b = bin_buffers();
+    TRY(before());
+    TRY(launch_bin_gather(a, b, !keys_ready_, st));
+    TRY(after());
+    TRY(before());
+    TRY(launch_bin_scan(a, b, st));
+    TRY(after());
+    TRY(before());
+    TRY(launch_bin_scatter(a, b, st));
+    TRY(after());
     a.state = spare_state_;
     a.id = spare_id_;
-    TRY(mark());
+    TRY(before());
     TRY(launch_p2g1(a, st));
-    TRY(mark());
+    TRY(after());
+    TRY(before());
     TRY(launch_reduce1(a, st));
-    TRY(mark());
+    TRY(after());
+    TRY(before());
     TRY(launch_p2g2(a, st));
-    TRY(mark());
+    TRY(after());
+    TRY(before());
     TRY(launch_reduce2(a, st));
-    TRY(mark());
+    TRY(after());
+    TRY(before());
     TRY(launch_g2p(a, st));
-    return mark();
+    TRY(after());
+    return hipSuccess;
 }
 
 hipError_t Batch::substep(hipStream_t st) {
@@ -422,27 +458,43 @@ hipError_t Batch::time_stages(hipStream_t st, unsigned substeps, float *ms, int 
                               const char **names, int capacity, int *count) {
     const bool fused = tun_.path == 0;
     const int nslots = fused ? kFusedSlots : 6;
+    static_assert(kFusedSlots <= SlotTimes::kMax, "slot events");
     if (capacity < nslots) return hipErrorInvalidValue;
-    hipEvent_t ev[kFusedSlots + 1];
-    for (int i = 0; i <= nslots; ++i) TRY(hipEventCreate(&ev[i]));
+    SlotTimes t{};
+    hipError_t e = hipSuccess;
+    int made = 0;  // slots whose two events exist
+    while (made < nslots) {
+        e = hipEventCreate(&t.start[made]);
+        if (e != hipSuccess) break;
+        e = hipEventCreate(&t.stop[made]);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(t.start[made]);
+            break;
+        }
+        ++made;
+    }
     for (int i = 0; i < nslots; ++i) {
         ms[i] = 0.0f;
         if (calls) calls[i] = 0;
         if (names) names[i] = fused ? kFusedNames[i] : kStageNames[i];
     }
-    hipError_t e = hipSuccess;
     for (unsigned k = 0; k < substeps && e == hipSuccess; ++k) {
-        e = launch_substep(st, ev);
-        if (e == hipSuccess) e = hipEventSynchronize(ev[nslots]);
+        e = launch_substep(st, &t);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         for (int i = 0; i < nslots && e == hipSuccess; ++i) {
-            float t = 0.0f;
-            e = hipEventElapsedTime(&t, ev[i], ev[i + 1]);
-            ms[i] += t;
+            if (!t.used[i]) continue;
+            float dt = 0.0f;
+            e = hipEventElapsedTime(&dt, t.start[i], t.stop[i]);
+            ms[i] += dt;
             if (calls) calls[i] += 1;
         }
         after_substep(st);
     }
-    for (int i = 0; i <= nslots; ++i) (void)hipEventDestroy(ev[i]);
+    (void)disarm_kernel_timer();
+    for (int i = 0; i < made; ++i) {
+        (void)hipEventDestroy(t.start[i]);
+        (void)hipEventDestroy(t.stop[i]);
+    }
     *count = nslots;
     return e;
 }

@@ -133,11 +133,22 @@ void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
 // counts are taken first (after clearing count); otherwise the previous
 // g2p's counts are used.
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st);
+// The three parts of launch_bin: (count and) per-tile gather of the counts;
+// key scan and tile chunks; scatter of the state.
+hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count,
+                             hipStream_t st);
+hipError_t launch_bin_scan(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
+hipError_t launch_bin_scatter(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
 hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st);
 hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st);
 hipError_t launch_g2p(const BatchArgs &a, hipStream_t st);
+// Kernel timing for time_stages: the next chunk/tile kernel launched on this
+// thread records `start`/`stop` at its own start and end (hipExtLaunchKernel).
+// disarm_kernel_timer() returns whether the events were still unused.
+void arm_kernel_timer(hipEvent_t start, hipEvent_t stop);
+bool disarm_kernel_timer();
 // Writes uniforms of one scene (stream-ordered, like queue.writeBuffer).
 hipError_t launch_set_scene_uniforms(SceneDesc *d, const Uniforms &u, hipStream_t st);
 

@@ -23,6 +23,8 @@
 // binned to a separate key tile and add into a separate grid-sized
 // accumulator with global atomics; the reduce kernels fold it into the cells
 // and clear it again, only for scenes that have wild particles.
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <utility>
@@ -1071,6 +1073,23 @@ __global__ __launch_bounds__(kBlock) void k_g2p(BatchArgs a) {
 
 __global__ void k_set_scene_uniforms(SceneDesc *d, Uniforms u) { d->uni = u; }
 
+// Events armed for the next chunk/tile kernel launch (time_stages): the
+// launch goes through hipExtLaunchKernel, whose events take the kernel's own
+// start and end timestamps, so a slot's time excludes the dispatch gaps that
+// events recorded between launches would add.
+thread_local hipEvent_t t_timer_start = nullptr, t_timer_stop = nullptr;
+
+template <typename K, typename... Args>
+void launch_timed(K kernel, unsigned grid, unsigned block, hipStream_t st, Args... args) {
+    if (t_timer_start) {
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, t_timer_start, t_timer_stop,
+                              0, args...);
+        t_timer_start = t_timer_stop = nullptr;
+    } else {
+        kernel<<<grid, block, 0, st>>>(args...);
+    }
+}
+
 inline unsigned blocks_for(uint64_t count, uint64_t per) {
     return (unsigned)((count + per - 1) / per);
 }
@@ -1084,9 +1103,9 @@ template <int PPT>
 hipError_t launch_ppt(int which, const BatchArgs &a, hipStream_t st) {
     const unsigned nb = walk_blocks(a);
     switch (which) {
-    case 1: k_p2g1<PPT><<<nb, kBlock, 0, st>>>(a); break;
-    case 2: k_p2g2<PPT><<<nb, kBlock, 0, st>>>(a); break;
-    case 3: k_g2p<PPT><<<nb, kBlock, 0, st>>>(a); break;
+    case 1: launch_timed(k_p2g1<PPT>, nb, kBlock, st, a); break;
+    case 2: launch_timed(k_p2g2<PPT>, nb, kBlock, st, a); break;
+    case 3: launch_timed(k_g2p<PPT>, nb, kBlock, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1195,21 +1214,39 @@ void scene_key_geometry(SceneDesc &d) {
     d.fast = integral ? 1u : 0u;
 }
 
-hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st) {
+hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count,
+                             hipStream_t st) {
     const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
-    const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
     const unsigned nseg = blocks_for(b.nkeys, kSegKeys);
     if (count) {
+        const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
         const unsigned zn = std::max(1u, std::min(blocks_for(b.nkeys, kBlock), 1024u));
         k_zero_u32<<<zn, kBlock, 0, st>>>(b.count, b.nkeys);
         k_zero_u32<<<1, kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(b.sup), 2 * b.nsup);
         if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a);
     }
-    k_bin_gather<<<(nseg + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, 0, st>>>(a, b, cap,
-                                                                             count ? 1 : 0);
-    k_scan_keys<<<blocks_for(b.nkeys, kWave), kScanWaves * kWave, 0, st>>>(a, b, cap);
-    if (a.nchunks) k_bin_scatter<<<nbin, kBlock, 0, st>>>(a, b);
+    launch_timed(k_bin_gather, (nseg + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, st, a, b,
+                 cap, count ? 1 : 0);
     return hipGetLastError();
+}
+
+hipError_t launch_bin_scan(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
+    const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
+    launch_timed(k_scan_keys, blocks_for(b.nkeys, kWave), kScanWaves * kWave, st, a, b, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin_scatter(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
+    const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
+    if (a.nchunks) launch_timed(k_bin_scatter, nbin, kBlock, st, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st) {
+    hipError_t e = launch_bin_gather(a, b, count, st);
+    if (e == hipSuccess) e = launch_bin_scan(a, b, st);
+    if (e == hipSuccess) e = launch_bin_scatter(a, b, st);
+    return e;
 }
 
 hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(1, a, st); }
@@ -1217,13 +1254,24 @@ hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st) { return launch_by_pp
 hipError_t launch_g2p(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(3, a, st); }
 
 hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st) {
-    if (a.stiles) k_reduce<1><<<(a.stiles + kXcds - 1) / kXcds * kXcds, kWave, 0, st>>>(a);
+    if (a.stiles) launch_timed(k_reduce<1>, (a.stiles + kXcds - 1) / kXcds * kXcds, kWave, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st) {
-    if (a.stiles) k_reduce<2><<<(a.stiles + kXcds - 1) / kXcds * kXcds, kWave, 0, st>>>(a);
+    if (a.stiles) launch_timed(k_reduce<2>, (a.stiles + kXcds - 1) / kXcds * kXcds, kWave, st, a);
     return hipGetLastError();
+}
+
+void arm_kernel_timer(hipEvent_t start, hipEvent_t stop) {
+    t_timer_start = start;
+    t_timer_stop = stop;
+}
+
+bool disarm_kernel_timer() {
+    const bool was = t_timer_start != nullptr;
+    t_timer_start = t_timer_stop = nullptr;
+    return was;
 }
 
 hipError_t launch_set_scene_uniforms(SceneDesc *d, const Uniforms &u, hipStream_t st) {
