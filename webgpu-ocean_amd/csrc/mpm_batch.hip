@@ -112,7 +112,9 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     seg_cap_ = nseg;
     sup_cap_ = nsup;
     TRY(hipMalloc(&tchunks_, sizeof(TileChunk) * tc));
-    TRY(hipMalloc(&ntchunks_, sizeof(uint32_t)));
+    // chunk count, then the kXcds + 1 XCD range boundaries of the chunk walk
+    TRY(hipMalloc(&ntchunks_, sizeof(uint32_t) * (2 + kXcds)));
+    TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t) * (2 + kXcds)));
     TRY(hipMalloc(&tile_chunk_, sizeof(uint2) * k));
     TRY(hipMemset(count_, 0, sizeof(uint32_t) * k));
     TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t)));
@@ -282,6 +284,8 @@ BatchArgs Batch::args() const {
     a.nchunks = (uint32_t)h_chunks_.size();
     a.tchunks = tchunks_;
     a.ntchunks = ntchunks_;
+    a.xcd_first = ntchunks_ + 1;
+    a.total_slots = total_n_;
     a.tchunk_cap = (uint32_t)cap_tchunks_;
     a.tile_chunk = tile_chunk_;
     a.stiles = total_tiles_;
@@ -333,6 +337,7 @@ BinBuffers Batch::bin_buffers() const {
     b.tchunks = tchunks_;
     b.tchunk_cap = (uint32_t)cap_tchunks_;
     b.ntchunks = ntchunks_;
+    b.xcd_first = ntchunks_ + 1;
     b.tile_chunk = tile_chunk_;
     return b;
 }

@@ -28,6 +28,7 @@ constexpr int kTileEdge = 1 << kTileBits;                      // 4
 constexpr int kTileCells = kTileEdge * kTileEdge * kTileEdge;  // 64
 constexpr int kWinEdge = kTileEdge + 2;                        // 6
 constexpr int kWinCells = kWinEdge * kWinEdge * kWinEdge;      // 216
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
 constexpr uint32_t kWildTile = 0xffffffffu;
 constexpr uint32_t kNoTile = 0xffffffffu;  // neighbour outside the grid
 
@@ -67,6 +68,9 @@ struct BatchArgs {
     uint32_t nchunks;
     const TileChunk *tchunks;    // tile chunks of the current binning
     const uint32_t *ntchunks;    // their number (device)
+    const uint32_t *xcd_first;   // kXcds + 1 chunk indices: XCD x walks chunks
+                                 // [xcd_first[x], xcd_first[x+1]) (equal particle shares)
+    uint32_t total_slots;        // particles of all scenes
     uint32_t tchunk_cap;         // capacity of tchunks, partial and mig (in chunks)
     const uint2 *tile_chunk;     // key -> (first tile chunk, count)
     uint32_t stiles;             // spatial tiles of all scenes (reduce launch)
@@ -111,6 +115,7 @@ struct BinBuffers {
     TileChunk *tchunks;
     uint32_t tchunk_cap;
     uint32_t *ntchunks;
+    uint32_t *xcd_first;       // kXcds + 1 entries, written by the scan
     uint2 *tile_chunk;
 };
 

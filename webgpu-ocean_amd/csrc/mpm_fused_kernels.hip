@@ -39,7 +39,6 @@ namespace mpm {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kXcds = 8;
 constexpr unsigned kDefaultWalkBlocks = 4096;
 
 // ---------------------------------------------------------------------------
@@ -132,7 +131,9 @@ __device__ __forceinline__ uint32_t key_scene_of(const BatchArgs &a, uint32_t k)
 
 // Persistent walk over the device-side tile-chunk list.  With xcd_remap the
 // blocks of one XCD (block id mod 8) take a contiguous range of chunks, so
-// neighbouring tiles' windows meet in the same L2.
+// neighbouring tiles' windows meet in the same L2; the ranges hold equal
+// shares of the particles (chunk fill varies a lot between the body of the
+// fluid and its surface), as the scan split them.
 struct TileWalk {
     uint32_t c, end, step;
 };
@@ -141,9 +142,9 @@ __device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
     const uint32_t n = min(*a.ntchunks, a.tchunk_cap);
     TileWalk w;
     if (a.xcd_remap && (gridDim.x % kXcds) == 0) {
-        const uint32_t per = (n + kXcds - 1) / kXcds, x = blockIdx.x % kXcds;
-        w.c = x * per + blockIdx.x / kXcds;
-        w.end = min(n, (x + 1) * per);
+        const uint32_t x = blockIdx.x % kXcds;
+        w.c = min(a.xcd_first[x], n) + blockIdx.x / kXcds;
+        w.end = min(a.xcd_first[x + 1], n);
         w.step = gridDim.x / kXcds;
     } else {
         w.c = blockIdx.x;
@@ -485,11 +486,24 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
         b.tile_chunk[k] = make_uint2(cb, nch);
         if (nch) {
             const uint32_t sc = key_scene_of(a, k);
-            for (uint32_t i = 0; i < nch; ++i)
-                if (cb + i < b.tchunk_cap)
-                    b.tchunks[cb + i] = make_uint4(sc, xm, off + i * cap, min(cap, c - i * cap));
+            for (uint32_t i = 0; i < nch; ++i) {
+                const uint32_t z = off + i * cap, zn = min(cap, c - i * cap);
+                if (cb + i < b.tchunk_cap) b.tchunks[cb + i] = make_uint4(sc, xm, z, zn);
+                // XCD ranges: slot s_x = ceil(x * total / kXcds) starts range x;
+                // the chunk holding slot s_x - 1 is the last of range x - 1
+                // (chunks tile the slots [0, total) in order).
+#pragma unroll
+                for (int x = 1; x < kXcds; ++x) {
+                    const uint32_t sx = (uint32_t)(((uint64_t)x * a.total_slots + kXcds - 1) / kXcds);
+                    if (z < sx && sx <= z + zn) b.xcd_first[x] = cb + i + 1;
+                }
+            }
         }
-        if (k == b.nkeys - 1) *b.ntchunks = cb + nch;
+        if (k == b.nkeys - 1) {
+            *b.ntchunks = cb + nch;
+            b.xcd_first[0] = 0;
+            b.xcd_first[kXcds] = cb + nch;
+        }
     }
     // Every tile key writes its reach flag; a scene with wild particles (or
     // a non-integral box) needs every tile visited.  Counts stay readable
