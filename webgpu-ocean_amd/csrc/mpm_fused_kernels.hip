@@ -63,20 +63,26 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-// Blocks of a one-wave-per-tile launch: the spatial tiles padded to a
-// multiple of 8 (tiles >= stiles are padding).
-__device__ __forceinline__ uint32_t tile_blocks(const BatchArgs &a) {
-    return (a.stiles + kXcds - 1) / kXcds * kXcds;
+// One-wave-per-tile launches (the reduce kernels) pack kTileWaves waves, i.e.
+// consecutive tiles, per block: fewer, larger blocks dispatch faster.
+constexpr uint32_t kTileWaves = 4;
+
+// Blocks of a one-wave-per-tile launch, padded to a multiple of 8 (tiles >=
+// stiles are padding).
+__host__ __device__ __forceinline__ uint32_t tile_blocks(uint32_t stiles) {
+    const uint32_t nb = (stiles + kTileWaves - 1) / kTileWaves;
+    return (nb + kXcds - 1) / kXcds * kXcds;
 }
 
-// Spatial tile of this block: with xcd_remap the blocks of one XCD take a
+// Spatial tile of this wave: with xcd_remap the blocks of one XCD take a
 // contiguous range of tiles, so neighbouring partials meet in one L2.
 __device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
+    uint32_t blk = blockIdx.x;
     if (a.xcd_remap) {
-        const uint32_t per = tile_blocks(a) / kXcds;
-        return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+        const uint32_t per = tile_blocks(a.stiles) / kXcds;
+        blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     }
-    return blockIdx.x;
+    return blk * kTileWaves + threadIdx.x / kWave;
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -761,7 +767,7 @@ struct CellPlan {
 
 __device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b, const int l[3],
                                                   bool cell_ok) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     uint32_t nfirst = 0, ncnt = 0;
     if (lane < 27) {
         const uint32_t kt = a.nbr[(size_t)b * 27 + lane];
@@ -822,7 +828,8 @@ __device__ __forceinline__ void take_wild(const BatchArgs &a, uint64_t gi, int4 
 }
 
 template <int PHASE>
-__global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
+__global__ __launch_bounds__(kTileWaves * kWave) void k_reduce(BatchArgs a) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t b = reduce_tile(a);
     if (b >= a.stiles) return;
     const uint4 ti = a.tile_info[b];
@@ -831,7 +838,7 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
     // the tile's cells are empty and already written as such
     if (!a.near[b] && !dirty) return;  // (its count is zero too)
     const SceneDesc *sd = a.scenes + ti.w;
-    if (PHASE == 2 && threadIdx.x == 0) {  // ready for the next counting pass
+    if (PHASE == 2 && lane == 0) {  // ready for the next counting pass
         a.count[sd->koff + (b - sd->soff)] = 0;
         if (b == sd->soff) a.count[sd->koff + sd->ntiles] = 0;  // the scene's wild key
     }
@@ -839,7 +846,7 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
     const Params prm = sd->prm;
     const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
     if (!sd->fast) {  // every particle is wild: cells in linear order
-        const uint32_t li = (b - sd->soff) * kTileCells + threadIdx.x;
+        const uint32_t li = (b - sd->soff) * kTileCells + lane;
         if (li >= sd->G) return;
         const uint64_t gi = (uint64_t)sd->goff + li;
         if (PHASE == 1) {
@@ -856,7 +863,7 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
         }
         return;
     }
-    const int l[3] = {(int)threadIdx.x >> 4, ((int)threadIdx.x >> 2) & 3, (int)threadIdx.x & 3};
+    const int l[3] = {(int)lane >> 4, ((int)lane >> 2) & 3, (int)lane & 3};
     const int cx = (int)ti.x * kTileEdge + l[0], cy = (int)ti.y * kTileEdge + l[1],
               cz = (int)ti.z * kTileEdge + l[2];
     const bool cell_ok = in_dims(cx, cy, cz, sd);
@@ -874,13 +881,13 @@ __global__ __launch_bounds__(kWave) void k_reduce(BatchArgs a) {
     } else {
         if (empty) {
             if (dirty && cell_ok) a.vel[gi] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (threadIdx.x == 0 && dirty) a.dirty[b] = 0;
+            if (lane == 0 && dirty) a.dirty[b] = 0;
             return;
         }
         int4 g = make_int4(0, 0, 0, 0);
         if (cell_ok) g = a.grid[gi];
         const int4 acc = sum_partials(a, plan);
-        if (threadIdx.x == 0 && !dirty) a.dirty[b] = 1;
+        if (lane == 0 && !dirty) a.dirty[b] = 1;
         if (!cell_ok) return;
         g = add4(g, make_int4(acc.x, acc.y, acc.z, 0));
         if (has_wild) take_wild(a, gi, g, false);
@@ -1278,12 +1285,12 @@ hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st) { return launch_by_pp
 hipError_t launch_g2p(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(3, a, st); }
 
 hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st) {
-    if (a.stiles) launch_timed(k_reduce<1>, (a.stiles + kXcds - 1) / kXcds * kXcds, kWave, st, a);
+    if (a.stiles) launch_timed(k_reduce<1>, tile_blocks(a.stiles), kTileWaves * kWave, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st) {
-    if (a.stiles) launch_timed(k_reduce<2>, (a.stiles + kXcds - 1) / kXcds * kXcds, kWave, st, a);
+    if (a.stiles) launch_timed(k_reduce<2>, tile_blocks(a.stiles), kTileWaves * kWave, st, a);
     return hipGetLastError();
 }
 
