@@ -49,12 +49,12 @@ def stage_bytes(name: str, n: int, g: int, binned_path: bool = True) -> int:
     binned = {
         "bin_gather": 8 * (g // 64),              # per tile: count in, running offset out
         "bin_scan": 16 * (g // 64),               # per tile: offset, chunk range
-        "bin_scatter": 2 * 64 * n + 12 * n,       # permute state + id; read key, rank, src
-        "p2g_1": 60 * n,                          # read x, v, C (cells written by reduce_1)
+        "bin_scatter": 16 * n,                    # read key, rank, src; write the permutation
+        "p2g_1": 64 * n,                          # permutation; read x, v, C (cells: reduce_1)
         "reduce_1": 16 * g,                       # write cells
-        "p2g_2": 48 * n + 4 * g,                  # read x, C; read masses
+        "p2g_2": 52 * n + 4 * g,                  # permutation; read x, C; read masses
         "reduce_2": 16 * g + 12 * g + 12 * g,     # read cells; write momentum, velocity
-        "g2p": (12 + 60 + 24) * n + 12 * g,       # read x; write x, v, C, PosVel; gather v
+        "g2p": (4 + 12 + 4 + 60 + 4 + 24) * n + 12 * g,  # perm, x, id in; x, v, C, id, PosVel out
     }
     return (binned if binned_path else table)[name]
 

@@ -971,8 +971,9 @@ hipError_t Batch::substep_taped(hipStream_t st, char *rec, const void *layout) {
     const TapeLayout &L = *static_cast<const TapeLayout *>(layout);
     BatchArgs a = args();
     TRY(launch_bin(a, bin_buffers(), !keys_ready_, st));
-    a.state = spare_state_;
-    a.id = spare_id_;
+    // the tape holds this substep's input state in bin order
+    TRY(launch_gather_binned(a, reinterpret_cast<float *>(rec + L.state),
+                             reinterpret_cast<uint32_t *>(rec + L.id), st));
     CopySegs cs{};
     auto seg = [&](size_t at, const void *src, size_t bytes) {
         cs.src[cs.count] = static_cast<const uint32_t *>(src);
@@ -980,8 +981,6 @@ hipError_t Batch::substep_taped(hipStream_t st, char *rec, const void *layout) {
         cs.words[cs.count] = bytes / 4;
         ++cs.count;
     };
-    seg(L.state, spare_state_, sizeof(float) * NCOMP * stride_);
-    seg(L.id, spare_id_, sizeof(uint32_t) * stride_);
     seg(L.tchunks, tchunks_, sizeof(TileChunk) * cap_tchunks_);
     seg(L.tile_chunk, tile_chunk_, sizeof(uint2) * total_keys_);
     seg(L.ntchunks, ntchunks_, sizeof(uint32_t));

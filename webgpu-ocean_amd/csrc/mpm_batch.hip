@@ -74,6 +74,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     free_ptr(src_);
     free_ptr(mig_);
     free_ptr(order_);
+    free_ptr(perm_);
     free_ptr(count_);
     free_ptr(offset_);
     free_ptr(seg_);
@@ -102,6 +103,8 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMalloc(&src_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&mig_, sizeof(uint32_t) * 27 * tc));
     TRY(hipMalloc(&order_, sizeof(uint16_t) * c));
+    TRY(hipMalloc(&perm_, sizeof(uint32_t) * c));
+    TRY(hipMemset(perm_, 0, sizeof(uint32_t) * c));  // every entry a valid slot
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
     TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
     const size_t nseg = (k + 15) / 16, nsup = (k + 4095) / 4096;
@@ -300,6 +303,9 @@ BatchArgs Batch::args() const {
     a.key_scene = d_key_scene_;
     a.state = state_;
     a.id = id_;
+    a.perm = perm_;
+    a.state_out = spare_state_;
+    a.id_out = spare_id_;
     a.stride = stride_;
     a.grid = grid_;
     a.vel = vel_;
@@ -332,8 +338,7 @@ BinBuffers Batch::bin_buffers() const {
     b.sup = sup_ + par * sup_cap_;
     b.sup_next = sup_ + (1 - par) * sup_cap_;
     b.nkeys = (uint32_t)total_keys_;
-    b.state_out = spare_state_;
-    b.id_out = spare_id_;
+    b.perm_out = perm_;
     b.tchunks = tchunks_;
     b.tchunk_cap = (uint32_t)cap_tchunks_;
     b.ntchunks = ntchunks_;
@@ -412,8 +417,6 @@ hipError_t Batch::launch_substep(hipStream_t st, SlotTimes *t) {
     TRY(before());
     TRY(launch_bin_scatter(a, b, st));
     TRY(after());
-    a.state = spare_state_;
-    a.id = spare_id_;
     TRY(before());
     TRY(launch_p2g1(a, st));
     TRY(after());
@@ -671,6 +674,7 @@ void Batch::release() {
     free_ptr(src_);
     free_ptr(mig_);
     free_ptr(order_);
+    free_ptr(perm_);
     free_ptr(count_);
     free_ptr(offset_);
     free_ptr(seg_);

@@ -81,8 +81,12 @@ struct BatchArgs {
                                  // or a scene with wild particles / a non-integral box
     uint8_t *near_prev;          // the same for the previous binning
     const uint32_t *key_scene;   // per key: its scene
-    float *state;                // SoA particle state, NCOMP x stride, bin order
-    uint32_t *id;                // slot -> global caller index (poff + local index)
+    float *state;                // SoA particle state, NCOMP x stride, in the order the
+                                 // previous substep left (read through perm)
+    uint32_t *id;                // slot of state -> global caller index (poff + local index)
+    const uint32_t *perm;        // slot of this substep's bin order -> slot of state
+    float *state_out;            // g2p's output state, in this substep's bin order
+    uint32_t *id_out;            // its caller indices
     uint32_t stride;
     int4 *grid;                  // all scenes' cells: raw P2G accumulators (vx,vy,vz,mass)
     float4 *vel;                 // decoded updated velocities (written by reduce_2)
@@ -110,8 +114,7 @@ struct BinBuffers {
     uint2 *seg_next, *sup_next;  // the next substep's (the scan clears sup_next)
     uint32_t nseg, nsup;
     uint32_t nkeys;            // all scenes
-    float *state_out;
-    uint32_t *id_out;
+    uint32_t *perm_out;        // bin-order slot -> slot of the state (the scatter writes it)
     TileChunk *tchunks;
     uint32_t tchunk_cap;
     uint32_t *ntchunks;
@@ -134,9 +137,11 @@ void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
                         const uint32_t *xmaj, std::vector<uint32_t> &nbr,
                         std::vector<uint4> &info);
 
-// Counting sort of a.state into b.state_out.  With `count`, the keys and
-// counts are taken first (after clearing count); otherwise the previous
-// g2p's counts are used.
+// Counting sort of the slots of a.state by bin key into the permutation
+// b.perm_out (the state itself stays where it is; the chunk kernels read it
+// through the permutation and g2p writes its output in bin order).  With
+// `count`, the keys and counts are taken first (after clearing count);
+// otherwise the previous g2p's counts are used.
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st);
 // The three parts of launch_bin: (count and) per-tile gather of the counts;
 // key scan and tile chunks; scatter of the state.
@@ -144,6 +149,10 @@ hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count
                              hipStream_t st);
 hipError_t launch_bin_scan(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
 hipError_t launch_bin_scatter(const BatchArgs &a, const BinBuffers &b, hipStream_t st);
+// Copies the state into bin order through a.perm (dst_state: NCOMP x a.stride,
+// dst_id: a.stride), for the adjoint's tape.
+hipError_t launch_gather_binned(const BatchArgs &a, float *dst_state, uint32_t *dst_id,
+                                hipStream_t st);
 hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st);
 hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st);

@@ -522,9 +522,10 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
     if (lane == 0 && (k0 & ((1u << kSupBits) - 1)) == 0) b.sup_next[sp] = make_uint2(0u, 0u);
 }
 
-// Keys of scene s lie in [koff_s, koff_s + nkeys_s) and scenes are packed in
-// slot order, so the scattered slot stays inside the scene's slot range (the
-// check only guards the buffers against inconsistent ranks).
+// Writes the permutation of the binning: perm[d] = slot of the particle that
+// takes bin-order slot d.  Keys of scene s lie in [koff_s, koff_s + nkeys_s)
+// and scenes are packed in slot order, so d stays inside the scene's slot
+// range (the check only guards the buffers against inconsistent ranks).
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers b) {
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         const Chunk ch = a.chunks[c];
@@ -537,10 +538,17 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers 
             if (src != kRankFinal) r += a.mig[src];
             const uint32_t d = b.offset[a.key[i]] + r;
             if (d < lo || d >= hi) continue;
-#pragma unroll
-            for (int k = 0; k < NCOMP; ++k) b.state_out[k * a.stride + d] = a.state[k * a.stride + i];
-            b.id_out[d] = a.id[i];
+            b.perm_out[d] = i;
         }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_binned(BatchArgs a, float *dst, uint32_t *dst_id) {
+    for (uint32_t d = blockIdx.x * kBlock + threadIdx.x; d < a.total_slots; d += gridDim.x * kBlock) {
+        const uint32_t i = a.perm[d];
+#pragma unroll
+        for (int k = 0; k < NCOMP; ++k) dst[k * a.stride + d] = a.state[k * a.stride + i];
+        dst_id[d] = a.id[i];
     }
 }
 
@@ -645,7 +653,7 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
         const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
         cell[k] = lev[k] = 0;
         if (j < ch.w) {
-            const uint32_t i = ch.z + j;
+            const uint32_t i = a.perm[ch.z + j];
             const int m = kTileEdge - 1;
             const int cx = (int)floorf(a.state[PX * a.stride + i]) & m;
             const int cy = (int)floorf(a.state[(PX + 1) * a.stride + i]) & m;
@@ -712,9 +720,10 @@ void k_p2g1(BatchArgs a) {
             if (valid[k]) {
                 const uint32_t j = ord.perm[pos];
                 a.order[ch.z + pos] = (uint16_t)j;  // p2g_2 takes the same order
-                load_x(a.state, a.stride, ch.z + j, x[k]);
-                load_v(a.state, a.stride, ch.z + j, v[k]);
-                load_C(a.state, a.stride, ch.z + j, C[k]);
+                const uint32_t i = a.perm[ch.z + j];
+                load_x(a.state, a.stride, i, x[k]);
+                load_v(a.state, a.stride, i, v[k]);
+                load_C(a.state, a.stride, i, C[k]);
             }
         }
 #pragma unroll
@@ -740,9 +749,10 @@ void k_p2g1(BatchArgs a) {
         const GridSink1 sink{(int32_t *)(a.wild + sd->goff), sd->G, sd->uni.Y, sd->uni.Z};
         for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
             float x[3], v[3], C[3][3];
-            load_x(a.state, a.stride, ch.z + j, x);
-            load_v(a.state, a.stride, ch.z + j, v);
-            load_C(a.state, a.stride, ch.z + j, C);
+            const uint32_t i = a.perm[ch.z + j];
+            load_x(a.state, a.stride, i, x);
+            load_v(a.state, a.stride, i, v);
+            load_C(a.state, a.stride, i, C);
             Node3 n;
             node3_init(x, n);
             p2g1_particle<true>(n, v, C, sd->prm.mult, sink);
@@ -917,9 +927,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
             const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
             valid[k] = pos < ch.w;
             if (valid[k]) {  // the level order p2g_1 recorded
-                const uint32_t j = a.order[ch.z + pos];
-                load_x(a.state, a.stride, ch.z + j, x[k]);
-                load_C(a.state, a.stride, ch.z + j, C[k]);
+                const uint32_t i = a.perm[ch.z + a.order[ch.z + pos]];
+                load_x(a.state, a.stride, i, x[k]);
+                load_C(a.state, a.stride, i, C[k]);
             }
         }
         int t[3];
@@ -955,8 +965,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
                              sd->G, sd->uni.Y, sd->uni.Z, prm.mult};
         for (uint32_t j = threadIdx.x; j < ch.w; j += kBlock) {
             float x[3], C[3][3];
-            load_x(a.state, a.stride, ch.z + j, x);
-            load_C(a.state, a.stride, ch.z + j, C);
+            const uint32_t i = a.perm[ch.z + j];
+            load_x(a.state, a.stride, i, x);
+            load_C(a.state, a.stride, i, C);
             Node3 n;
             node3_init(x, n);
             p2g2_particle<true>(n, C, prm, sink);
@@ -967,18 +978,22 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // g2p with copyPosition fused (mls-mpm/g2p.wgsl:26-92, copyPosition.wgsl:15-21)
 
-__device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, const float xo[3],
-                                          const float v[3], const float Cn[3][3]) {
+// Writes particle `src` (a slot of a.state) updated, at bin-order slot i of
+// the output state.
+__device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, uint32_t src,
+                                          const float xo[3], const float v[3],
+                                          const float Cn[3][3]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) a.state[(C00 + 3 * c + r) * a.stride + i] = Cn[c][r];
+        for (int r = 0; r < 3; ++r) a.state_out[(C00 + 3 * c + r) * a.stride + i] = Cn[c][r];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        a.state[(PX + ax) * a.stride + i] = xo[ax];
-        a.state[(VX + ax) * a.stride + i] = v[ax];
+        a.state_out[(PX + ax) * a.stride + i] = xo[ax];
+        a.state_out[(VX + ax) * a.stride + i] = v[ax];
     }
-    const size_t o = a.id[i];
+    const uint32_t o = a.id[src];
+    a.id_out[i] = o;
     a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
     a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
 }
@@ -1028,13 +1043,18 @@ void k_g2p(BatchArgs a) {
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
         if (threadIdx.x < kDirWild + 1) lcnt[threadIdx.x] = 0;
         float x[PPT][3];
+        uint32_t src[PPT];
         bool valid[PPT];
         // contiguous slots per task: coalesced state stores
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const uint32_t j = task_of(k) * 64 + (threadIdx.x & 63);
             valid[k] = j < ch.w;
-            if (valid[k]) load_x(a.state, a.stride, ch.z + j, x[k]);
+            src[k] = 0;
+            if (valid[k]) {
+                src[k] = a.perm[ch.z + j];
+                load_x(a.state, a.stride, src[k], x[k]);
+            }
         }
         int t[3];
         tile_coords(ch.y, sd, t);
@@ -1054,7 +1074,7 @@ void k_g2p(BatchArgs a) {
                 node3_init(x[k], n);
                 g2p_gather(n, WinSinkG{lvel, win_l0(x[k])}, v, B);
                 g2p_finish(x[k], rb, dt, v, B, xo, Cn);
-                g2p_store(a, i, xo, v, Cn);
+                g2p_store(a, i, src[k], xo, v, Cn);
                 key = bin_key(a, sd, xo, t, &dir);
             }
             wild[k] = valid[k] && dir == kDirWild;
@@ -1088,12 +1108,13 @@ void k_g2p(BatchArgs a) {
             uint32_t key = 0, dir = kDirFar;
             if (active) {
                 float x[3], v[3], B[3][3], xo[3], Cn[3][3];
-                load_x(a.state, a.stride, i, x);
+                const uint32_t si = a.perm[i];
+                load_x(a.state, a.stride, si, x);
                 Node3 n;
                 node3_init(x, n);
                 g2p_gather<true>(n, sink, v, B);
                 g2p_finish(x, rb, dt, v, B, xo, Cn);
-                g2p_store(a, i, xo, v, Cn);
+                g2p_store(a, i, si, xo, v, Cn);
                 key = bin_key(a, sd, xo, nowhere, &dir);
                 dir = kDirFar;
             }
@@ -1270,6 +1291,14 @@ hipError_t launch_bin_scan(const BatchArgs &a, const BinBuffers &b, hipStream_t 
 hipError_t launch_bin_scatter(const BatchArgs &a, const BinBuffers &b, hipStream_t st) {
     const unsigned nbin = std::max(1u, std::min(a.nchunks, 4096u));
     if (a.nchunks) launch_timed(k_bin_scatter, nbin, kBlock, st, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_binned(const BatchArgs &a, float *dst_state, uint32_t *dst_id,
+                                hipStream_t st) {
+    if (a.total_slots)
+        k_gather_binned<<<std::min(blocks_for(a.total_slots, kBlock), 4096u), kBlock, 0, st>>>(
+            a, dst_state, dst_id);
     return hipGetLastError();
 }
 
