@@ -122,6 +122,30 @@ def test_config2_two_million_mass_and_one_substep():
         assert_same(sim, ref, ("particles", "posvel"), ctx="cfg2 substep 1")
 
 
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+def test_dense_cells_match_the_oracle(ppt):
+    """Hundreds of particles per cell (the pile-up at a wall after a box
+    shrink): tiles split into several chunks, the P2G level order runs to
+    hundreds of levels, and the i32 mass accumulators wrap.  Integer adds
+    wrap the same on both sides, so results stay bit-exact."""
+    rng = np.random.default_rng(11)
+    box = [24.0, 20.0, 24.0]
+    dense = rng.uniform([9.0, 6.0, 9.0], [11.0, 8.0, 11.0], size=(1800, 3))  # 8 cells
+    plane = np.stack([np.full(700, 14.0), rng.uniform(4.0, 7.0, 700),
+                      rng.uniform(4.0, 20.0, 700)], axis=1)  # one x plane, like a clamp
+    spread = rng.uniform(3.5, 16.5, size=(500, 3))
+    pos = np.concatenate([dense, plane, spread]).astype(np.float32)
+    vel = rng.normal(0.0, 0.3, size=pos.shape).astype(np.float32)
+    rec = scenes.records(pos, vel)
+    sim, ref = pair(rec, box)
+    with sim:
+        sim.set_tuning(particles_per_thread=ppt)
+        for frame in range(2):
+            sim.execute()
+            ref.execute(2)
+            assert_same(sim, ref, ("particles", "posvel"), ctx=f"ppt {ppt} frame {frame}")
+
+
 def test_particles_outside_the_grid_are_dropped_like_the_oracle():
     box = [12.0, 12.0, 12.0]
     pos = np.array([[0.2, 5.0, 5.0], [11.8, 5.0, 5.0], [5.0, 5.0, 11.9], [5.0, -3.0, 5.0],
