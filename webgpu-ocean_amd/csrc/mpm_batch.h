@@ -140,7 +140,7 @@ private:
     uint32_t *key_ = nullptr, *rank_ = nullptr, *src_ = nullptr, *count_ = nullptr,
              *offset_ = nullptr;
     uint32_t *mig_ = nullptr;  // 27 per tile chunk
-    uint16_t *order_ = nullptr;  // per slot: P2G level order within its chunk
+    uint32_t *order_ = nullptr;  // per bin-order slot: state slot in P2G level order
     uint32_t *perm_ = nullptr;   // bin-order slot -> slot of state_ (written by the binning)
     uint2 *seg_ = nullptr, *sup_ = nullptr;  // scan totals, two parities
     size_t seg_cap_ = 0, sup_cap_ = 0;

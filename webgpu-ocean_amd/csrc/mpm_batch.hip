@@ -102,7 +102,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMalloc(&rank_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&src_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&mig_, sizeof(uint32_t) * 27 * tc));
-    TRY(hipMalloc(&order_, sizeof(uint16_t) * c));
+    TRY(hipMalloc(&order_, sizeof(uint32_t) * c));
     TRY(hipMalloc(&perm_, sizeof(uint32_t) * c));
     TRY(hipMemset(perm_, 0, sizeof(uint32_t) * c));  // every entry a valid slot
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));

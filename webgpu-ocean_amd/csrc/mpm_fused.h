@@ -99,8 +99,8 @@ struct BatchArgs {
     uint32_t *src;               // per slot: chunk * 27 + direction, or final-rank marker
     uint32_t *mig;               // 27 per tile chunk: particles moving each direction,
                                  // then their offset within the destination tile
-    uint16_t *order;             // per slot: level order of its window chunk (slot within
-                                 // the chunk of each position), written by p2g_1
+    uint32_t *order;             // per bin-order position of a window chunk: the state slot
+                                 // of the particle p2g_1 processed there (its level order)
     uint32_t *count;             // per key: particles (zero after each scan)
     int count_next;              // g2p also counts the keys of its output
     int ppt;                     // particles per thread: chunk <= kBlock * ppt

@@ -638,6 +638,7 @@ struct LevelOrder {
     uint32_t base[kBlock * PPT];   // first position of each level
     uint64_t mask[kBlock * PPT];   // cells present in each level
     uint16_t perm[kBlock * PPT];   // position -> slot within the chunk
+    uint32_t slot[kBlock * PPT];   // slot within the chunk -> state slot (a.perm)
 };
 
 // Fills o.perm for window chunk ch; every thread of the block calls it.
@@ -654,6 +655,7 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
         cell[k] = lev[k] = 0;
         if (j < ch.w) {
             const uint32_t i = a.perm[ch.z + j];
+            o.slot[j] = i;
             const int m = kTileEdge - 1;
             const int cx = (int)floorf(a.state[PX * a.stride + i]) & m;
             const int cy = (int)floorf(a.state[(PX + 1) * a.stride + i]) & m;
@@ -718,9 +720,8 @@ void k_p2g1(BatchArgs a) {
             const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
             valid[k] = pos < ch.w;
             if (valid[k]) {
-                const uint32_t j = ord.perm[pos];
-                a.order[ch.z + pos] = (uint16_t)j;  // p2g_2 takes the same order
-                const uint32_t i = a.perm[ch.z + j];
+                const uint32_t i = ord.slot[ord.perm[pos]];
+                a.order[ch.z + pos] = i;  // p2g_2 takes the same order
                 load_x(a.state, a.stride, i, x[k]);
                 load_v(a.state, a.stride, i, v[k]);
                 load_C(a.state, a.stride, i, C[k]);
@@ -927,7 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
             const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
             valid[k] = pos < ch.w;
             if (valid[k]) {  // the level order p2g_1 recorded
-                const uint32_t i = a.perm[ch.z + a.order[ch.z + pos]];
+                const uint32_t i = a.order[ch.z + pos];
                 load_x(a.state, a.stride, i, x[k]);
                 load_C(a.state, a.stride, i, C[k]);
             }
