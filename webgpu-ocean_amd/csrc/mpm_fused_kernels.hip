@@ -2,11 +2,13 @@
 //
 //   bin -> p2g_1 -> reduce_1 -> p2g_2 -> reduce_2 -> g2p
 //
-// Particles live in SoA storage permuted into bin order (tile-major cell keys,
-// tiles along a Hilbert curve); id[] maps a storage slot to the caller's
-// particle index, so readout order never changes.  The binning scan also
-// emits "tile chunks": runs of at most 256*ppt particles of ONE 4^3-cell
-// tile.  A P2G block accumulates its chunk into the tile's 6^3-cell LDS
+// Particles live in SoA storage in the bin order of the previous substep
+// (tiles along a Hilbert curve); id[] maps a storage slot to the caller's
+// particle index, so readout order never changes.  The binning computes a
+// permutation into this substep's bin order without moving the state: the
+// chunk kernels read through it, and g2p writes its output state in bin
+// order into the other buffer.  The binning scan also emits "tile chunks":
+// runs of at most 256*ppt particles of ONE 4^3-cell tile.  A P2G block accumulates its chunk into the tile's 6^3-cell LDS
 // window (tile + one-cell halo) with i32 fixed-point LDS adds, then stores the
 // whole window with coalesced plain stores as a partial.  The reduce kernels
 // give every grid cell exactly one writer: one wave per tile sums, for each of
