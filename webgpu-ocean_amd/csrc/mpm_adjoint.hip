@@ -602,15 +602,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
     __shared__ uint32_t s_max;
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
-        if (ch.y == kWildTile) {
-            const GlobalAcc acc(q);
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
-                G2PAdj p;
-                adj_g2p_a(q, ch.z + j, acc, p);
-                adj_g2p_b(q, acc, p);
-            }
-            continue;
-        }
+        if (ch.y == kWildTile) continue;  // second loop
         int t[3];
         adj_tile_coords(q, ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
@@ -655,6 +647,18 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
             out[l] = make_float4(s_sum[l], s_sum[kWinCells + l], s_sum[2 * kWinCells + l], 0.0f);
         __syncthreads();
     }
+    // Wild chunks (global float atomics) in a loop of their own: in the
+    // window loop they would set its register budget.
+    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = q.tchunks[w.c];
+        if (ch.y != kWildTile) continue;
+        const GlobalAcc acc(q);
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+            G2PAdj p;
+            adj_g2p_a(q, ch.z + j, acc, p);
+            adj_g2p_b(q, acc, p);
+        }
+    }
 }
 
 // P2G-2 adjoint: window of (gmv, mass) in, window of mass adjoints out
@@ -668,15 +672,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
     float gk = 0.0f, gmu = 0.0f;
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
-        if (ch.y == kWildTile) {
-            const GlobalAcc acc(q);
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
-                P2G2Adj p;
-                adj_p2g2_a(q, ch.z + j, acc, p, gk, gmu);
-                adj_p2g2_b(q, acc, p);
-            }
-            continue;
-        }
+        if (ch.y == kWildTile) continue;  // second loop
         int t[3];
         adj_tile_coords(q, ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
@@ -715,6 +711,18 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
             out[l] = make_float4(s_sum[l], 0.0f, 0.0f, 0.0f);
         __syncthreads();
     }
+    // Wild chunks (global float atomics) in a loop of their own: in the
+    // window loop they would set its register budget.
+    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = q.tchunks[w.c];
+        if (ch.y != kWildTile) continue;
+        const GlobalAcc acc(q);
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock) {
+            P2G2Adj p;
+            adj_p2g2_a(q, ch.z + j, acc, p, gk, gmu);
+            adj_p2g2_b(q, acc, p);
+        }
+    }
     // block reduction of the scalar parameter adjoints
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -742,12 +750,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
     __shared__ float4 s_w4[kWinCells];
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
-        if (ch.y == kWildTile) {
-            const GlobalAcc acc(q);
-            for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
-                adj_p2g1_particle(q, ch.z + j, acc);
-            continue;
-        }
+        if (ch.y == kWildTile) continue;  // second loop
         int t[3];
         adj_tile_coords(q, ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
@@ -769,6 +772,15 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
             adj_p2g1_particle(q, i, acc);
         }
         __syncthreads();
+    }
+    // Wild chunks (global float atomics) in a loop of their own: in the
+    // window loop they would set its register budget.
+    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+        const TileChunk ch = q.tchunks[w.c];
+        if (ch.y != kWildTile) continue;
+        const GlobalAcc acc(q);
+        for (uint32_t j = threadIdx.x; j < ch.w; j += kAdjBlock)
+            adj_p2g1_particle(q, ch.z + j, acc);
     }
 }
 
