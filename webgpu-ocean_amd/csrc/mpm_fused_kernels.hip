@@ -696,9 +696,9 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
 // p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65) into tile partials
 
 // Register budget of the chunk kernels at the default 2 particles per thread:
-// 80 VGPRs (6 waves per SIMD).  At this budget k_p2g1 needs no spills and
+// 80 VGPRs (6 waves per SIMD).  At this budget k_p2g1 spills a few bytes and
 // k_g2p spills only in its wild-chunk loop; measured 6% (p2g_1) and 8% (g2p)
-// faster than the compiler's own 92-VGPR choice.  k_p2g2 gains nothing from
+// faster than the compiler's own ~92-VGPR choice.  k_p2g2 gains nothing from
 // it and keeps the compiler's budget.
 template <int PPT>
 constexpr int kChunkWaves = PPT == 2 ? 6 : 1;  // amdgpu_waves_per_eu minimum (1: no constraint)
