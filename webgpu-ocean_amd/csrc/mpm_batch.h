@@ -142,8 +142,8 @@ private:
     uint32_t *mig_ = nullptr;  // 27 per tile chunk
     uint32_t *order_ = nullptr;  // per bin-order slot: state slot in P2G level order
     uint32_t *perm_ = nullptr;   // bin-order slot -> slot of state_ (written by the binning)
-    uint2 *seg_ = nullptr, *sup_ = nullptr;  // scan totals, two parities
-    size_t seg_cap_ = 0, sup_cap_ = 0;
+    uint2 *sup_ = nullptr;  // scan totals per 256 keys, two parities
+    size_t sup_cap_ = 0;
     float *state_buf0_ = nullptr;  // parity 0 = this buffer is the current state
     TileChunk *tchunks_ = nullptr;
     uint32_t *ntchunks_ = nullptr;

@@ -77,7 +77,6 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     free_ptr(perm_);
     free_ptr(count_);
     free_ptr(offset_);
-    free_ptr(seg_);
     free_ptr(sup_);
     free_ptr(tchunks_);
     free_ptr(ntchunks_);
@@ -107,12 +106,9 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMemset(perm_, 0, sizeof(uint32_t) * c));  // every entry a valid slot
     TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
     TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
-    const size_t nseg = (k + 15) / 16, nsup = (k + 4095) / 4096;
-    TRY(hipMalloc(&seg_, sizeof(uint2) * 2 * nseg));  // two substep parities
-    TRY(hipMalloc(&sup_, sizeof(uint2) * 2 * nsup));
-    TRY(hipMemset(seg_, 0, sizeof(uint2) * 2 * nseg));
+    const size_t nsup = (k + (1u << kSupBits) - 1) >> kSupBits;
+    TRY(hipMalloc(&sup_, sizeof(uint2) * 2 * nsup));  // two substep parities
     TRY(hipMemset(sup_, 0, sizeof(uint2) * 2 * nsup));
-    seg_cap_ = nseg;
     sup_cap_ = nsup;
     TRY(hipMalloc(&tchunks_, sizeof(TileChunk) * tc));
     // chunk count, then the kXcds + 1 XCD range boundaries of the chunk walk
@@ -301,6 +297,10 @@ BatchArgs Batch::args() const {
         a.near_prev = d_near_ + (1 - par) * tile_tab_cap_;
     }
     a.key_scene = d_key_scene_;
+    {
+        const size_t par = state_ == state_buf0_ ? 0 : 1;  // g2p fills the other parity
+        a.sup_acc = sup_ + (1 - par) * sup_cap_;
+    }
     a.state = state_;
     a.id = id_;
     a.perm = perm_;
@@ -317,6 +317,11 @@ BatchArgs Batch::args() const {
     a.order = order_;
     a.count = count_;
     a.count_next = 1;
+    // Key spaces much larger than the particle chunks (many sparse scenes,
+    // big grids) make the per-key gather expensive; g2p's atomics cost per
+    // chunk instead.  Measured crossover between 10 and 33 keys per 256
+    // particles.
+    a.count_atomic = (uint64_t)total_keys_ > 16ull * ((uint64_t)total_n_ / kBlock + 1) ? 1 : 0;
     a.partial = partial_;
     a.posvel = posvel_;
     a.ppt = tun_.ppt;
@@ -331,10 +336,7 @@ BinBuffers Batch::bin_buffers() const {
     b.offset = offset_;
     // totals of this substep's parity (state buffer 0 or 1), the other one next
     const size_t par = state_ == state_buf0_ ? 0 : 1;
-    b.nseg = (uint32_t)((total_keys_ + 15) / 16);
-    b.nsup = (uint32_t)((total_keys_ + 4095) / 4096);
-    b.seg = seg_ + par * seg_cap_;
-    b.seg_next = seg_ + (1 - par) * seg_cap_;
+    b.nsup = (uint32_t)((total_keys_ + (1u << kSupBits) - 1) >> kSupBits);
     b.sup = sup_ + par * sup_cap_;
     b.sup_next = sup_ + (1 - par) * sup_cap_;
     b.nkeys = (uint32_t)total_keys_;
@@ -677,7 +679,6 @@ void Batch::release() {
     free_ptr(perm_);
     free_ptr(count_);
     free_ptr(offset_);
-    free_ptr(seg_);
     free_ptr(sup_);
     free_ptr(tchunks_);
     free_ptr(ntchunks_);

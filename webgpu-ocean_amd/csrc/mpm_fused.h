@@ -29,6 +29,9 @@ constexpr int kTileCells = kTileEdge * kTileEdge * kTileEdge;  // 64
 constexpr int kWinEdge = kTileEdge + 2;                        // 6
 constexpr int kWinCells = kWinEdge * kWinEdge * kWinEdge;      // 216
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
+constexpr uint32_t kSegBits = 4;   // 16 keys per block of the count-path totals kernel
+constexpr uint32_t kSupBits = 8;   // binning scan totals per 256 keys (super-segment; g2p
+                                   // adds to them with atomics, so they must not be too few)
 constexpr uint32_t kWildTile = 0xffffffffu;
 constexpr uint32_t kNoTile = 0xffffffffu;  // neighbour outside the grid
 
@@ -101,8 +104,11 @@ struct BatchArgs {
                                  // then their offset within the destination tile
     uint32_t *order;             // per bin-order position of a window chunk: the state slot
                                  // of the particle p2g_1 processed there (its level order)
-    uint32_t *count;             // per key: particles (zero after each scan)
+    uint32_t *count;             // per key: particles (reduce_2 clears it for g2p's counting)
+    uint2 *sup_acc;              // the next binning's (particles, chunks) totals per 256 keys,
+                                 // accumulated by g2p (the scan clears them)
     int count_next;              // g2p also counts the keys of its output
+    int count_atomic;            // ... and publishes them with atomics (no k_bin_gather)
     int ppt;                     // particles per thread: chunk <= kBlock * ppt
     int grid_blocks;             // persistent blocks of the chunk kernels (0: default)
     int xcd_remap;               // contiguous chunk ranges per XCD
@@ -110,9 +116,9 @@ struct BatchArgs {
 
 struct BinBuffers {
     uint32_t *count, *offset;  // per key; reduce_2 clears count
-    uint2 *seg, *sup;          // totals per 64 keys / per 4096 keys, this substep
-    uint2 *seg_next, *sup_next;  // the next substep's (the scan clears sup_next)
-    uint32_t nseg, nsup;
+    uint2 *sup;                // (particles, chunks) totals per 256 keys, this substep
+    uint2 *sup_next;           // the next substep's (the scan clears them; g2p adds)
+    uint32_t nsup;
     uint32_t nkeys;            // all scenes
     uint32_t *perm_out;        // bin-order slot -> slot of the state (the scatter writes it)
     TileChunk *tchunks;
@@ -140,11 +146,12 @@ void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
 // Counting sort of the slots of a.state by bin key into the permutation
 // b.perm_out (the state itself stays where it is; the chunk kernels read it
 // through the permutation and g2p writes its output in bin order).  With
-// `count`, the keys and counts are taken first (after clearing count);
-// otherwise the previous g2p's counts are used.
+// `count`, the keys, counts and key-range totals are taken first (after
+// clearing count); otherwise the previous g2p left them.
 hipError_t launch_bin(const BatchArgs &a, const BinBuffers &b, bool count, hipStream_t st);
-// The three parts of launch_bin: (count and) per-tile gather of the counts;
-// key scan and tile chunks; scatter of the state.
+// The three parts of launch_bin: (with `count`, keys and ranks from scratch,
+// then) the per-key gather of the counts (skipped when g2p published them
+// with atomics); key scan and tile chunks; the permutation.
 hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count,
                              hipStream_t st);
 hipError_t launch_bin_scan(const BatchArgs &a, const BinBuffers &b, hipStream_t st);

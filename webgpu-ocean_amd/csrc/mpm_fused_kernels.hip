@@ -309,15 +309,21 @@ struct GridSinkG {  // g2p straight from the decoded velocities
 // binning: one key per (scene, tile) in Hilbert order plus one wild key per
 // scene; counting sort; tile-chunk list.
 //
-// The counts come from the previous substep's g2p, without global atomics in
-// the common case: a g2p block counts, in LDS, how many of its chunk's
-// particles moved to each of the 27 tiles around the chunk's tile ("mig",
-// 27 per chunk) and gives each particle its rank within its (chunk,
-// direction).  k_bin_gather then visits every tile and turns the counts of
-// the chunks that can move into it into running offsets.  Particles that
-// jump further than one tile, or become wild, take a rank from a global
-// atomic on their key instead (rank "final").  After the state was replaced
-// from outside, k_bin_count ranks every particle that way.
+// The counts come from the previous substep's g2p: a g2p block counts, in
+// LDS, how many of its chunk's particles moved to each of the 27 tiles around
+// the chunk's tile ("mig", 27 per chunk) and gives each particle its rank
+// within its (chunk, direction).  Two ways turn the counts into offsets
+// within the destination tiles (count_atomic, chosen from the ratio of keys
+// to particles):
+//  - k_bin_gather visits every key and sums, in a fixed order, the counts of
+//    the chunks that can move into it (no atomics; cost grows with the keys);
+//  - with count_atomic, g2p itself adds each (chunk, direction) count to the
+//    destination key's counter with one global atomic whose old value is the
+//    group's offset (cost grows with the chunks; no gather kernel), and adds
+//    the deltas to the per-256-key totals the scan starts from.
+// Particles that jump further than one tile, or become wild, take a rank
+// from a global atomic on their key instead (rank "final").  After the state
+// was replaced from outside, k_bin_count ranks every particle that way.
 
 constexpr uint32_t kDirs = 27;              // near directions (dx+1)*9 + (dy+1)*3 + dz+1
 constexpr uint32_t kDirWild = kDirs;        // became wild: block-level global atomic
@@ -369,6 +375,40 @@ __device__ __forceinline__ uint32_t count_rank(uint32_t *count, uint32_t key, bo
     return __shfl(base, my_head, kWave) + (uint32_t)(lane - my_head);
 }
 
+__device__ __forceinline__ uint32_t chunks_of(uint32_t n, uint32_t cap) {
+    return (n + cap - 1) / cap;
+}
+
+// Adds m particles that joined key k at rank `base` to the scan totals of
+// the next binning: particles, and the chunks they add to the key.
+__device__ __forceinline__ uint2 total_delta(uint32_t base, uint32_t m, uint32_t cap) {
+    return make_uint2(m, chunks_of(base + m, cap) - chunks_of(base, cap));
+}
+
+// count_rank on the global counters of the next binning, also adding each
+// run's particles to the super-segment totals of the scan.
+__device__ __forceinline__ uint32_t count_rank_totals(const BatchArgs &a, uint32_t key, bool active) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t prev_key = __shfl_up(key, 1, kWave);
+    const int prev_active = __shfl_up(active ? 1 : 0, 1, kWave);
+    const bool head = lane == 0 || !active || !prev_active || prev_key != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t le = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes <= this
+    const uint64_t above = heads & ~le;
+    const uint32_t end = above ? (uint32_t)__ffsll((unsigned long long)above) - 1 : (uint32_t)kWave;
+    uint32_t base = 0;
+    if (head && active) {
+        const uint32_t n = end - (uint32_t)lane;
+        const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
+        base = atomicAdd(&a.count[key], n);
+        const uint2 d = total_delta(base, n, cap);
+        atomicAdd(&a.sup_acc[key >> kSupBits].x, d.x);
+        if (d.y) atomicAdd(&a.sup_acc[key >> kSupBits].y, d.y);
+    }
+    const int my_head = 63 - __clzll((unsigned long long)(heads & le));  // lane 0 is a head
+    return __shfl(base, my_head, kWave) + (uint32_t)(lane - my_head);
+}
+
 __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
 }
@@ -416,18 +456,12 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *total) {
     return wbase + inc - v;
 }
 
-__device__ __forceinline__ uint32_t chunks_of(uint32_t count, uint32_t cap) {
-    return (count + cap - 1) / cap;
-}
-
-// The scan runs one wave per 64 keys.  Its prefix comes from totals of
-// (particles, tile chunks) per segment of 16 keys, written by the gather
-// kernel (one block per segment), and their sums per super-segment of 4096
-// keys, accumulated with atomics and double-buffered by substep parity: a
-// scan clears the other parity's, which the next substep fills.
-constexpr uint32_t kSegBits = 4;
-constexpr uint32_t kSegKeys = 1u << kSegBits;
-constexpr uint32_t kSupBits = 12;
+// The scan's prefix comes from totals of (particles, tile chunks) per
+// super-segment of 256 keys, double-buffered by substep parity: a scan
+// clears the other parity's, which the next k_bin_gather (or, with
+// count_atomic, this substep's g2p) fills with atomics, plus the counts of the
+// super-segment's keys before the block's.
+constexpr uint32_t kSegKeys = 1u << kSegBits;  // keys per k_bin_gather block
 constexpr uint32_t kSegPerSup = 1u << (kSupBits - kSegBits);
 
 // One block of 4 waves per 64 keys.  All waves first check whether a
@@ -466,7 +500,7 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
     }
     __syncthreads();
     if (part != 0) return;
-    const uint32_t sp = k0 >> kSupBits, sg = k0 >> kSegBits;  // first segment of the block
+    const uint32_t sp = k0 >> kSupBits;  // super-segment of the block
     uint32_t ps = 0, pc = 0;  // totals of all earlier keys
     for (uint32_t i0 = 0; i0 < sp; i0 += kWave) {
         const uint32_t i = i0 + (uint32_t)lane;
@@ -476,12 +510,12 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
             pc += v.y;
         }
     }
-    for (uint32_t i0 = sp * kSegPerSup; i0 < sg; i0 += kWave) {
+    for (uint32_t i0 = sp << kSupBits; i0 < k0; i0 += kWave) {  // <= 3 rounds
         const uint32_t i = i0 + (uint32_t)lane;
-        if (i < sg) {
-            const uint2 v = b.seg[i];
-            ps += v.x;
-            pc += v.y;
+        if (i < k0) {
+            const uint32_t v = b.count[i];
+            ps += v;
+            pc += chunks_of(v, cap);
         }
     }
     ps = wave_sum(ps);
@@ -496,7 +530,9 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
             const uint32_t sc = key_scene_of(a, k);
             for (uint32_t i = 0; i < nch; ++i) {
                 const uint32_t z = off + i * cap, zn = min(cap, c - i * cap);
-                if (cb + i < b.tchunk_cap) b.tchunks[cb + i] = make_uint4(sc, xm, z, zn);
+                // (the slot bound only guards the buffers against inconsistent totals)
+                if (cb + i < b.tchunk_cap && z + zn <= a.total_slots)
+                    b.tchunks[cb + i] = make_uint4(sc, xm, z, zn);
                 // XCD ranges: slot s_x = ceil(x * total / kXcds) starts range x;
                 // the chunk holding slot s_x - 1 is the last of range x - 1
                 // (chunks tile the slots [0, total) in order).
@@ -521,6 +557,7 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
         const bool wild = !sd->fast || b.count[sd->koff + sd->ntiles] != 0;
         a.near[sd->soff + (k - sd->koff)] = (c != 0 || s_any[lane] != 0 || wild) ? 1 : 0;
     }
+    // clear the next binning's totals (g2p adds to them)
     if (lane == 0 && (k0 & ((1u << kSupBits) - 1)) == 0) b.sup_next[sp] = make_uint2(0u, 0u);
 }
 
@@ -561,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_binned(BatchArgs a, float *ds
 // global atomics (already in count[T]).  Each (chunk, direction) entry
 // belongs to exactly one destination tile.  With `final_counts` (after
 // k_bin_count) or for wild keys the count is already complete.  The block
-// then writes the segment's totals for the scan.
+// then adds its keys' totals to their super-segment's.
 __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, BinBuffers bb,
                                                                  uint32_t cap, int final_counts) {
     __shared__ uint32_t s_cnt[kSegKeys], s_nch[kSegKeys];
@@ -616,7 +653,6 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, Bi
             ts += s_cnt[i];
             tc += s_nch[i];
         }
-        bb.seg[sg] = make_uint2(ts, tc);
         if (ts) {
             atomicAdd(&bb.sup[sg / kSegPerSup].x, ts);
             atomicAdd(&bb.sup[sg / kSegPerSup].y, tc);
@@ -1005,13 +1041,16 @@ __device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, uint32
 // Near moves rank within (chunk c, direction) through the block's LDS
 // counters; wild ones too, finished after the chunk (wild_rank); far moves
 // take a final rank from a global atomic on their key.
+template <bool kAtomic>
 __device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, uint32_t c,
                                           uint32_t i, uint32_t key, uint32_t dir, bool active) {
     const bool local = active && dir <= kDirWild;
     const uint32_t lr = count_rank(lcnt, dir, local);
     uint32_t fr = 0;
-    if (__ballot(active && dir == kDirFar))  // rare: a particle left its tile's neighbourhood
-        fr = count_rank(a.count, key, active && dir == kDirFar);
+    if (__ballot(active && dir == kDirFar)) {  // rare: a particle left its tile's neighbourhood
+        const bool far = active && dir == kDirFar;
+        fr = kAtomic ? count_rank_totals(a, key, far) : count_rank(a.count, key, far);
+    }
     if (active) {
         a.key[i] = key;
         a.rank[i] = local ? lr : fr;
@@ -1019,21 +1058,66 @@ __device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, ui
     }
 }
 
-// After a chunk: publish its near counts, reserve the wild particles' ranks
-// with one atomic, and finish their ranks.  lcnt[kDirWild + 1] carries the
-// reserved base between the two barriers.
+// After a chunk, with count_atomic: 27 lanes add its near counts to the
+// destination keys' counters (the old values are the groups' offsets within
+// the destination tiles, kept in mig), one lane reserves the wild particles'
+// ranks; their deltas go to the super-segment totals of the next binning,
+// summed over the lanes first (few super-segments, many chunks).  Without
+// it: the counts go to mig for k_bin_gather, and only the wild particles'
+// ranks are reserved here.
+// lcnt[kDirWild + 1] carries the wild base between the two barriers.  b is
+// the chunk's spatial tile.
+template <bool kAtomic>
 __device__ __forceinline__ void finish_chunk_counts(const BatchArgs &a, const SceneDesc *sd,
-                                                    uint32_t *lcnt, uint32_t c) {
+                                                    uint32_t *lcnt, uint32_t c, uint32_t b) {
     __syncthreads();
-    if (threadIdx.x < kDirs) a.mig[(size_t)c * kDirs + threadIdx.x] = lcnt[threadIdx.x];
-    if (threadIdx.x == kDirWild) {
-        const uint32_t nw = lcnt[kDirWild];
-        lcnt[kDirWild + 1] = nw ? atomicAdd(&a.count[sd->koff + sd->ntiles], nw) : 0u;
+    if (!kAtomic) {  // the counts go to k_bin_gather, which makes them offsets
+        if (threadIdx.x < kDirs) a.mig[(size_t)c * kDirs + threadIdx.x] = lcnt[threadIdx.x];
+        if (threadIdx.x == kDirWild) {
+            const uint32_t nw = lcnt[kDirWild];
+            lcnt[kDirWild + 1] = nw ? atomicAdd(&a.count[sd->koff + sd->ntiles], nw) : 0u;
+        }
+    } else if (threadIdx.x < kWave) {
+        const uint32_t lane = threadIdx.x;
+        const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
+        uint32_t key = kNoTile, m = 0;
+        if (lane < kDirs) {
+            m = lcnt[lane];
+            if (m) key = a.nbr[(size_t)b * kDirs + lane];  // the tile the group moved to
+        } else if (lane == kDirWild) {
+            m = lcnt[kDirWild];
+            key = sd->koff + sd->ntiles;  // the scene's wild key
+        }
+        const bool has = m != 0 && key != kNoTile;
+        uint32_t base = 0;
+        uint2 d = make_uint2(0u, 0u);
+        if (has) {
+            base = atomicAdd(&a.count[key], m);
+            d = total_delta(base, m, cap);
+        }
+        if (lane < kDirs) a.mig[(size_t)c * kDirs + lane] = base;
+        if (lane == kDirWild) lcnt[kDirWild + 1] = base;
+        // super-segment totals: one atomic per distinct super-segment
+        const uint32_t sp = has ? key >> kSupBits : 0xffffffffu;
+        uint64_t pending = __ballot(has);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t lsp = __shfl(sp, leader, kWave);
+            const bool mine = has && sp == lsp;
+            const uint32_t tp = wave_sum(mine ? d.x : 0u), tc = wave_sum(mine ? d.y : 0u);
+            if ((int)lane == leader) {
+                atomicAdd(&a.sup_acc[lsp].x, tp);
+                if (tc) atomicAdd(&a.sup_acc[lsp].y, tc);
+            }
+            pending &= ~__ballot(mine);
+        }
     }
     __syncthreads();
 }
 
-template <int PPT>
+// kAtomic = a.count_atomic (a template parameter, so that each variant keeps
+// only its own counting code in the register budget).
+template <int PPT, bool kAtomic>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
 void k_g2p(BatchArgs a) {
     __shared__ float4 lvel[kWinCells];
@@ -1081,10 +1165,11 @@ void k_g2p(BatchArgs a) {
                 key = bin_key(a, sd, xo, t, &dir);
             }
             wild[k] = valid[k] && dir == kDirWild;
-            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, valid[k]);
+            if (a.count_next) rank_next<kAtomic>(a, lcnt, w.c, i, key, dir, valid[k]);
         }
         if (a.count_next) {
-            finish_chunk_counts(a, sd, lcnt, w.c);
+            finish_chunk_counts<kAtomic>(a, sd, lcnt, w.c,
+                                sd->soff + a.tile_rank[sd->rank_off + ch.y]);
 #pragma unroll
             for (int k = 0; k < PPT; ++k)
                 if (wild[k]) {
@@ -1121,7 +1206,7 @@ void k_g2p(BatchArgs a) {
                 key = bin_key(a, sd, xo, nowhere, &dir);
                 dir = kDirFar;
             }
-            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, active);
+            if (a.count_next) rank_next<kAtomic>(a, lcnt, w.c, i, key, dir, active);
         }
     }
 }
@@ -1160,7 +1245,12 @@ hipError_t launch_ppt(int which, const BatchArgs &a, hipStream_t st) {
     switch (which) {
     case 1: launch_timed(k_p2g1<PPT>, nb, kBlock, st, a); break;
     case 2: launch_timed(k_p2g2<PPT>, nb, kBlock, st, a); break;
-    case 3: launch_timed(k_g2p<PPT>, nb, kBlock, st, a); break;
+    case 3:
+        if (a.count_atomic)
+            launch_timed(k_g2p<PPT, true>, nb, kBlock, st, a);
+        else
+            launch_timed(k_g2p<PPT, false>, nb, kBlock, st, a);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1271,6 +1361,7 @@ void scene_key_geometry(SceneDesc &d) {
 
 hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count,
                              hipStream_t st) {
+    if (!count && a.count_atomic) return hipSuccess;  // g2p left counts, offsets and totals
     const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
     const unsigned nseg = blocks_for(b.nkeys, kSegKeys);
     if (count) {
