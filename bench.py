@@ -198,8 +198,8 @@ def measured_traffic(slot: str):
     try:
         for line in open(TRAFFIC_FILE):
             t = line.split()
-            if len(t) == 3 and t[0] == "TRAFFIC" and t[1] == kernel:
-                return float(t[2])
+            if len(t) >= 3 and t[0] == "TRAFFIC" and " ".join(t[1:-1]) == kernel:
+                return float(t[-1])
     except OSError:
         return None
     return None
