@@ -32,8 +32,10 @@ def stage_bytes(name: str, n: int, g: int, binned_path: bool = True) -> int:
     """Algorithmic HBM bytes of one launch of each kernel.
 
     Particle fields: x 12 B, v 12 B, C 36 B; cells 16 B = (vx, vy, vz, mass).
-    These are the minimum bytes the data dependencies force per kernel; the
-    reference's six kernels total 204 B/particle + 84 B/cell per substep.
+    These are the minimum bytes the data dependencies force per kernel; with
+    clearGrid fused into p2g_1's flush and copyPosition into g2p (which then
+    need not re-read x and v), the reference's substep needs 204 B/particle +
+    84 B/cell.
     The binned path's intermediates (tile partials, bin keys) are not
     counted: they are overhead the roofline fraction should expose.
     """

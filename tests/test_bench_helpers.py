@@ -1,0 +1,77 @@
+"""CPU checks of bench.py's measurement helpers: the per-kernel byte model,
+the profile-summary parsers and the one-GPU contract."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_reference_stage_bytes_give_the_fused_substep_minimum():
+    # clearGrid fused into p2g_1's flush, copyPosition into g2p (no re-read of
+    # x and v): 204 B per particle + 84 B per cell
+    def part(n, g, kernels):
+        return sum(bench.stage_bytes(k, n, g, binned_path=False) for k in kernels)
+    per_particle = part(1, 0, ("p2g_1", "p2g_2", "g2p", "copy_position")) - 24
+    per_cell = part(0, 1, ("p2g_1", "p2g_2", "update_grid", "g2p"))
+    assert (per_particle, per_cell) == (204, 84)
+
+
+def test_every_production_slot_has_bytes_and_a_kernel():
+    for slot in ("bin_gather", "bin_scan", "bin_scatter", "p2g_1", "reduce_1", "p2g_2",
+                 "reduce_2", "g2p"):
+        assert bench.stage_bytes(slot, 300_000, 512_000) > 0
+        assert slot in bench.SLOT_KERNEL
+
+
+def test_committed_profile_gives_the_dominant_kernels_traffic():
+    t = bench.measured_traffic("g2p")
+    assert t is not None and t > bench.stage_bytes("g2p", 300_000, 512_000)
+
+
+def _write_counters(path, rows):
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, "run_counter_collection.csv"), "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n")
+        for i, (k, c, v) in enumerate(rows):
+            f.write(f'{i},"{k}",{c},{v}\n')
+
+
+def test_summary_scales_counters_by_the_calibration_copy(tmp_path):
+    trace = tmp_path / "trace"
+    trace.mkdir()
+    (trace / "run_kernel_stats.csv").write_text(
+        'Name,Calls,TotalDurationNs,AverageNs,Percentage\n'
+        '"void mpm::(anonymous namespace)::k_g2p<2, false>(mpm::BatchArgs)",10,200000,20000,100.0\n')
+    kib = bench.CALIBRATION_BYTES // 1024
+    copy = "__amd_rocclr_copyBuffer"
+    _write_counters(str(tmp_path / "fetch"), [
+        (copy, "FETCH_SIZE", kib // 2),
+        ("void mpm::(anonymous namespace)::k_g2p<2, false>(mpm::BatchArgs)", "FETCH_SIZE", 100)])
+    _write_counters(str(tmp_path / "write"), [
+        (copy, "WRITE_SIZE", kib),
+        ("void mpm::(anonymous namespace)::k_g2p<2, false>(mpm::BatchArgs)", "WRITE_SIZE", 300)])
+    out = bench.summarize_profiles(str(trace), str(tmp_path / "fetch"), str(tmp_path / "write"))
+    line = [l for l in out.splitlines() if l.startswith("TRAFFIC")][0]
+    # FETCH reads half the true bytes on the copy, WRITE all of them
+    assert line.split()[1:-1] == ["k_g2p<2,", "false>"]
+    assert float(line.split()[-1]) == pytest.approx((100 / 0.5 + 300) * 1024)
+
+
+def test_counter_table_averages_per_kernel(tmp_path):
+    _write_counters(str(tmp_path / "c"), [("k_a", "SQ_WAVES", 4), ("k_a", "SQ_WAVES", 8),
+                                          ("k_b", "SQ_WAVES", 1)])
+    rows = bench.counter_table([str(tmp_path / "c")]).splitlines()
+    assert rows[0].split() == ["kernel", "SQ_WAVES"]
+    assert float(rows[1].split()[1]) == 6.0 and rows[1].split()[0] == "k_a"
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stderr
