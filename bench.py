@@ -206,7 +206,6 @@ def measured_traffic(slot: str):
     except OSError:
         return None
     return None
-    return None
 
 
 def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
@@ -434,6 +433,17 @@ def main() -> int:
         },
         "stages_us_per_launch": per_launch_us,
         "stages_us_per_substep": per_substep_us,
+        # every kernel against the HBM peak (G2P, updateGrid = reduce_2, and
+        # the P2G flushes = reduce_1 / reduce_2 included)
+        "stages_roofline": {
+            k: {
+                "algorithmic_bytes": stage_bytes(k, n, g, tuning["path"] == 0),
+                "achieved_GBs": stage_bytes(k, n, g, tuning["path"] == 0) / (us * 1e-6) / 1e9,
+                "frac": stage_bytes(k, n, g, tuning["path"] == 0) / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                "traffic": measured_traffic(k) if tuning == DEFAULT_TUNING else None,
+            }
+            for k, us in per_launch_us.items()
+        },
         "tuning": tuning,
         "substep_roofline": {
             "bytes": total_bytes,
