@@ -464,42 +464,44 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *total) {
 constexpr uint32_t kSegKeys = 1u << kSegBits;  // keys per k_bin_gather block
 constexpr uint32_t kSegPerSup = 1u << (kSupBits - kSegBits);
 
-// One block of 4 waves per 64 keys.  All waves first check whether a
-// particle is within reach of each key's tile (its own, or a quarter of its
-// 27 neighbours' counts each); then the first wave computes the key offsets,
-// the keys' tile-chunk ranges and the tile chunks themselves, and clears the
-// wild keys' counts for the next pass.
+// One block of 4 waves per 64 keys.  Wave 0 computes the key offsets, the
+// keys' tile-chunk ranges and the tile chunks themselves; meanwhile waves
+// 1..3 check, for each empty tile key, whether a particle is within reach
+// (a third of its 27 neighbours' counts each).  After one barrier wave 0
+// writes the reach flags and clears the next pass's totals.
 constexpr int kScanWaves = 4;
+constexpr int kReachWaves = kScanWaves - 1;
+constexpr int kReachPer = (kDirs + kReachWaves - 1) / kReachWaves;
 
 __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, BinBuffers b,
                                                                   uint32_t cap) {
-    __shared__ uint32_t s_any[kWave];
+    __shared__ uint32_t s_any[kReachWaves][kWave];
     const int lane = threadIdx.x & (kWave - 1), part = threadIdx.x / kWave;
     const uint32_t k0 = blockIdx.x * kWave, k = k0 + (uint32_t)lane;
     const bool in = k < b.nkeys;
     const uint32_t c = in ? b.count[k] : 0u;
     const uint32_t xm = in ? a.tile_xmaj[k] : 0u;
-    if (part == 0) s_any[lane] = 0u;
-    __syncthreads();
-    if (in && xm != kWildTile && c == 0) {
-        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
-        if (sd->fast) {
-            const uint32_t *nb = a.nbr + (size_t)(sd->soff + (k - sd->koff)) * kDirs;
-            uint32_t nk[(kDirs + kScanWaves - 1) / kScanWaves];
+    if (part != 0) {
+        bool any = false;
+        if (in && xm != kWildTile && c == 0) {
+            const SceneDesc *sd = a.scenes + key_scene_of(a, k);
+            if (sd->fast) {
+                const uint32_t *nb = a.nbr + (size_t)(sd->soff + (k - sd->koff)) * kDirs;
+                uint32_t nk[kReachPer];
 #pragma unroll
-            for (int j = 0; j < (int)((kDirs + kScanWaves - 1) / kScanWaves); ++j) {
-                const uint32_t d = (uint32_t)(part + kScanWaves * j);
-                nk[j] = d < kDirs ? nb[d] : kNoTile;
+                for (int j = 0; j < kReachPer; ++j) {
+                    const uint32_t d = (uint32_t)((part - 1) + kReachWaves * j);
+                    nk[j] = d < kDirs ? nb[d] : kNoTile;
+                }
+#pragma unroll
+                for (int j = 0; j < kReachPer; ++j)
+                    any = any || (nk[j] != kNoTile && b.count[nk[j]] != 0);
             }
-            bool any = false;
-#pragma unroll
-            for (int j = 0; j < (int)((kDirs + kScanWaves - 1) / kScanWaves); ++j)
-                any = any || (nk[j] != kNoTile && b.count[nk[j]] != 0);
-            if (any) s_any[lane] = 1u;
         }
+        s_any[part - 1][lane] = any ? 1u : 0u;
+        __syncthreads();
+        return;
     }
-    __syncthreads();
-    if (part != 0) return;
     const uint32_t sp = k0 >> kSupBits;  // super-segment of the block
     uint32_t ps = 0, pc = 0;  // totals of all earlier keys
     for (uint32_t i0 = 0; i0 < sp; i0 += kWave) {
@@ -517,6 +519,12 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
             ps += v;
             pc += chunks_of(v, cap);
         }
+    }
+    // a scene with wild particles (or a non-integral box) needs every tile visited
+    bool wild = false;
+    if (in && xm != kWildTile) {
+        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
+        wild = !sd->fast || b.count[sd->koff + sd->ntiles] != 0;
     }
     ps = wave_sum(ps);
     pc = wave_sum(pc);
@@ -549,16 +557,18 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
             b.xcd_first[kXcds] = cb + nch;
         }
     }
-    // Every tile key writes its reach flag; a scene with wild particles (or
-    // a non-integral box) needs every tile visited.  Counts stay readable
-    // until reduce_2 clears them.
-    if (in && xm != kWildTile) {
-        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
-        const bool wild = !sd->fast || b.count[sd->koff + sd->ntiles] != 0;
-        a.near[sd->soff + (k - sd->koff)] = (c != 0 || s_any[lane] != 0 || wild) ? 1 : 0;
-    }
     // clear the next binning's totals (g2p adds to them)
     if (lane == 0 && (k0 & ((1u << kSupBits) - 1)) == 0) b.sup_next[sp] = make_uint2(0u, 0u);
+    __syncthreads();
+    // Every tile key writes its reach flag.  Counts stay readable until
+    // reduce_2 clears them.
+    if (in && xm != kWildTile) {
+        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
+        bool any = c != 0 || wild;
+#pragma unroll
+        for (int w = 0; w < kReachWaves; ++w) any = any || s_any[w][lane] != 0;
+        a.near[sd->soff + (k - sd->koff)] = any ? 1 : 0;
+    }
 }
 
 // Writes the permutation of the binning: perm[d] = slot of the particle that
