@@ -105,6 +105,35 @@ def test_config1_full_size_substeps_bit_exact():
         assert n * 10**7 - 27 * n <= mass <= n * 10**7
 
 
+def test_rollouts_while_the_host_is_busy_match_the_oracle():
+    """Regression: many-substep calls made while every host core is busy (the
+    oracle's own threads) must still give the oracle's result.  A counting
+    variant that published the next binning's counts with atomics from g2p
+    failed exactly this way (lost particles in most such rollouts)."""
+    import threading
+
+    rec, box = scenes.config1()
+    substeps = 200
+    ref = oracle.Scene(rec, box)
+    worker = threading.Thread(target=ref.execute, args=(substeps, 16))
+    worker.start()
+    finals = []
+    try:
+        while worker.is_alive() or not finals:
+            with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+                for _ in range(substeps // 20):
+                    sim.execute(20)
+                    sim.synchronize()
+                finals.append(sim.read_particles())
+            if len(finals) >= 12:
+                break
+    finally:
+        worker.join()
+    for k, p in enumerate(finals):
+        bad = np.nonzero(~np.all(bits_equal(p, ref.particles), axis=1))[0]
+        assert bad.size == 0, f"rollout {k} of {len(finals)}: {bad.size} particles differ"
+
+
 def test_config2_two_million_mass_and_one_substep():
     rec, box = scenes.config2()
     sim, ref = pair(rec, box)

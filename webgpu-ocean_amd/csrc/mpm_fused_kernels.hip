@@ -313,8 +313,8 @@ struct GridSinkG {  // g2p straight from the decoded velocities
 // LDS, how many of its chunk's particles moved to each of the 27 tiles around
 // the chunk's tile ("mig", 27 per chunk) and gives each particle its rank
 // within its (chunk, direction).  Two ways turn the counts into offsets
-// within the destination tiles (count_atomic, chosen from the ratio of keys
-// to particles):
+// within the destination tiles (count_atomic; the host currently never
+// selects the second: rollouts made with it lost particles now and then):
 //  - k_bin_gather visits every key and sums, in a fixed order, the counts of
 //    the chunks that can move into it (no atomics; cost grows with the keys);
 //  - with count_atomic, g2p itself adds each (chunk, direction) count to the
