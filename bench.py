@@ -179,7 +179,7 @@ def counter_table(dirs) -> str:
     return "\n".join(out)
 
 
-DEFAULT_TUNING = {"path": 0, "particles_per_thread": 2, "grid_blocks": 0, "xcd_remap": 1}
+DEFAULT_TUNING = {"path": 0, "particles_per_thread": 2, "grid_blocks": 0, "xcd_remap": 0}
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "substep_kernels_dambreak_300k.txt")
