@@ -152,7 +152,8 @@ int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls
  *   grid_blocks          persistent blocks of the chunk kernels, 0 = 4096,
  *                        at most 2^20
  *   xcd_remap            1 = each XCD walks a contiguous range of chunks,
- *                        0 = chunks dealt round robin over the XCDs (default)
+ *                        0 = runs of 16 consecutive chunks dealt round robin
+ *                            over the XCDs (default)
  * Results do not depend on these (integer adds commute; per-particle float
  * math is unchanged); only speed does. */
 typedef struct {

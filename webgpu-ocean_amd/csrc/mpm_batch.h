@@ -14,7 +14,7 @@ struct Tuning {
     int path = 0;          // 0: binned tile-partial kernels, 1: reference-shaped stage kernels
     int ppt = 2;           // particles per thread: tile chunks of <= 256 * ppt particles
     int grid_blocks = 0;   // persistent blocks of the chunk kernels (0: 4096)
-    int xcd_remap = 0;     // 1: contiguous chunk ranges per XCD (0: round robin)
+    int xcd_remap = 0;     // 1: contiguous chunk ranges per XCD (0: runs of 16, round robin)
 };
 
 // Uniforms of a scene from its init and real boxes.

@@ -137,11 +137,15 @@ __device__ __forceinline__ uint32_t key_scene_of(const BatchArgs &a, uint32_t k)
     return a.nscenes == 1 ? 0u : a.key_scene[k];
 }
 
-// Persistent walk over the device-side tile-chunk list.  With xcd_remap the
-// blocks of one XCD (block id mod 8) take a contiguous range of chunks, so
-// neighbouring tiles' windows meet in the same L2; the ranges hold equal
-// shares of the particles (chunk fill varies a lot between the body of the
-// fluid and its surface), as the scan split them.
+// Persistent walk over the device-side tile-chunk list.  Blocks are
+// dispatched round robin over the XCDs (block id mod 8).  By default the
+// chunks are dealt to the XCDs in runs of kXcdRun consecutive (Hilbert-
+// ordered, so neighbouring) chunks: neighbouring windows share an L2 within
+// a run while every XCD gets an even mix of the fluid's dense body and
+// sparse surface.  With xcd_remap each XCD takes one contiguous range of
+// chunks holding an equal share of the particles, as the scan split them.
+constexpr uint32_t kXcdRun = 16;
+
 struct TileWalk {
     uint32_t c, end, step;
 };
@@ -156,6 +160,10 @@ __device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
         w.step = gridDim.x / kXcds;
     } else {
         w.c = blockIdx.x;
+        if ((gridDim.x % (kXcds * kXcdRun)) == 0) {  // a bijection of [0, gridDim)
+            const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+            w.c = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
+        }
         w.end = n;
         w.step = gridDim.x;
     }
