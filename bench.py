@@ -317,6 +317,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched and adjoint comparisons")
+    ap.add_argument("--extras-only", action="store_true",
+                    help="run only the batched and adjoint comparisons (one JSON line; "
+                         "for profiling them)")
     ap.add_argument("--stage-substeps", type=int, default=0,
                     help="substeps of the per-kernel timing replay (default: all of the timed "
                          "frames)")
@@ -349,6 +352,10 @@ def main() -> int:
 
     if args.calibrate:
         calibration_copy()
+
+    if args.extras_only:
+        print(json.dumps({"batched": bench_batched(), "adjoint": bench_adjoint()}), flush=True)
+        return 0
 
     rec, box = scenes.config1(n=args.particles)
     n = rec.shape[0]
