@@ -1077,10 +1077,13 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
             keys_ready_ = false;
             return hipMemcpyAsync(id_, src + sbytes, ibytes, hipMemcpyDeviceToDevice, st);
         };
-        // forward rollout with checkpoints every K substeps
+        // forward rollout with checkpoints every K substeps; the last segment
+        // is taped on the way, so the backward pass need not recompute it
+        const uint32_t last0 = (nseg - 1) * K;
         for (uint32_t t = 0; t < T && e == hipSuccess; ++t) {
             if (t % K == 0) e = save(ckpt + per_ck * (t / K));
-            if (e == hipSuccess) e = substep(st);
+            if (e == hipSuccess)
+                e = t >= last0 ? substep_taped(st, tape + L.bytes * (t - last0), &L) : substep(st);
         }
         if (e != hipSuccess) break;
         if ((e = save(ckpt + per_ck * nseg)) != hipSuccess) break;  // final state
@@ -1134,9 +1137,11 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         const unsigned nchunk_blocks = 2048;
         for (int32_t sgi = (int32_t)nseg - 1; sgi >= 0 && e == hipSuccess; --sgi) {
             const uint32_t t0 = (uint32_t)sgi * K, t1 = std::min(T, t0 + K);
-            if ((e = restore(ckpt + per_ck * sgi)) != hipSuccess) break;
-            for (uint32_t t = t0; t < t1 && e == hipSuccess; ++t)
-                e = substep_taped(st, tape + L.bytes * (t - t0), &L);
+            if (sgi != (int32_t)nseg - 1) {  // (the last segment's tape is the forward's)
+                if ((e = restore(ckpt + per_ck * sgi)) != hipSuccess) break;
+                for (uint32_t t = t0; t < t1 && e == hipSuccess; ++t)
+                    e = substep_taped(st, tape + L.bytes * (t - t0), &L);
+            }
             for (int32_t t = (int32_t)t1 - 1; t >= (int32_t)t0 && e == hipSuccess; --t) {
                 const char *rec = tape + L.bytes * (t - t0);
                 q.st = reinterpret_cast<const float *>(rec + L.state);
