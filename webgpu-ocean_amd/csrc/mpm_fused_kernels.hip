@@ -69,21 +69,30 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 // consecutive tiles, per block: fewer, larger blocks dispatch faster.
 constexpr uint32_t kTileWaves = 4;
 
-// Blocks of a one-wave-per-tile launch, padded to a multiple of 8 (tiles >=
-// stiles are padding).
+// Runs of consecutive work items dealt round robin over the XCDs (see
+// tile_walk).
+constexpr uint32_t kXcdRun = 16;
+
+// Blocks of a one-wave-per-tile launch, padded to a multiple of 8 runs
+// (tiles >= stiles are padding).
 __host__ __device__ __forceinline__ uint32_t tile_blocks(uint32_t stiles) {
     const uint32_t nb = (stiles + kTileWaves - 1) / kTileWaves;
-    return (nb + kXcds - 1) / kXcds * kXcds;
+    return (nb + kXcds * kXcdRun - 1) / (kXcds * kXcdRun) * (kXcds * kXcdRun);
 }
 
-// Spatial tile of this wave: with xcd_remap the blocks of one XCD take a
-// contiguous range of tiles, so neighbouring partials meet in one L2.
+// Spatial tile of this wave.  The dispatcher puts block b on XCD b mod 8;
+// blocks take Hilbert-consecutive tiles in runs of kXcdRun blocks per XCD
+// turn, so neighbouring partial windows mostly meet in one L2 while every
+// XCD gets a share of the fluid's dense and sparse regions (with xcd_remap:
+// one contiguous range of tiles per XCD).  Both are bijections of the
+// padded block range.
 __device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
-    uint32_t blk = blockIdx.x;
-    if (a.xcd_remap) {
-        const uint32_t per = tile_blocks(a.stiles) / kXcds;
-        blk = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-    }
+    const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    uint32_t blk;
+    if (a.xcd_remap)
+        blk = x * (tile_blocks(a.stiles) / kXcds) + j;
+    else
+        blk = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
     return blk * kTileWaves + threadIdx.x / kWave;
 }
 
@@ -144,8 +153,6 @@ __device__ __forceinline__ uint32_t key_scene_of(const BatchArgs &a, uint32_t k)
 // a run while every XCD gets an even mix of the fluid's dense body and
 // sparse surface.  With xcd_remap each XCD takes one contiguous range of
 // chunks holding an equal share of the particles, as the scan split them.
-constexpr uint32_t kXcdRun = 16;
-
 struct TileWalk {
     uint32_t c, end, step;
 };
