@@ -569,7 +569,9 @@ __device__ __forceinline__ void adj_tile_coords(const AdjArgs &q, uint32_t xm, i
     t[2] = (int)(xm % n2);
 }
 
-// Persistent walk over the taped tile chunks (XCD-contiguous ranges).
+// Persistent walk over the taped tile chunks: runs of 16 consecutive chunks
+// dealt round robin over the XCDs (block b runs on XCD b mod 8), or with
+// xcd_remap one contiguous range per XCD.
 struct AdjWalk {
     uint32_t c, end, step;
 };
@@ -583,6 +585,10 @@ __device__ __forceinline__ AdjWalk adj_walk(const AdjArgs &q) {
         w.step = gridDim.x / 8;
     } else {
         w.c = blockIdx.x;
+        if ((gridDim.x % (8 * 16)) == 0) {  // a bijection of [0, gridDim)
+            const uint32_t x = blockIdx.x % 8, j = blockIdx.x / 8;
+            w.c = ((j / 16) * 8 + x) * 16 + j % 16;
+        }
         w.end = n;
         w.step = gridDim.x;
     }
