@@ -134,18 +134,23 @@ def summarize_profiles(trace_dir: str, fetch_dir: str = "", write_dir: str = "")
     out.append(f"calibration: 64 MiB copy read FETCH_SIZE x{fcal:.3f} and WRITE_SIZE x{wcal:.3f} "
                "of its true size" if fcal and wcal else "calibration: none")
     out.append(f"{'kernel':28s} {'calls':>6s} {'avg_us':>8s} {'pct':>6s} {'FETCH_KiB':>10s} "
-               f"{'WRITE_KiB':>10s} {'HBM_bytes':>12s}")
+               f"{'WRITE_KiB':>10s} {'HBM_bytes':>12s} {'GB/s':>7s} {'of_peak':>7s}")
     for r in rows:
         k = _short_kernel(r["Name"])
         if k.startswith("__amd"):
             continue
         f, w = fetch.get(k), write.get(k)
-        hbm = ""
+        hbm, rate, frac = "", "", ""
+        avg_us = float(r["AverageNs"]) / 1e3
         if f is not None and w is not None and fcal and wcal:
-            hbm = f"{(f / fcal + w / wcal) * 1024:.4g}"
-        out.append(f"{k[:28]:28s} {r['Calls']:>6s} {float(r['AverageNs']) / 1e3:8.2f} "
+            nbytes = (f / fcal + w / wcal) * 1024
+            hbm = f"{nbytes:.4g}"
+            if avg_us > 0:  # counted bytes over the kernel's average duration
+                rate = f"{nbytes / avg_us / 1e3:.0f}"
+                frac = f"{nbytes / avg_us / 1e3 / HBM_PEAK_GBS:.3f}"
+        out.append(f"{k[:28]:28s} {r['Calls']:>6s} {avg_us:8.2f} "
                    f"{float(r['Percentage']):6.2f} {'' if f is None else f'{f:.0f}':>10s} "
-                   f"{'' if w is None else f'{w:.0f}':>10s} {hbm:>12s}")
+                   f"{'' if w is None else f'{w:.0f}':>10s} {hbm:>12s} {rate:>7s} {frac:>7s}")
         if hbm:
             out.append(f"TRAFFIC {k} {hbm}")
     return "\n".join(out)
