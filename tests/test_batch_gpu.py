@@ -119,3 +119,36 @@ def test_large_key_space_scan_matches_oracle():
         for s, r in enumerate(refs):
             assert np.all(bits_equal(bt.read_posvel(s), r.posvel)), f"scene {s} posvel"
             assert np.all(bits_equal(bt.read_particles(s), r.particles)), f"scene {s} state"
+
+
+def test_batched_runs_while_the_host_is_busy_are_identical():
+    """Regression companion of the single-scene host-load test: 16 uneven
+    scenes of the 64-scene mix, stepped in 20-substep calls while the oracle
+    keeps every host core busy, give the same particles on every run."""
+    import threading
+
+    mix = scenes.config3()[:16]
+    rec1, box1 = scenes.config1()
+    burner = oracle.Scene(rec1, box1)
+    stop = threading.Event()
+
+    def burn():
+        while not stop.is_set():
+            burner.substep(threads=16)
+
+    worker = threading.Thread(target=burn)
+    worker.start()
+    runs = []
+    try:
+        for _ in range(3):
+            with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in mix]) as bt:
+                for _ in range(5):
+                    bt.step(20)
+                    bt.synchronize()
+                runs.append([bt.read_particles(s) for s in range(len(mix))])
+    finally:
+        stop.set()
+        worker.join()
+    for k, run in enumerate(runs[1:], 1):
+        for s, (a, b) in enumerate(zip(run, runs[0])):
+            assert np.all(bits_equal(a, b)), f"run {k} scene {s} differs from run 0"
