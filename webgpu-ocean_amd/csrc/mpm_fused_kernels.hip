@@ -616,14 +616,61 @@ __global__ __launch_bounds__(kBlock) void k_gather_binned(BatchArgs a, float *ds
     }
 }
 
-// One block of 16 waves per segment of 16 keys, one wave per key.  For a
-// tile key T: the chunks of the 27 tiles around T (previous binning) that
-// sent particles into T, in a fixed order; their counts in mig become
-// running offsets within T, after the particles that reached T through
-// global atomics (already in count[T]).  Each (chunk, direction) entry
-// belongs to exactly one destination tile.  With `final_counts` (after
-// k_bin_count) or for wild keys the count is already complete.  The block
-// then adds its keys' totals to their super-segment's.
+// Tile key k (rank `rank` in scene sd), whose count so far is `total` (the
+// particles that reached it through global atomics): the chunks of the 27
+// tiles around it (previous binning) that sent particles into it, in a fixed
+// order; their counts in mig become running offsets within the tile.  Each
+// (chunk, direction) entry belongs to exactly one destination tile.  Every
+// lane of the wave calls this with the same key; returns the key's total,
+// which lane 0 also stores.
+__device__ __forceinline__ uint32_t gather_key(const BatchArgs &a, const BinBuffers &bb,
+                                               uint32_t k, const SceneDesc *sd, uint32_t rank,
+                                               uint32_t total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t b = sd->soff + rank;  // spatial tile of key k
+    uint32_t first = 0, cnt = 0, dir = 0;
+    if (lane < (int)kDirs) {
+        const uint32_t s = a.nbr[(size_t)b * kDirs + lane];  // source tile T + d
+        if (s != kNoTile) {
+            const uint2 tc = a.tile_chunk[s];
+            first = tc.x;
+            cnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
+        }
+        dir = kDirs - 1 - lane;  // from the source, T lies at -d
+    }
+    const uint32_t rounds = wave_max(cnt);
+    for (uint32_t j = 0; j < rounds; ++j) {
+        const bool has = j < cnt;
+        const size_t e = (size_t)(first + j) * kDirs + dir;
+        const uint32_t v = has ? a.mig[e] : 0u;
+        const uint32_t inc = wave_incl_scan(v, lane);
+        if (has) a.mig[e] = total + inc - v;
+        total += __shfl(inc, kWave - 1, kWave);
+    }
+    if (lane == 0) bb.count[k] = total;
+    return total;
+}
+
+// Adds a block's key totals to their super-segment's.
+__device__ __forceinline__ void add_segment_totals(const BinBuffers &bb, const uint32_t *s_cnt,
+                                                   const uint32_t *s_nch, uint32_t sup) {
+    uint32_t ts = 0, tc = 0;
+#pragma unroll
+    for (int i = 0; i < (int)kSegKeys; ++i) {
+        ts += s_cnt[i];
+        tc += s_nch[i];
+    }
+    if (ts) {
+        atomicAdd(&bb.sup[sup].x, ts);
+        atomicAdd(&bb.sup[sup].y, tc);
+    }
+}
+
+// Dense key spaces: one block of 16 waves per segment of 16 keys, one wave
+// per key (gather_key when a particle came near the tile last substep).
+// With `final_counts` (after k_bin_count) or for wild keys the count is
+// already complete.  The block then adds its keys' totals to their
+// super-segment's.
 __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, BinBuffers bb,
                                                                  uint32_t cap, int final_counts) {
     __shared__ uint32_t s_cnt[kSegKeys], s_nch[kSegKeys];
@@ -642,47 +689,62 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, Bi
         total = bb.count[k];
         // a tile no particle came near last substep has no source chunk
         const bool sources = rank < sd->ntiles && a.near_prev[sd->soff + rank] != 0;
-        if (!final_counts && sd->fast && rank < sd->ntiles && sources) {
-            const uint32_t b = sd->soff + rank;  // spatial tile of key k
-            uint32_t first = 0, cnt = 0, dir = 0;
-            if (lane < (int)kDirs) {
-                const uint32_t s = a.nbr[(size_t)b * kDirs + lane];  // source tile T + d
-                if (s != kNoTile) {
-                    const uint2 tc = a.tile_chunk[s];
-                    first = tc.x;
-                    cnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
-                }
-                dir = kDirs - 1 - lane;  // from the source, T lies at -d
-            }
-            const uint32_t rounds = wave_max(cnt);
-            for (uint32_t j = 0; j < rounds; ++j) {
-                const bool has = j < cnt;
-                const size_t e = (size_t)(first + j) * kDirs + dir;
-                const uint32_t v = has ? a.mig[e] : 0u;
-                const uint32_t inc = wave_incl_scan(v, lane);
-                if (has) a.mig[e] = total + inc - v;
-                total += __shfl(inc, kWave - 1, kWave);
-            }
-            if (lane == 0) bb.count[k] = total;
-        }
+        if (!final_counts && sd->fast && rank < sd->ntiles && sources)
+            total = gather_key(a, bb, k, sd, rank, total);
     }
     if (lane == 0) {
         s_cnt[w] = total;
         s_nch[w] = chunks_of(total, cap);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && sg < nseg) {
-        uint32_t ts = 0, tc = 0;
-#pragma unroll
-        for (int i = 0; i < (int)kSegKeys; ++i) {
-            ts += s_cnt[i];
-            tc += s_nch[i];
-        }
-        if (ts) {
-            atomicAdd(&bb.sup[sg / kSegPerSup].x, ts);
-            atomicAdd(&bb.sup[sg / kSegPerSup].y, tc);
-        }
+    if (threadIdx.x == 0 && sg < nseg) add_segment_totals(bb, s_cnt, s_nch, sg / kSegPerSup);
+}
+
+// Sparse key spaces (keys far outnumber particle chunks, as in many small
+// scenes): most keys have no source chunk, and one wave per key would spend
+// the launch on waves that do nothing.  Here a block of 16 waves takes one
+// super-segment (256 keys), 16 keys per wave: the wave's lanes first read
+// their keys' counts and near flags in parallel, then the wave gathers only
+// the keys that need it, one after another.  Same results as k_bin_gather.
+constexpr uint32_t kSparseKeysPerWave = 1u << (kSupBits - kSegBits);
+
+__global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather_sparse(BatchArgs a, BinBuffers bb,
+                                                                        uint32_t cap,
+                                                                        int final_counts) {
+    __shared__ uint32_t s_cnt[kSegKeys], s_nch[kSegKeys];
+    const uint32_t nsup = (bb.nkeys + (1u << kSupBits) - 1) >> kSupBits;
+    uint32_t sp = blockIdx.x;  // super-segment of this block
+    if (a.xcd_remap) {
+        const uint32_t per = gridDim.x / kXcds;
+        sp = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     }
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const uint32_t k = (sp * kSegKeys + (uint32_t)w) * kSparseKeysPerWave + (uint32_t)lane;
+    const bool mine = sp < nsup && lane < (int)kSparseKeysPerWave && k < bb.nkeys;
+    uint32_t total = 0, rank = 0, sc = 0;
+    bool todo = false;
+    if (mine) {
+        sc = key_scene_of(a, k);
+        const SceneDesc *sd = a.scenes + sc;
+        rank = k - sd->koff;
+        total = bb.count[k];
+        todo = !final_counts && sd->fast && rank < sd->ntiles && a.near_prev[sd->soff + rank] != 0;
+    }
+    for (uint64_t pending = __ballot(todo); pending; pending &= pending - 1) {  // wave-uniform
+        const int i = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t kk = __shfl(k, i, kWave), rr = __shfl(rank, i, kWave);
+        const uint32_t ss = __shfl(sc, i, kWave), t0 = __shfl(total, i, kWave);
+        const uint32_t t = gather_key(a, bb, kk, a.scenes + ss, rr, t0);
+        if (lane == i) total = t;
+    }
+    const uint32_t wt = wave_sum(mine ? total : 0u);
+    const uint32_t wc = wave_sum(mine ? chunks_of(total, cap) : 0u);
+    if (lane == 0) {
+        s_cnt[w] = wt;
+        s_nch[w] = wc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sp < nsup) add_segment_totals(bb, s_cnt, s_nch, sp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,8 +1458,17 @@ hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count
         k_zero_u32<<<1, kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(b.sup), 2 * b.nsup);
         if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a);
     }
-    launch_timed(k_bin_gather, (nseg + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, st, a, b,
-                 cap, count ? 1 : 0);
+    // many more keys than particle chunks: the sparse variant (measured on the
+    // 64-scene mix); otherwise one wave per key
+    const bool sparse = (uint64_t)b.nkeys > 16ull * ((uint64_t)a.total_slots / kBlock + 1);
+    if (sparse) {
+        const unsigned nsup = blocks_for(b.nkeys, 1u << kSupBits);
+        launch_timed(k_bin_gather_sparse, (nsup + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, st,
+                     a, b, cap, count ? 1 : 0);
+    } else {
+        launch_timed(k_bin_gather, (nseg + kXcds - 1) / kXcds * kXcds, kSegKeys * kWave, st, a, b,
+                     cap, count ? 1 : 0);
+    }
     return hipGetLastError();
 }
 
