@@ -297,10 +297,6 @@ BatchArgs Batch::args() const {
         a.near_prev = d_near_ + (1 - par) * tile_tab_cap_;
     }
     a.key_scene = d_key_scene_;
-    {
-        const size_t par = state_ == state_buf0_ ? 0 : 1;  // g2p fills the other parity
-        a.sup_acc = sup_ + (1 - par) * sup_cap_;
-    }
     a.state = state_;
     a.id = id_;
     a.perm = perm_;
@@ -317,14 +313,6 @@ BatchArgs Batch::args() const {
     a.order = order_;
     a.count = count_;
     a.count_next = 1;
-    // g2p's atomic counting (count_atomic: no k_bin_gather) is switched off.
-    // It was faster for key spaces much larger than the particle chunks (many
-    // sparse scenes, big grids: 4.2e9 vs 3.7e9 particle-substeps/s on the
-    // 64-scene mix), but with it 27 of 40 rollouts of the 300k dam break made
-    // while the host was busy ended with lost particles (per-key counts off by
-    // a few), where the gather counting stayed exact in 40 of 40.  The cause
-    // is not established, so the kernels' atomic variant is never selected.
-    a.count_atomic = 0;
     a.partial = partial_;
     a.posvel = posvel_;
     a.ppt = tun_.ppt;

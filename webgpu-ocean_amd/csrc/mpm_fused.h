@@ -105,10 +105,7 @@ struct BatchArgs {
     uint32_t *order;             // per bin-order position of a window chunk: the state slot
                                  // of the particle p2g_1 processed there (its level order)
     uint32_t *count;             // per key: particles (reduce_2 clears it for g2p's counting)
-    uint2 *sup_acc;              // the next binning's (particles, chunks) totals per 256 keys,
-                                 // accumulated by g2p (the scan clears them)
     int count_next;              // g2p also counts the keys of its output
-    int count_atomic;            // ... and publishes them with atomics (no k_bin_gather)
     int ppt;                     // particles per thread: chunk <= kBlock * ppt
     int grid_blocks;             // persistent blocks of the chunk kernels (0: default)
     int xcd_remap;               // contiguous chunk ranges per XCD
@@ -117,7 +114,7 @@ struct BatchArgs {
 struct BinBuffers {
     uint32_t *count, *offset;  // per key; reduce_2 clears count
     uint2 *sup;                // (particles, chunks) totals per 256 keys, this substep
-    uint2 *sup_next;           // the next substep's (the scan clears them; g2p adds)
+    uint2 *sup_next;           // the next substep's (the scan clears them; k_bin_gather adds)
     uint32_t nsup;
     uint32_t nkeys;            // all scenes
     uint32_t *perm_out;        // bin-order slot -> slot of the state (the scatter writes it)

@@ -327,15 +327,9 @@ struct GridSinkG {  // g2p straight from the decoded velocities
 // The counts come from the previous substep's g2p: a g2p block counts, in
 // LDS, how many of its chunk's particles moved to each of the 27 tiles around
 // the chunk's tile ("mig", 27 per chunk) and gives each particle its rank
-// within its (chunk, direction).  Two ways turn the counts into offsets
-// within the destination tiles (count_atomic; the host currently never
-// selects the second: rollouts made with it lost particles now and then):
-//  - k_bin_gather visits every key and sums, in a fixed order, the counts of
-//    the chunks that can move into it (no atomics; cost grows with the keys);
-//  - with count_atomic, g2p itself adds each (chunk, direction) count to the
-//    destination key's counter with one global atomic whose old value is the
-//    group's offset (cost grows with the chunks; no gather kernel), and adds
-//    the deltas to the per-256-key totals the scan starts from.
+// within its (chunk, direction).  k_bin_gather turns the counts into offsets
+// within the destination tiles: it visits every key and sums, in a fixed
+// order, the counts of the chunks that can move into it (no atomics).
 // Particles that jump further than one tile, or become wild, take a rank
 // from a global atomic on their key instead (rank "final").  After the state
 // was replaced from outside, k_bin_count ranks every particle that way.
@@ -394,36 +388,6 @@ __device__ __forceinline__ uint32_t chunks_of(uint32_t n, uint32_t cap) {
     return (n + cap - 1) / cap;
 }
 
-// Adds m particles that joined key k at rank `base` to the scan totals of
-// the next binning: particles, and the chunks they add to the key.
-__device__ __forceinline__ uint2 total_delta(uint32_t base, uint32_t m, uint32_t cap) {
-    return make_uint2(m, chunks_of(base + m, cap) - chunks_of(base, cap));
-}
-
-// count_rank on the global counters of the next binning, also adding each
-// run's particles to the super-segment totals of the scan.
-__device__ __forceinline__ uint32_t count_rank_totals(const BatchArgs &a, uint32_t key, bool active) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t prev_key = __shfl_up(key, 1, kWave);
-    const int prev_active = __shfl_up(active ? 1 : 0, 1, kWave);
-    const bool head = lane == 0 || !active || !prev_active || prev_key != key;
-    const uint64_t heads = __ballot(head);
-    const uint64_t le = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1);  // lanes <= this
-    const uint64_t above = heads & ~le;
-    const uint32_t end = above ? (uint32_t)__ffsll((unsigned long long)above) - 1 : (uint32_t)kWave;
-    uint32_t base = 0;
-    if (head && active) {
-        const uint32_t n = end - (uint32_t)lane;
-        const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
-        base = atomicAdd(&a.count[key], n);
-        const uint2 d = total_delta(base, n, cap);
-        atomicAdd(&a.sup_acc[key >> kSupBits].x, d.x);
-        if (d.y) atomicAdd(&a.sup_acc[key >> kSupBits].y, d.y);
-    }
-    const int my_head = 63 - __clzll((unsigned long long)(heads & le));  // lane 0 is a head
-    return __shfl(base, my_head, kWave) + (uint32_t)(lane - my_head);
-}
-
 __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
 }
@@ -473,9 +437,8 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *total) {
 
 // The scan's prefix comes from totals of (particles, tile chunks) per
 // super-segment of 256 keys, double-buffered by substep parity: a scan
-// clears the other parity's, which the next k_bin_gather (or, with
-// count_atomic, this substep's g2p) fills with atomics, plus the counts of the
-// super-segment's keys before the block's.
+// clears the other parity's, which the next k_bin_gather fills with atomics,
+// plus the counts of the super-segment's keys before the block's.
 constexpr uint32_t kSegKeys = 1u << kSegBits;  // keys per k_bin_gather block
 constexpr uint32_t kSegPerSup = 1u << (kSupBits - kSegBits);
 
@@ -572,7 +535,7 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
             b.xcd_first[kXcds] = cb + nch;
         }
     }
-    // clear the next binning's totals (g2p adds to them)
+    // clear the next binning's totals (its k_bin_gather adds to them)
     if (lane == 0 && (k0 & ((1u << kSupBits) - 1)) == 0) b.sup_next[sp] = make_uint2(0u, 0u);
     __syncthreads();
     // Every tile key writes its reach flag.  Counts stay readable until
@@ -1128,7 +1091,6 @@ __device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, uint32
 // Near moves rank within (chunk c, direction) through the block's LDS
 // counters; wild ones too, finished after the chunk (wild_rank); far moves
 // take a final rank from a global atomic on their key.
-template <bool kAtomic>
 __device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, uint32_t c,
                                           uint32_t i, uint32_t key, uint32_t dir, bool active) {
     const bool local = active && dir <= kDirWild;
@@ -1136,7 +1098,7 @@ __device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, ui
     uint32_t fr = 0;
     if (__ballot(active && dir == kDirFar)) {  // rare: a particle left its tile's neighbourhood
         const bool far = active && dir == kDirFar;
-        fr = kAtomic ? count_rank_totals(a, key, far) : count_rank(a.count, key, far);
+        fr = count_rank(a.count, key, far);
     }
     if (active) {
         a.key[i] = key;
@@ -1145,66 +1107,21 @@ __device__ __forceinline__ void rank_next(const BatchArgs &a, uint32_t *lcnt, ui
     }
 }
 
-// After a chunk, with count_atomic: 27 lanes add its near counts to the
-// destination keys' counters (the old values are the groups' offsets within
-// the destination tiles, kept in mig), one lane reserves the wild particles'
-// ranks; their deltas go to the super-segment totals of the next binning,
-// summed over the lanes first (few super-segments, many chunks).  Without
-// it: the counts go to mig for k_bin_gather, and only the wild particles'
-// ranks are reserved here.
-// lcnt[kDirWild + 1] carries the wild base between the two barriers.  b is
-// the chunk's spatial tile.
-template <bool kAtomic>
+// After a chunk: its near counts go to mig for k_bin_gather, and one lane
+// reserves the wild particles' ranks on the scene's wild key.
+// lcnt[kDirWild + 1] carries the wild base between the two barriers.
 __device__ __forceinline__ void finish_chunk_counts(const BatchArgs &a, const SceneDesc *sd,
-                                                    uint32_t *lcnt, uint32_t c, uint32_t b) {
+                                                    uint32_t *lcnt, uint32_t c) {
     __syncthreads();
-    if (!kAtomic) {  // the counts go to k_bin_gather, which makes them offsets
-        if (threadIdx.x < kDirs) a.mig[(size_t)c * kDirs + threadIdx.x] = lcnt[threadIdx.x];
-        if (threadIdx.x == kDirWild) {
-            const uint32_t nw = lcnt[kDirWild];
-            lcnt[kDirWild + 1] = nw ? atomicAdd(&a.count[sd->koff + sd->ntiles], nw) : 0u;
-        }
-    } else if (threadIdx.x < kWave) {
-        const uint32_t lane = threadIdx.x;
-        const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
-        uint32_t key = kNoTile, m = 0;
-        if (lane < kDirs) {
-            m = lcnt[lane];
-            if (m) key = a.nbr[(size_t)b * kDirs + lane];  // the tile the group moved to
-        } else if (lane == kDirWild) {
-            m = lcnt[kDirWild];
-            key = sd->koff + sd->ntiles;  // the scene's wild key
-        }
-        const bool has = m != 0 && key != kNoTile;
-        uint32_t base = 0;
-        uint2 d = make_uint2(0u, 0u);
-        if (has) {
-            base = atomicAdd(&a.count[key], m);
-            d = total_delta(base, m, cap);
-        }
-        if (lane < kDirs) a.mig[(size_t)c * kDirs + lane] = base;
-        if (lane == kDirWild) lcnt[kDirWild + 1] = base;
-        // super-segment totals: one atomic per distinct super-segment
-        const uint32_t sp = has ? key >> kSupBits : 0xffffffffu;
-        uint64_t pending = __ballot(has);
-        while (pending) {
-            const int leader = __ffsll((unsigned long long)pending) - 1;
-            const uint32_t lsp = __shfl(sp, leader, kWave);
-            const bool mine = has && sp == lsp;
-            const uint32_t tp = wave_sum(mine ? d.x : 0u), tc = wave_sum(mine ? d.y : 0u);
-            if ((int)lane == leader) {
-                atomicAdd(&a.sup_acc[lsp].x, tp);
-                if (tc) atomicAdd(&a.sup_acc[lsp].y, tc);
-            }
-            pending &= ~__ballot(mine);
-        }
+    if (threadIdx.x < kDirs) a.mig[(size_t)c * kDirs + threadIdx.x] = lcnt[threadIdx.x];
+    if (threadIdx.x == kDirWild) {
+        const uint32_t nw = lcnt[kDirWild];
+        lcnt[kDirWild + 1] = nw ? atomicAdd(&a.count[sd->koff + sd->ntiles], nw) : 0u;
     }
     __syncthreads();
 }
 
-// kAtomic = a.count_atomic (a template parameter, so that each variant keeps
-// only its own counting code in the register budget).
-template <int PPT, bool kAtomic>
+template <int PPT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
 void k_g2p(BatchArgs a) {
     __shared__ float4 lvel[kWinCells];
@@ -1252,11 +1169,10 @@ void k_g2p(BatchArgs a) {
                 key = bin_key(a, sd, xo, t, &dir);
             }
             wild[k] = valid[k] && dir == kDirWild;
-            if (a.count_next) rank_next<kAtomic>(a, lcnt, w.c, i, key, dir, valid[k]);
+            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, valid[k]);
         }
         if (a.count_next) {
-            finish_chunk_counts<kAtomic>(a, sd, lcnt, w.c,
-                                sd->soff + a.tile_rank[sd->rank_off + ch.y]);
+            finish_chunk_counts(a, sd, lcnt, w.c);
 #pragma unroll
             for (int k = 0; k < PPT; ++k)
                 if (wild[k]) {
@@ -1293,7 +1209,7 @@ void k_g2p(BatchArgs a) {
                 key = bin_key(a, sd, xo, nowhere, &dir);
                 dir = kDirFar;
             }
-            if (a.count_next) rank_next<kAtomic>(a, lcnt, w.c, i, key, dir, active);
+            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, active);
         }
     }
 }
@@ -1332,12 +1248,7 @@ hipError_t launch_ppt(int which, const BatchArgs &a, hipStream_t st) {
     switch (which) {
     case 1: launch_timed(k_p2g1<PPT>, nb, kBlock, st, a); break;
     case 2: launch_timed(k_p2g2<PPT>, nb, kBlock, st, a); break;
-    case 3:
-        if (a.count_atomic)
-            launch_timed(k_g2p<PPT, true>, nb, kBlock, st, a);
-        else
-            launch_timed(k_g2p<PPT, false>, nb, kBlock, st, a);
-        break;
+    case 3: launch_timed(k_g2p<PPT>, nb, kBlock, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1448,7 +1359,6 @@ void scene_key_geometry(SceneDesc &d) {
 
 hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count,
                              hipStream_t st) {
-    if (!count && a.count_atomic) return hipSuccess;  // g2p left counts, offsets and totals
     const uint32_t cap = (uint32_t)kBlock * (uint32_t)a.ppt;
     const unsigned nseg = blocks_for(b.nkeys, kSegKeys);
     if (count) {
