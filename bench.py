@@ -213,6 +213,34 @@ def measured_traffic(slot: str):
     return None
 
 
+def host_transfers(sim, rec, frame_ms: float, reps: int = 5) -> dict:
+    """Host <-> device transfers at the boundary, outside the metric: the
+    PosVel readback (N x 32 B, what a host-side consumer of every frame would
+    add) and the particle upload (N x 80 B records, as at construction and
+    reset).  Medians of `reps`; the upload replaces the simulator's state."""
+    def med(fn):
+        ts = []
+        for _ in range(reps):
+            sim.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            sim.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * sorted(ts)[len(ts) // 2]
+
+    n = rec.shape[0]
+    read_ms = med(sim.read_posvel)
+    write_ms = med(lambda: sim.write_particles(rec))
+    return {
+        "read_posvel_ms": read_ms,
+        "read_posvel_GBs": 32 * n / (read_ms * 1e-3) / 1e9,
+        "write_particles_ms": write_ms,
+        "write_particles_GBs": 80 * n / (write_ms * 1e-3) / 1e9,
+        # particle-substeps/s if every frame's PosVel also went to the host
+        "value_with_posvel_readback": n * SUBSTEPS_PER_FRAME / ((frame_ms + read_ms) * 1e-3),
+    }
+
+
 def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
     """Oracle (CPU restatement of the reference WGSL) timed on the host cores."""
     import oracle
@@ -414,6 +442,7 @@ def main() -> int:
     substep_us = sum(per_substep_us.values())
     tuning = sim.tuning
     total_bytes = 204 * n + 84 * g
+    host = host_transfers(sim, rec, 1e3 * elapsed / args.steps)
     sim.close()
 
     out = {
@@ -469,6 +498,7 @@ def main() -> int:
             "achieved_GBs": total_bytes / (substep_us * 1e-6) / 1e9,
             "wall_us_per_substep": 1e6 * elapsed / substeps,
         },
+        "host_transfers": host,
     }
     if not args.no_extras:
         out["batched"] = bench_batched()
