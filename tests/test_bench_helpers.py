@@ -75,3 +75,32 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_host_transfers_report_rates_and_the_readback_inclusive_value():
+    import time
+    import numpy as np
+
+    class FakeSim:  # each transfer takes ~2 ms of host time
+        calls = []
+
+        def synchronize(self):
+            pass
+
+        def read_posvel(self):
+            self.calls.append("read")
+            time.sleep(0.002)
+
+        def write_particles(self, rec):
+            self.calls.append("write")
+            time.sleep(0.002)
+
+    rec = np.zeros((1000, 20), dtype=np.float32)
+    h = bench.host_transfers(FakeSim(), rec, frame_ms=8.0, reps=3)
+    assert FakeSim.calls == ["read"] * 3 + ["write"] * 3
+    assert 2.0 <= h["read_posvel_ms"] < 8.0 and 2.0 <= h["write_particles_ms"] < 8.0
+    assert h["read_posvel_GBs"] == pytest.approx(32 * 1000 / (h["read_posvel_ms"] * 1e-3) / 1e9)
+    assert h["write_particles_GBs"] == pytest.approx(80 * 1000 / (h["write_particles_ms"] * 1e-3) / 1e9)
+    # two substeps per frame, the readback added to every frame
+    assert h["value_with_posvel_readback"] == pytest.approx(
+        1000 * 2 / ((8.0 + h["read_posvel_ms"]) * 1e-3))
