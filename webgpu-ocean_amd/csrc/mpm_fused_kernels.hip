@@ -1067,9 +1067,9 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // g2p with copyPosition fused (mls-mpm/g2p.wgsl:26-92, copyPosition.wgsl:15-21)
 
-// Writes particle `src` (a slot of a.state) updated, at bin-order slot i of
-// the output state.
-__device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, uint32_t src,
+// Writes particle o (its caller index, a.id of its input slot) updated, at
+// bin-order slot i of the output state.
+__device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, uint32_t o,
                                           const float xo[3], const float v[3],
                                           const float Cn[3][3]) {
 #pragma unroll
@@ -1081,7 +1081,6 @@ __device__ __forceinline__ void g2p_store(const BatchArgs &a, uint32_t i, uint32
         a.state_out[(PX + ax) * a.stride + i] = xo[ax];
         a.state_out[(VX + ax) * a.stride + i] = v[ax];
     }
-    const uint32_t o = a.id[src];
     a.id_out[i] = o;
     a.posvel[2 * o] = make_float4(xo[0], xo[1], xo[2], 0.0f);
     a.posvel[2 * o + 1] = make_float4(v[0], v[1], v[2], 0.0f);
@@ -1134,17 +1133,18 @@ void k_g2p(BatchArgs a) {
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
         if (threadIdx.x < kDirWild + 1) lcnt[threadIdx.x] = 0;
         float x[PPT][3];
-        uint32_t src[PPT];
+        uint32_t oid[PPT];  // caller index of each particle
         bool valid[PPT];
         // contiguous slots per task: coalesced state stores
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const uint32_t j = task_of(k) * 64 + (threadIdx.x & 63);
             valid[k] = j < ch.w;
-            src[k] = 0;
+            oid[k] = 0;
             if (valid[k]) {
-                src[k] = a.perm[ch.z + j];
-                load_x(a.state, a.stride, src[k], x[k]);
+                const uint32_t src = a.perm[ch.z + j];
+                load_x(a.state, a.stride, src, x[k]);
+                oid[k] = a.id[src];
             }
         }
         int t[3];
@@ -1165,7 +1165,7 @@ void k_g2p(BatchArgs a) {
                 node3_init(x[k], n);
                 g2p_gather(n, WinSinkG{lvel, win_l0(x[k])}, v, B);
                 g2p_finish(x[k], rb, dt, v, B, xo, Cn);
-                g2p_store(a, i, src[k], xo, v, Cn);
+                g2p_store(a, i, oid[k], xo, v, Cn);
                 key = bin_key(a, sd, xo, t, &dir);
             }
             wild[k] = valid[k] && dir == kDirWild;
@@ -1205,7 +1205,7 @@ void k_g2p(BatchArgs a) {
                 node3_init(x, n);
                 g2p_gather<true>(n, sink, v, B);
                 g2p_finish(x, rb, dt, v, B, xo, Cn);
-                g2p_store(a, i, si, xo, v, Cn);
+                g2p_store(a, i, a.id[si], xo, v, Cn);
                 key = bin_key(a, sd, xo, nowhere, &dir);
                 dir = kDirFar;
             }
