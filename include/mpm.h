@@ -112,7 +112,10 @@ int mpm_synchronize(mpm_sim *sim);
 int mpm_read_posvel(mpm_sim *sim, void *dst, uint32_t n);
 
 /* Device address of the N x 32-B PosVel buffer, for consumers that read it
- * on the GPU as the reference's renderer does (render/sphere.wgsl:28-33). */
+ * on the GPU as the reference's renderer does (render/sphere.wgsl:28-33).
+ * The address stays the same across mpm_step and mpm_set_box; mpm_reset and
+ * mpm_destroy invalidate it.  Steps are asynchronous: a reader on another
+ * stream waits for mpm_synchronize (or an event after the handle's work). */
 int mpm_posvel_device_ptr(mpm_sim *sim, void **out);
 
 /* Particle state in the reference's 80-B layout (read synchronizes). */

@@ -231,6 +231,31 @@ def test_empty_scene_and_reset():
         assert sim.grid_count == 4096
 
 
+def test_posvel_device_pointer_is_stable_and_holds_posvel():
+    import ctypes
+
+    box = [24.0, 24.0, 24.0]
+    rec = scenes.small_scene(2000, box, seed=11, velocity=1.0, affine=0.3)
+    sim, ref = pair(rec, box)
+    with sim:
+        hip = ctypes.CDLL("libamdhip64.so.7")  # the HIP runtime libmpm355 already loaded
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipMemcpy.restype = ctypes.c_int
+        p0 = sim.posvel_device_ptr()
+        assert p0
+        for f in range(3):
+            if f == 1:  # a box change keeps the buffer
+                sim.change_box_size([20.0, 24.0, 24.0])
+                ref.change_box_size([20.0, 24.0, 24.0])
+            sim.execute()
+            ref.execute(2)
+            sim.synchronize()
+            assert sim.posvel_device_ptr() == p0
+            out = np.zeros((2000, 8), dtype=np.float32)
+            assert hip.hipMemcpy(out.ctypes.data, p0, out.nbytes, 2) == 0  # device to host
+            assert np.all(bits_equal(out, ref.posvel)), f"frame {f}"
+
+
 def test_errors_are_status_codes():
     with MLSMPMSimulator(init_box_size=(8, 8, 8)) as sim:
         with pytest.raises(MPMError) as ei:
