@@ -794,9 +794,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWa
 void k_p2g1(BatchArgs a) {
     __shared__ int lds[4 * kWinCells];  // [mass | vx | vy | vz]
     __shared__ LevelOrder<PPT> ord;
+    bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
-        if (ch.y == kWildTile) continue;  // second loop
+        if (ch.y == kWildTile) {  // second loop
+            wild_seen = true;
+            continue;
+        }
         const SceneDesc *sd = a.scenes + ch.x;
         const float mult = sd->prm.mult;
         for (int l = threadIdx.x; l < 4 * kWinCells; l += kBlock) lds[l] = 0;
@@ -831,6 +835,7 @@ void k_p2g1(BatchArgs a) {
     }
     // Wild chunks (rare): global atomics into the wild accumulator.  A loop of
     // their own keeps their registers out of the window loop's budget.
+    if (!wild_seen) return;
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
         if (ch.y != kWildTile) continue;
@@ -1004,9 +1009,13 @@ template <int PPT>
 __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
     __shared__ int lds[4 * kWinCells];  // [decoded mass (f32) | vx | vy | vz]
     float *dm = reinterpret_cast<float *>(lds);
+    bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
-        if (ch.y == kWildTile) continue;  // second loop
+        if (ch.y == kWildTile) {  // second loop
+            wild_seen = true;
+            continue;
+        }
         const SceneDesc *sd = a.scenes + ch.x;
         const Params prm = sd->prm;
         float x[PPT][3], C[PPT][3][3];
@@ -1045,6 +1054,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
         __syncthreads();
     }
     // Wild chunks (rare), in a loop of their own like k_p2g1's.
+    if (!wild_seen) return;
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
         if (ch.y != kWildTile) continue;
