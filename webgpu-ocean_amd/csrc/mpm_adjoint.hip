@@ -606,9 +606,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
     __shared__ int s_acc[3 * kWinCells];
     __shared__ float s_sum[3 * kWinCells];
     __shared__ uint32_t s_max;
+    bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
-        if (ch.y == kWildTile) continue;  // second loop
+        if (ch.y == kWildTile) {  // second loop
+            wild_seen = true;
+            continue;
+        }
         int t[3];
         adj_tile_coords(q, ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
@@ -655,6 +659,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
     }
     // Wild chunks (global float atomics) in a loop of their own: in the
     // window loop they would set its register budget.
+    if (!wild_seen) return;
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
         if (ch.y != kWildTile) continue;
@@ -676,9 +681,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
     __shared__ uint32_t s_max;
     __shared__ float s_red[2][kAdjBlock / 64];
     float gk = 0.0f, gmu = 0.0f;
+    bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
-        if (ch.y == kWildTile) continue;  // second loop
+        if (ch.y == kWildTile) {  // second loop
+            wild_seen = true;
+            continue;
+        }
         int t[3];
         adj_tile_coords(q, ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
@@ -718,8 +727,9 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
         __syncthreads();
     }
     // Wild chunks (global float atomics) in a loop of their own: in the
-    // window loop they would set its register budget.
-    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
+    // window loop they would set its register budget.  (No early return:
+    // the block reduction below needs every block.)
+    for (AdjWalk w = adj_walk(q); wild_seen && w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
         if (ch.y != kWildTile) continue;
         const GlobalAcc acc(q);
@@ -754,9 +764,13 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
 // P2G-1 adjoint: window of (gmv, gm) in, particle adjoints out.
 __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
     __shared__ float4 s_w4[kWinCells];
+    bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
-        if (ch.y == kWildTile) continue;  // second loop
+        if (ch.y == kWildTile) {  // second loop
+            wild_seen = true;
+            continue;
+        }
         int t[3];
         adj_tile_coords(q, ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
@@ -781,6 +795,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
     }
     // Wild chunks (global float atomics) in a loop of their own: in the
     // window loop they would set its register budget.
+    if (!wild_seen) return;
     for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
         const TileChunk ch = q.tchunks[w.c];
         if (ch.y != kWildTile) continue;
