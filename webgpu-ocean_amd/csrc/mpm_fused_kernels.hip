@@ -1135,6 +1135,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWa
 void k_g2p(BatchArgs a) {
     __shared__ float4 lvel[kWinCells];
     __shared__ uint32_t lcnt[kDirWild + 2];  // per-direction counts, wild base
+    // whether the (single) scene has wild chunks at all, read up front
+    const bool any_wild =
+        a.nscenes != 1 || a.tile_chunk[a.scenes[0].toff + a.scenes[0].ntiles].y != 0;
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
         if (ch.y == kWildTile) continue;  // second loop
@@ -1195,6 +1198,7 @@ void k_g2p(BatchArgs a) {
     // Wild chunks (rare), in a loop of their own like k_p2g1's.  Every move
     // from a wild chunk is "far" (no tile to count from): final ranks from
     // global atomics, no LDS counts.
+    if (!any_wild) return;
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
         if (ch.y != kWildTile) continue;
