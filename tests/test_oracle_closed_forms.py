@@ -197,6 +197,106 @@ def test_p2g2_stress_formula_single_cell_hand_value():
     assert c[3, 4, 2, 0] == 0   # d = 0 at the centre node
 
 
+# Viscous stress, mu * (C + C^T) (mls-mpm/p2g_2.wgsl:65-68).  The particle sits
+# at the centre of cell (3, 4, 2) on a pre-filled uniform grid, so every axis
+# has weights (.125, .75, .125) and the node (+1,0,0) has w = .125*.75*.75 =
+# .0703125 with cell_dist (1, 0, 0).  The momentum added there is column 0 of
+# M*w, i.e. (M[0][0], M[0][1], M[0][2]) * w (M stored column-major like C);
+# node (0,+1,0) receives column 1.  Each test is built so that a dropped
+# transpose (C + C), mu * C alone, or C - C^T changes some asserted integer.
+
+CENTRE = (3.5, 4.5, 2.5)
+W_X1 = f32(0.0703125)
+
+
+def _p2g2_cells(C, mass, viscosity=0.1):
+    sc = oracle.Scene(one_particle(CENTRE, (0, 0, 0), C=C), [8.0, 8.0, 8.0],
+                      oracle.default_params(dynamic_viscosity=viscosity))
+    sc.cells[:, 3] = mass
+    sc.run_stage(2)
+    return sc.cells.reshape(8, 8, 8, 4)
+
+
+def _offdiag_mom(strain, mass):
+    """Hand value of one off-diagonal momentum integer at a node with w =
+    W_X1 and |d| = 1 along the node's axis, for the given strain entry: with
+    a uniform mass m per node the density is m, volume 1/m, and
+    M = ((-volume * 4) * (0 + mu * strain)) * dt, mom = (M * w) * 1."""
+    volume = f32(1.0) / f32(mass)
+    stress = f32(f32(0.0) + f32(f32(0.1) * f32(strain)))
+    M = f32(f32(f32(-volume * f32(4.0)) * stress) * f32(0.2))
+    return int(f32(f32(M * W_X1) * f32(MULT)))
+
+
+@pytest.mark.parametrize("mass", [40000000, 80000000])
+def test_p2g2_antisymmetric_C_has_no_viscous_stress(mass):
+    # C + C^T = 0: the result must equal the C = 0 result exactly.
+    a, b, c = 0.75, -0.375, 0.5
+    C = [[0, a, b], [-a, 0, c], [-b, -c, 0]]  # columns; C^T = -C
+    base = _p2g2_cells(None, mass)
+    got = _p2g2_cells(C, mass)
+    assert np.array_equal(got, base)
+    # the particle's own momentum scatter is non-trivial only through pressure
+    if mass == 80000000:
+        assert got[4, 4, 2, 0] > 0
+
+
+@pytest.mark.parametrize("mass", [40000000, 80000000])
+def test_p2g2_symmetric_offdiagonal_C_hand_integers(mass):
+    # C[0][1] = C[1][0] = s: strain (0,1) = (1,0) = 2s.
+    s = 0.625
+    C = [[0, s, 0], [s, 0, 0], [0, 0, 0]]
+    got = _p2g2_cells(C, mass)
+    want = _offdiag_mom(f32(2 * s), mass // 10**7)
+    assert want != 0
+    assert got[4, 4, 2, 1] == want and got[2, 4, 2, 1] == -want   # vy at (+-1, 0, 0)
+    assert got[3, 5, 2, 0] == want and got[3, 3, 2, 0] == -want   # vx at (0, +-1, 0)
+    # nothing couples into z, and the diagonal carries only the pressure
+    assert got[3, 4, 3, 0] == 0 and got[3, 4, 3, 1] == 0
+    base = _p2g2_cells(None, mass)
+    assert got[4, 4, 2, 0] == base[4, 4, 2, 0]
+
+
+def test_p2g2_single_C_entry_reaches_both_stress_entries():
+    # Only C[0][1] = a (column 0, row 1): strain (0,1) = C[0][1] + C[1][0] = a
+    # and strain (1,0) = C[1][0] + C[0][1] = a.  mu*C alone would leave (1,0)
+    # at 0; C + C would give 2a at (0,1) and 0 at (1,0); C - C^T gives -a.
+    a = 0.8125
+    mass = 40000000  # density 4 = rest density: no pressure, viscosity only
+    got = _p2g2_cells([[0, a, 0], [0, 0, 0], [0, 0, 0]], mass)
+    want = _offdiag_mom(f32(a), 4)
+    assert want != 0
+    assert got[4, 4, 2, 1] == want     # column 0 of M: vy at (+1, 0, 0)
+    assert got[3, 5, 2, 0] == want     # column 1 of M: vx at (0, +1, 0)
+    assert got[4, 4, 2, 0] == 0 and got[3, 5, 2, 1] == 0   # diagonal: no pressure
+    # and the viscosity scales it: mu = 0 removes it
+    assert np.all(_p2g2_cells([[0, a, 0], [0, 0, 0], [0, 0, 0]], mass, viscosity=0.0)[..., :3] == 0)
+
+
+def test_p2g2_viscous_momentum_matches_the_continuum_formula():
+    # Random non-symmetric C at rest density (no pressure): every node's
+    # momentum is -4 dt vol mu (A + A^T) d w with A[r][c] = C[c][r], computed
+    # here in float64 from the physics, not in the oracle's f32 order.
+    rng = np.random.default_rng(17)
+    mu = 0.1
+    for trial in range(5):
+        C = rng.uniform(-1.0, 1.0, size=(3, 3)).astype(np.float32)   # columns
+        got = _p2g2_cells(C.tolist(), 40000000, viscosity=mu)
+        A = C.astype(np.float64).T
+        S = mu * (A + A.T)
+        M = -4.0 * 0.2 * 0.25 * S
+        w1 = [0.125, 0.75, 0.125]
+        base = [3, 4, 2]
+        for gx in range(3):
+            for gy in range(3):
+                for gz in range(3):
+                    w = w1[gx] * w1[gy] * w1[gz]
+                    d = np.array([gx - 1, gy - 1, gz - 1], dtype=np.float64)
+                    mom = M @ d * w * MULT
+                    cell = got[base[0] + gx - 1, base[1] + gy - 1, base[2] + gz - 1, :3]
+                    assert np.all(np.abs(cell - mom) <= 3), (trial, gx, gy, gz, cell, mom)
+
+
 # --- updateGrid ------------------------------------------------------------
 
 def test_update_grid_hand_cases():
