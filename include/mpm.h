@@ -39,6 +39,8 @@ typedef enum {
     MPM_ERR_OUT_OF_MEMORY = 3,
     MPM_ERR_DEVICE = 4,         /* a HIP runtime call failed */
     MPM_ERR_NO_DEVICE = 5,
+    MPM_ERR_NOT_READY = 6,      /* the handle holds no scene: its last mpm_reset failed
+                                   (everything was released; mpm_reset again) */
 } mpm_status;
 
 /* The reference's pipeline-override constants (mls-mpm/mls-mpm.ts:53-59). */
@@ -123,11 +125,20 @@ int mpm_read_particles(mpm_sim *sim, void *dst, uint32_t n);
 int mpm_write_particles(mpm_sim *sim, const void *src, uint32_t n);
 
 /* Grid accumulators, ncells x (vx, vy, vz, mass) i32 (read synchronizes).
- * After mpm_step on the default path the grid holds the last substep's P2G
- * accumulators (updateGrid is applied inside g2p's LDS staging); after
- * mpm_run_stage it holds that stage's output exactly as the reference's. */
+ * After mpm_step on the default path the grid holds the last substep's raw
+ * P2G accumulators (after p2g_1 and p2g_2, before updateGrid: the production
+ * path keeps the updated velocities in a separate buffer, see
+ * mpm_read_grid_velocity); after mpm_run_stage it holds that stage's output
+ * exactly as the reference's cell buffer (mls-mpm/mls-mpm.ts:122-130). */
 int mpm_read_grid(mpm_sim *sim, int32_t *dst, uint64_t ncells);
 int mpm_write_grid(mpm_sim *sim, const int32_t *src, uint64_t ncells);
+
+/* Grid velocities after updateGrid, decoded: ncells x (vx, vy, vz, 0) f32 =
+ * f32(cell.v) / multiplier of the reference's cells after updateGrid.wgsl
+ * (what g2p.wgsl:51-53 reads), as left by the last mpm_step on the default
+ * path (read synchronizes).  Test hook for the production path's updateGrid;
+ * undefined after mpm_run_stage. */
+int mpm_read_grid_velocity(mpm_sim *sim, float *dst, uint64_t ncells);
 
 /* numParticles and gridCount (mls-mpm/mls-mpm.ts:19-20). */
 int mpm_get_counts(mpm_sim *sim, uint32_t *num_particles, uint64_t *grid_count);

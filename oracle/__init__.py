@@ -76,7 +76,7 @@ def lib():
         L.oracle_weights.argtypes = [ctypes.c_float, F3]
         D3 = ctypes.POINTER(ctypes.c_double)
         L.oracle_rollout_loss_f64.restype = ctypes.c_double
-        L.oracle_rollout_loss_f64.argtypes = [P, P, u32, F3, ctypes.c_double, ctypes.c_double,
+        L.oracle_rollout_loss_f64.argtypes = [P, P, P, u32, F3, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_double, u32, D3,
                                               ctypes.c_double, P]
         _lib = L
@@ -168,15 +168,18 @@ def weights(x: float) -> np.ndarray:
 
 def rollout_loss_f64(particles, box, substeps, target, ke_weight=1e-3, v0=None,
                      stiffness=3.0, viscosity=0.1, rest_density=4.0, dt=0.2,
-                     return_state=False):
+                     return_state=False, x0=None):
     """FP64 unquantized rollout (oracle_rollout_loss_f64): the loss whose
-    central differences check the HIP adjoint."""
+    central differences check the HIP adjoint.  x0 / v0 (n x 3 float64)
+    replace the records' initial positions / velocities."""
     rec = np.ascontiguousarray(particles, dtype=np.float32).reshape(-1, 20)
     n = rec.shape[0]
     v = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64).reshape(n, 3)
+    x = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).reshape(n, 3)
     tgt = (ctypes.c_double * 3)(*[float(t) for t in target])
     out = np.zeros((n, 20), dtype=np.float32) if return_state else None
     loss = lib().oracle_rollout_loss_f64(
-        _ptr(rec), None if v is None else _ptr(v), n, _f3(box), stiffness, viscosity,
-        rest_density, dt, substeps, tgt, ke_weight, None if out is None else _ptr(out))
+        _ptr(rec), None if x is None else _ptr(x), None if v is None else _ptr(v), n, _f3(box),
+        stiffness, viscosity, rest_density, dt, substeps, tgt, ke_weight,
+        None if out is None else _ptr(out))
     return (loss, out) if return_state else loss

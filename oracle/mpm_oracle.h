@@ -70,10 +70,12 @@ void oracle_substep_omp(oracle_particle *p, uint32_t n, oracle_cell *cells, uint
 
 /* FP64, unquantized rollout of `substeps` substeps (real box = init box);
  * returns |com - target|^2 + ke_weight * mean|v|^2 of the final state.
- * v0 (n x 3, may be NULL) overrides the records' velocities; final_state
- * (may be NULL) receives the final particles as 80-B f32 records. */
-double oracle_rollout_loss_f64(const float *particles, const double *v0, uint32_t n,
-                               const float init_box[3], double stiffness, double viscosity,
+ * x0 and v0 (n x 3 each, may be NULL) override the records' positions and
+ * velocities; final_state (may be NULL) receives the final particles as 80-B
+ * f32 records. */
+double oracle_rollout_loss_f64(const float *particles, const double *x0, const double *v0,
+                               uint32_t n, const float init_box[3], double stiffness,
+                               double viscosity,
                                double rest_density, double dt, uint32_t substeps,
                                const double target[3], double ke_weight, float *final_state);
 

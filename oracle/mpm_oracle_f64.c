@@ -166,8 +166,9 @@ static void substep64(p64 *p, uint32_t n, double *grid /* G x 4: vx,vy,vz,m */, 
 
 /* Loss of mls-mpm rollouts used by the differentiable mode:
  * |centre of mass - target|^2 + ke_weight * mean |v|^2 at the final step. */
-double oracle_rollout_loss_f64(const float *particles, const double *v0, uint32_t n,
-                               const float init_box[3], double stiffness, double viscosity,
+double oracle_rollout_loss_f64(const float *particles, const double *x0, const double *v0,
+                               uint32_t n, const float init_box[3], double stiffness,
+                               double viscosity,
                                double rest_density, double dt, uint32_t substeps,
                                const double target[3], double ke_weight, float *final_state)
 {
@@ -184,7 +185,7 @@ double oracle_rollout_loss_f64(const float *particles, const double *v0, uint32_
     for (uint32_t i = 0; i < n; ++i) {
         const float *rec = particles + 20 * (size_t)i;
         for (int a = 0; a < 3; ++a) {
-            p[i].x[a] = rec[a];
+            p[i].x[a] = x0 ? x0[3 * i + a] : rec[a];
             p[i].v[a] = v0 ? v0[3 * i + a] : rec[4 + a];
         }
         for (int c = 0; c < 3; ++c)

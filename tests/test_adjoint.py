@@ -46,6 +46,31 @@ def test_central_differences_converge_in_h():
     assert abs(d[0] - d[2]) < 1e-2 * abs(d[2])
 
 
+# Positions: the loss is only C1 in x0 (the quadratic B-spline's second
+# derivative jumps at its knots, and g2p clamps positions), so central
+# differences converge in O(h) rather than O(h^2): at h = 1e-4 they are still
+# ~0.7% off on this scene.  The position check therefore uses H_X = 1e-6,
+# where the differences have converged to ~1e-7 relative.
+H_X = 1e-6
+
+
+def test_position_central_differences_converge_in_h():
+    rec, box, target = fd_scene()
+    x0 = rec[:, 0:3].astype(np.float64)
+    u = np.random.default_rng(4).normal(size=x0.shape)
+    d = []
+    for h in (1e-4, 1e-5, 3e-6, H_X):
+        lp = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, x0=x0 + h * u)
+        lm = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, x0=x0 - h * u)
+        d.append((lp - lm) / (2 * h))
+    assert abs(d[2] - d[3]) < 1e-5 * abs(d[3])   # converged at H_X
+    assert abs(d[1] - d[3]) < 1e-4 * abs(d[3])
+    assert abs(d[0] - d[3]) < 2e-2 * abs(d[3])   # h = 1e-4: first-order error
+    # the override itself: x0 equal to the records' positions changes nothing
+    assert oracle.rollout_loss_f64(rec, box, 4, target, x0=x0) == \
+        oracle.rollout_loss_f64(rec, box, 4, target)
+
+
 # --- GPU: the adjoint ------------------------------------------------------
 
 def fd_dir(rec, box, target, u, h=H):
@@ -85,6 +110,38 @@ def test_adjoint_matches_fp64_central_differences():
         assert abs(ad - fd) <= 5 * tol * abs(fd), f"component {f}: adjoint {ad} fd {fd}"
 
 
+def fd_dir_x(rec, box, target, u, h=H_X):
+    x0 = rec[:, 0:3].astype(np.float64)
+    lp = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, x0=x0 + h * u)
+    lm = oracle.rollout_loss_f64(rec, box, SUBSTEPS, target, x0=x0 - h * u)
+    return (lp - lm) / (2 * h)
+
+
+@pytest.mark.gpu
+def test_position_adjoint_matches_fp64_central_differences():
+    # d loss / d x0 runs through the weight derivatives, the -w term of
+    # d(dist)/dx and the clamp and wall masks of g2p.
+    rec, box, target = fd_scene()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        g = sim.backward(SUBSTEPS, target, ke_weight=1e-3, checkpoint_every=CKPT)
+    gx = g["grad_x0"].astype(np.float64)
+    assert np.all(np.isfinite(gx)) and np.any(gx != 0)
+    rng = np.random.default_rng(3)
+    tol = 1e-3
+    for trial in range(3):
+        u = rng.normal(size=(rec.shape[0], 3))
+        fd = fd_dir_x(rec, box, target, u)
+        ad = float(np.sum(gx * u))
+        assert abs(ad - fd) <= tol * abs(fd), f"direction {trial}: adjoint {ad} fd {fd}"
+    flat = np.argsort(-np.abs(gx).ravel())[:4]
+    for f in flat:
+        u = np.zeros(rec.shape[0] * 3)
+        u[f] = 1.0
+        fd = fd_dir_x(rec, box, target, u.reshape(-1, 3))
+        ad = float(gx.ravel()[f])
+        assert abs(ad - fd) <= 5 * tol * abs(fd), f"component {f}: adjoint {ad} fd {fd}"
+
+
 @pytest.mark.gpu
 def test_checkpoint_interval_does_not_change_gradients():
     rec, box, target = fd_scene()
@@ -97,6 +154,7 @@ def test_checkpoint_interval_does_not_change_gradients():
     for r in res[1:]:
         assert r["loss"] == res[0]["loss"]
         np.testing.assert_allclose(r["grad_v0"], res[0]["grad_v0"], rtol=1e-4, atol=1e-7)
+        np.testing.assert_allclose(r["grad_x0"], res[0]["grad_x0"], rtol=1e-4, atol=1e-7)
         assert abs(r["grad_stiffness"] - res[0]["grad_stiffness"]) <= 1e-4 * abs(res[0]["grad_stiffness"])
 
 

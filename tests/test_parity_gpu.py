@@ -105,6 +105,51 @@ def test_config1_full_size_substeps_bit_exact():
         assert n * 10**7 - 27 * n <= mass <= n * 10**7
 
 
+def _production_step_matches_oracle_stages(sim, box, real_box, ctx, threads=0):
+    """One production substep (mpm_step(1) on the default path) from the
+    simulator's current state, checked against the oracle's stages from the
+    same state: the raw P2G accumulators that mpm_read_grid returns (clear +
+    p2g_1 + p2g_2, mls-mpm/p2g_1.wgsl:57-60, p2g_2.wgsl:87-89) bit for bit,
+    the decoded updateGrid velocities (updateGrid.wgsl:26-43) bit for bit,
+    then g2p + copyPosition."""
+    ref = oracle.Scene(sim.read_particles(), box)
+    ref.change_box_size(real_box)
+    sim.execute(1)
+    g = sim.read_grid()
+    vel = sim.read_grid_velocity()
+    for stage in (0, 1, 2):
+        ref.run_stage(stage)
+    bad = np.nonzero(np.any(g != ref.cells, axis=1))[0]
+    assert bad.size == 0, f"{ctx}: {bad.size} raw accumulators differ, first {bad[:5]}: " \
+                          f"{g[bad[:3]].tolist()} vs {ref.cells[bad[:3]].tolist()}"
+    assert np.count_nonzero(g[:, 3]) > 0
+    ref.run_stage(3)
+    want = ref.cells[:, :3].astype(np.float32) / np.float32(1e7)   # decodeFixedPoint
+    bad = np.nonzero(~np.all(bits_equal(vel[:, :3], want), axis=1))[0]
+    assert bad.size == 0, f"{ctx}: {bad.size} updated velocities differ, first {bad[:5]}: " \
+                          f"{vel[bad[:3]].tolist()} vs {want[bad[:3]].tolist()}"
+    assert np.all(vel[:, 3] == 0)
+    ref.run_stage(4)
+    ref.run_stage(5)
+    assert_same(sim, ref, ("particles", "posvel"), ctx=ctx)
+
+
+@pytest.mark.parametrize("cfg,frames,resize", [("config0", 3, None), ("config1", 2, None),
+                                               ("config1", 2, [64.0, 80.0, 80.0]),
+                                               ("config2", 1, None)],
+                         ids=["40k", "300k", "300k-resized", "2M"])
+def test_production_grid_accumulators_bit_exact(cfg, frames, resize):
+    rec, box = getattr(scenes, cfg)()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        sim.execute(2 * frames)
+        real = list(box)
+        if resize is not None:
+            sim.change_box_size(resize)
+            real = resize
+        for k in range(2):
+            _production_step_matches_oracle_stages(sim, box, real, f"{cfg} substep {k}")
+
+
 def test_rollouts_while_the_host_is_busy_match_the_oracle():
     """Regression: many-substep calls made while every host core is busy (the
     oracle's own threads) must still give the oracle's result.  A counting
@@ -269,6 +314,39 @@ def test_errors_are_status_codes():
         sim.reset(10, [8.0, 8.0, 8.0], particles=scenes.small_scene(10, [8.0, 8.0, 8.0], 1))
         sim.execute()
         assert sim.read_posvel().shape == (10, 8)
+
+
+@pytest.mark.parametrize("fail_after", [0, 7, 21])
+def test_failed_reset_leaves_a_handle_that_refuses_work(monkeypatch, fail_after):
+    """A reset whose device allocation fails (forced from the host with the
+    library's MPM355_FAIL_ALLOC_AFTER knob) releases everything: every later
+    call reports MPM_ERR_NOT_READY instead of launching on missing buffers,
+    until a reset succeeds."""
+    box, big_box = [16.0, 16.0, 16.0], [32.0, 32.0, 32.0]
+    rec = scenes.small_scene(500, box, seed=4)
+    big = scenes.small_scene(5000, big_box, seed=5)   # larger buffers: reallocation
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        sim.execute()
+        monkeypatch.setenv("MPM355_FAIL_ALLOC_AFTER", str(fail_after))
+        with pytest.raises(MPMError) as ei:
+            sim.reset(5000, big_box, particles=big)
+        assert ei.value.status == 3   # MPM_ERR_OUT_OF_MEMORY
+        monkeypatch.delenv("MPM355_FAIL_ALLOC_AFTER")
+        calls = [lambda: sim.execute(), lambda: sim.read_posvel(), lambda: sim.read_particles(),
+                 lambda: sim.read_grid(), lambda: sim.change_box_size([8.0, 8.0, 8.0]),
+                 lambda: sim.run_stage(0), lambda: sim.backward(2, [1.0, 1.0, 1.0]),
+                 lambda: sim.posvel_device_ptr(), lambda: sim.time_stages(1),
+                 lambda: sim.num_particles]
+        for k, call in enumerate(calls):
+            with pytest.raises(MPMError) as ei:
+                call()
+            assert ei.value.status == 6, f"call {k}"   # MPM_ERR_NOT_READY
+        sim.synchronize()
+        sim.reset(5000, big_box, particles=big)
+        ref = oracle.Scene(big, big_box)
+        sim.execute()
+        ref.execute(2)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="after the failed reset")
 
 
 TUNINGS = [

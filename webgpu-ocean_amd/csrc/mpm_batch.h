@@ -52,9 +52,12 @@ public:
     ~Batch() { release(); }
 
     // (Re)creates the batch: packs every scene's particles, grid and keys.
+    // On failure everything is released and ready() is false.
     hipError_t init(const SceneInit *scenes, uint32_t count, hipStream_t st);
     void release();
 
+    // False until init succeeds, and again after a failed init.
+    bool ready() const { return !h_scenes_.empty(); }
     uint32_t scenes() const { return (uint32_t)h_scenes_.size(); }
     const SceneDesc &scene(uint32_t s) const { return h_scenes_[s]; }
 
@@ -67,6 +70,9 @@ public:
     hipError_t read_particles(uint32_t s, void *dst, uint32_t n, hipStream_t st);
     hipError_t write_particles(uint32_t s, const void *src, uint32_t n, hipStream_t st);
     hipError_t read_grid(uint32_t s, int32_t *dst, uint64_t ncells, hipStream_t st);
+    // Decoded updated grid velocities (vx, vy, vz, 0) of the last production
+    // substep, G x 4 floats.
+    hipError_t read_vel(uint32_t s, float *dst, uint64_t ncells, hipStream_t st);
     hipError_t write_grid(uint32_t s, const int32_t *src, uint64_t ncells, hipStream_t st);
     void *posvel_ptr(uint32_t s) const { return posvel_ + 2 * (size_t)h_scenes_[s].poff; }
 
@@ -89,6 +95,7 @@ public:
     hipError_t backward(const AdjointConfig &cfg, hipStream_t st, AdjointResult &res);
 
 private:
+    hipError_t init_impl(const SceneInit *scenes, uint32_t count, hipStream_t st);
     hipError_t ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys,
                                uint64_t tchunks);
     hipError_t upload_chunks();

@@ -2,6 +2,7 @@
 // batch of scenes.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -53,11 +54,32 @@ Uniforms derive_uniforms(const float init_box[3], const float real_box[3]) {
     return u;
 }
 
+namespace {
+
+// Device allocation of the batch buffers.  MPM355_FAIL_ALLOC_AFTER=k (a
+// fault-injection knob for the failure-handling tests) makes the allocation
+// after the first k of one (re)allocation pass fail as out of memory.
+struct Allocator {
+    long budget = -1;
+    Allocator() {
+        if (const char *e = std::getenv("MPM355_FAIL_ALLOC_AFTER")) budget = std::atol(e);
+    }
+    template <typename T>
+    hipError_t operator()(T **p, size_t bytes) {
+        if (budget == 0) return hipErrorOutOfMemory;
+        if (budget > 0) --budget;
+        return hipMalloc(p, bytes);
+    }
+};
+
+}  // namespace
+
 hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys,
                                   uint64_t tchunks) {
     const uint32_t cap = std::max<uint32_t>(64u, (total_n + 63u) & ~63u);
     if (cap <= cap_n_ && cells <= cap_cells_ && keys <= cap_keys_ && tchunks <= cap_tchunks_)
         return hipSuccess;
+    Allocator alloc;
     drop_graphs();
     free_ptr(state_);
     free_ptr(spare_state_);
@@ -85,36 +107,36 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     cap_cells_ = cap_keys_ = cap_tchunks_ = 0;
     const size_t c = cap, g = std::max<uint64_t>(cells, 1), k = std::max<uint64_t>(keys, 1),
                  tc = std::max<uint64_t>(tchunks, 1);
-    TRY(hipMalloc(&state_, sizeof(float) * NCOMP * c));
+    TRY(alloc(&state_, sizeof(float) * NCOMP * c));
     state_buf0_ = state_;
-    TRY(hipMalloc(&spare_state_, sizeof(float) * NCOMP * c));
-    TRY(hipMalloc(&id_, sizeof(uint32_t) * c));
-    TRY(hipMalloc(&spare_id_, sizeof(uint32_t) * c));
-    TRY(hipMalloc(&grid_, sizeof(int4) * g));
-    TRY(hipMalloc(&vel_, sizeof(float4) * g));
-    TRY(hipMalloc(&wild_, sizeof(int4) * g));
+    TRY(alloc(&spare_state_, sizeof(float) * NCOMP * c));
+    TRY(alloc(&id_, sizeof(uint32_t) * c));
+    TRY(alloc(&spare_id_, sizeof(uint32_t) * c));
+    TRY(alloc(&grid_, sizeof(int4) * g));
+    TRY(alloc(&vel_, sizeof(float4) * g));
+    TRY(alloc(&wild_, sizeof(int4) * g));
     TRY(hipMemset(wild_, 0, sizeof(int4) * g));
-    TRY(hipMalloc(&partial_, sizeof(int4) * kWinCells * tc));
-    TRY(hipMalloc(&posvel_, 2 * sizeof(float4) * c));
-    TRY(hipMalloc(&staging_, 80 * c));
-    TRY(hipMalloc(&key_, sizeof(uint32_t) * c));
-    TRY(hipMalloc(&rank_, sizeof(uint32_t) * c));
-    TRY(hipMalloc(&src_, sizeof(uint32_t) * c));
-    TRY(hipMalloc(&mig_, sizeof(uint32_t) * 27 * tc));
-    TRY(hipMalloc(&order_, sizeof(uint32_t) * c));
-    TRY(hipMalloc(&perm_, sizeof(uint32_t) * c));
+    TRY(alloc(&partial_, sizeof(int4) * kWinCells * tc));
+    TRY(alloc(&posvel_, 2 * sizeof(float4) * c));
+    TRY(alloc(&staging_, 80 * c));
+    TRY(alloc(&key_, sizeof(uint32_t) * c));
+    TRY(alloc(&rank_, sizeof(uint32_t) * c));
+    TRY(alloc(&src_, sizeof(uint32_t) * c));
+    TRY(alloc(&mig_, sizeof(uint32_t) * 27 * tc));
+    TRY(alloc(&order_, sizeof(uint32_t) * c));
+    TRY(alloc(&perm_, sizeof(uint32_t) * c));
     TRY(hipMemset(perm_, 0, sizeof(uint32_t) * c));  // every entry a valid slot
-    TRY(hipMalloc(&count_, sizeof(uint32_t) * k));
-    TRY(hipMalloc(&offset_, sizeof(uint32_t) * k));
+    TRY(alloc(&count_, sizeof(uint32_t) * k));
+    TRY(alloc(&offset_, sizeof(uint32_t) * k));
     const size_t nsup = (k + (1u << kSupBits) - 1) >> kSupBits;
-    TRY(hipMalloc(&sup_, sizeof(uint2) * 2 * nsup));  // two substep parities
+    TRY(alloc(&sup_, sizeof(uint2) * 2 * nsup));  // two substep parities
     TRY(hipMemset(sup_, 0, sizeof(uint2) * 2 * nsup));
     sup_cap_ = nsup;
-    TRY(hipMalloc(&tchunks_, sizeof(TileChunk) * tc));
+    TRY(alloc(&tchunks_, sizeof(TileChunk) * tc));
     // chunk count, then the kXcds + 1 XCD range boundaries of the chunk walk
-    TRY(hipMalloc(&ntchunks_, sizeof(uint32_t) * (2 + kXcds)));
+    TRY(alloc(&ntchunks_, sizeof(uint32_t) * (2 + kXcds)));
     TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t) * (2 + kXcds)));
-    TRY(hipMalloc(&tile_chunk_, sizeof(uint2) * k));
+    TRY(alloc(&tile_chunk_, sizeof(uint2) * k));
     TRY(hipMemset(count_, 0, sizeof(uint32_t) * k));
     TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t)));
     cap_n_ = cap;
@@ -124,7 +146,18 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     return hipSuccess;
 }
 
+// A failed (re)initialisation releases everything: the batch then holds no
+// scene (ready() is false) rather than scene records over missing buffers.
 hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
+    const hipError_t e = init_impl(sc, count, st);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        release();
+    }
+    return e;
+}
+
+hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st) {
     TRY(hipStreamSynchronize(st));
     drop_graphs();
     h_scenes_.assign(count, SceneDesc{});
@@ -532,6 +565,13 @@ hipError_t Batch::write_particles(uint32_t s, const void *src, uint32_t n, hipSt
 hipError_t Batch::read_grid(uint32_t s, int32_t *dst, uint64_t ncells, hipStream_t st) {
     if (ncells)
         TRY(hipMemcpyAsync(dst, grid_ + h_scenes_[s].goff, 16 * (size_t)ncells,
+                           hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t Batch::read_vel(uint32_t s, float *dst, uint64_t ncells, hipStream_t st) {
+    if (ncells)
+        TRY(hipMemcpyAsync(dst, vel_ + h_scenes_[s].goff, 16 * (size_t)ncells,
                            hipMemcpyDeviceToHost, st));
     return hipStreamSynchronize(st);
 }

@@ -62,6 +62,15 @@ int select_device(const Context &c) {
     return MPM_OK;
 }
 
+// Every call that touches the scene goes through this: after a failed reset
+// the handle holds no scene and no buffers, and must not launch anything.
+int ready(const Context &c) {
+    if (!c.batch.ready())
+        return fail(MPM_ERR_NOT_READY,
+                    "the handle holds no scene (its last reset failed); call mpm_reset again");
+    return select_device(c);
+}
+
 int check_params(const mpm_params &p) {
     const float vals[5] = {p.stiffness, p.rest_density, p.dynamic_viscosity, p.dt,
                            p.fixed_point_multiplier};
@@ -139,7 +148,7 @@ int from_c_tuning(const mpm_tuning *in, mpm::Batch &b) {
 int time_stages_impl(Context &c, uint32_t substeps, float *stage_ms, int *calls,
                      const char **names, int capacity, int *count) {
     if (!stage_ms || !count) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
-    int rc = select_device(c);
+    int rc = ready(c);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c.stream));
     hipError_t e =
@@ -149,7 +158,7 @@ int time_stages_impl(Context &c, uint32_t substeps, float *stage_ms, int *calls,
 }
 
 int time_substeps(Context &c, uint32_t substeps, float *ms) {
-    int rc = select_device(c);
+    int rc = ready(c);
     if (rc) return rc;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
@@ -243,7 +252,7 @@ int mpm_reset(mpm_sim *sim, const void *particles, uint32_t n, const float init_
                       to_params(sim->params)};
     rc = check_scenes(&si, 1);
     if (rc) return rc;
-    rc = select_device(sim->ctx);
+    rc = select_device(sim->ctx);  // (a reset is what makes a failed handle usable again)
     if (rc) return rc;
     hipError_t e = sim->ctx.batch.init(&si, 1, sim->ctx.stream);
     if (e != hipSuccess) return hip_fail(e, "reset");
@@ -254,7 +263,7 @@ int mpm_set_box(mpm_sim *sim, const float real_box[3]) {
     if (!sim) return fail(MPM_ERR_INVALID_ARGUMENT, "sim is NULL");
     int rc = check_box(real_box, "real_box");
     if (rc) return rc;
-    rc = select_device(sim->ctx);
+    rc = ready(sim->ctx);
     if (rc) return rc;
     HIP_TRY(sim->ctx.batch.set_box(0, real_box, sim->ctx.stream));
     return MPM_OK;
@@ -262,7 +271,7 @@ int mpm_set_box(mpm_sim *sim, const float real_box[3]) {
 
 int mpm_step(mpm_sim *sim, uint32_t substeps) {
     if (!sim) return fail(MPM_ERR_INVALID_ARGUMENT, "sim is NULL");
-    int rc = select_device(sim->ctx);
+    int rc = ready(sim->ctx);
     if (rc) return rc;
     for (uint32_t k = 0; k < substeps; ++k) HIP_TRY(sim->ctx.batch.substep(sim->ctx.stream));
     return MPM_OK;
@@ -272,7 +281,7 @@ int mpm_run_stage(mpm_sim *sim, int stage) {
     if (!sim) return fail(MPM_ERR_INVALID_ARGUMENT, "sim is NULL");
     if (stage < 0 || stage >= MPM_STAGE_COUNT)
         return fail(MPM_ERR_INVALID_ARGUMENT, "stage out of range");
-    int rc = select_device(sim->ctx);
+    int rc = ready(sim->ctx);
     if (rc) return rc;
     HIP_TRY(mpm::launch_stage(stage, sim->ctx.batch.stage_view(), sim->ctx.stream));
     HIP_TRY(sim->ctx.batch.external_change(sim->ctx.stream));
@@ -289,62 +298,80 @@ int mpm_synchronize(mpm_sim *sim) {
 
 int mpm_read_posvel(mpm_sim *sim, void *dst, uint32_t n) {
     if (!sim || (!dst && n)) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
     if (n > sim->ctx.batch.scene(0).n)
         return fail(MPM_ERR_INVALID_ARGUMENT, "n exceeds numParticles");
-    int rc = select_device(sim->ctx);
-    if (rc) return rc;
     HIP_TRY(sim->ctx.batch.read_posvel(0, dst, n, sim->ctx.stream));
     return MPM_OK;
 }
 
 int mpm_posvel_device_ptr(mpm_sim *sim, void **out) {
     if (!sim || !out) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
     *out = sim->ctx.batch.posvel_ptr(0);
     return MPM_OK;
 }
 
 int mpm_read_particles(mpm_sim *sim, void *dst, uint32_t n) {
     if (!sim || (!dst && n)) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
     if (n > sim->ctx.batch.scene(0).n)
         return fail(MPM_ERR_INVALID_ARGUMENT, "n exceeds numParticles");
-    int rc = select_device(sim->ctx);
-    if (rc) return rc;
     HIP_TRY(sim->ctx.batch.read_particles(0, dst, n, sim->ctx.stream));
     return MPM_OK;
 }
 
 int mpm_write_particles(mpm_sim *sim, const void *src, uint32_t n) {
     if (!sim || (!src && n)) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
     if (n != sim->ctx.batch.scene(0).n)
         return fail(MPM_ERR_INVALID_ARGUMENT, "n must equal numParticles (use mpm_reset)");
-    int rc = select_device(sim->ctx);
-    if (rc) return rc;
     HIP_TRY(sim->ctx.batch.write_particles(0, src, n, sim->ctx.stream));
     return MPM_OK;
 }
 
 int mpm_read_grid(mpm_sim *sim, int32_t *dst, uint64_t ncells) {
     if (!sim || (!dst && ncells)) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
     if (ncells > sim->ctx.batch.scene(0).G)
         return fail(MPM_ERR_INVALID_ARGUMENT, "ncells exceeds gridCount");
-    int rc = select_device(sim->ctx);
-    if (rc) return rc;
     HIP_TRY(sim->ctx.batch.read_grid(0, dst, ncells, sim->ctx.stream));
+    return MPM_OK;
+}
+
+int mpm_read_grid_velocity(mpm_sim *sim, float *dst, uint64_t ncells) {
+    if (!sim || (!dst && ncells)) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
+    if (ncells > sim->ctx.batch.scene(0).G)
+        return fail(MPM_ERR_INVALID_ARGUMENT, "ncells exceeds gridCount");
+    HIP_TRY(sim->ctx.batch.read_vel(0, dst, ncells, sim->ctx.stream));
     return MPM_OK;
 }
 
 int mpm_write_grid(mpm_sim *sim, const int32_t *src, uint64_t ncells) {
     if (!sim || (!src && ncells)) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
+    int rc = ready(sim->ctx);
+    if (rc) return rc;
     if (ncells > sim->ctx.batch.scene(0).G)
         return fail(MPM_ERR_INVALID_ARGUMENT, "ncells exceeds gridCount");
-    int rc = select_device(sim->ctx);
-    if (rc) return rc;
     HIP_TRY(sim->ctx.batch.write_grid(0, src, ncells, sim->ctx.stream));
     return MPM_OK;
 }
 
 int mpm_get_counts(mpm_sim *sim, uint32_t *num_particles, uint64_t *grid_count) {
     if (!sim) return fail(MPM_ERR_INVALID_ARGUMENT, "sim is NULL");
+    if (!sim->ctx.batch.ready()) {
+        if (num_particles) *num_particles = 0;
+        if (grid_count) *grid_count = 0;
+        return fail(MPM_ERR_NOT_READY, "the handle holds no scene (its last reset failed)");
+    }
     if (num_particles) *num_particles = sim->ctx.batch.scene(0).n;
     if (grid_count) *grid_count = sim->ctx.batch.scene(0).G;
     return MPM_OK;
@@ -369,7 +396,7 @@ int mpm_get_tuning(mpm_sim *sim, mpm_tuning *out) {
 
 int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in) {
     if (!sim || !in) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
-    int rc = select_device(sim->ctx);
+    int rc = ready(sim->ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(sim->ctx.stream));
     return from_c_tuning(in, sim->ctx.batch);
@@ -377,7 +404,7 @@ int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in) {
 
 int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen) {
     if (!sim) return fail(MPM_ERR_INVALID_ARGUMENT, "sim is NULL");
-    int rc = select_device(sim->ctx);
+    int rc = ready(sim->ctx);
     if (rc) return rc;
     mpm::Tuning t;
     HIP_TRY(sim->ctx.batch.autotune(sim->ctx.stream, substeps_per_trial ? substeps_per_trial : 4,
@@ -395,7 +422,7 @@ int mpm_backward(mpm_sim *sim, const mpm_adjoint_config *cfg, float *grad_v0, fl
         if (!std::isfinite(cfg->target[a]))
             return fail(MPM_ERR_INVALID_ARGUMENT, "target must be finite");
     if (!std::isfinite(cfg->ke_weight)) return fail(MPM_ERR_INVALID_ARGUMENT, "ke_weight must be finite");
-    int rc = select_device(sim->ctx);
+    int rc = ready(sim->ctx);
     if (rc) return rc;
     mpm::AdjointConfig ac{cfg->substeps, cfg->checkpoint_every,
                           {cfg->target[0], cfg->target[1], cfg->target[2]}, cfg->ke_weight};
@@ -458,7 +485,7 @@ static int check_scene_index(mpm_batch *b, uint32_t scene) {
 
 int mpm_batch_step(mpm_batch *b, uint32_t substeps) {
     if (!b) return fail(MPM_ERR_INVALID_ARGUMENT, "batch is NULL");
-    int rc = select_device(b->ctx);
+    int rc = ready(b->ctx);
     if (rc) return rc;
     for (uint32_t k = 0; k < substeps; ++k) HIP_TRY(b->ctx.batch.substep(b->ctx.stream));
     return MPM_OK;
@@ -469,7 +496,7 @@ int mpm_batch_set_box(mpm_batch *b, uint32_t scene, const float real_box[3]) {
     if (rc) return rc;
     rc = check_box(real_box, "real_box");
     if (rc) return rc;
-    rc = select_device(b->ctx);
+    rc = ready(b->ctx);
     if (rc) return rc;
     HIP_TRY(b->ctx.batch.set_box(scene, real_box, b->ctx.stream));
     return MPM_OK;
@@ -477,7 +504,7 @@ int mpm_batch_set_box(mpm_batch *b, uint32_t scene, const float real_box[3]) {
 
 int mpm_batch_synchronize(mpm_batch *b) {
     if (!b) return fail(MPM_ERR_INVALID_ARGUMENT, "batch is NULL");
-    int rc = select_device(b->ctx);
+    int rc = ready(b->ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->ctx.stream));
     return MPM_OK;
@@ -498,7 +525,7 @@ int mpm_batch_read_posvel(mpm_batch *b, uint32_t scene, void *dst, uint32_t n) {
     if (!dst && n) return fail(MPM_ERR_INVALID_ARGUMENT, "dst is NULL");
     if (n > b->ctx.batch.scene(scene).n)
         return fail(MPM_ERR_INVALID_ARGUMENT, "n exceeds the scene's particles");
-    rc = select_device(b->ctx);
+    rc = ready(b->ctx);
     if (rc) return rc;
     HIP_TRY(b->ctx.batch.read_posvel(scene, dst, n, b->ctx.stream));
     return MPM_OK;
@@ -510,7 +537,7 @@ int mpm_batch_read_particles(mpm_batch *b, uint32_t scene, void *dst, uint32_t n
     if (!dst && n) return fail(MPM_ERR_INVALID_ARGUMENT, "dst is NULL");
     if (n > b->ctx.batch.scene(scene).n)
         return fail(MPM_ERR_INVALID_ARGUMENT, "n exceeds the scene's particles");
-    rc = select_device(b->ctx);
+    rc = ready(b->ctx);
     if (rc) return rc;
     HIP_TRY(b->ctx.batch.read_particles(scene, dst, n, b->ctx.stream));
     return MPM_OK;
@@ -524,7 +551,7 @@ int mpm_batch_get_tuning(mpm_batch *b, mpm_tuning *out) {
 
 int mpm_batch_set_tuning(mpm_batch *b, const mpm_tuning *in) {
     if (!b || !in) return fail(MPM_ERR_INVALID_ARGUMENT, "NULL argument");
-    int rc = select_device(b->ctx);
+    int rc = ready(b->ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->ctx.stream));
     return from_c_tuning(in, b->ctx.batch);
@@ -532,7 +559,7 @@ int mpm_batch_set_tuning(mpm_batch *b, const mpm_tuning *in) {
 
 int mpm_batch_autotune(mpm_batch *b, uint32_t substeps_per_trial, mpm_tuning *chosen) {
     if (!b) return fail(MPM_ERR_INVALID_ARGUMENT, "batch is NULL");
-    int rc = select_device(b->ctx);
+    int rc = ready(b->ctx);
     if (rc) return rc;
     mpm::Tuning t;
     HIP_TRY(b->ctx.batch.autotune(b->ctx.stream, substeps_per_trial ? substeps_per_trial : 4, &t));

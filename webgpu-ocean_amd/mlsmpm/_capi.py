@@ -20,6 +20,7 @@ STATUS_NAMES = {
     3: "MPM_ERR_OUT_OF_MEMORY",
     4: "MPM_ERR_DEVICE",
     5: "MPM_ERR_NO_DEVICE",
+    6: "MPM_ERR_NOT_READY",
 }
 STAGES = ("clear_grid", "p2g_1", "p2g_2", "update_grid", "g2p", "copy_position")
 
@@ -29,6 +30,7 @@ EXPORTED = (
     "mpm_destroy", "mpm_reset", "mpm_set_box", "mpm_step", "mpm_run_stage",
     "mpm_synchronize", "mpm_read_posvel", "mpm_posvel_device_ptr",
     "mpm_read_particles", "mpm_write_particles", "mpm_read_grid", "mpm_write_grid",
+    "mpm_read_grid_velocity",
     "mpm_get_counts", "mpm_time_steps", "mpm_time_stages", "mpm_get_tuning",
     "mpm_set_tuning", "mpm_autotune", "mpm_backward",
     "mpm_batch_create", "mpm_batch_destroy", "mpm_batch_step", "mpm_batch_set_box",
@@ -112,6 +114,7 @@ def load(path: str = LIB_PATH):
         "mpm_write_particles": ([P, P, u32], i32),
         "mpm_read_grid": ([P, P, u64], i32),
         "mpm_write_grid": ([P, P, u64], i32),
+        "mpm_read_grid_velocity": ([P, P, u64], i32),
         "mpm_get_counts": ([P, ctypes.POINTER(u32), ctypes.POINTER(u64)], i32),
         "mpm_time_steps": ([P, u32, u32, ctypes.POINTER(ctypes.c_float)], i32),
         "mpm_time_stages": ([P, u32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32),

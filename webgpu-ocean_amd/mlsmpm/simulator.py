@@ -134,6 +134,13 @@ class MLSMPMSimulator:
         _capi.check(self._L.mpm_read_grid(self._h, self._data(out), out.shape[0]))
         return out
 
+    def read_grid_velocity(self) -> np.ndarray:
+        """(G, 4) float32: decoded grid velocities after updateGrid of the last
+        production substep (x, y, z, 0)."""
+        out = np.zeros((self.grid_count, 4), dtype=np.float32)
+        _capi.check(self._L.mpm_read_grid_velocity(self._h, self._data(out), out.shape[0]))
+        return out
+
     def write_grid(self, cells: np.ndarray):
         cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
         _capi.check(self._L.mpm_write_grid(self._h, self._data(cells), cells.shape[0]))
