@@ -3,13 +3,14 @@
 One "step" is one reference frame: execute() = 2 substeps
 (mls-mpm/mls-mpm.ts:245-268).  The workload is the 300,000-particle dam break
 on an 80^3 grid: the timed region replays its frames from the initial state,
-resizing the box to 64 cells in x at frame 500 (when the run reaches it).
-Inputs are resident in HBM before timing starts.
+resizing the box to 64 cells in x at frame min(500, frames / 2).  Inputs are
+resident in HBM before timing starts.
 
 Prints ONE JSON line with the throughput (particle-substeps/s), the roofline
-of the dominant kernel (algorithmic bytes / HIP-event kernel time, averaged
-over an eager replay of the timed frames) and the CPU baseline (the oracle's
-OpenMP restatement on the host cores).
+of the slowest kernel and of G2P (algorithmic bytes / HIP-event kernel time,
+averaged over an eager replay of the timed frames), the 2M-particle scene,
+the batched and adjoint comparisons, and the CPU baseline (the oracle's
+OpenMP restatement on the host cores, and on one core).
 """
 from __future__ import annotations
 
@@ -29,37 +30,62 @@ SUBSTEPS_PER_FRAME = 2
 RESIZE_FRAME = 500
 
 
+# Algorithmic HBM bytes per launch, per kernel: the minimum traffic of the
+# four barrier-separated phases of the reference's substep, 204 B per particle
+# + 84 B per cell in total (particle fields: x 12 B, v 12 B, C 36 B; cells
+# 16 B = vx, vy, vz, mass):
+#   p2g_1     reads x, v, C                          60 / particle
+#   p2g_1 flush writes the cells (clear fused)       16 / cell
+#   p2g_2     reads x, C                             48 / particle
+#             reads masses, RMW of vx, vy, vz         4 + 24 / cell
+#   updateGrid reads 16, writes 12                   28 / cell
+#   g2p       reads x, writes x, v, C, PosVel(24)    96 / particle
+#             gathers updated velocities             12 / cell
+# Each production kernel is charged the phases it performs; the binning and
+# the intermediates (permutation, ids, tile partials) carry no algorithmic
+# bytes: they are listed apart as overhead_bytes.
 def stage_bytes(name: str, n: int, g: int, binned_path: bool = True) -> int:
-    """Algorithmic HBM bytes of one launch of each kernel.
-
-    Particle fields: x 12 B, v 12 B, C 36 B; cells 16 B = (vx, vy, vz, mass).
-    These are the minimum bytes the data dependencies force per kernel; with
-    clearGrid fused into p2g_1's flush and copyPosition into g2p (which then
-    need not re-read x and v), the reference's substep needs 204 B/particle +
-    84 B/cell.
-    The binned path's intermediates (tile partials, bin keys) are not
-    counted: they are overhead the roofline fraction should expose.
-    """
-    table = {
-        # reference-shaped stage kernels
-        "clear_grid": 16 * g,                     # write 4 x i32
-        "p2g_1": 60 * n + 16 * g,                 # read x, v, C; write cells
-        "p2g_2": 48 * n + 28 * g,                 # read x, C; mass read + v RMW
-        "update_grid": 28 * g,                    # read 16, write 12
-        "g2p": (12 + 60) * n + 12 * g,            # read x; write x, v, C; gather v
-        "copy_position": 48 * n,                  # read x, v; write PosVel (24 useful)
+    """Algorithmic HBM bytes of one launch of kernel slot `name`."""
+    reference_shaped = {
+        "clear_grid": 16 * g,
+        "p2g_1": 60 * n,
+        "p2g_2": 48 * n + 28 * g,
+        "update_grid": 28 * g,
+        "g2p": (12 + 60) * n + 12 * g,
+        "copy_position": 24 * n,
     }
     binned = {
-        "bin_gather": 8 * (g // 64),              # per tile: count in, running offset out
-        "bin_scan": 16 * (g // 64),               # per tile: offset, chunk range
-        "bin_scatter": 16 * n,                    # read key, rank, src; write the permutation
-        "p2g_1": 64 * n,                          # permutation; read x, v, C (cells: reduce_1)
-        "reduce_1": 16 * g,                       # write cells
-        "p2g_2": 52 * n + 4 * g,                  # permutation; read x, C; read masses
-        "reduce_2": 16 * g + 12 * g + 12 * g,     # read cells; write momentum, velocity
-        "g2p": (4 + 12 + 4 + 60 + 4 + 24) * n + 12 * g,  # perm, x, id in; x, v, C, id, PosVel out
+        "bin_gather": 0,
+        "bin_scan": 0,
+        "bin_scatter": 0,
+        "p2g_1": 60 * n,
+        "reduce_1": 16 * g,                 # the p2g_1 flush
+        "p2g_2": 48 * n + 4 * g,
+        "reduce_2": 24 * g + 28 * g,        # p2g_2's momentum RMW + updateGrid
+        "g2p": 96 * n + 12 * g,
     }
-    return (binned if binned_path else table)[name]
+    return (binned if binned_path else reference_shaped).get(name, 0)
+
+
+def overhead_bytes(name: str, n: int, g: int) -> int:
+    """Bytes of the production path's intermediates per launch (not part of
+    the algorithmic bytes): per particle the binning key/rank/source and
+    permutation, the level order, the caller ids; per 4^3 tile its count,
+    offset and chunk range.  Tile partial windows are not included (their
+    size depends on the particle distribution)."""
+    tiles = g // 64
+    table = {
+        "bin_gather": 8 * tiles,
+        "bin_scan": 16 * tiles,
+        "bin_scatter": 16 * n,
+        "p2g_1": 8 * n,        # permutation in, level order out
+        "p2g_2": 4 * n,        # level order in
+        "g2p": 24 * n,         # permutation, id in; id, key, rank, source out
+    }
+    return table.get(name, 0)
+
+
+G2P_BYTES_DOC = "G2P: 96 B/particle + 12 B/cell (x in; x, v, C, PosVel out; 27-node gather)"
 
 
 CALIBRATION_BYTES = 64 << 20
@@ -241,32 +267,57 @@ def host_transfers(sim, rec, frame_ms: float, reps: int = 5) -> dict:
     }
 
 
-def cpu_baseline(rec, box, seconds: float = 12.0) -> dict:
-    """Oracle (CPU restatement of the reference WGSL) timed on the host cores."""
+def cpu_threads() -> tuple[int, int]:
+    """(threads used, host CPUs this process may run on).  All CPUs of the
+    affinity mask, capped by OMP_NUM_THREADS where the host sets it (a
+    shared GPU host grants each GPU job a CPU share and says so there)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    used = min(avail, int(cap)) if cap.isdigit() and int(cap) > 0 else avail
+    return max(1, used), avail
+
+
+def cpu_baseline(rec, box, seconds: float = 10.0, reps: int = 5) -> dict:
+    """Oracle (CPU restatement of the reference WGSL) timed on the host cores:
+    median of `reps` runs of k substeps each, on every allowed core and on
+    one core, about `seconds` of work per leg."""
+    import statistics
+
     import oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    sc = oracle.Scene(rec, box)
-    t0 = time.perf_counter()
-    sc.substep(threads=threads)
-    one = time.perf_counter() - t0
-    k = max(2, min(400, int(seconds / max(one, 1e-4))))
-    t0 = time.perf_counter()
-    for _ in range(k):
-        sc.substep(threads=threads)
-    dt = time.perf_counter() - t0
     n = rec.shape[0]
+    threads, avail = cpu_threads()
+
+    def leg(th):
+        sc = oracle.Scene(rec, box)
+        t0 = time.perf_counter()
+        sc.substep(threads=th)
+        one = time.perf_counter() - t0
+        k = max(1, min(200, int(seconds / reps / max(one, 1e-4))))
+        rates = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                sc.substep(threads=th)
+            rates.append(n * k / (time.perf_counter() - t0))
+        return statistics.median(rates), k
+
+    value, k = leg(threads)
+    value1, k1 = leg(1)
     return {
-        "value": n * k / dt,
+        "value": value,
         "unit": "particle-substeps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{k} substeps of the same {n}-particle 80^3 scene "
-                  f"({dt:.1f} s, OpenMP over particles, atomic i32 P2G)",
+        "sample": f"median of {reps} runs of {k} substeps ({threads} threads) and of {k1} "
+                  f"substeps (1 thread) of the same {n}-particle 80^3 dam break from its "
+                  "initial state (OpenMP over particles, atomic i32 P2G)",
+        "value_1_thread": value1,
+        "host_cpus_visible": avail,
     }
 
 
-def bench_batched(substeps: int = 20) -> dict:
+def bench_batched(substeps: int = 500) -> dict:
     """64 uneven scenes in one launch per stage (autotuned) vs the same scenes
     advanced one after another by single simulators; results are identical
     per scene (tests/test_batch_gpu.py)."""
@@ -341,6 +392,82 @@ def bench_adjoint(substeps: int = 64, every: int = 8) -> dict:
     }
 
 
+def timed_rollout(sim, frames: int, resize_frame: int, resize_box) -> float:
+    """Wall seconds of `frames` frames (hipGraph-replayed substeps), the box
+    changed at `resize_frame`; inputs are already on the device."""
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        if f == resize_frame:
+            sim.change_box_size(resize_box)
+        sim.execute()
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(sim, substeps: int, resize_frame: int, resize_box):
+    """Per-kernel HIP-event times of `substeps` eager substeps (frames of 2,
+    the box changed at `resize_frame`): ({slot: us per launch},
+    {slot: us per substep})."""
+    stage = {}
+    for f in range((substeps + SUBSTEPS_PER_FRAME - 1) // SUBSTEPS_PER_FRAME):
+        if f == resize_frame:
+            sim.change_box_size(resize_box)
+        k = min(SUBSTEPS_PER_FRAME, substeps - f * SUBSTEPS_PER_FRAME)
+        for name, (ms, c) in sim.time_stages(k).items():
+            acc = stage.setdefault(name, [0.0, 0])
+            acc[0] += ms
+            acc[1] += c
+    per_launch = {k: 1e3 * ms / max(c, 1) for k, (ms, c) in stage.items()}
+    per_substep = {k: 1e3 * ms / substeps for k, (ms, c) in stage.items()}
+    return per_launch, per_substep
+
+
+def roofline_entry(name, n, g, per_launch_us, binned, traffic=lambda k: None) -> dict:
+    """Algorithmic bytes of one launch of `name` over its average launch time."""
+    b = stage_bytes(name, n, g, binned)
+    us = per_launch_us[name]
+    achieved = b / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    return {
+        "kernel": name,
+        "achieved": achieved,
+        "frac": achieved / HBM_PEAK_GBS,
+        "avg_launch_us": us,
+        "algorithmic_bytes_per_launch": b,
+        "overhead_bytes_per_launch": overhead_bytes(name, n, g) if binned else 0,
+        "traffic": traffic(name),
+    }
+
+
+def bench_config2(frames: int = 300) -> dict:
+    """2,000,000 particles on 160^3 (two colliding blocks): a working set
+    (about 230 MB of particles and cells, ~500 MB with the buffers of the
+    binned path) beyond the 256 MB MALL, so the roofline is read where the
+    traffic goes to HBM.  Graph-replayed frames, then the same frames eagerly
+    with per-kernel events."""
+    from mlsmpm import MLSMPMSimulator, scenes
+
+    rec, box = scenes.config2()
+    n = rec.shape[0]
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        g = sim.grid_count
+        sim.execute(4)
+        sim.reset(n, box, particles=rec)
+        elapsed = timed_rollout(sim, frames, frames, box)
+        sim.reset(n, box, particles=rec)
+        per_launch, per_substep = kernel_times(sim, 2 * frames, frames, box)
+    dom = max(per_substep, key=per_substep.get)
+    return {
+        "workload": f"2M particles, 160^3 grid, two colliding blocks, {frames} frames",
+        "value": n * 2 * frames / elapsed,
+        "unit": "particle-substeps/s",
+        "wall_us_per_substep": 1e6 * elapsed / (2 * frames),
+        "roofline": roofline_entry(dom, n, g, per_launch, True),
+        "g2p": roofline_entry("g2p", n, g, per_launch, True),
+        "stages_us_per_launch": per_launch,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +523,9 @@ def main() -> int:
     g = sim.grid_count
     if args.tuning:
         sim.set_tuning(**json.loads(args.tuning))
+    # the box change lands inside the timed frames whatever their number
+    resize_frame = min(RESIZE_FRAME, args.steps // 2)
+    resize_box = [64.0, box[1], box[2]]
 
     # warmup on the same scene, then restart from the initial state
     for _ in range(args.warmup):
@@ -404,14 +534,7 @@ def main() -> int:
     sim.reset(n, box, particles=rec)
     sim.synchronize()
 
-    t0 = time.perf_counter()
-    for f in range(args.steps):
-        if f == RESIZE_FRAME:
-            sim.change_box_size([64.0, box[1], box[2]])
-        sim.execute()
-    sim.synchronize()
-    elapsed = time.perf_counter() - t0
-
+    elapsed = timed_rollout(sim, args.steps, resize_frame, resize_box)
     substeps = args.steps * SUBSTEPS_PER_FRAME
     value = n * substeps / elapsed
 
@@ -420,30 +543,23 @@ def main() -> int:
     # events bound to every kernel
     sim.reset(n, box, particles=rec)
     replay = min(args.stage_substeps or substeps, substeps)
-    stage = {}
-    for f in range((replay + SUBSTEPS_PER_FRAME - 1) // SUBSTEPS_PER_FRAME):
-        if f == RESIZE_FRAME:
-            sim.change_box_size([64.0, box[1], box[2]])
-        for k, (ms, c) in sim.time_stages(min(SUBSTEPS_PER_FRAME, replay - f * SUBSTEPS_PER_FRAME)).items():
-            acc = stage.setdefault(k, [0.0, 0])
-            acc[0] += ms
-            acc[1] += c
-    per_substep_us = {k: 1e3 * ms / replay for k, (ms, c) in stage.items()}
-    per_launch_us = {k: 1e3 * ms / max(c, 1) for k, (ms, c) in stage.items()}
-    # The dominant kernel; p2g_1, p2g_2 and g2p take about the same time, so
-    # within 10% of the slowest the roofline names G2P, the kernel the HBM
-    # target is set for, rather than whichever wins by noise.
-    dom = max(per_substep_us, key=per_substep_us.get)
-    for pref in ("g2p",):
-        if pref in per_substep_us and per_substep_us[pref] >= 0.9 * per_substep_us[dom]:
-            dom = pref
-    dom_bytes = stage_bytes(dom, n, g, sim.tuning["path"] == 0)
-    achieved = dom_bytes / (per_launch_us[dom] * 1e-6) / 1e9
-    substep_us = sum(per_substep_us.values())
+    per_launch_us, per_substep_us = kernel_times(sim, replay, resize_frame, resize_box)
     tuning = sim.tuning
-    total_bytes = 204 * n + 84 * g
+    binned = tuning["path"] == 0
     host = host_transfers(sim, rec, 1e3 * elapsed / args.steps)
     sim.close()
+
+    # the dominant kernel is the slowest one per substep; G2P, the kernel the
+    # HBM target is set for, is reported beside it
+    dom = max(per_substep_us, key=per_substep_us.get)
+    traffic = (lambda k: measured_traffic(k)) if tuning == DEFAULT_TUNING else (lambda k: None)
+    roof = roofline_entry(dom, n, g, per_launch_us, binned, traffic)
+    roof.update({"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s"})
+    roof["g2p"] = roofline_entry("g2p", n, g, per_launch_us, binned, traffic)
+    roof["bytes_basis"] = ("algorithmic bytes of the reference's phases (204 B/particle + "
+                           "84 B/cell per substep) charged per kernel; " + G2P_BYTES_DOC)
+    substep_us = sum(per_substep_us.values())
+    total_bytes = 204 * n + 84 * g
 
     out = {
         "metric": METRIC,
@@ -459,48 +575,35 @@ def main() -> int:
         "dtype": "f32 (i32 fixed-point grid)",
         "data": "synthetic",
         "config": {
-            "workload": "MLS-MPM dam break, 300k particles, 80^3 grid, 2 substeps/frame, "
-                        "box x resized to 64 at frame 500",
+            "workload": f"MLS-MPM dam break, 300k particles, 80^3 grid, 2 substeps/frame, "
+                        f"{args.steps} frames from the initial state, box x resized to 64 at "
+                        f"frame {resize_frame}",
             "particles": n,
             "grid": [int(b) for b in box],
             "grid_cells": g,
             "substeps_per_step": SUBSTEPS_PER_FRAME,
             "timed_frames": args.steps,
+            "resize_frame": resize_frame,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom) if tuning == DEFAULT_TUNING else None,
-            "algorithmic_bytes_per_launch": dom_bytes,
-            "avg_launch_us": per_launch_us[dom],
-        },
+        "roofline": roof,
         "stages_us_per_launch": per_launch_us,
         "stages_us_per_substep": per_substep_us,
-        # every kernel against the HBM peak (G2P, updateGrid = reduce_2, and
-        # the P2G flushes = reduce_1 / reduce_2 included)
-        "stages_roofline": {
-            k: {
-                "algorithmic_bytes": stage_bytes(k, n, g, tuning["path"] == 0),
-                "achieved_GBs": stage_bytes(k, n, g, tuning["path"] == 0) / (us * 1e-6) / 1e9,
-                "frac": stage_bytes(k, n, g, tuning["path"] == 0) / (us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                "traffic": measured_traffic(k) if tuning == DEFAULT_TUNING else None,
-            }
-            for k, us in per_launch_us.items()
-        },
+        "kernels_per_substep": len([k for k, v in per_substep_us.items() if v > 0]),
+        "stages_roofline": {k: roofline_entry(k, n, g, per_launch_us, binned, traffic)
+                            for k in per_launch_us},
         "tuning": tuning,
         "substep_roofline": {
             "bytes": total_bytes,
             "sum_kernel_us": substep_us,
             "achieved_GBs": total_bytes / (substep_us * 1e-6) / 1e9,
+            "frac": total_bytes / (substep_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
             "wall_us_per_substep": 1e6 * elapsed / substeps,
+            "wall_frac": total_bytes / (elapsed / substeps) / 1e9 / HBM_PEAK_GBS,
         },
         "host_transfers": host,
     }
     if not args.no_extras:
+        out["config2"] = bench_config2()
         out["batched"] = bench_batched()
         out["adjoint"] = bench_adjoint()
     if not args.no_cpu_baseline:

@@ -12,21 +12,37 @@ sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
 
-def test_reference_stage_bytes_give_the_fused_substep_minimum():
-    # clearGrid fused into p2g_1's flush, copyPosition into g2p (no re-read of
-    # x and v): 204 B per particle + 84 B per cell
-    def part(n, g, kernels):
-        return sum(bench.stage_bytes(k, n, g, binned_path=False) for k in kernels)
-    per_particle = part(1, 0, ("p2g_1", "p2g_2", "g2p", "copy_position")) - 24
-    per_cell = part(0, 1, ("p2g_1", "p2g_2", "update_grid", "g2p"))
-    assert (per_particle, per_cell) == (204, 84)
+def test_stage_bytes_add_up_to_the_substep_minimum():
+    # 204 B per particle + 84 B per cell per substep on either path: clearGrid
+    # is charged as the p2g_1 flush, copyPosition as g2p's PosVel write
+    def part(n, g, kernels, binned):
+        return sum(bench.stage_bytes(k, n, g, binned_path=binned) for k in kernels)
+    ref = ("clear_grid", "p2g_1", "p2g_2", "update_grid", "g2p", "copy_position")
+    prod = ("bin_gather", "bin_scan", "bin_scatter", "p2g_1", "reduce_1", "p2g_2", "reduce_2",
+            "g2p")
+    for kernels, binned in ((ref, False), (prod, True)):
+        assert (part(1, 0, kernels, binned), part(0, 1, kernels, binned)) == (204, 84)
+    # G2P: x in; x, v, C and PosVel out; the 27-node gather of 12-B velocities
+    assert bench.stage_bytes("g2p", 1, 0) == 12 + 60 + 24
+    assert bench.stage_bytes("g2p", 0, 1) == 12
 
 
-def test_every_production_slot_has_bytes_and_a_kernel():
+def test_every_production_slot_has_a_kernel_and_the_binning_is_overhead():
     for slot in ("bin_gather", "bin_scan", "bin_scatter", "p2g_1", "reduce_1", "p2g_2",
                  "reduce_2", "g2p"):
-        assert bench.stage_bytes(slot, 300_000, 512_000) > 0
         assert slot in bench.SLOT_KERNEL
+        assert bench.stage_bytes(slot, 300_000, 512_000) + \
+            bench.overhead_bytes(slot, 300_000, 512_000) > 0
+    for slot in ("bin_gather", "bin_scan", "bin_scatter"):
+        assert bench.stage_bytes(slot, 300_000, 512_000) == 0
+
+
+def test_roofline_entry_is_bytes_over_launch_time():
+    e = bench.roofline_entry("g2p", 300_000, 512_000, {"g2p": 20.0}, True)
+    want = (96 * 300_000 + 12 * 512_000) / 20e-6 / 1e9
+    assert e["achieved"] == pytest.approx(want)
+    assert e["frac"] == pytest.approx(want / 8000.0)
+    assert e["overhead_bytes_per_launch"] == 24 * 300_000
 
 
 def test_committed_profile_gives_the_dominant_kernels_traffic():

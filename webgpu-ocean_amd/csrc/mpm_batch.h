@@ -164,12 +164,19 @@ private:
     // the last g2p); false after anything else changed the particles.
     bool keys_ready_ = false;
 
+    // A captured substep is valid exactly while the kernel arguments it was
+    // captured with are the current ones (they are value-initialised, so
+    // padding compares equal too).
     struct GraphEntry {
-        float *state;
+        BatchArgs args;
+        BinBuffers bins;
+        DeviceScene stage;  // the reference-shaped path's arguments
+        int path;
         bool keys_ready;
         hipGraphExec_t exec;
     };
-    GraphEntry graphs_[4] = {};
+    static constexpr int kMaxGraphs = 6;
+    GraphEntry graphs_[kMaxGraphs] = {};
     int ngraphs_ = 0;
 };
 

@@ -158,8 +158,7 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
 }
 
 hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st) {
-    TRY(hipStreamSynchronize(st));
-    drop_graphs();
+    TRY(hipStreamSynchronize(st));  // (graphs stay: they match only their own arguments)
     h_scenes_.assign(count, SceneDesc{});
     uint64_t poff = 0, goff = 0, koff = 0, soff = 0;
     for (uint32_t s = 0; s < count; ++s) {
@@ -463,10 +462,17 @@ hipError_t Batch::launch_substep(hipStream_t st, SlotTimes *t) {
 
 hipError_t Batch::substep(hipStream_t st) {
     GraphEntry *g = nullptr;
+    const BatchArgs a = args();
+    const BinBuffers b = bin_buffers();
+    const DeviceScene v = stage_view();
     for (int i = 0; i < ngraphs_; ++i)
-        if (graphs_[i].state == state_ && graphs_[i].keys_ready == keys_ready_) g = &graphs_[i];
+        if (graphs_[i].keys_ready == keys_ready_ && graphs_[i].path == tun_.path &&
+            std::memcmp(&graphs_[i].args, &a, sizeof a) == 0 &&
+            std::memcmp(&graphs_[i].bins, &b, sizeof b) == 0 &&
+            std::memcmp(&graphs_[i].stage, &v, sizeof v) == 0)
+            g = &graphs_[i];
     if (!g) {
-        if (ngraphs_ == 4) drop_graphs();
+        if (ngraphs_ == kMaxGraphs) drop_graphs();
         hipGraph_t graph = nullptr;
         TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t le = launch_substep(st, nullptr);
@@ -480,7 +486,7 @@ hipError_t Batch::substep(hipStream_t st) {
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return e;
-        graphs_[ngraphs_] = GraphEntry{state_, keys_ready_, exec};
+        graphs_[ngraphs_] = GraphEntry{a, b, v, tun_.path, keys_ready_, exec};
         g = &graphs_[ngraphs_++];
     }
     TRY(hipGraphLaunch(g->exec, st));
