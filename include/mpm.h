@@ -206,6 +206,26 @@ typedef struct {
 int mpm_backward(mpm_sim *sim, const mpm_adjoint_config *cfg, float *grad_v0, float *grad_x0,
                  double *loss, float *grad_stiffness, float *grad_viscosity);
 
+/* ---- Allocation-free step.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Runs `substeps`
+ * substeps of the reference-shaped kernels (clearGrid, p2g_1, p2g_2,
+ * updateGrid, g2p, copyPosition; mls-mpm/mls-mpm.ts:248-265) on the caller's
+ * device buffers of the current device: particles = n 80-B records, read and
+ * written in place; posvel = n 32-B PosVel records, written (record order).
+ * Results equal mpm_step's on the same state and boxes.  The grid and the
+ * uniforms live in a library-owned static device array holding at most 64^3
+ * cells (the reference's cell-buffer cap, mls-mpm/mls-mpm.ts:13-15): a larger
+ * ceil(bx)*ceil(by)*ceil(bz) is rejected with MPM_ERR_GRID_TOO_LARGE.  No
+ * device memory is allocated or freed.  stream: a hipStream_t (NULL = the
+ * default stream); asynchronous.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller.  params NULL = mpm_default_params. */
+#define MPM_NOALLOC_MAX_CELLS (64u * 64u * 64u)
+int mpm_step_noalloc(const mpm_params *params, void *particles, void *posvel, uint32_t n,
+                     const float init_box[3], const float real_box[3], uint32_t substeps,
+                     void *stream);
+
 /* ---- Batched scenes (no reference counterpart: the reference runs one scene).
  * Many independent scenes with their own particle counts, boxes and
  * parameters, packed into shared buffers and advanced by ONE launch per stage

@@ -436,6 +436,49 @@ int mpm_backward(mpm_sim *sim, const mpm_adjoint_config *cfg, float *grad_v0, fl
     return MPM_OK;
 }
 
+// ---- allocation-free step ------------------------------------------------------
+
+int mpm_step_noalloc(const mpm_params *params, void *particles, void *posvel, uint32_t n,
+                     const float init_box[3], const float real_box[3], uint32_t substeps,
+                     void *stream) {
+    mpm_params p;
+    if (params)
+        p = *params;
+    else
+        mpm_default_params(&p);
+    int rc = check_params(p);
+    if (rc) return rc;
+    rc = check_box(init_box, "init_box");
+    if (rc) return rc;
+    rc = check_box(real_box, "real_box");
+    if (rc) return rc;
+    if (n && (!particles || !posvel))
+        return fail(MPM_ERR_INVALID_ARGUMENT, "particles and posvel must be device buffers");
+    for (int a = 0; a < 3; ++a)
+        if (init_box[a] < 1.0f || init_box[a] > 1048576.0f)
+            return fail(MPM_ERR_INVALID_ARGUMENT, "init_box components must lie in [1, 2^20]");
+    const uint64_t G = (uint64_t)std::ceil(init_box[0]) * (uint64_t)std::ceil(init_box[1]) *
+                       (uint64_t)std::ceil(init_box[2]);
+    mpm::Uniforms *uni = nullptr;
+    int4 *grid = nullptr;
+    uint32_t max_cells = 0;
+    HIP_TRY(mpm::noalloc_scratch(&uni, &grid, &max_cells));
+    if (G > max_cells)
+        return fail(MPM_ERR_GRID_TOO_LARGE,
+                    "gridCount " + std::to_string(G) + " exceeds the " +
+                        std::to_string(max_cells) +
+                        " cells of the allocation-free scratch grid (64^3, the reference's cap)");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const mpm::Params prm = to_params(p);
+    HIP_TRY(mpm::launch_set_uniforms(uni, mpm::derive_uniforms(init_box, real_box), st));
+    float *rec = static_cast<float *>(particles);
+    float4 *pv = static_cast<float4 *>(posvel);
+    for (uint32_t k = 0; k < substeps; ++k)
+        for (int stage = 0; stage < MPM_STAGE_COUNT; ++stage)
+            HIP_TRY(mpm::launch_stage_records(stage, rec, n, grid, (uint32_t)G, uni, prm, pv, st));
+    return MPM_OK;
+}
+
 // ---- batch of independent scenes ---------------------------------------------
 
 int mpm_batch_create(const mpm_scene *scenes, uint32_t count, mpm_batch **out) {

@@ -25,6 +25,17 @@ struct DeviceScene {
 // particle or cell, global i32 atomics (mls-mpm/mls-mpm.ts:248-265).
 hipError_t launch_stage(int stage, const DeviceScene &s, hipStream_t st);
 
+// The same kernels over n particles in the reference's 80-B records (read and
+// written in place) with PosVel in record order: the allocation-free step.
+hipError_t launch_stage_records(int stage, float *records, uint32_t n, int4 *grid, uint32_t G,
+                                const Uniforms *uni, const Params &prm, float4 *posvel,
+                                hipStream_t st);
+
+// Scratch of the allocation-free step on the current device: uniforms at
+// *uni, then max_cells grid cells at *grid (a static device array; its
+// address is looked up once per device).
+hipError_t noalloc_scratch(Uniforms **uni, int4 **grid, uint32_t *max_cells);
+
 // Writes the uniforms with a one-thread kernel so the update is ordered on
 // the stream like queue.writeBuffer (mls-mpm/mls-mpm.ts:274).
 hipError_t launch_set_uniforms(Uniforms *dst, const Uniforms &src, hipStream_t st);

@@ -13,18 +13,34 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__device__ __forceinline__ void load_particle(const float *__restrict__ st, uint32_t stride,
-                                              uint32_t i, float x[3], float v[3],
+// Where component k (PX..C22) of particle i lives.
+struct SoALayout {  // the library's own state: state[k * stride + i]
+    float *st;
+    uint32_t stride;
+    __device__ __forceinline__ float &at(int k, uint32_t i) const { return st[k * stride + i]; }
+};
+
+struct RecordLayout {  // the reference's 80-B records (mls-mpm/p2g_1.wgsl:1-5): position @0,
+                       // v @16, C column c @32 + 16c, each vec3 followed by 4 B of pad
+    float *rec;
+    __device__ __forceinline__ float &at(int k, uint32_t i) const {
+        const int off = k < VX ? k : (k < C00 ? k + 1 : 8 + 4 * ((k - C00) / 3) + (k - C00) % 3);
+        return rec[20 * (size_t)i + off];
+    }
+};
+
+template <class L>
+__device__ __forceinline__ void load_particle(const L &p, uint32_t i, float x[3], float v[3],
                                               float C[3][3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        x[a] = st[(PX + a) * stride + i];
-        v[a] = st[(VX + a) * stride + i];
+        x[a] = p.at(PX + a, i);
+        v[a] = p.at(VX + a, i);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) C[c][r] = st[(C00 + 3 * c + r) * stride + i];
+        for (int r = 0; r < 3; ++r) C[c][r] = p.at(C00 + 3 * c + r, i);
 }
 
 // clearGrid -- mls-mpm/clearGrid.wgsl:10-18
@@ -34,14 +50,14 @@ __global__ __launch_bounds__(kBlock) void k_clear_grid(int4 *__restrict__ grid, 
 }
 
 // p2g_1 -- mls-mpm/p2g_1.wgsl:24-65
-__global__ __launch_bounds__(kBlock) void k_p2g_1(const float *__restrict__ st, uint32_t stride,
-                                                  uint32_t n, int32_t *__restrict__ grid,
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_p2g_1(L p, uint32_t n, int32_t *__restrict__ grid,
                                                   uint32_t G, const Uniforms *__restrict__ u,
                                                   Params prm) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float x[3], v[3], C[3][3];
-    load_particle(st, stride, i, x, v, C);
+    load_particle(p, i, x, v, C);
     Stencil s;
     stencil_init(x, s);
     const int32_t Y = u->Y, Z = u->Z;
@@ -71,14 +87,14 @@ __global__ __launch_bounds__(kBlock) void k_p2g_1(const float *__restrict__ st, 
 }
 
 // p2g_2 -- mls-mpm/p2g_2.wgsl:30-94
-__global__ __launch_bounds__(kBlock) void k_p2g_2(const float *__restrict__ st, uint32_t stride,
-                                                  uint32_t n, int32_t *__restrict__ grid,
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_p2g_2(L p, uint32_t n, int32_t *__restrict__ grid,
                                                   uint32_t G, const Uniforms *__restrict__ u,
                                                   Params prm) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float x[3], v[3], C[3][3];
-    load_particle(st, stride, i, x, v, C);
+    load_particle(p, i, x, v, C);
     Stencil s;
     stencil_init(x, s);
     const int32_t Y = u->Y, Z = u->Z;
@@ -162,15 +178,15 @@ __global__ __launch_bounds__(kBlock) void k_update_grid(int4 *__restrict__ grid,
 }
 
 // g2p -- mls-mpm/g2p.wgsl:26-92
-__global__ __launch_bounds__(kBlock) void k_g2p(float *__restrict__ st, uint32_t stride,
-                                                uint32_t n, const int4 *__restrict__ grid,
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_g2p(L p, uint32_t n, const int4 *__restrict__ grid,
                                                 uint32_t G, const Uniforms *__restrict__ u,
                                                 Params prm) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float x[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) x[a] = st[(PX + a) * stride + i];
+    for (int a = 0; a < 3; ++a) x[a] = p.at(PX + a, i);
     Stencil s;
     stencil_init(x, s);
     const int32_t Y = u->Y, Z = u->Z;
@@ -198,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_g2p(float *__restrict__ st, uint32_t
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) st[(C00 + 3 * c + r) * stride + i] = B[c][r] * 4.0f;
+        for (int r = 0; r < 3; ++r) p.at(C00 + 3 * c + r, i) = B[c][r] * 4.0f;
     const float dt = prm.dt;
     float xn_[3];
 #pragma unroll
@@ -215,23 +231,22 @@ __global__ __launch_bounds__(kBlock) void k_g2p(float *__restrict__ st, uint32_t
         float va = v[a];
         if (xn < 3.0f) va = va + wall_stiffness * (3.0f - xn);
         if (xn > wall_max) va = va + wall_stiffness * (wall_max - xn);
-        st[(PX + a) * stride + i] = xn_[a];
-        st[(VX + a) * stride + i] = va;
+        p.at(PX + a, i) = xn_[a];
+        p.at(VX + a, i) = va;
     }
 }
 
-// copyPosition -- mls-mpm/copyPosition.wgsl:15-21 (PosVel in caller order)
-__global__ __launch_bounds__(kBlock) void k_copy_position(const float *__restrict__ st,
-                                                          const uint32_t *__restrict__ id,
-                                                          uint32_t stride, uint32_t n,
+// copyPosition -- mls-mpm/copyPosition.wgsl:15-21 (PosVel in caller order: particle i of
+// the storage is caller particle id[i], or i itself without an id map)
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_copy_position(L p, const uint32_t *__restrict__ id,
+                                                          uint32_t n,
                                                           float4 *__restrict__ posvel) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const size_t o = id[i];
-    posvel[2 * o] = make_float4(st[PX * stride + i], st[PY * stride + i], st[PZ * stride + i],
-                                0.0f);
-    posvel[2 * o + 1] = make_float4(st[VX * stride + i], st[VY * stride + i],
-                                    st[VZ * stride + i], 0.0f);
+    const size_t o = id ? id[i] : i;
+    posvel[2 * o] = make_float4(p.at(PX, i), p.at(PY, i), p.at(PZ, i), 0.0f);
+    posvel[2 * o + 1] = make_float4(p.at(VX, i), p.at(VY, i), p.at(VZ, i), 0.0f);
 }
 
 __global__ void k_set_uniforms(Uniforms *dst, Uniforms src) { *dst = src; }
@@ -282,38 +297,48 @@ inline unsigned blocks_for(uint64_t count) { return (unsigned)((count + kBlock -
 
 }  // namespace
 
-hipError_t launch_stage(int stage, const DeviceScene &s, hipStream_t st) {
+namespace {
+
+// One reference kernel over particles stored as L (mls-mpm/mls-mpm.ts:248-265).
+template <class L>
+hipError_t stage_on(int stage, L p, const uint32_t *id, uint32_t n, int4 *grid, uint32_t G,
+                    const Uniforms *uni, const Params &prm, float4 *posvel, hipStream_t st) {
     switch (stage) {
     case 0:
-        if (s.G) k_clear_grid<<<blocks_for(s.G), kBlock, 0, st>>>(s.grid, s.G);
+        if (G) k_clear_grid<<<blocks_for(G), kBlock, 0, st>>>(grid, G);
         break;
     case 1:
-        if (s.n)
-            k_p2g_1<<<blocks_for(s.n), kBlock, 0, st>>>(s.state, s.stride, s.n,
-                                                        (int32_t *)s.grid, s.G, s.uni, s.prm);
+        if (n) k_p2g_1<<<blocks_for(n), kBlock, 0, st>>>(p, n, (int32_t *)grid, G, uni, prm);
         break;
     case 2:
-        if (s.n)
-            k_p2g_2<<<blocks_for(s.n), kBlock, 0, st>>>(s.state, s.stride, s.n,
-                                                        (int32_t *)s.grid, s.G, s.uni, s.prm);
+        if (n) k_p2g_2<<<blocks_for(n), kBlock, 0, st>>>(p, n, (int32_t *)grid, G, uni, prm);
         break;
     case 3:
-        if (s.G) k_update_grid<<<blocks_for(s.G), kBlock, 0, st>>>(s.grid, s.G, s.uni, s.prm);
+        if (G) k_update_grid<<<blocks_for(G), kBlock, 0, st>>>(grid, G, uni, prm);
         break;
     case 4:
-        if (s.n)
-            k_g2p<<<blocks_for(s.n), kBlock, 0, st>>>(s.state, s.stride, s.n, s.grid, s.G, s.uni,
-                                                      s.prm);
+        if (n) k_g2p<<<blocks_for(n), kBlock, 0, st>>>(p, n, grid, G, uni, prm);
         break;
     case 5:
-        if (s.n)
-            k_copy_position<<<blocks_for(s.n), kBlock, 0, st>>>(s.state, s.id, s.stride, s.n,
-                                                                s.posvel);
+        if (n) k_copy_position<<<blocks_for(n), kBlock, 0, st>>>(p, id, n, posvel);
         break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_stage(int stage, const DeviceScene &s, hipStream_t st) {
+    return stage_on(stage, SoALayout{s.state, s.stride}, s.id, s.n, s.grid, s.G, s.uni, s.prm,
+                    s.posvel, st);
+}
+
+hipError_t launch_stage_records(int stage, float *records, uint32_t n, int4 *grid, uint32_t G,
+                                const Uniforms *uni, const Params &prm, float4 *posvel,
+                                hipStream_t st) {
+    return stage_on(stage, RecordLayout{records}, nullptr, n, grid, G, uni, prm, posvel, st);
 }
 
 hipError_t launch_set_uniforms(Uniforms *dst, const Uniforms &src, hipStream_t st) {

@@ -32,7 +32,7 @@ EXPORTED = (
     "mpm_read_particles", "mpm_write_particles", "mpm_read_grid", "mpm_write_grid",
     "mpm_read_grid_velocity",
     "mpm_get_counts", "mpm_time_steps", "mpm_time_stages", "mpm_get_tuning",
-    "mpm_set_tuning", "mpm_autotune", "mpm_backward",
+    "mpm_set_tuning", "mpm_autotune", "mpm_backward", "mpm_step_noalloc",
     "mpm_batch_create", "mpm_batch_destroy", "mpm_batch_step", "mpm_batch_set_box",
     "mpm_batch_synchronize", "mpm_batch_get_counts", "mpm_batch_read_posvel",
     "mpm_batch_read_particles", "mpm_batch_get_tuning", "mpm_batch_set_tuning",
@@ -124,6 +124,7 @@ def load(path: str = LIB_PATH):
         "mpm_autotune": ([P, u32, ctypes.POINTER(Tuning)], i32),
         "mpm_backward": ([P, ctypes.POINTER(AdjointConfig), P, P, ctypes.POINTER(ctypes.c_double),
                           ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)], i32),
+        "mpm_step_noalloc": ([ctypes.POINTER(Params), P, P, u32, F3, F3, u32, P], i32),
         "mpm_batch_create": ([ctypes.POINTER(Scene), u32, ctypes.POINTER(P)], i32),
         "mpm_batch_destroy": ([P], i32),
         "mpm_batch_step": ([P, u32], i32),
