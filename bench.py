@@ -119,94 +119,127 @@ def _short_kernel(name: str) -> str:
     return n.split("(")[0].replace("void ", "").strip()
 
 
-def summarize_profiles(trace_dir: str, fetch_dir: str = "", write_dir: str = "") -> str:
-    """Table of a rocprofv3 --kernel-trace --stats run (csv output), with the
-    per-dispatch FETCH_SIZE / WRITE_SIZE of two --pmc runs of the same
-    command with --calibrate, scaled by what the counters read for the
-    calibration copy (its true size is CALIBRATION_BYTES each way)."""
+def _csv_rows(d: str, suffix: str):
     import csv
     import glob
 
-    def one(pattern):
-        hits = glob.glob(os.path.join(pattern, "**", "*"), recursive=True)
-        return hits
+    files = sorted(f for f in glob.glob(os.path.join(d, "**", "*"), recursive=True)
+                   if f.endswith(suffix))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    rows = []
+    for f in files:
+        rows.extend(csv.DictReader(open(f)))
+    return rows
 
-    stats = [f for f in one(trace_dir) if f.endswith("kernel_stats.csv")]
-    if not stats:
-        raise SystemExit(f"no kernel_stats.csv under {trace_dir}")
-    rows = list(csv.DictReader(open(stats[0])))
 
-    def counters(d, cname):
-        per, calib = {}, None
-        files = [f for f in one(d) if f.endswith("counter_collection.csv")] if d else []
-        if not files:
-            return per, calib
-        acc = {}
-        for r in csv.DictReader(open(files[0])):
-            if r["Counter_Name"] != cname:
-                continue
+def _select(per_kernel: dict, lo: int, hi: int) -> dict:
+    """Dispatches lo..hi-1 (in dispatch order) of every kernel that has them,
+    else all of its dispatches."""
+    out = {}
+    for k, items in per_kernel.items():
+        items = sorted(items)
+        out[k] = items[lo:hi] if len(items) >= hi > lo else items
+    return out
+
+
+def trace_durations(trace_dir: str, lo: int = 0, hi: int = 0) -> dict:
+    """{kernel: (average us, dispatches)} of a rocprofv3 --kernel-trace run
+    (csv), over each kernel's dispatches lo..hi-1 (hi = 0: all)."""
+    per = {}
+    for r in _csv_rows(trace_dir, "kernel_trace.csv"):
+        k = _short_kernel(r["Kernel_Name"])
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        per.setdefault(k, []).append((int(r["Dispatch_Id"]), us))
+    sel = _select(per, lo, hi) if hi else {k: sorted(v) for k, v in per.items()}
+    return {k: (sum(u for _, u in v) / len(v), len(v)) for k, v in sel.items() if v}
+
+
+def counter_values(dirs, lo: int = 0, hi: int = 0, reduce=None) -> dict:
+    """{kernel: {counter: average per dispatch}} of rocprofv3 --pmc runs (csv),
+    over each kernel's dispatches lo..hi-1 (hi = 0: all); `reduce` replaces
+    the average (e.g. max)."""
+    out = {}
+    for d in dirs:
+        per = {}
+        for r in _csv_rows(d, "counter_collection.csv"):
             k = _short_kernel(r["Kernel_Name"])
-            acc.setdefault(k, []).append(float(r["Counter_Value"]))
-        for k, v in acc.items():
-            per[k] = sum(v) / len(v)
-        copies = [max(v) for k, v in acc.items() if "copyBuffer" in k]
-        if copies:
-            calib = max(copies) * 1024.0 / CALIBRATION_BYTES
-        return per, calib
+            per.setdefault((k, r["Counter_Name"]), []).append(
+                (int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+        for (k, c), v in per.items():
+            v = sorted(v)
+            if hi and len(v) >= hi > lo:
+                v = v[lo:hi]
+            xs = [x for _, x in v]
+            out.setdefault(k, {})[c] = reduce(xs) if reduce else sum(xs) / len(xs)
+    return out
 
-    fetch, fcal = counters(fetch_dir, "FETCH_SIZE")
-    write, wcal = counters(write_dir, "WRITE_SIZE")
+
+def summarize_profiles(trace_dir: str, counter_dirs=(), lo: int = 0, hi: int = 0) -> str:
+    """Per-kernel table of one profiled command: average duration from the
+    kernel trace and bytes per dispatch from counter runs of the same command,
+    both over each kernel's dispatches lo..hi-1.
+
+    - FETCH_SIZE / WRITE_SIZE (KiB), each divided by what the same counter
+      reads for the calibration copy of CALIBRATION_BYTES (bench.py
+      --calibrate): the `traffic` column, which bench.py reports;
+    - request sizes: reads 32 n32 + 64 n64 + 128 n128 (TCC_EA0_RDREQ_32B/64B/
+      128B), writes 32 (n - n64) + 64 n64 (TCC_EA0_WRREQ / _64B);
+    - the share of read and write requests that went to DRAM rather than the
+      MALL (TCC_EA0_RDREQ_DRAM / TCC_EA0_RDREQ, TCC_EA0_WRREQ_DRAM / TCC_EA0_WRREQ)."""
+    dur = trace_durations(trace_dir, lo, hi)
+    cnt = counter_values(counter_dirs, lo, hi) if counter_dirs else {}
+    # the calibration copy is the largest copy dispatch of the run
+    copies = [v for k, v in (counter_values(counter_dirs, reduce=max) if counter_dirs
+                             else {}).items() if "copyBuffer" in k]
+
+    def calib(name):
+        vals = [c[name] for c in copies if name in c]
+        return max(vals) * 1024.0 / CALIBRATION_BYTES if vals else None
+
+    fcal, wcal = calib("FETCH_SIZE"), calib("WRITE_SIZE")
     out = []
-    out.append(f"calibration: 64 MiB copy read FETCH_SIZE x{fcal:.3f} and WRITE_SIZE x{wcal:.3f} "
-               "of its true size" if fcal and wcal else "calibration: none")
-    out.append(f"{'kernel':28s} {'calls':>6s} {'avg_us':>8s} {'pct':>6s} {'FETCH_KiB':>10s} "
-               f"{'WRITE_KiB':>10s} {'HBM_bytes':>12s} {'GB/s':>7s} {'of_peak':>7s}")
-    for r in rows:
-        k = _short_kernel(r["Name"])
+    if fcal and wcal:
+        out.append(f"calibration: the {CALIBRATION_BYTES >> 20} MiB copy reads FETCH_SIZE x{fcal:.3f} "
+                   f"and WRITE_SIZE x{wcal:.3f} of its true size")
+    out.append(f"{'kernel':24s} {'disp':>5s} {'avg_us':>8s} {'traffic_B':>11s} {'GB/s':>6s} "
+               f"{'rd_req_B':>10s} {'wr_req_B':>10s} {'rd_DRAM':>7s} {'wr_DRAM':>7s}")
+    for k, (us, n) in sorted(dur.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
         if k.startswith("__amd"):
             continue
-        f, w = fetch.get(k), write.get(k)
-        hbm, rate, frac = "", "", ""
-        avg_us = float(r["AverageNs"]) / 1e3
-        if f is not None and w is not None and fcal and wcal:
-            nbytes = (f / fcal + w / wcal) * 1024
-            hbm = f"{nbytes:.4g}"
-            if avg_us > 0:  # counted bytes over the kernel's average duration
-                rate = f"{nbytes / avg_us / 1e3:.0f}"
-                frac = f"{nbytes / avg_us / 1e3 / HBM_PEAK_GBS:.3f}"
-        out.append(f"{k[:28]:28s} {r['Calls']:>6s} {avg_us:8.2f} "
-                   f"{float(r['Percentage']):6.2f} {'' if f is None else f'{f:.0f}':>10s} "
-                   f"{'' if w is None else f'{w:.0f}':>10s} {hbm:>12s} {rate:>7s} {frac:>7s}")
-        if hbm:
-            out.append(f"TRAFFIC {k} {hbm}")
+        c = cnt.get(k, {})
+        traffic = rate = rd = wr = rdd = wrd = ""
+        if "FETCH_SIZE" in c and "WRITE_SIZE" in c and fcal and wcal:
+            t = (c["FETCH_SIZE"] / fcal + c["WRITE_SIZE"] / wcal) * 1024
+            traffic, rate = f"{t:.4g}", f"{t / us / 1e3:.0f}"
+        if all(x in c for x in ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum",
+                                "TCC_EA0_RDREQ_128B_sum")):
+            rd = f"{32 * c['TCC_EA0_RDREQ_32B_sum'] + 64 * c['TCC_EA0_RDREQ_64B_sum'] + 128 * c['TCC_EA0_RDREQ_128B_sum']:.4g}"
+        if "TCC_EA0_WRREQ_sum" in c and "TCC_EA0_WRREQ_64B_sum" in c:
+            n64 = c["TCC_EA0_WRREQ_64B_sum"]
+            wr = f"{32 * (c['TCC_EA0_WRREQ_sum'] - n64) + 64 * n64:.4g}"
+        if c.get("TCC_EA0_RDREQ_sum") and "TCC_EA0_RDREQ_DRAM_sum" in c:
+            rdd = f"{c['TCC_EA0_RDREQ_DRAM_sum'] / c['TCC_EA0_RDREQ_sum']:.2f}"
+        if c.get("TCC_EA0_WRREQ_sum") and "TCC_EA0_WRREQ_DRAM_sum" in c:
+            wrd = f"{c['TCC_EA0_WRREQ_DRAM_sum'] / c['TCC_EA0_WRREQ_sum']:.2f}"
+        out.append(f"{k[:24]:24s} {n:5d} {us:8.2f} {traffic:>11s} {rate:>6s} {rd:>10s} {wr:>10s} "
+                   f"{rdd:>7s} {wrd:>7s}")
+        if traffic:
+            out.append(f"TRAFFIC {k} {traffic}")
     return "\n".join(out)
 
 
-def counter_table(dirs) -> str:
+def counter_table(dirs, lo: int = 0, hi: int = 0) -> str:
     """Per-kernel averages (per dispatch) of every counter of one or more
     rocprofv3 --pmc runs (csv output), one column per counter."""
-    import csv
-    import glob
-
-    acc = {}
-    names = []
-    for d in dirs:
-        files = [f for f in glob.glob(os.path.join(d, "**", "*"), recursive=True)
-                 if f.endswith("counter_collection.csv")]
-        if not files:
-            raise SystemExit(f"no counter_collection.csv under {d}")
-        for r in csv.DictReader(open(files[0])):
-            c = r["Counter_Name"]
-            if c not in names:
-                names.append(c)
-            acc.setdefault(_short_kernel(r["Kernel_Name"]), {}).setdefault(c, []).append(
-                float(r["Counter_Value"]))
+    vals = counter_values(dirs, lo, hi)
+    names = sorted({c for per in vals.values() for c in per})
     out = [f"{'kernel':24s}" + "".join(f"{c[:18]:>19s}" for c in names)]
-    for k, per in acc.items():
+    for k, per in vals.items():
         if k.startswith("__amd") or "copyBuffer" in k or "fill" in k.lower():
             continue
         out.append(f"{k[:24]:24s}" + "".join(
-            f"{(sum(per[c]) / len(per[c]) if c in per else float('nan')):19.4g}" for c in names))
+            f"{per.get(c, float('nan')):19.4g}" for c in names))
     return "\n".join(out)
 
 
@@ -485,21 +518,24 @@ def main() -> int:
                          "frames)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 64 MiB device copy (counter calibration under a profiler)")
-    ap.add_argument("--summarize", nargs="+", metavar="DIR",
-                    help="print the table of a rocprofv3 kernel-trace run DIR "
-                         "[FETCH_SIZE-run DIR, WRITE_SIZE-run DIR] and exit")
-    ap.add_argument("--counters", nargs="+", metavar="DIR",
-                    help="print per-kernel averages of the counters of rocprofv3 --pmc runs "
-                         "DIR... and exit")
+    ap.add_argument("--summarize", metavar="TRACE_DIR",
+                    help="print the per-kernel table of a rocprofv3 kernel-trace run (with "
+                         "--pmc-dirs: bytes from counter runs of the same command) and exit")
+    ap.add_argument("--pmc-dirs", nargs="*", default=[], metavar="DIR",
+                    help="rocprofv3 --pmc output directories for --summarize / --counters")
+    ap.add_argument("--counters", action="store_true",
+                    help="print per-kernel averages of every counter in --pmc-dirs and exit")
+    ap.add_argument("--dispatches", type=str, default="",
+                    help="LO:HI -- summarize each kernel's dispatches LO..HI-1 only")
     ap.add_argument("--tuning", type=str, default="",
                     help='JSON launch configuration, e.g. {"particles_per_thread": 1}')
     args = ap.parse_args()
+    lo, hi = (int(x) for x in args.dispatches.split(":")) if args.dispatches else (0, 0)
     if args.summarize:
-        d = list(args.summarize) + ["", ""]
-        print(summarize_profiles(d[0], d[1], d[2]))
+        print(summarize_profiles(args.summarize, args.pmc_dirs, lo, hi))
         return 0
     if args.counters:
-        print(counter_table(args.counters))
+        print(counter_table(args.pmc_dirs, lo, hi))
         return 0
 
     if args.gpus != 1:

@@ -54,29 +54,54 @@ def _write_counters(path, rows):
     os.makedirs(path, exist_ok=True)
     with open(os.path.join(path, "run_counter_collection.csv"), "w") as f:
         f.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n")
-        for i, (k, c, v) in enumerate(rows):
-            f.write(f'{i},"{k}",{c},{v}\n')
+        for i, row in enumerate(rows):
+            k, c, v = row[:3]
+            d = row[3] if len(row) > 3 else i
+            f.write(f'{d},"{k}",{c},{v}\n')
 
 
-def test_summary_scales_counters_by_the_calibration_copy(tmp_path):
-    trace = tmp_path / "trace"
-    trace.mkdir()
-    (trace / "run_kernel_stats.csv").write_text(
-        'Name,Calls,TotalDurationNs,AverageNs,Percentage\n'
-        '"void mpm::(anonymous namespace)::k_g2p<2, false>(mpm::BatchArgs)",10,200000,20000,100.0\n')
+def _write_trace(path, rows):
+    os.makedirs(path, exist_ok=True)
+    with open(os.path.join(path, "run_kernel_trace.csv"), "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for d, k, ns in rows:
+            f.write(f'{d},"{k}",{1000 * d},{1000 * d + ns}\n')
+
+
+G2P = "void mpm::(anonymous namespace)::k_g2p<2>(mpm::BatchArgs)"
+COPY = "__amd_rocclr_copyBuffer"
+
+
+def test_summary_selects_dispatches_and_scales_counters(tmp_path):
+    # four g2p dispatches of 10, 20, 30, 40 us; dispatches 1..2 selected
+    _write_trace(str(tmp_path / "trace"), [(1, G2P, 10000), (2, G2P, 20000), (3, G2P, 30000),
+                                           (4, G2P, 40000), (5, COPY, 5000)])
     kib = bench.CALIBRATION_BYTES // 1024
-    copy = "__amd_rocclr_copyBuffer"
     _write_counters(str(tmp_path / "fetch"), [
-        (copy, "FETCH_SIZE", kib // 2),
-        ("void mpm::(anonymous namespace)::k_g2p<2, false>(mpm::BatchArgs)", "FETCH_SIZE", 100)])
+        (COPY, "FETCH_SIZE", kib // 2, 5),
+        (G2P, "FETCH_SIZE", 100, 1), (G2P, "FETCH_SIZE", 200, 2), (G2P, "FETCH_SIZE", 400, 3),
+        (G2P, "FETCH_SIZE", 800, 4)])
     _write_counters(str(tmp_path / "write"), [
-        (copy, "WRITE_SIZE", kib),
-        ("void mpm::(anonymous namespace)::k_g2p<2, false>(mpm::BatchArgs)", "WRITE_SIZE", 300)])
-    out = bench.summarize_profiles(str(trace), str(tmp_path / "fetch"), str(tmp_path / "write"))
-    line = [l for l in out.splitlines() if l.startswith("TRAFFIC")][0]
+        (COPY, "WRITE_SIZE", kib, 5),
+        (G2P, "WRITE_SIZE", 300, 1), (G2P, "WRITE_SIZE", 300, 2), (G2P, "WRITE_SIZE", 300, 3),
+        (G2P, "WRITE_SIZE", 300, 4)])
+    _write_counters(str(tmp_path / "req"), [
+        (G2P, "TCC_EA0_RDREQ_32B_sum", 10, 2), (G2P, "TCC_EA0_RDREQ_64B_sum", 20, 2),
+        (G2P, "TCC_EA0_RDREQ_128B_sum", 30, 2), (G2P, "TCC_EA0_RDREQ_sum", 60, 2),
+        (G2P, "TCC_EA0_RDREQ_DRAM_sum", 15, 2), (G2P, "TCC_EA0_WRREQ_sum", 50, 2),
+        (G2P, "TCC_EA0_WRREQ_64B_sum", 40, 2), (G2P, "TCC_EA0_WRREQ_DRAM_sum", 25, 2)])
+    dirs = [str(tmp_path / d) for d in ("fetch", "write", "req")]
+    out = bench.summarize_profiles(str(tmp_path / "trace"), dirs, 1, 3)
+    row = [l for l in out.splitlines() if l.startswith("k_g2p")][0].split()
+    assert row[1] == "2" and float(row[2]) == pytest.approx(25.0)   # dispatches 2, 3
     # FETCH reads half the true bytes on the copy, WRITE all of them
-    assert line.split()[1:-1] == ["k_g2p<2,", "false>"]
-    assert float(line.split()[-1]) == pytest.approx((100 / 0.5 + 300) * 1024)
+    want = ((200 + 400) / 2 / 0.5 + 300) * 1024
+    line = [l for l in out.splitlines() if l.startswith("TRAFFIC")][0]
+    assert float(line.split()[-1]) == pytest.approx(want, rel=1e-3)
+    # the request-size counters exist only for dispatch 2: all dispatches used
+    assert float(row[5]) == pytest.approx(32 * 10 + 64 * 20 + 128 * 30)
+    assert float(row[6]) == pytest.approx(32 * 10 + 64 * 40)
+    assert float(row[7]) == pytest.approx(0.25) and float(row[8]) == pytest.approx(0.5)
 
 
 def test_counter_table_averages_per_kernel(tmp_path):
