@@ -727,6 +727,7 @@ struct LevelOrder {
     uint64_t mask[kBlock * PPT];   // cells present in each level
     uint16_t perm[kBlock * PPT];   // position -> slot within the chunk
     uint32_t slot[kBlock * PPT];   // slot within the chunk -> state slot (a.perm)
+    float x[3][kBlock * PPT];      // position of each slot within the chunk (read once)
 };
 
 // Fills o.perm for window chunk ch; every thread of the block calls it.
@@ -744,10 +745,14 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
         if (j < ch.w) {
             const uint32_t i = a.perm[ch.z + j];
             o.slot[j] = i;
+            float x[3];
+            load_x(a.state, a.stride, i, x);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) o.x[ax][j] = x[ax];
             const int m = kTileEdge - 1;
-            const int cx = (int)floorf(a.state[PX * a.stride + i]) & m;
-            const int cy = (int)floorf(a.state[(PX + 1) * a.stride + i]) & m;
-            const int cz = (int)floorf(a.state[(PX + 2) * a.stride + i]) & m;
+            const int cx = (int)floorf(x[0]) & m;
+            const int cy = (int)floorf(x[1]) & m;
+            const int cz = (int)floorf(x[2]) & m;
             cell[k] = (uint32_t)((cx * kTileEdge + cy) * kTileEdge + cz);
             lev[k] = atomicAdd(&o.cnt[cell[k]], 1u);
         }
@@ -812,9 +817,11 @@ void k_p2g1(BatchArgs a) {
             const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
             valid[k] = pos < ch.w;
             if (valid[k]) {
-                const uint32_t i = ord.slot[ord.perm[pos]];
+                const uint32_t j = ord.perm[pos];
+                const uint32_t i = ord.slot[j];
                 a.order[ch.z + pos] = i;  // p2g_2 takes the same order
-                load_x(a.state, a.stride, i, x[k]);
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) x[k][ax] = ord.x[ax][j];
                 load_v(a.state, a.stride, i, v[k]);
                 load_C(a.state, a.stride, i, C[k]);
             }
@@ -1182,7 +1189,8 @@ void k_g2p(BatchArgs a) {
                 key = bin_key(a, sd, xo, t, &dir);
             }
             wild[k] = valid[k] && dir == kDirWild;
-            if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, valid[k]);
+            // (a wave whose k-th task holds no particle skips the ranking together)
+            if (a.count_next && __ballot(valid[k])) rank_next(a, lcnt, w.c, i, key, dir, valid[k]);
         }
         if (a.count_next) {
             finish_chunk_counts(a, sd, lcnt, w.c);
