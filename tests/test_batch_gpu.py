@@ -72,21 +72,31 @@ def test_autotuned_batch_equals_scenes_run_alone():
                 assert np.all(bits_equal(bt.read_posvel(s), sim.read_posvel())), f"scene {s}"
 
 
-def test_config3_full_batch_two_substeps_equals_sequential():
+def test_config3_full_batch_equals_every_scene_alone_and_the_oracle():
+    """The 64-scene mix (about 3.4M particles, uneven counts, boxes and
+    parameters) in one batch: after 4 substeps every scene equals a single
+    simulator run alone on it, and 8 scenes spread over the mix (the
+    smallest and largest included) equal the oracle, bit for bit."""
     sc = scenes.config3()
+    assert len(sc) == 64
+    steps = 4
     with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in sc]) as bt:
-        bt.step(2)
-        picks = [0, 17, 40, 63]
+        bt.step(steps)
+        for s, (r, b, p) in enumerate(sc):
+            with MLSMPMSimulator(params=params_of(p), particles=r, init_box_size=b) as sim:
+                sim.execute(steps)
+                assert np.all(bits_equal(bt.read_posvel(s), sim.read_posvel())), f"scene {s}"
+                assert np.all(bits_equal(bt.read_particles(s), sim.read_particles())), \
+                    f"scene {s} state"
+        sizes = [r.shape[0] for r, _, _ in sc]
+        picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 5, 13, 27, 38, 50, 61})
+        assert len(picks) >= 8
         for s in picks:
             r, b, p = sc[s]
-            with MLSMPMSimulator(params=params_of(p), particles=r, init_box_size=b) as sim:
-                sim.execute(2)
-                assert np.all(bits_equal(bt.read_posvel(s), sim.read_posvel())), f"scene {s}"
-        # one scene against the oracle as well
-        r, b, p = sc[picks[1]]
-        ref = oracle_scene(r, b, p)
-        ref.execute(2, threads=8)
-        assert np.all(bits_equal(bt.read_posvel(picks[1]), ref.posvel))
+            ref = oracle_scene(r, b, p)
+            ref.execute(steps, threads=8)
+            assert np.all(bits_equal(bt.read_posvel(s), ref.posvel)), f"scene {s} vs oracle"
+            assert np.all(bits_equal(bt.read_particles(s), ref.particles)), f"scene {s} state"
 
 
 def test_batch_errors():

@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "mpm_fused.h"
+#include "mpm_index.h"
 #include "mpm_particle.h"
 
 #pragma clang fp contract(off)
@@ -65,35 +66,8 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
-// One-wave-per-tile launches (the reduce kernels) pack kTileWaves waves, i.e.
-// consecutive tiles, per block: fewer, larger blocks dispatch faster.
-constexpr uint32_t kTileWaves = 4;
-
-// Runs of consecutive work items dealt round robin over the XCDs (see
-// tile_walk).
-constexpr uint32_t kXcdRun = 16;
-
-// Blocks of a one-wave-per-tile launch, padded to a multiple of 8 runs
-// (tiles >= stiles are padding).
-__host__ __device__ __forceinline__ uint32_t tile_blocks(uint32_t stiles) {
-    const uint32_t nb = (stiles + kTileWaves - 1) / kTileWaves;
-    return (nb + kXcds * kXcdRun - 1) / (kXcds * kXcdRun) * (kXcds * kXcdRun);
-}
-
-// Spatial tile of this wave.  The dispatcher puts block b on XCD b mod 8;
-// blocks take Hilbert-consecutive tiles in runs of kXcdRun blocks per XCD
-// turn, so neighbouring partial windows mostly meet in one L2 while every
-// XCD gets a share of the fluid's dense and sparse regions (with xcd_remap:
-// one contiguous range of tiles per XCD).  Both are bijections of the
-// padded block range.
 __device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
-    const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-    uint32_t blk;
-    if (a.xcd_remap)
-        blk = x * (tile_blocks(a.stiles) / kXcds) + j;
-    else
-        blk = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
-    return blk * kTileWaves + threadIdx.x / kWave;
+    return reduce_tile_of(blockIdx.x, threadIdx.x / kWave, a.stiles, a.xcd_remap);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -102,27 +76,11 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
-// (mls-mpm/p2g_1.wgsl:53-56).
-__device__ __forceinline__ uint32_t cell_linear(int32_t x, int32_t y, int32_t z, int32_t Y,
-                                                int32_t Z) {
-    return (uint32_t)x * (uint32_t)Y * (uint32_t)Z + (uint32_t)y * (uint32_t)Z + (uint32_t)z;
-}
-
 __device__ __forceinline__ int4 add4(int4 a, int4 b) {
     return make_int4((int32_t)((uint32_t)a.x + (uint32_t)b.x),
                      (int32_t)((uint32_t)a.y + (uint32_t)b.y),
                      (int32_t)((uint32_t)a.z + (uint32_t)b.z),
                      (int32_t)((uint32_t)a.w + (uint32_t)b.w));
-}
-
-// Tile coordinates of an x-major tile index.
-__device__ __forceinline__ void tile_coords(uint32_t xm, const SceneDesc *sd, int t[3]) {
-    const uint32_t n12 = sd->knt[1] * sd->knt[2];
-    t[0] = (int)(xm / n12);
-    const uint32_t r = xm - (uint32_t)t[0] * n12;
-    t[1] = (int)(r / sd->knt[2]);
-    t[2] = (int)(r - (uint32_t)t[1] * sd->knt[2]);
 }
 
 // Scene owning element `v` of a per-scene offset field (binary search; the
@@ -146,58 +104,9 @@ __device__ __forceinline__ uint32_t key_scene_of(const BatchArgs &a, uint32_t k)
     return a.nscenes == 1 ? 0u : a.key_scene[k];
 }
 
-// Persistent walk over the device-side tile-chunk list.  Blocks are
-// dispatched round robin over the XCDs (block id mod 8).  By default the
-// chunks are dealt to the XCDs in runs of kXcdRun consecutive (Hilbert-
-// ordered, so neighbouring) chunks: neighbouring windows share an L2 within
-// a run while every XCD gets an even mix of the fluid's dense body and
-// sparse surface.  With xcd_remap each XCD takes one contiguous range of
-// chunks holding an equal share of the particles, as the scan split them.
-struct TileWalk {
-    uint32_t c, end, step;
-};
-
 __device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
-    const uint32_t n = min(*a.ntchunks, a.tchunk_cap);
-    TileWalk w;
-    if (a.xcd_remap && (gridDim.x % kXcds) == 0) {
-        const uint32_t x = blockIdx.x % kXcds;
-        w.c = min(a.xcd_first[x], n) + blockIdx.x / kXcds;
-        w.end = min(a.xcd_first[x + 1], n);
-        w.step = gridDim.x / kXcds;
-    } else {
-        w.c = blockIdx.x;
-        if ((gridDim.x % (kXcds * kXcdRun)) == 0) {  // a bijection of [0, gridDim)
-            const uint32_t x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-            w.c = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
-        }
-        w.end = n;
-        w.step = gridDim.x;
-    }
-    return w;
-}
-
-__device__ __forceinline__ bool in_dims(int x, int y, int z, const SceneDesc *sd) {
-    return (uint32_t)x < (uint32_t)sd->kdim[0] && (uint32_t)y < (uint32_t)sd->kdim[1] &&
-           (uint32_t)z < (uint32_t)sd->kdim[2];
-}
-
-// Grid cell of window cell l (0..215) of the tile with coordinates t, or -1
-// outside the scene's cells.
-__device__ __forceinline__ int64_t win_cell_index(int l, const int t[3], const SceneDesc *sd) {
-    const int x = t[0] * kTileEdge - 1 + l / (kWinEdge * kWinEdge);
-    const int y = t[1] * kTileEdge - 1 + (l / kWinEdge) % kWinEdge;
-    const int z = t[2] * kTileEdge - 1 + l % kWinEdge;
-    if (!in_dims(x, y, z, sd)) return -1;
-    return (int64_t)sd->goff + cell_linear(x, y, z, sd->uni.Y, sd->uni.Z);
-}
-
-// LDS cell of node (0,0,0) of a particle with base cell `x` in its tile's
-// window (the window starts one cell before the tile).
-__device__ __forceinline__ int win_l0(const float x[3]) {
-    const int m = kTileEdge - 1;
-    const int cx = (int)floorf(x[0]), cy = (int)floorf(x[1]), cz = (int)floorf(x[2]);
-    return ((cx & m) * kWinEdge + (cy & m)) * kWinEdge + (cz & m);
+    return tile_walk_of(blockIdx.x, gridDim.x, min(*a.ntchunks, a.tchunk_cap), a.xcd_first,
+                        a.xcd_remap);
 }
 
 __device__ __forceinline__ void load_x(const float *__restrict__ st, uint32_t stride, uint32_t i,
@@ -334,35 +243,11 @@ struct GridSinkG {  // g2p straight from the decoded velocities
 // from a global atomic on their key instead (rank "final").  After the state
 // was replaced from outside, k_bin_count ranks every particle that way.
 
-constexpr uint32_t kDirs = 27;              // near directions (dx+1)*9 + (dy+1)*3 + dz+1
-constexpr uint32_t kDirWild = kDirs;        // became wild: block-level global atomic
-constexpr uint32_t kDirFar = kDirs + 1;     // moved further: per-particle global atomic
 constexpr uint32_t kRankFinal = 0xffffffffu;
 
-// Bin key of a particle at x: its tile's key, or the scene's wild key when
-// the 3x3x3 stencil is not inside [0, kdim) of an integral box (NaN fails
-// every comparison).  *dir receives the direction from tile `from` (kDirFar
-// when further away or when `from` is not a tile, kDirWild when wild).
 __device__ __forceinline__ uint32_t bin_key(const BatchArgs &a, const SceneDesc *sd,
                                             const float x[3], const int from[3], uint32_t *dir) {
-    bool tame = sd->fast != 0;
-    int t[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const float f = floorf(x[ax]);
-        tame = tame && f >= 1.0f && f <= (float)(sd->kdim[ax] - 2);
-        t[ax] = tame ? (int32_t)f >> kTileBits : 0;
-    }
-    if (!tame) {
-        *dir = kDirWild;
-        return sd->koff + sd->ntiles;
-    }
-    const int dx = t[0] - from[0], dy = t[1] - from[1], dz = t[2] - from[2];
-    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
-    *dir = near ? (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) : kDirFar;
-    return sd->koff + a.tile_rank[sd->rank_off +
-                                  ((uint32_t)t[0] * sd->knt[1] + (uint32_t)t[1]) * sd->knt[2] +
-                                  (uint32_t)t[2]];
+    return bin_key_of(a.tile_rank, sd, x, from, dir);
 }
 
 // Counts one particle per active lane into count[key] and returns the lane's
@@ -384,9 +269,6 @@ __device__ __forceinline__ uint32_t count_rank(uint32_t *count, uint32_t key, bo
     return __shfl(base, my_head, kWave) + (uint32_t)(lane - my_head);
 }
 
-__device__ __forceinline__ uint32_t chunks_of(uint32_t n, uint32_t cap) {
-    return (n + cap - 1) / cap;
-}
 
 __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
@@ -524,7 +406,7 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
                 // (chunks tile the slots [0, total) in order).
 #pragma unroll
                 for (int x = 1; x < kXcds; ++x) {
-                    const uint32_t sx = (uint32_t)(((uint64_t)x * a.total_slots + kXcds - 1) / kXcds);
+                    const uint32_t sx = xcd_split_slot((uint32_t)x, a.total_slots);
                     if (z < sx && sx <= z + zn) b.xcd_first[x] = cb + i + 1;
                 }
             }
@@ -888,26 +770,15 @@ __device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b
             ncnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
         }
     }
-    int ext[3];
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) ext[ax] = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
     CellPlan p;
     uint32_t mc = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        bool valid = cell_ok;
-        int d[3];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const bool bit = (s >> ax) & 1;
-            d[ax] = bit ? ext[ax] : 0;
-            valid = valid && !(bit && ext[ax] == 0);
-        }
-        const int q = (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1);
+        int q;
+        bool valid;
+        const int widx = cover_slot(l, s, &q, &valid);
+        valid = valid && cell_ok;
         const uint32_t f = __shfl(nfirst, q, kWave), c = __shfl(ncnt, q, kWave);
-        const int widx = ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) *
-                             kWinEdge +
-                         (l[2] + 1 - kTileEdge * d[2]);
         p.base[s] = f * (uint32_t)kWinCells + (uint32_t)widx;
         p.cnt[s] = valid ? c : 0u;
         mc = max(mc, p.cnt[s]);
@@ -1286,98 +1157,6 @@ hipError_t launch_by_ppt(int which, const BatchArgs &a, hipStream_t st) {
 }
 
 }  // namespace
-
-// 3D Hilbert index of (x, y, z) in a 2^bits cube: J. Skilling, "Programming
-// the Hilbert curve", AIP Conf. Proc. 707, 381 (2004) -- axes to transposed
-// form, then the transposed bits interleaved most significant first.
-static uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
-    uint32_t X[3] = {x, y, z};
-    if (bits <= 0) return 0;
-    const uint32_t M = 1u << (bits - 1);
-    for (uint32_t Q = M; Q > 1; Q >>= 1) {
-        const uint32_t P = Q - 1;
-        for (int i = 0; i < 3; ++i) {
-            if (X[i] & Q) {
-                X[0] ^= P;
-            } else {
-                const uint32_t t = (X[0] ^ X[i]) & P;
-                X[0] ^= t;
-                X[i] ^= t;
-            }
-        }
-    }
-    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
-    uint32_t t = 0;
-    for (uint32_t Q = M; Q > 1; Q >>= 1)
-        if (X[2] & Q) t ^= Q - 1;
-    for (int i = 0; i < 3; ++i) X[i] ^= t;
-    uint64_t h = 0;
-    for (int b = bits - 1; b >= 0; --b)
-        for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
-    return h;
-}
-
-void append_hilbert_ranks(const SceneDesc &d, std::vector<uint32_t> &ranks,
-                          std::vector<uint32_t> &xmaj) {
-    const uint32_t n0 = d.knt[0], n1 = d.knt[1], n2 = d.knt[2];
-    const uint32_t nt = n0 * n1 * n2;
-    int bits = 0;
-    while ((1u << bits) < std::max(n0, std::max(n1, n2))) ++bits;
-    std::vector<std::pair<uint64_t, uint32_t>> order(nt);
-    for (uint32_t x = 0; x < n0; ++x)
-        for (uint32_t y = 0; y < n1; ++y)
-            for (uint32_t z = 0; z < n2; ++z) {
-                const uint32_t t = (x * n1 + y) * n2 + z;
-                order[t] = {hilbert3(x, y, z, bits), t};
-            }
-    std::sort(order.begin(), order.end());
-    const size_t base = ranks.size();
-    ranks.resize(base + nt);
-    for (uint32_t r = 0; r < nt; ++r) {
-        ranks[base + order[r].second] = r;
-        xmaj.push_back(order[r].second);
-    }
-    xmaj.push_back(kWildTile);
-}
-
-void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
-                        const uint32_t *xmaj, std::vector<uint32_t> &nbr,
-                        std::vector<uint4> &info) {
-    const uint32_t n1 = d.knt[1], n2 = d.knt[2];
-    for (uint32_t r = 0; r < d.ntiles; ++r) {
-        const uint32_t xm = xmaj[r];
-        const uint32_t t[3] = {xm / (n1 * n2), (xm / n2) % n1, xm % n2};
-        info.push_back(make_uint4(t[0], t[1], t[2], s));
-        for (int dx = -1; dx <= 1; ++dx)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int64_t nx = (int64_t)t[0] + dx, ny = (int64_t)t[1] + dy,
-                                  nz = (int64_t)t[2] + dz;
-                    if (nx < 0 || ny < 0 || nz < 0 || nx >= d.knt[0] || ny >= n1 || nz >= n2) {
-                        nbr.push_back(kNoTile);
-                        continue;
-                    }
-                    nbr.push_back(d.toff + ranks[((uint32_t)nx * n1 + (uint32_t)ny) * n2 +
-                                                 (uint32_t)nz]);
-                }
-    }
-}
-
-void scene_key_geometry(SceneDesc &d) {
-    uint64_t nt = 1;
-    bool integral = true;
-    for (int a = 0; a < 3; ++a) {
-        const float b = d.uni.init_box[a];
-        const int32_t dim = (int32_t)std::ceil(b);
-        d.kdim[a] = dim < 1 ? 1 : dim;
-        d.knt[a] = (uint32_t)((d.kdim[a] + kTileEdge - 1) >> kTileBits);
-        nt *= d.knt[a];
-        integral = integral && std::floor(b) == b;
-    }
-    d.ntiles = (uint32_t)nt;
-    d.nkeys = (uint32_t)(nt + 1);  // one key per tile plus the wild key
-    d.fast = integral ? 1u : 0u;
-}
 
 hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count,
                              hipStream_t st) {

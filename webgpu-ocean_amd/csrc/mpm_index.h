@@ -1,0 +1,183 @@
+// mpm_index.h -- index arithmetic of the production substep.
+//
+// Pure functions of their arguments, shared by the gfx950 kernels
+// (mpm_fused_kernels.hip, which pass blockIdx / gridDim) and by the host-side
+// emulation that checks every index they form stays inside its buffer
+// (tests/emu/chunk_index_emu.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpm_fused.h"
+
+namespace mpm {
+
+// One-wave-per-tile launches (the reduce kernels) pack kTileWaves waves, i.e.
+// consecutive tiles, per block: fewer, larger blocks dispatch faster.
+constexpr uint32_t kTileWaves = 4;
+
+// Runs of consecutive work items dealt round robin over the XCDs (see
+// tile_walk_of).
+constexpr uint32_t kXcdRun = 16;
+
+constexpr uint32_t kDirs = 27;              // near directions (dx+1)*9 + (dy+1)*3 + dz+1
+constexpr uint32_t kDirWild = kDirs;        // became wild: block-level global atomic
+constexpr uint32_t kDirFar = kDirs + 1;     // moved further: per-particle global atomic
+
+// Blocks of a one-wave-per-tile launch, padded to a multiple of 8 runs
+// (tiles >= stiles are padding).
+__host__ __device__ __forceinline__ uint32_t tile_blocks(uint32_t stiles) {
+    const uint32_t nb = (stiles + kTileWaves - 1) / kTileWaves;
+    return (nb + kXcds * kXcdRun - 1) / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+}
+
+// Spatial tile of wave `wave` of block `block` of a one-wave-per-tile launch.
+// The dispatcher puts block b on XCD b mod 8; blocks take Hilbert-consecutive
+// tiles in runs of kXcdRun blocks per XCD turn, so neighbouring partial
+// windows mostly meet in one L2 while every XCD gets a share of the fluid's
+// dense and sparse regions (with xcd_remap: one contiguous range of tiles per
+// XCD).  Both are bijections of the padded block range.
+__host__ __device__ __forceinline__ uint32_t reduce_tile_of(uint32_t block, uint32_t wave,
+                                                           uint32_t stiles, int xcd_remap) {
+    const uint32_t x = block % kXcds, j = block / kXcds;
+    uint32_t blk;
+    if (xcd_remap)
+        blk = x * (tile_blocks(stiles) / kXcds) + j;
+    else
+        blk = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
+    return blk * kTileWaves + wave;
+}
+
+__host__ __device__ __forceinline__ uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// Persistent walk of block `block` of `grid` over the n tile chunks of the
+// current binning (n = min(chunk count, capacity)).  By default the chunks
+// are dealt to the XCDs in runs of kXcdRun consecutive (Hilbert-ordered, so
+// neighbouring) chunks: neighbouring windows share an L2 within a run while
+// every XCD gets an even mix of the fluid's dense body and sparse surface.
+// With xcd_remap each XCD takes one contiguous range of chunks holding an
+// equal share of the particles, as the scan split them (xcd_first).
+struct TileWalk {
+    uint32_t c, end, step;
+};
+
+__host__ __device__ __forceinline__ TileWalk tile_walk_of(uint32_t block, uint32_t grid,
+                                                         uint32_t n, const uint32_t *xcd_first,
+                                                         int xcd_remap) {
+    TileWalk w;
+    if (xcd_remap && (grid % kXcds) == 0) {
+        const uint32_t x = block % kXcds;
+        w.c = min_u32(xcd_first[x], n) + block / kXcds;
+        w.end = min_u32(xcd_first[x + 1], n);
+        w.step = grid / kXcds;
+    } else {
+        w.c = block;
+        if ((grid % (kXcds * kXcdRun)) == 0) {  // a bijection of [0, grid)
+            const uint32_t x = block % kXcds, j = block / kXcds;
+            w.c = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
+        }
+        w.end = n;
+        w.step = grid;
+    }
+    return w;
+}
+
+// Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
+// (mls-mpm/p2g_1.wgsl:53-56).
+__host__ __device__ __forceinline__ uint32_t cell_linear(int32_t x, int32_t y, int32_t z,
+                                                         int32_t Y, int32_t Z) {
+    return (uint32_t)x * (uint32_t)Y * (uint32_t)Z + (uint32_t)y * (uint32_t)Z + (uint32_t)z;
+}
+
+// Tile coordinates of an x-major tile index.
+__host__ __device__ __forceinline__ void tile_coords(uint32_t xm, const SceneDesc *sd, int t[3]) {
+    const uint32_t n12 = sd->knt[1] * sd->knt[2];
+    t[0] = (int)(xm / n12);
+    const uint32_t r = xm - (uint32_t)t[0] * n12;
+    t[1] = (int)(r / sd->knt[2]);
+    t[2] = (int)(r - (uint32_t)t[1] * sd->knt[2]);
+}
+
+__host__ __device__ __forceinline__ bool in_dims(int x, int y, int z, const SceneDesc *sd) {
+    return (uint32_t)x < (uint32_t)sd->kdim[0] && (uint32_t)y < (uint32_t)sd->kdim[1] &&
+           (uint32_t)z < (uint32_t)sd->kdim[2];
+}
+
+// Grid cell of window cell l (0..215) of the tile with coordinates t, or -1
+// outside the scene's cells.
+__host__ __device__ __forceinline__ int64_t win_cell_index(int l, const int t[3],
+                                                          const SceneDesc *sd) {
+    const int x = t[0] * kTileEdge - 1 + l / (kWinEdge * kWinEdge);
+    const int y = t[1] * kTileEdge - 1 + (l / kWinEdge) % kWinEdge;
+    const int z = t[2] * kTileEdge - 1 + l % kWinEdge;
+    if (!in_dims(x, y, z, sd)) return -1;
+    return (int64_t)sd->goff + cell_linear(x, y, z, sd->uni.Y, sd->uni.Z);
+}
+
+// LDS cell of node (0,0,0) of a particle with base cell `x` in its tile's
+// window (the window starts one cell before the tile).
+__host__ __device__ __forceinline__ int win_l0(const float x[3]) {
+    const int m = kTileEdge - 1;
+    const int cx = (int)floorf(x[0]), cy = (int)floorf(x[1]), cz = (int)floorf(x[2]);
+    return ((cx & m) * kWinEdge + (cy & m)) * kWinEdge + (cz & m);
+}
+
+// Bin key of a particle at x: its tile's key, or the scene's wild key when
+// the 3x3x3 stencil is not inside [0, kdim) of an integral box (NaN fails
+// every comparison).  *dir receives the direction from tile `from` (kDirFar
+// when further away or when `from` is not a tile, kDirWild when wild).
+__host__ __device__ __forceinline__ uint32_t bin_key_of(const uint32_t *tile_rank,
+                                                       const SceneDesc *sd, const float x[3],
+                                                       const int from[3], uint32_t *dir) {
+    bool tame = sd->fast != 0;
+    int t[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float f = floorf(x[ax]);
+        tame = tame && f >= 1.0f && f <= (float)(sd->kdim[ax] - 2);
+        t[ax] = tame ? (int32_t)f >> kTileBits : 0;
+    }
+    if (!tame) {
+        *dir = kDirWild;
+        return sd->koff + sd->ntiles;
+    }
+    const int dx = t[0] - from[0], dy = t[1] - from[1], dz = t[2] - from[2];
+    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+    *dir = near ? (uint32_t)((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)) : kDirFar;
+    return sd->koff + tile_rank[sd->rank_off +
+                                ((uint32_t)t[0] * sd->knt[1] + (uint32_t)t[1]) * sd->knt[2] +
+                                (uint32_t)t[2]];
+}
+
+__host__ __device__ __forceinline__ uint32_t chunks_of(uint32_t n, uint32_t cap) {
+    return (n + cap - 1) / cap;
+}
+
+// The covering slot s (0..7) of tile-local cell l[3] in a per-tile reduce:
+// per axis bit s>>ax selects the tile itself or its neighbour below (local
+// coordinate 0) or above (3).  *q receives the neighbour (27-entry order) and
+// the return value the cell's index in that neighbour's 6^3 window; *valid is
+// false when the slot needs a neighbour the cell does not touch.
+__host__ __device__ __forceinline__ int cover_slot(const int l[3], int s, int *q, bool *valid) {
+    bool ok = true;
+    int d[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int ext = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
+        const bool bit = (s >> ax) & 1;
+        d[ax] = bit ? ext : 0;
+        ok = ok && !(bit && ext == 0);
+    }
+    *q = (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1);
+    *valid = ok;
+    return ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) * kWinEdge +
+           (l[2] + 1 - kTileEdge * d[2]);
+}
+
+// Slot where range x (1..kXcds-1) of the XCD split of `total` particles
+// starts: ceil(x * total / kXcds).
+__host__ __device__ __forceinline__ uint32_t xcd_split_slot(uint32_t x, uint32_t total) {
+    return (uint32_t)(((uint64_t)x * total + kXcds - 1) / kXcds);
+}
+
+}  // namespace mpm
