@@ -440,6 +440,10 @@ int main(int argc, char **argv) {
         for (uint32_t i = 0; i < n; ++i) far_moves += e.src[i] == kFinal;
         wild_substeps += e.count[e.d.koff + e.d.ntiles] != 0;
         p.swap(next);
+        if ((s + 1) % 50 == 0) {
+            std::fprintf(stderr, "substep %d checks=%ld\n", s + 1, g_checks);
+            std::fflush(stderr);
+        }
     }
     std::printf("ok substeps=%d particles=%u tiles=%u max_chunks=%u cap_chunks=%llu "
                 "final_ranks=%u wild_substeps=%u checks=%ld\n",

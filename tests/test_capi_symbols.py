@@ -59,3 +59,45 @@ def test_no_device_is_reported_not_faked():
     with pytest.raises(mlsmpm.MPMError) as ei:
         mlsmpm.MLSMPMSimulator(init_box_size=(8, 8, 8))
     assert ei.value.status == 5
+
+
+SPH_HEADER = os.path.join(ROOT, "include", "sph.h")
+
+
+def sph_declared_functions():
+    text = open(SPH_HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_sph_header_binding_and_library_agree():
+    import sph
+    from sph import _capi as sph_capi
+    assert sph_declared_functions() == sorted(sph_capi.EXPORTED)
+    L = sph.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", sph.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(sph_\w+)", out))
+    for name in sph_declared_functions():
+        assert hasattr(L, name) and name in exported, name
+
+
+def test_sph_default_params_and_null_arguments():
+    import sph
+    p = sph.default_params()
+    assert (p.kernel_radius, p.mass, p.dt, p.stiffness, p.near_stiffness, p.rest_density,
+            p.viscosity) == (0.07, 1.0, 0.006, 20.0, 1.0, 15000.0, 100.0)
+    L = sph.load()
+    assert L.sph_create(None, None, 0, None, None) == 1
+    assert b"NULL" in L.sph_last_error()
+    assert L.sph_step(None, 2) == 1
+    assert L.sph_destroy(None) == 0
+    assert L.sph_default_params(None) == 1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_sph_no_device_is_reported_not_faked():
+    import sph
+    with pytest.raises(sph.SPHError) as ei:
+        sph.SPHSimulator()
+    assert ei.value.status == 5
