@@ -109,13 +109,14 @@ def test_edge_cases_outside_the_grid_nan_inf_isolated(sphmod):
     with sphmod.SPHSimulator(particles=rec, init_half_box_size=half) as sim:
         sc = osph.Scene(rec, half)
         for _ in range(3):
+            start, order, _ = sc.bin()        # the binning this substep starts from
             sim.execute(1)
             sc.substep()
             got, want = sim.read_particles(), sc.particles
             np.testing.assert_array_equal(np.isnan(got[:, FIELDS]), np.isnan(want[:, FIELDS]))
             _same(np.nan_to_num(got[:, FIELDS]), np.nan_to_num(want[:, FIELDS]), "edge")
-        start, order, _ = sc.bin()
-        np.testing.assert_array_equal(sim.read_cell_start(), start)
+            np.testing.assert_array_equal(sim.read_cell_start(), start)
+            np.testing.assert_array_equal(sim.read_sorted_order(len(order)), order)
 
 
 def test_no_particles_and_reset(sphmod):

@@ -186,7 +186,10 @@ __global__ void __launch_bounds__(kBlock) k_scatter(Bufs b) {
     if (i >= b.n) return;
     const int32_t c = b.cell[i];
     if (c < 0 || (uint32_t)c >= b.u->G) return;
-    b.order[b.start[c + 1] - b.rank[i] - 1] = i;
+    // the slot is below start[G] <= n whenever the counts were cleared before
+    // count ran (stages in order); the test mirrors `targetIndex < params.n`
+    const uint32_t t = b.start[c + 1] - b.rank[i] - 1;
+    if (t < b.n) b.order[t] = i;
 }
 
 // Slot d of particle i = its cell's start + the number of the cell's
@@ -196,9 +199,10 @@ __global__ void __launch_bounds__(kBlock) k_sort_gather(Bufs b) {
     if (k >= b.n || k >= b.start[b.u->G]) return;
     const uint32_t i = b.order[k];
     const int32_t c = b.cell[i];
-    const uint32_t s = b.start[c], e = b.start[c + 1];
+    const uint32_t s = b.start[c], e = min(b.start[c + 1], b.n);
     uint32_t d = s;
     for (uint32_t j = s; j < e; ++j) d += b.order[j] > i ? 1u : 0u;
+    if (d >= b.n) return;
     b.order2[d] = i;
     b.spos[d] = b.pos[i];
     b.svel[d] = b.vel[i];
@@ -243,7 +247,7 @@ __global__ void __launch_bounds__(kBlock) k_density(Bufs b) {
             for (int dy = r.dy0; dy <= r.dy1; ++dy) {
                 const int32_t row = (r.v[1] + dy) * u.X + (r.v[2] + dz) * u.X * u.Y;
                 const uint32_t lo = b.start[row + r.v[0] + r.dx0];
-                const uint32_t hi = b.start[row + r.v[0] + r.dx1 + 1];
+                const uint32_t hi = min(b.start[row + r.v[0] + r.dx1 + 1], b.n);
                 for (uint32_t j = lo; j < hi; ++j) {
                     const float4 q = b.spos[j];
                     const float dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
@@ -287,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) k_force(Bufs b) {
             for (int dy = r.dy0; dy <= r.dy1; ++dy) {
                 const int32_t row = (r.v[1] + dy) * u.X + (r.v[2] + dz) * u.X * u.Y;
                 const uint32_t lo = b.start[row + r.v[0] + r.dx0];
-                const uint32_t hi = b.start[row + r.v[0] + r.dx1 + 1];
+                const uint32_t hi = min(b.start[row + r.v[0] + r.dx1 + 1], b.n);
                 for (uint32_t j = lo; j < hi; ++j) {
                     const float2 dnj = b.sdn[j];
                     const float4 q = b.spos[j];
