@@ -501,6 +501,107 @@ def bench_config2(frames: int = 300) -> dict:
     }
 
 
+# Algorithmic bytes per launch of the SPH kernels (sph.hip): what each kernel
+# must move for its own inputs and outputs, per particle (n) and per cell of
+# the 62^3 grid (g); neighbour reads are reuse of the sorted copies (cache),
+# not counted.
+SPH_BYTES = {
+    "count": (28, 0),          # pos 16 in; cell 4 + rank 4 out (+ the count atomics, 4)
+    "scan_reduce": (0, 4),     # counts in
+    "scan": (0, 12),           # counts in, prefix out, counts cleared
+    "scatter": (16, 0),        # cell, rank, cell end in; slot out
+    "sort_gather": (84, 0),    # slot, cell, 2 prefix entries, pos, vel in; order, 2 copies out
+    "density": (36, 0),        # order, own position in; densities + sorted copy out
+    "force": (124, 0),         # order, position, velocity, densities in; force, state, posvel out
+}
+
+
+def sph_roofline(stage: str, n: int, g: int, us: float) -> dict:
+    per_p, per_c = SPH_BYTES[stage]
+    b = per_p * n + per_c * g
+    ach = b / (us * 1e-6) / 1e9 if us > 0 else 0.0
+    return {"kernel": stage, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_launch": b, "us_per_launch": us}
+
+
+def sph_cpu_baseline(rec, half, seconds: float = 6.0, reps: int = 3) -> dict:
+    """The SPH oracle (CPU restatement of sph/*.wgsl, OpenMP over particles)
+    on the host cores: median of `reps` runs, all allowed threads and one."""
+    import statistics
+
+    from oracle import sph as osph
+
+    n = rec.shape[0]
+    threads, _ = cpu_threads()
+
+    def leg(th):
+        sc = osph.Scene(rec, half)
+        t0 = time.perf_counter()
+        sc.substep(threads=th)
+        one = time.perf_counter() - t0
+        k = max(1, min(100, int(seconds / reps / max(one, 1e-4))))
+        rates = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                sc.substep(threads=th)
+            rates.append(n * k / (time.perf_counter() - t0))
+        return statistics.median(rates), k
+
+    value, k = leg(threads)
+    value1, k1 = leg(1)
+    return {"value": value, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"median of {reps} runs of {k} substeps ({threads} threads) and of {k1} "
+                      f"substeps (1 thread) of the same {n}-particle dam break from its "
+                      "initial state",
+            "value_1_thread": value1}
+
+
+def bench_sph(frames: int = 500, cpu: bool = True) -> dict:
+    """SPH path (sph/sph.ts): the reference's largest setting (40,000
+    particles, half box 1.4 x 2 x 1.4, main.ts:131-132) and its particle cap
+    (200,000, common.ts:11) in the same box, `frames` frames x 2 substeps from
+    the initial state with the z wall moved in to 0.7 of its extent at the
+    middle frame (the box slider, main.ts:253-260)."""
+    from sph import SPHSimulator, scenes
+
+    out = {}
+    half = scenes.INIT_HALF_BOXES[3]
+    narrow = [half[0], half[1], half[2] * 0.7]
+    for label, count in (("40k", 40000), ("200k", 200000)):
+        rec = scenes.dam_break(3, seed=0, num_particles=count)
+        with SPHSimulator(particles=rec, init_half_box_size=half) as sim:
+            g = sim.grid_count
+            sim.execute(10)
+            sim.reset(count, half, particles=rec)
+            sim.synchronize()
+            t0 = time.perf_counter()
+            for f in range(frames):
+                if f == frames // 2:
+                    sim.change_box_size(narrow)
+                sim.execute()
+            sim.synchronize()
+            elapsed = time.perf_counter() - t0
+            sim.reset(count, half, particles=rec)
+            stages_ms = sim.time_stages(2 * frames)
+        per_launch = {k: 1e3 * v / (2 * frames) for k, v in stages_ms.items()}
+        dom = max(per_launch, key=per_launch.get)
+        out[label] = {
+            "workload": f"SPH dam break, {count} particles, 62^3 neighbour grid, {frames} "
+                        f"frames x 2 substeps, z wall at 0.7 from frame {frames // 2}",
+            "value": count * 2 * frames / elapsed,
+            "unit": "particle-substeps/s",
+            "wall_us_per_substep": 1e6 * elapsed / (2 * frames),
+            "stages_us_per_launch": per_launch,
+            "sum_kernel_us": sum(per_launch.values()),
+            "roofline": sph_roofline(dom, count, g, per_launch[dom]),
+        }
+        if cpu and label == "40k":
+            out[label]["cpu_baseline"] = sph_cpu_baseline(rec, half)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,6 +614,8 @@ def main() -> int:
     ap.add_argument("--extras-only", action="store_true",
                     help="run only the batched and adjoint comparisons (one JSON line; "
                          "for profiling them)")
+    ap.add_argument("--sph-only", action="store_true",
+                    help="run only the SPH section (--steps frames; one JSON line)")
     ap.add_argument("--stage-substeps", type=int, default=0,
                     help="substeps of the per-kernel timing replay (default: all of the timed "
                          "frames)")
@@ -549,6 +652,10 @@ def main() -> int:
     if args.calibrate:
         calibration_copy()
 
+    if args.sph_only:
+        print(json.dumps({"sph": bench_sph(frames=args.steps, cpu=not args.no_cpu_baseline)}),
+              flush=True)
+        return 0
     if args.extras_only:
         print(json.dumps({"batched": bench_batched(), "adjoint": bench_adjoint()}), flush=True)
         return 0
@@ -642,6 +749,7 @@ def main() -> int:
         out["config2"] = bench_config2()
         out["batched"] = bench_batched()
         out["adjoint"] = bench_adjoint()
+        out["sph"] = bench_sph(cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rec, box)
     print(json.dumps(out), flush=True)
