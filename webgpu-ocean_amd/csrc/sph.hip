@@ -230,40 +230,117 @@ __device__ __forceinline__ Rows rows_of(const Uniforms &u, float4 p) {
     return r;
 }
 
+// The nine neighbour rows of a particle as sorted-slot ranges [lo, hi) in
+// the reference's order (dz outer, dy inner); rows outside the grid, and all
+// rows of a particle whose cell is outside the grid, are empty.
+struct RowRanges {
+    uint32_t lo[9], hi[9];
+};
+
+__device__ __forceinline__ RowRanges row_ranges(const Bufs &b, const Uniforms &u, const Rows &r) {
+    RowRanges rr;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int dz = t / 3 - 1, dy = t % 3 - 1;
+        const bool ok = r.inside && dz >= r.dz0 && dz <= r.dz1 && dy >= r.dy0 && dy <= r.dy1;
+        uint32_t lo = 0, hi = 0;
+        if (ok) {
+            const int32_t row = (r.v[1] + dy) * u.X + (r.v[2] + dz) * u.X * u.Y;
+            lo = b.start[row + r.v[0] + r.dx0];
+            hi = min(b.start[row + r.v[0] + r.dx1 + 1], b.n);
+        }
+        rr.lo[t] = lo;
+        rr.hi[t] = hi;
+    }
+    return rr;
+}
+
+// Neighbour walks: four lanes (a quad) per particle.  The candidates of the
+// nine rows form one stream; lane q of the quad evaluates stream entries q,
+// q + 4, q + 8, ... (the distance test, the kernels and the IEEE divisions,
+// i.e. nearly all of the work), and every lane then adds the quad's four
+// terms to its accumulators in stream order, reading them through DPP quad
+// broadcasts.  The sequence of float additions is exactly the sequential
+// walk's: a candidate that does not interact adds +0.0, which leaves any
+// accumulator unchanged (an accumulator starting at +0 never becomes -0
+// under round-to-nearest, and x + 0 == x for every other x).  Four times the
+// lanes of a particle-per-lane walk, and a quarter of its dependent steps.
+constexpr unsigned kNbrBlock = 64;
+constexpr unsigned kQuad = 4;
+
+struct Stream {
+    uint32_t lo[9], pre[10];  // row starts; pre[t] = candidates in rows < t
+};
+
+__device__ __forceinline__ Stream stream_of(const RowRanges &rr) {
+    Stream s;
+    s.pre[0] = 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        s.lo[t] = rr.lo[t];
+        s.pre[t + 1] = s.pre[t] + (rr.hi[t] > rr.lo[t] ? rr.hi[t] - rr.lo[t] : 0u);
+    }
+    return s;
+}
+
+// Sorted slot of stream entry x < pre[9]: in the last row t with pre[t] <= x
+// (that row holds x: the rows after it start beyond x).
+__device__ __forceinline__ uint32_t slot_at(const Stream &s, uint32_t x) {
+    uint32_t j = 0;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+        if (x >= s.pre[t]) j = s.lo[t] + (x - s.pre[t]);
+    return j;
+}
+
+// Lane Q's value of the calling lane's quad (DPP quad_perm [Q,Q,Q,Q]).
+template <int Q>
+__device__ __forceinline__ float quad_lane(float v) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), Q * 0x55, 0xF, 0xF, false));
+}
+
+// acc = (((acc + t[0]) + t[1]) + t[2]) + t[3], t[q] = lane q's term.
+__device__ __forceinline__ float quad_accumulate(float acc, float t) {
+    acc = acc + quad_lane<0>(t);
+    acc = acc + quad_lane<1>(t);
+    acc = acc + quad_lane<2>(t);
+    acc = acc + quad_lane<3>(t);
+    return acc;
+}
+
 // computeDensity -- sph/density.wgsl:41-51 (kernels), :65-106, over the sorted
 // slots.
-__global__ void __launch_bounds__(kBlock) k_density(Bufs b) {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+__global__ void __launch_bounds__(kNbrBlock) k_density(Bufs b) {
+    const uint32_t g = blockIdx.x * kNbrBlock + threadIdx.x;
+    const uint32_t k = g / kQuad, q = g % kQuad;
     const Uniforms &u = *b.u;
-    if (k >= b.n || k >= b.start[u.G]) return;
-    const uint32_t i = b.order2[k];
+    if (k >= b.n || k >= b.start[u.G]) return;  // the whole quad leaves together
     const float4 p = b.spos[k];
     const float scale_d = 315.0f / ((float)(64. * 3.1415926535) * u.h9);
     const float scale_n = 15.0f / ((float)3.1415926535 * u.h6);
     float density = 0.0f, near_density = 0.0f;
-    const Rows r = rows_of(u, p);
-    if (r.inside) {
-        for (int dz = r.dz0; dz <= r.dz1; ++dz)
-            for (int dy = r.dy0; dy <= r.dy1; ++dy) {
-                const int32_t row = (r.v[1] + dy) * u.X + (r.v[2] + dz) * u.X * u.Y;
-                const uint32_t lo = b.start[row + r.v[0] + r.dx0];
-                const uint32_t hi = min(b.start[row + r.v[0] + r.dx1 + 1], b.n);
-                for (uint32_t j = lo; j < hi; ++j) {
-                    const float4 q = b.spos[j];
-                    const float dx = p.x - q.x, dy2 = p.y - q.y, dz2 = p.z - q.z;
-                    const float r2 = dot3(dx, dy2, dz2, dx, dy2, dz2);
-                    if (r2 < u.h2) {
-                        const float rr = sqrtf(r2);
-                        const float dd = u.h2 - rr * rr;
-                        const float w = ((scale_d * dd) * dd) * dd;
-                        density = density + u.mass * w;
-                        const float d = u.h - rr;
-                        const float wn = ((scale_n * d) * d) * d;
-                        near_density = near_density + u.mass * wn;
-                    }
-                }
-            }
+    const Stream st = stream_of(row_ranges(b, u, rows_of(u, p)));
+    const uint32_t total = st.pre[9];
+    for (uint32_t base = 0; base < total; base += kQuad) {
+        const uint32_t x = base + q;
+        const bool valid = x < total;
+        const float4 c = b.spos[valid ? slot_at(st, x) : k];
+        const float dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+        const float r2 = dot3(dx, dy, dz, dx, dy, dz);
+        float td = 0.0f, tn = 0.0f;
+        if (valid && r2 < u.h2) {
+            const float r = sqrtf(r2);
+            const float dd = u.h2 - r * r;
+            td = u.mass * (((scale_d * dd) * dd) * dd);
+            const float d = u.h - r;
+            tn = u.mass * (((scale_n * d) * d) * d);
+        }
+        density = quad_accumulate(density, td);
+        near_density = quad_accumulate(near_density, tn);
     }
+    if (q != 0) return;
+    const uint32_t i = b.order2[k];
     const float2 out = make_float2(density, near_density);
     b.dn[i] = out;
     b.sdn[k] = out;
@@ -271,12 +348,14 @@ __global__ void __launch_bounds__(kBlock) k_density(Bufs b) {
 
 // computeForce (sph/force.wgsl:41-59 kernels, :73-152) + integrate
 // (sph/integrate.wgsl:35-65) + copyPosition (sph/copyPosition.wgsl:34-38),
-// over the sorted slots.
-__global__ void __launch_bounds__(kBlock) k_force(Bufs b) {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+// over the sorted slots.  |pos_j - pos_i| is the r of the pair: the two
+// differences are negations of each other, exactly, so dot(dir, dir) == r2
+// bit for bit and normalize's length is sqrt(r2).
+__global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
+    const uint32_t g = blockIdx.x * kNbrBlock + threadIdx.x;
+    const uint32_t k = g / kQuad, q = g % kQuad;
     const Uniforms &u = *b.u;
-    if (k >= b.n || k >= b.start[u.G]) return;
-    const uint32_t i = b.order2[k];
+    if (k >= b.n || k >= b.start[u.G]) return;  // the whole quad leaves together
     const float4 p = b.spos[k];
     float4 v = b.svel[k];
     const float2 dni = b.sdn[k];
@@ -284,52 +363,64 @@ __global__ void __launch_bounds__(kBlock) k_force(Bufs b) {
     const float scale_g = 45.0f / ((float)3.1415926535 * u.h6);
     const float scale_ng = 45.0f / ((float)3.1415926535 * u.h5);
     const float scale_l = 45.0f / ((float)3.1415926535 * u.h6);
+    const float pressure_i = u.stiff * (density_i - u.rest);
+    const float near_pressure_i = u.near_stiff * near_density_i;
     float fpx = 0.0f, fpy = 0.0f, fpz = 0.0f, fvx = 0.0f, fvy = 0.0f, fvz = 0.0f;
-    const Rows r = rows_of(u, p);
-    if (r.inside) {
-        for (int dz = r.dz0; dz <= r.dz1; ++dz)
-            for (int dy = r.dy0; dy <= r.dy1; ++dy) {
-                const int32_t row = (r.v[1] + dy) * u.X + (r.v[2] + dz) * u.X * u.Y;
-                const uint32_t lo = b.start[row + r.v[0] + r.dx0];
-                const uint32_t hi = min(b.start[row + r.v[0] + r.dx1 + 1], b.n);
-                for (uint32_t j = lo; j < hi; ++j) {
-                    const float2 dnj = b.sdn[j];
-                    const float4 q = b.spos[j];
-                    const float ex = p.x - q.x, ey = p.y - q.y, ez = p.z - q.z;
-                    const float r2 = dot3(ex, ey, ez, ex, ey, ez);
-                    if (dnj.x == 0.0f || dnj.y == 0.0f) continue;
-                    if (r2 < u.h2 && 0.0f < r2) {
-                        const float rr = sqrtf(r2);
-                        const float pressure_i = u.stiff * (density_i - u.rest);
-                        const float pressure_j = u.stiff * (dnj.x - u.rest);
-                        const float near_pressure_i = u.near_stiff * near_density_i;
-                        const float near_pressure_j = u.near_stiff * dnj.y;
-                        const float shared = (pressure_i + pressure_j) / 2.0f;
-                        const float near_shared = (near_pressure_i + near_pressure_j) / 2.0f;
-                        float dirx = q.x - p.x, diry = q.y - p.y, dirz = q.z - p.z;
-                        const float len = sqrtf(dot3(dirx, diry, dirz, dirx, diry, dirz));
-                        dirx = dirx / len;
-                        diry = diry / len;
-                        dirz = dirz / len;
-                        const float dg = u.h - rr;
-                        const float grad = (scale_g * dg) * dg;
-                        const float near_grad = (scale_ng * dg) * dg;
-                        const float lap = scale_l * dg;
-                        const float sp = -u.mass * shared, nsp = -u.mass * near_shared;
-                        fpx = fpx + ((sp * dirx) * grad) / dnj.x;
-                        fpy = fpy + ((sp * diry) * grad) / dnj.x;
-                        fpz = fpz + ((sp * dirz) * grad) / dnj.x;
-                        fpx = fpx + ((nsp * dirx) * near_grad) / dnj.y;
-                        fpy = fpy + ((nsp * diry) * near_grad) / dnj.y;
-                        fpz = fpz + ((nsp * dirz) * near_grad) / dnj.y;
-                        const float4 w = b.svel[j];
-                        fvx = fvx + ((u.mass * (w.x - v.x)) * lap) / dnj.x;
-                        fvy = fvy + ((u.mass * (w.y - v.y)) * lap) / dnj.x;
-                        fvz = fvz + ((u.mass * (w.z - v.z)) * lap) / dnj.x;
-                    }
-                }
-            }
+    const Stream st = stream_of(row_ranges(b, u, rows_of(u, p)));
+    const uint32_t total = st.pre[9];
+    for (uint32_t base = 0; base < total; base += kQuad) {
+        const uint32_t x = base + q;
+        const bool valid = x < total;
+        const uint32_t j = valid ? slot_at(st, x) : k;
+        const float4 c = b.spos[j];
+        const float2 dnj = b.sdn[j];
+        const float4 w = b.svel[j];
+        const float ex = p.x - c.x, ey = p.y - c.y, ez = p.z - c.z;
+        const float r2 = dot3(ex, ey, ez, ex, ey, ez);
+        float ax = 0.0f, ay = 0.0f, az = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+        float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+        if (valid && dnj.x != 0.0f && dnj.y != 0.0f && r2 < u.h2 && 0.0f < r2) {
+            const float r = sqrtf(r2);
+            const float pressure_j = u.stiff * (dnj.x - u.rest);
+            const float near_pressure_j = u.near_stiff * dnj.y;
+            const float shared = (pressure_i + pressure_j) / 2.0f;
+            const float near_shared = (near_pressure_i + near_pressure_j) / 2.0f;
+            const float dirx = (c.x - p.x) / r, diry = (c.y - p.y) / r, dirz = (c.z - p.z) / r;
+            const float dg = u.h - r;
+            const float grad = (scale_g * dg) * dg;
+            const float near_grad = (scale_ng * dg) * dg;
+            const float lap = scale_l * dg;
+            const float sp = -u.mass * shared, nsp = -u.mass * near_shared;
+            ax = ((sp * dirx) * grad) / dnj.x;
+            ay = ((sp * diry) * grad) / dnj.x;
+            az = ((sp * dirz) * grad) / dnj.x;
+            nx = ((nsp * dirx) * near_grad) / dnj.y;
+            ny = ((nsp * diry) * near_grad) / dnj.y;
+            nz = ((nsp * dirz) * near_grad) / dnj.y;
+            vx = ((u.mass * (w.x - v.x)) * lap) / dnj.x;
+            vy = ((u.mass * (w.y - v.y)) * lap) / dnj.x;
+            vz = ((u.mass * (w.z - v.z)) * lap) / dnj.x;
+        }
+        // per candidate, in stream order: fPress += pressure term, then
+        // += near-pressure term (force.wgsl:113-114); fVisc += (:116)
+        fpx = fpx + quad_lane<0>(ax); fpx = fpx + quad_lane<0>(nx);
+        fpx = fpx + quad_lane<1>(ax); fpx = fpx + quad_lane<1>(nx);
+        fpx = fpx + quad_lane<2>(ax); fpx = fpx + quad_lane<2>(nx);
+        fpx = fpx + quad_lane<3>(ax); fpx = fpx + quad_lane<3>(nx);
+        fpy = fpy + quad_lane<0>(ay); fpy = fpy + quad_lane<0>(ny);
+        fpy = fpy + quad_lane<1>(ay); fpy = fpy + quad_lane<1>(ny);
+        fpy = fpy + quad_lane<2>(ay); fpy = fpy + quad_lane<2>(ny);
+        fpy = fpy + quad_lane<3>(ay); fpy = fpy + quad_lane<3>(ny);
+        fpz = fpz + quad_lane<0>(az); fpz = fpz + quad_lane<0>(nz);
+        fpz = fpz + quad_lane<1>(az); fpz = fpz + quad_lane<1>(nz);
+        fpz = fpz + quad_lane<2>(az); fpz = fpz + quad_lane<2>(nz);
+        fpz = fpz + quad_lane<3>(az); fpz = fpz + quad_lane<3>(nz);
+        fvx = quad_accumulate(fvx, vx);
+        fvy = quad_accumulate(fvy, vy);
+        fvz = quad_accumulate(fvz, vz);
     }
+    if (q != 0) return;
+    const uint32_t i = b.order2[k];
     fvx = fvx * u.visc;
     fvy = fvy * u.visc;
     fvz = fvz * u.visc;
@@ -444,26 +535,27 @@ inline unsigned blocks_for(uint64_t count) { return (unsigned)((count + kBlock -
 // One kernel of the substep; with events, they take the kernel's own start
 // and end timestamps (hipExtLaunchKernel).
 hipError_t launch_kernel(void (*kernel)(Bufs), unsigned grid, const Bufs &b, hipStream_t st,
-                         hipEvent_t e0, hipEvent_t e1) {
+                         hipEvent_t e0, hipEvent_t e1, unsigned block = kBlock) {
     if (grid == 0) return hipSuccess;
     if (e0)
-        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, st, e0, e1, 0, b);
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, e0, e1, 0, b);
     else
-        kernel<<<grid, kBlock, 0, st>>>(b);
+        kernel<<<grid, block, 0, st>>>(b);
     return hipGetLastError();
 }
 
 hipError_t launch_stage(int stage, const Bufs &b, unsigned ntiles, hipStream_t st,
                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const unsigned pb = blocks_for(b.n);
+    const unsigned nb = (unsigned)(((uint64_t)b.n * kQuad + kNbrBlock - 1) / kNbrBlock);
     switch (stage) {
     case SPH_STAGE_COUNT: return launch_kernel(k_count, pb, b, st, e0, e1);
     case SPH_STAGE_SCAN_REDUCE: return launch_kernel(k_scan_reduce, ntiles, b, st, e0, e1);
     case SPH_STAGE_SCAN: return launch_kernel(k_scan, ntiles, b, st, e0, e1);
     case SPH_STAGE_SCATTER: return launch_kernel(k_scatter, pb, b, st, e0, e1);
     case SPH_STAGE_SORT_GATHER: return launch_kernel(k_sort_gather, pb, b, st, e0, e1);
-    case SPH_STAGE_DENSITY: return launch_kernel(k_density, pb, b, st, e0, e1);
-    case SPH_STAGE_FORCE: return launch_kernel(k_force, pb, b, st, e0, e1);
+    case SPH_STAGE_DENSITY: return launch_kernel(k_density, nb, b, st, e0, e1, kNbrBlock);
+    case SPH_STAGE_FORCE: return launch_kernel(k_force, nb, b, st, e0, e1, kNbrBlock);
     default: return hipErrorInvalidValue;
     }
 }
