@@ -243,7 +243,8 @@ def counter_table(dirs, lo: int = 0, hi: int = 0) -> str:
     return "\n".join(out)
 
 
-DEFAULT_TUNING = {"path": 0, "particles_per_thread": 2, "grid_blocks": 0, "xcd_remap": 0}
+DEFAULT_TUNING = {"path": 0, "particles_per_thread": 2, "grid_blocks": 0, "xcd_remap": 0,
+                  "tile_edge": 4}
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "substep_kernels_dambreak_300k.txt")

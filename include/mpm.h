@@ -162,26 +162,30 @@ int mpm_time_stages(mpm_sim *sim, uint32_t substeps, float *stage_ms, int *calls
  *                        1 = reference-shaped kernels (one launch per WGSL kernel;
  *                            single simulators only)
  *   particles_per_thread 1, 2 or 4: a P2G/G2P block takes a chunk of at most
- *                        256 x this particles of one 4^3-cell tile
+ *                        256 x this particles of one tile
  *   grid_blocks          persistent blocks of the chunk kernels, 0 = 4096,
  *                        at most 2^20
  *   xcd_remap            1 = each XCD walks a contiguous range of chunks,
  *                        0 = runs of 16 consecutive chunks dealt round robin
  *                            over the XCDs (default)
+ *   tile_edge            4 (default) or 8: P2G tiles of tile_edge^3 cells, whose
+ *                        LDS windows are (tile_edge + 2)^3 cells
  * Results do not depend on these (integer adds commute; per-particle float
- * math is unchanged); only speed does. */
+ * math is unchanged); only speed does.  mpm_backward runs its reverse
+ * kernels on 4^3 tiles and restores the tuning afterwards. */
 typedef struct {
     int path;
     int particles_per_thread;
     int grid_blocks;
     int xcd_remap;
+    int tile_edge;
 } mpm_tuning;
 
 int mpm_get_tuning(mpm_sim *sim, mpm_tuning *out);
 int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in);
 
-/* Autotuner: times candidate tunings (particles_per_thread x grid_blocks x
- * xcd_remap) for `substeps_per_trial` substeps each on a snapshot of the
+/* Autotuner: times candidate tunings (tile_edge x particles_per_thread x
+ * grid_blocks x xcd_remap) for `substeps_per_trial` substeps each on a snapshot of the
  * current state, restores the state and keeps the fastest.  No reference
  * counterpart. */
 int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen);

@@ -82,7 +82,7 @@ struct Emu {
         d.n = n;
         d.G = (uint32_t)((uint64_t)std::ceil(box[0]) * (uint64_t)std::ceil(box[1]) *
                          (uint64_t)std::ceil(box[2]));
-        scene_key_geometry(d);
+        scene_key_geometry(d, kTileBits);
         append_hilbert_ranks(d, rank_of, xmaj);
         append_tile_tables(d, 0, rank_of.data(), xmaj.data(), nbr, info);
         nkeys = d.nkeys;

@@ -172,3 +172,22 @@ def test_backward_leaves_the_rollout_final_state():
         pa, pb = a.read_particles(), b.read_particles()
         assert np.array_equal(pa.view(np.int32), pb.view(np.int32))
         assert np.array_equal(a.read_posvel().view(np.int32), b.read_posvel().view(np.int32))
+
+
+@pytest.mark.gpu
+def test_backward_from_8cube_tiles_runs_on_4cube_tiles_and_restores_the_tuning():
+    rec, box, target = fd_scene()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as a, \
+            MLSMPMSimulator(particles=rec, init_box_size=box) as b:
+        a.set_tuning(tile_edge=8)
+        ga = a.backward(16, target, checkpoint_every=4)
+        gb = b.backward(16, target, checkpoint_every=4)
+        assert a.tuning["tile_edge"] == 8
+        assert ga["loss"] == gb["loss"]
+        for key in ("grad_v0", "grad_x0"):
+            atol = 2e-5 * float(np.max(np.abs(gb[key])))
+            np.testing.assert_allclose(ga[key], gb[key], rtol=1e-4, atol=atol)
+        # and the 8^3 forward path continues from the rollout's final state
+        a.execute(2)
+        b.execute(2)
+        assert np.array_equal(a.read_posvel().view(np.int32), b.read_posvel().view(np.int32))

@@ -41,7 +41,8 @@ def oracle_scene(rec, box, phys):
 
 
 @pytest.mark.parametrize("tuning", [None, dict(particles_per_thread=1, grid_blocks=5, xcd_remap=0),
-                                    dict(particles_per_thread=4, grid_blocks=64)])
+                                    dict(particles_per_thread=4, grid_blocks=64),
+                                    dict(particles_per_thread=2, grid_blocks=40, tile_edge=8)])
 def test_batch_matches_oracle_per_scene(tuning):
     sc = small_batch()
     refs = [oracle_scene(r, b, p) for r, b, p in sc]
@@ -65,6 +66,7 @@ def test_autotuned_batch_equals_scenes_run_alone():
     with MLSMPMBatch([(r, b, params_of(p)) for r, b, p in sc]) as bt:
         chosen = bt.autotune(2)
         assert chosen["particles_per_thread"] in (1, 2, 4)
+        assert chosen["tile_edge"] in (4, 8)
         bt.step(4)
         for s, (r, b, p) in enumerate(sc):
             with MLSMPMSimulator(params=params_of(p), particles=r, init_box_size=b) as sim:

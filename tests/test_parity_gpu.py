@@ -134,13 +134,15 @@ def _production_step_matches_oracle_stages(sim, box, real_box, ctx, threads=0):
     assert_same(sim, ref, ("particles", "posvel"), ctx=ctx)
 
 
-@pytest.mark.parametrize("cfg,frames,resize", [("config0", 3, None), ("config1", 2, None),
-                                               ("config1", 2, [64.0, 80.0, 80.0]),
-                                               ("config2", 1, None)],
-                         ids=["40k", "300k", "300k-resized", "2M"])
-def test_production_grid_accumulators_bit_exact(cfg, frames, resize):
+@pytest.mark.parametrize("cfg,frames,resize,edge", [
+    ("config0", 3, None, 4), ("config1", 2, None, 4), ("config1", 2, [64.0, 80.0, 80.0], 4),
+    ("config2", 1, None, 4), ("config0", 3, None, 8), ("config1", 2, [64.0, 80.0, 80.0], 8),
+    ("config2", 1, None, 8)],
+    ids=["40k", "300k", "300k-resized", "2M", "40k-8^3", "300k-resized-8^3", "2M-8^3"])
+def test_production_grid_accumulators_bit_exact(cfg, frames, resize, edge):
     rec, box = getattr(scenes, cfg)()
     with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        sim.set_tuning(tile_edge=edge)
         sim.execute(2 * frames)
         real = list(box)
         if resize is not None:
@@ -355,6 +357,9 @@ TUNINGS = [
     dict(path=0, particles_per_thread=4, grid_blocks=3, xcd_remap=0),
     dict(path=0, particles_per_thread=1, grid_blocks=4096, xcd_remap=0),
     dict(path=1, particles_per_thread=2, grid_blocks=0, xcd_remap=1),
+    dict(path=0, particles_per_thread=1, grid_blocks=0, xcd_remap=1, tile_edge=8),
+    dict(path=0, particles_per_thread=2, grid_blocks=24, xcd_remap=0, tile_edge=8),
+    dict(path=0, particles_per_thread=4, grid_blocks=3, xcd_remap=1, tile_edge=8),
 ]
 # integral init box: tame particles take the tile-partial path; a non-integral
 # one sends every particle down the global-atomic path
@@ -392,3 +397,18 @@ def test_time_stages_reports_every_kernel_slot():
         assert t["p2g_1"][1] == 8
         ms = sim.time_steps(4)
         assert ms > 0
+
+
+def test_switching_the_tile_edge_mid_rollout_keeps_results():
+    """4^3 -> 8^3 -> 4^3 tiles between frames (the autotuner's switch): the
+    key geometry and tile tables are rebuilt under the running state, and
+    every frame still equals the oracle."""
+    rec, box = scenes.config0()
+    sim, ref = pair(rec, box)
+    with sim:
+        for frame, edge in enumerate((4, 8, 8, 4, 8)):
+            sim.set_tuning(tile_edge=edge)
+            assert sim.tuning["tile_edge"] == edge
+            sim.execute()
+            ref.execute(2)
+            assert_same(sim, ref, ("particles", "posvel"), ctx=f"frame {frame} edge {edge}")

@@ -1033,6 +1033,14 @@ hipError_t Batch::substep_taped(hipStream_t st, char *rec, const void *layout) {
 hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResult &res) {
     if (h_scenes_.size() != 1 || cfg.substeps == 0 || cfg.checkpoint_every == 0)
         return hipErrorInvalidValue;
+    // the reverse kernels work on 4^3 tiles
+    const Tuning saved_tuning = tun_;
+    if (tun_.path != 0 || tun_.tile_bits != kTileBits) {
+        Tuning t = tun_;
+        t.path = 0;
+        t.tile_bits = kTileBits;
+        TRY(set_tuning(t, st));
+    }
     const SceneDesc &d = h_scenes_[0];
     const uint32_t n = d.n, G = d.G, T = cfg.substeps, K = cfg.checkpoint_every;
     const uint32_t nseg = (T + K - 1) / K;
@@ -1045,12 +1053,6 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
     L.tile_chunk = align256(L.tchunks + sizeof(TileChunk) * cap_tchunks_);
     L.ntchunks = align256(L.tile_chunk + sizeof(uint2) * total_keys_);
     L.bytes = align256(L.ntchunks + sizeof(uint32_t));
-    const Tuning saved_tuning = tun_;
-    if (tun_.path != 0) {
-        Tuning t = tun_;
-        t.path = 0;
-        TRY(set_tuning(t));
-    }
     // Device workspace (kept between calls): checkpoints (state + id) per
     // segment start, the segment tape, grid linearisation, adjoints.
     hipError_t e = hipSuccess;
@@ -1140,7 +1142,7 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         AdjArgs q{};
         q.sd = d_scenes_;
         q.stride = stride_;
-        q.tchunk_cap = (uint32_t)cap_tchunks_;
+        q.tchunk_cap = tchunk_cap();  // (4^3 tiles here)
         q.nbr = d_nbr_;
         q.tile_info = d_tile_info_;
         q.stiles = total_tiles_;
@@ -1218,7 +1220,8 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         if (e == hipSuccess) e = launch_stage(5, stage_view(), st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     } while (false);
-    if (saved_tuning.path != tun_.path) (void)set_tuning(saved_tuning);
+    if (saved_tuning.path != tun_.path || saved_tuning.tile_bits != tun_.tile_bits)
+        (void)set_tuning(saved_tuning, st);
     return e;
 }
 

@@ -15,6 +15,7 @@ struct Tuning {
     int ppt = 2;           // particles per thread: tile chunks of <= 256 * ppt particles
     int grid_blocks = 0;   // persistent blocks of the chunk kernels (0: 4096)
     int xcd_remap = 0;     // 1: contiguous chunk ranges per XCD (0: runs of 16, round robin)
+    int tile_bits = 2;     // P2G tiles of (2^tile_bits)^3 cells: 2 (4^3) or 3 (8^3)
 };
 
 // Uniforms of a scene from its init and real boxes.
@@ -84,7 +85,9 @@ public:
     hipError_t external_change(hipStream_t st);
 
     const Tuning &tuning() const { return tun_; }
-    hipError_t set_tuning(const Tuning &t);
+    // A new tile size rebuilds the key geometry and tile tables (after
+    // waiting for `st`); the buffers are sized for both tile sizes.
+    hipError_t set_tuning(const Tuning &t, hipStream_t st);
     // Times candidate tunings on a snapshot of the current state (restored
     // afterwards) and keeps the fastest.
     hipError_t autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *chosen);
@@ -97,10 +100,17 @@ public:
 private:
     hipError_t init_impl(const SceneInit *scenes, uint32_t count, hipStream_t st);
     hipError_t ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys,
-                               uint64_t tchunks);
+                               uint64_t tchunks, uint64_t partial_cells);
+    // Key geometry of every scene for tiles of (2^tb)^3 cells (koff, soff,
+    // ntiles, nkeys); returns the keys and spatial tiles of all scenes.
+    void layout_keys(int tb, uint64_t *keys, uint64_t *tiles);
+    // Makes (2^tb)^3 tiles current: key geometry, Hilbert ranks, neighbour
+    // and tile tables, scene records; every tile marked dirty.
+    hipError_t apply_tile_geometry(int tb);
     hipError_t upload_chunks();
     BatchArgs args() const;
     BinBuffers bin_buffers() const;
+    uint32_t tchunk_cap() const;
     // Per-slot events of time_stages (slots in launch order).
     struct SlotTimes {
         static constexpr int kMax = 8;
@@ -133,7 +143,7 @@ private:
     uint64_t total_cells_ = 0, total_keys_ = 0;
     uint32_t total_tiles_ = 0;  // spatial tiles of all scenes
     uint32_t cap_n_ = 0;
-    uint64_t cap_cells_ = 0, cap_keys_ = 0, cap_tchunks_ = 0;
+    uint64_t cap_cells_ = 0, cap_keys_ = 0, cap_tchunks_ = 0, cap_partial_ = 0;
     uint32_t scene_cap_ = 0;
 
     float *state_ = nullptr, *spare_state_ = nullptr;
@@ -143,7 +153,7 @@ private:
     float *staging_ = nullptr;  // 80-B records of every particle
     float4 *vel_ = nullptr;     // decoded updated grid velocities
     int4 *wild_ = nullptr;      // wild-particle accumulators, zero between substeps
-    int4 *partial_ = nullptr;   // kWinCells per tile chunk
+    int4 *partial_ = nullptr;   // one window (216 or 1000 cells) per tile chunk
     uint32_t *key_ = nullptr, *rank_ = nullptr, *src_ = nullptr, *count_ = nullptr,
              *offset_ = nullptr;
     uint32_t *mig_ = nullptr;  // 27 per tile chunk

@@ -75,9 +75,10 @@ struct Allocator {
 }  // namespace
 
 hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t keys,
-                                  uint64_t tchunks) {
+                                  uint64_t tchunks, uint64_t partial_cells) {
     const uint32_t cap = std::max<uint32_t>(64u, (total_n + 63u) & ~63u);
-    if (cap <= cap_n_ && cells <= cap_cells_ && keys <= cap_keys_ && tchunks <= cap_tchunks_)
+    if (cap <= cap_n_ && cells <= cap_cells_ && keys <= cap_keys_ && tchunks <= cap_tchunks_ &&
+        partial_cells <= cap_partial_)
         return hipSuccess;
     Allocator alloc;
     drop_graphs();
@@ -104,9 +105,10 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     free_ptr(ntchunks_);
     free_ptr(tile_chunk_);
     cap_n_ = 0;
-    cap_cells_ = cap_keys_ = cap_tchunks_ = 0;
+    cap_cells_ = cap_keys_ = cap_tchunks_ = cap_partial_ = 0;
     const size_t c = cap, g = std::max<uint64_t>(cells, 1), k = std::max<uint64_t>(keys, 1),
-                 tc = std::max<uint64_t>(tchunks, 1);
+                 tc = std::max<uint64_t>(tchunks, 1),
+                 pc = std::max<uint64_t>(partial_cells, (uint64_t)kMaxWinCells);
     TRY(alloc(&state_, sizeof(float) * NCOMP * c));
     state_buf0_ = state_;
     TRY(alloc(&spare_state_, sizeof(float) * NCOMP * c));
@@ -116,7 +118,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(alloc(&vel_, sizeof(float4) * g));
     TRY(alloc(&wild_, sizeof(int4) * g));
     TRY(hipMemset(wild_, 0, sizeof(int4) * g));
-    TRY(alloc(&partial_, sizeof(int4) * kWinCells * tc));
+    TRY(alloc(&partial_, sizeof(int4) * pc));
     TRY(alloc(&posvel_, 2 * sizeof(float4) * c));
     TRY(alloc(&staging_, 80 * c));
     TRY(alloc(&key_, sizeof(uint32_t) * c));
@@ -143,6 +145,7 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     cap_cells_ = g;
     cap_keys_ = k;
     cap_tchunks_ = tc;
+    cap_partial_ = pc;
     return hipSuccess;
 }
 
@@ -157,44 +160,27 @@ hipError_t Batch::init(const SceneInit *sc, uint32_t count, hipStream_t st) {
     return e;
 }
 
-hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st) {
-    TRY(hipStreamSynchronize(st));  // (graphs stay: they match only their own arguments)
-    h_scenes_.assign(count, SceneDesc{});
-    uint64_t poff = 0, goff = 0, koff = 0, soff = 0;
-    for (uint32_t s = 0; s < count; ++s) {
-        SceneDesc &d = h_scenes_[s];
-        d.uni = derive_uniforms(sc[s].init_box, sc[s].init_box);
-        d.prm = sc[s].prm;
-        d.poff = (uint32_t)poff;
-        d.n = sc[s].n;
-        d.goff = (uint32_t)goff;
-        d.G = (uint32_t)((uint64_t)std::ceil(sc[s].init_box[0]) *
-                         (uint64_t)std::ceil(sc[s].init_box[1]) *
-                         (uint64_t)std::ceil(sc[s].init_box[2]));
-        scene_key_geometry(d);
+void Batch::layout_keys(int tb, uint64_t *keys, uint64_t *tiles) {
+    uint64_t koff = 0, soff = 0;
+    for (SceneDesc &d : h_scenes_) {
+        scene_key_geometry(d, tb);
         d.koff = (uint32_t)koff;
         d.toff = (uint32_t)koff;
         d.soff = (uint32_t)soff;
-        d.rank_off = 0;  // set below
-        poff += d.n;
-        goff += d.G;
+        d.rank_off = 0;  // set by apply_tile_geometry
         koff += d.nkeys;
         soff += d.ntiles;
     }
-    total_n_ = (uint32_t)poff;
-    total_cells_ = goff;
-    total_keys_ = koff;
-    total_tiles_ = (uint32_t)soff;
-    // Tile chunks of one binning: sum over keys of ceil(count / 256) <=
-    // n / 256 + (non-empty keys).
-    const uint64_t max_tchunks = (poff + kBlock - 1) / kBlock + std::min<uint64_t>(poff, koff);
-    TRY(ensure_capacity(total_n_, total_cells_, total_keys_, max_tchunks));
-    stride_ = cap_n_;
-    if (count > scene_cap_) {
-        free_ptr(d_scenes_);
-        TRY(hipMalloc(&d_scenes_, sizeof(SceneDesc) * count));
-        scene_cap_ = count;
-    }
+    *keys = koff;
+    *tiles = soff;
+}
+
+hipError_t Batch::apply_tile_geometry(int tb) {
+    uint64_t keys = 0, tiles = 0;
+    layout_keys(tb, &keys, &tiles);
+    total_keys_ = keys;
+    total_tiles_ = (uint32_t)tiles;
+    const uint32_t count = (uint32_t)h_scenes_.size();
     std::vector<uint32_t> ranks, xmaj, nbr;
     std::vector<uint4> info;
     for (uint32_t s = 0; s < count; ++s) {
@@ -209,6 +195,7 @@ hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st)
         free_ptr(d_tile_info_);
         free_ptr(d_dirty_);
         free_ptr(d_near_);
+        tile_tab_cap_ = 0;
         TRY(hipMalloc(&d_nbr_, sizeof(uint32_t) * nbr.size()));
         TRY(hipMalloc(&d_tile_info_, sizeof(uint4) * info.size()));
         TRY(hipMalloc(&d_dirty_, info.size()));
@@ -225,12 +212,12 @@ hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st)
         for (uint32_t k = 0; k < h_scenes_[s].nkeys; ++k) key_scene[h_scenes_[s].koff + k] = s;
     if (key_scene.size() > key_scene_cap_) {
         free_ptr(d_key_scene_);
+        key_scene_cap_ = 0;
         TRY(hipMalloc(&d_key_scene_, sizeof(uint32_t) * key_scene.size()));
         key_scene_cap_ = key_scene.size();
     }
     TRY(hipMemcpy(d_key_scene_, key_scene.data(), sizeof(uint32_t) * key_scene.size(),
                   hipMemcpyHostToDevice));
-
     if (!info.empty()) {
         TRY(hipMemcpy(d_nbr_, nbr.data(), sizeof(uint32_t) * nbr.size(), hipMemcpyHostToDevice));
         TRY(hipMemcpy(d_tile_info_, info.data(), sizeof(uint4) * info.size(),
@@ -238,11 +225,13 @@ hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st)
     }
     if (ranks.size() > tile_rank_cap_) {
         free_ptr(d_tile_rank_);
+        tile_rank_cap_ = 0;
         TRY(hipMalloc(&d_tile_rank_, sizeof(uint32_t) * ranks.size()));
         tile_rank_cap_ = ranks.size();
     }
     if (xmaj.size() > tile_xmaj_cap_) {
         free_ptr(d_tile_xmaj_);
+        tile_xmaj_cap_ = 0;
         TRY(hipMalloc(&d_tile_xmaj_, sizeof(uint32_t) * xmaj.size()));
         tile_xmaj_cap_ = xmaj.size();
     }
@@ -253,6 +242,48 @@ hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st)
                   hipMemcpyHostToDevice));
     TRY(hipMemcpy(d_scenes_, h_scenes_.data(), sizeof(SceneDesc) * count,
                   hipMemcpyHostToDevice));
+    keys_ready_ = false;  // keys and counts were taken for the other tiles
+    return hipSuccess;
+}
+
+hipError_t Batch::init_impl(const SceneInit *sc, uint32_t count, hipStream_t st) {
+    TRY(hipStreamSynchronize(st));  // (graphs stay: they match only their own arguments)
+    h_scenes_.assign(count, SceneDesc{});
+    uint64_t poff = 0, goff = 0;
+    for (uint32_t s = 0; s < count; ++s) {
+        SceneDesc &d = h_scenes_[s];
+        d.uni = derive_uniforms(sc[s].init_box, sc[s].init_box);
+        d.prm = sc[s].prm;
+        d.poff = (uint32_t)poff;
+        d.n = sc[s].n;
+        d.goff = (uint32_t)goff;
+        d.G = (uint32_t)((uint64_t)std::ceil(sc[s].init_box[0]) *
+                         (uint64_t)std::ceil(sc[s].init_box[1]) *
+                         (uint64_t)std::ceil(sc[s].init_box[2]));
+        poff += d.n;
+        goff += d.G;
+    }
+    total_n_ = (uint32_t)poff;
+    total_cells_ = goff;
+    // Buffers for both tile sizes, so that the autotuner can switch without
+    // reallocating.  Tile chunks of one binning: sum over keys of
+    // ceil(count / 256) <= n / 256 + (non-empty keys).
+    uint64_t keys2 = 0, tiles2 = 0, keys3 = 0, tiles3 = 0;
+    layout_keys(3, &keys3, &tiles3);
+    layout_keys(2, &keys2, &tiles2);
+    const uint64_t tc2 = (poff + kBlock - 1) / kBlock + std::min<uint64_t>(poff, keys2);
+    const uint64_t tc3 = (poff + kBlock - 1) / kBlock + std::min<uint64_t>(poff, keys3);
+    TRY(ensure_capacity(total_n_, total_cells_, std::max(keys2, keys3), std::max(tc2, tc3),
+                        std::max(tc2 * (uint64_t)TileGeo<2>::win_cells,
+                                 tc3 * (uint64_t)TileGeo<3>::win_cells)));
+    stride_ = cap_n_;
+    if (count > scene_cap_) {
+        free_ptr(d_scenes_);
+        scene_cap_ = 0;
+        TRY(hipMalloc(&d_scenes_, sizeof(SceneDesc) * count));
+        scene_cap_ = count;
+    }
+    TRY(apply_tile_geometry(tun_.tile_bits));
     TRY(hipMemsetAsync(grid_, 0, sizeof(int4) * std::max<uint64_t>(total_cells_, 1), st));
     TRY(hipMemsetAsync(posvel_, 0, 2 * sizeof(float4) * (size_t)stride_, st));
     for (uint32_t s = 0; s < count; ++s)
@@ -305,6 +336,15 @@ DeviceScene Batch::stage_view() const {
     return v;
 }
 
+// Tile chunks one binning may emit: the chunk buffers' capacity, bounded by
+// the partial windows the partial buffer holds at the current tile size (a
+// binning emits at most n / 256 + (non-empty keys) chunks, which both bounds
+// cover for either tile size).
+uint32_t Batch::tchunk_cap() const {
+    const uint64_t wc = tun_.tile_bits == 3 ? TileGeo<3>::win_cells : TileGeo<2>::win_cells;
+    return (uint32_t)std::min<uint64_t>(cap_tchunks_, cap_partial_ / wc);
+}
+
 BatchArgs Batch::args() const {
     BatchArgs a{};
     a.scenes = d_scenes_;
@@ -317,7 +357,7 @@ BatchArgs Batch::args() const {
     a.ntchunks = ntchunks_;
     a.xcd_first = ntchunks_ + 1;
     a.total_slots = total_n_;
-    a.tchunk_cap = (uint32_t)cap_tchunks_;
+    a.tchunk_cap = tchunk_cap();
     a.tile_chunk = tile_chunk_;
     a.stiles = total_tiles_;
     a.nbr = d_nbr_;
@@ -350,6 +390,7 @@ BatchArgs Batch::args() const {
     a.ppt = tun_.ppt;
     a.grid_blocks = tun_.grid_blocks;
     a.xcd_remap = tun_.xcd_remap;
+    a.tile_bits = tun_.tile_bits;
     return a;
 }
 
@@ -365,7 +406,7 @@ BinBuffers Batch::bin_buffers() const {
     b.nkeys = (uint32_t)total_keys_;
     b.perm_out = perm_;
     b.tchunks = tchunks_;
-    b.tchunk_cap = (uint32_t)cap_tchunks_;
+    b.tchunk_cap = tchunk_cap();
     b.ntchunks = ntchunks_;
     b.xcd_first = ntchunks_ + 1;
     b.tile_chunk = tile_chunk_;
@@ -590,13 +631,20 @@ hipError_t Batch::write_grid(uint32_t s, const int32_t *src, uint64_t ncells, hi
     return hipStreamSynchronize(st);
 }
 
-hipError_t Batch::set_tuning(const Tuning &t) {
+hipError_t Batch::set_tuning(const Tuning &t, hipStream_t st) {
     if (t.path < 0 || t.path > 1) return hipErrorInvalidValue;
     if (t.ppt != 1 && t.ppt != 2 && t.ppt != 4) return hipErrorInvalidValue;
     if (t.grid_blocks < 0 || t.grid_blocks > (1 << 20) || t.xcd_remap < 0 || t.xcd_remap > 1)
         return hipErrorInvalidValue;
+    if (t.tile_bits != 2 && t.tile_bits != 3) return hipErrorInvalidValue;
     if (t.path == 1 && h_scenes_.size() > 1) return hipErrorInvalidValue;
     drop_graphs();
+    if (t.tile_bits != tun_.tile_bits && ready()) {
+        TRY(hipStreamSynchronize(st));  // the tables change under no running kernel
+        tun_.tile_bits = t.tile_bits;
+        const hipError_t e = apply_tile_geometry(t.tile_bits);
+        if (e != hipSuccess) return e;
+    }
     tun_ = t;
     keys_ready_ = false;  // counts were taken for the previous chunk size
     return hipSuccess;
@@ -624,18 +672,21 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
+    const int tbs[2] = {2, 3};
     const int ppts[3] = {1, 2, 4};
     const int blocks[3] = {1024, 2048, 4096};
     const int remaps[2] = {1, 0};
+    for (int ti = 0; ti < 2 && e == hipSuccess; ++ti)
     for (int pi = 0; pi < 3 && e == hipSuccess; ++pi)
         for (int bi = 0; bi < 3 && e == hipSuccess; ++bi)
             for (int ri = 0; ri < 2 && e == hipSuccess; ++ri) {
                 Tuning t = orig;
                 t.path = 0;
+                t.tile_bits = tbs[ti];
                 t.ppt = ppts[pi];
                 t.grid_blocks = blocks[bi];
                 t.xcd_remap = remaps[ri];
-                if ((e = set_tuning(t)) != hipSuccess) break;
+                if ((e = set_tuning(t, st)) != hipSuccess) break;
                 // restore the snapshot into the current buffers
                 if (state_ != orig_state) {
                     std::swap(state_, spare_state_);
@@ -681,10 +732,10 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
     if (e1) (void)hipEventDestroy(e1);
     if (e == hipSuccess) e = r;
     if (e != hipSuccess) {
-        (void)set_tuning(orig);
+        (void)set_tuning(orig, st);
         return e;
     }
-    e = set_tuning(best_ms < 1e30f ? best : orig);
+    e = set_tuning(best_ms < 1e30f ? best : orig, st);
     if (chosen) *chosen = tun_;
     return e;
 }
@@ -732,7 +783,7 @@ void Batch::release() {
     key_scene_cap_ = 0;
     tile_rank_cap_ = tile_xmaj_cap_ = tile_tab_cap_ = 0;
     cap_n_ = 0;
-    cap_cells_ = cap_keys_ = cap_tchunks_ = 0;
+    cap_cells_ = cap_keys_ = cap_tchunks_ = cap_partial_ = 0;
     scene_cap_ = 0;
     chunk_cap_ = 0;
     h_scenes_.clear();

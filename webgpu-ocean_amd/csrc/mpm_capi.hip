@@ -130,18 +130,23 @@ void to_c_tuning(const mpm::Tuning &t, mpm_tuning *out) {
     out->particles_per_thread = t.ppt;
     out->grid_blocks = t.grid_blocks;
     out->xcd_remap = t.xcd_remap;
+    out->tile_edge = 1 << t.tile_bits;
 }
 
-int from_c_tuning(const mpm_tuning *in, mpm::Batch &b) {
+int from_c_tuning(const mpm_tuning *in, mpm::Batch &b, hipStream_t st) {
     mpm::Tuning t;
     t.path = in->path;
     t.ppt = in->particles_per_thread;
     t.grid_blocks = in->grid_blocks;
     t.xcd_remap = in->xcd_remap;
-    if (b.set_tuning(t) != hipSuccess)
+    t.tile_bits = in->tile_edge == 8 ? 3 : in->tile_edge == 4 ? 2 : -1;
+    const hipError_t e = t.tile_bits < 0 ? hipErrorInvalidValue : b.set_tuning(t, st);
+    if (e == hipErrorInvalidValue)
         return fail(MPM_ERR_INVALID_ARGUMENT,
                     "tuning out of range: path 0|1 (1 only for one scene), "
-                    "particles_per_thread 1|2|4, grid_blocks 0..2^20, xcd_remap 0|1");
+                    "particles_per_thread 1|2|4, grid_blocks 0..2^20, xcd_remap 0|1, "
+                    "tile_edge 4|8");
+    if (e != hipSuccess) return hip_fail(e, "set_tuning");
     return MPM_OK;
 }
 
@@ -399,7 +404,7 @@ int mpm_set_tuning(mpm_sim *sim, const mpm_tuning *in) {
     int rc = ready(sim->ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(sim->ctx.stream));
-    return from_c_tuning(in, sim->ctx.batch);
+    return from_c_tuning(in, sim->ctx.batch, sim->ctx.stream);
 }
 
 int mpm_autotune(mpm_sim *sim, uint32_t substeps_per_trial, mpm_tuning *chosen) {
@@ -597,7 +602,7 @@ int mpm_batch_set_tuning(mpm_batch *b, const mpm_tuning *in) {
     int rc = ready(b->ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(b->ctx.stream));
-    return from_c_tuning(in, b->ctx.batch);
+    return from_c_tuning(in, b->ctx.batch, b->ctx.stream);
 }
 
 int mpm_batch_autotune(mpm_batch *b, uint32_t substeps_per_trial, mpm_tuning *chosen) {

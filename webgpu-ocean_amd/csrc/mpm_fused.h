@@ -19,15 +19,29 @@
 
 namespace mpm {
 
-// Tiles are (2^kTileBits)^3 cells, ordered along a 3D Hilbert curve per scene;
-// bin key = koff + tile rank.  Each scene has one extra key (rank = ntiles)
-// collecting the particles whose stencil leaves the grid ("wild" particles).
+// Tiles are (2^TB)^3 cells, TB = 2 (4^3, the default) or 3 (8^3), chosen per
+// batch (Tuning::tile_bits, autotuned), ordered along a 3D Hilbert curve per
+// scene; bin key = koff + tile rank.  Each scene has one extra key (rank =
+// ntiles) collecting the particles whose stencil leaves the grid ("wild"
+// particles).  A tile chunk's window is the tile plus one cell on every side.
+template <int TB>
+struct TileGeo {
+    static_assert(TB == 2 || TB == 3, "tiles of 4^3 or 8^3 cells");
+    static constexpr int bits = TB;
+    static constexpr int edge = 1 << TB;                          // 4 | 8
+    static constexpr int cells = edge * edge * edge;              // 64 | 512
+    static constexpr int win_edge = edge + 2;                     // 6 | 10
+    static constexpr int win_cells = win_edge * win_edge * win_edge;  // 216 | 1000
+};
+constexpr int kMaxWinCells = TileGeo<3>::win_cells;  // partial window stride bound
+
+// The 4^3 tile: the default, and the one the adjoint's reverse kernels use.
 constexpr int kTileBits = 2;
 constexpr int kBlock = 256;
-constexpr int kTileEdge = 1 << kTileBits;                      // 4
-constexpr int kTileCells = kTileEdge * kTileEdge * kTileEdge;  // 64
-constexpr int kWinEdge = kTileEdge + 2;                        // 6
-constexpr int kWinCells = kWinEdge * kWinEdge * kWinEdge;      // 216
+constexpr int kTileEdge = TileGeo<kTileBits>::edge;
+constexpr int kTileCells = TileGeo<kTileBits>::cells;
+constexpr int kWinEdge = TileGeo<kTileBits>::win_edge;
+constexpr int kWinCells = TileGeo<kTileBits>::win_cells;
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
 constexpr uint32_t kSegBits = 4;   // 16 keys per block of the count-path totals kernel
 constexpr uint32_t kSupBits = 8;   // binning scan totals per 256 keys (super-segment; g2p
@@ -50,6 +64,7 @@ struct SceneDesc {
     uint32_t nkeys;      // ntiles + 1
     uint32_t rank_off;   // first entry of the scene's x-major tile -> rank table
     uint32_t fast;       // 1: integer box (cells map to distinct indices), 0: all wild
+    uint32_t tile_bits;  // tiles of (2^tile_bits)^3 cells (the batch's)
 };
 
 // A run of consecutive particle slots of one scene (static partition used by
@@ -95,7 +110,7 @@ struct BatchArgs {
     float4 *vel;                 // decoded updated velocities (written by reduce_2)
     int4 *wild;                  // global-atomic accumulators of wild particles;
                                  // all zero between substeps
-    int4 *partial;               // kWinCells entries per tile chunk
+    int4 *partial;               // TileGeo<tile_bits>::win_cells entries per tile chunk
     float4 *posvel;              // 2 float4 per global caller index
     uint32_t *key, *rank;        // per slot: bin key and rank within the key (or within
                                  // its source chunk and direction, see src)
@@ -109,6 +124,7 @@ struct BatchArgs {
     int ppt;                     // particles per thread: chunk <= kBlock * ppt
     int grid_blocks;             // persistent blocks of the chunk kernels (0: default)
     int xcd_remap;               // contiguous chunk ranges per XCD
+    int tile_bits;               // 2: 4^3 tiles, 3: 8^3 tiles
 };
 
 struct BinBuffers {
@@ -125,8 +141,9 @@ struct BinBuffers {
     uint2 *tile_chunk;
 };
 
-// Derives key/tile geometry of one scene from its init box.
-void scene_key_geometry(SceneDesc &d);
+// Derives key/tile geometry of one scene from its init box, for tiles of
+// (2^tile_bits)^3 cells.
+void scene_key_geometry(SceneDesc &d, int tile_bits);
 
 // Appends the scene's x-major tile -> Hilbert-rank table (ntiles entries) and
 // its inverse (ntiles + 1 entries, the last = kWildTile).

@@ -8,11 +8,13 @@
 // permutation into this substep's bin order without moving the state: the
 // chunk kernels read through it, and g2p writes its output state in bin
 // order into the other buffer.  The binning scan also emits "tile chunks":
-// runs of at most 256*ppt particles of ONE 4^3-cell tile.  A P2G block accumulates its chunk into the tile's 6^3-cell LDS
-// window (tile + one-cell halo) with i32 fixed-point LDS adds, then stores the
-// whole window with coalesced plain stores as a partial.  The reduce kernels
-// give every grid cell exactly one writer: one wave per tile sums, for each of
-// its 64 cells, the partials of the chunks of the 27 surrounding tiles whose
+// runs of at most 256*ppt particles of ONE tile (4^3 or 8^3 cells, the
+// batch's tile_bits; kernels are instantiated for both).  A P2G block
+// accumulates its chunk into the tile's LDS window (tile + one-cell halo:
+// 6^3 or 10^3 cells) with i32 fixed-point LDS adds, then stores the whole
+// window with coalesced plain stores as a partial.  The reduce kernels give
+// every grid cell exactly one writer: one lane per cell sums the partials of
+// the chunks of the 27 surrounding tiles whose
 // windows cover it, then writes the cell, so there is no clear pass and no
 // global atomic.  reduce_2 also applies updateGrid and stores the decoded
 // velocities that g2p stages in LDS for its 27-node gathers.
@@ -66,8 +68,9 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return v;
 }
 
+template <int TB>
 __device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
-    return reduce_tile_of(blockIdx.x, threadIdx.x / kWave, a.stiles, a.xcd_remap);
+    return reduce_tile_of<TB>(blockIdx.x, threadIdx.x / kWave, a.stiles, a.xcd_remap);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -132,17 +135,19 @@ __device__ __forceinline__ void load_C(const float *__restrict__ st, uint32_t st
 // ---------------------------------------------------------------------------
 // sinks: where a particle's 27 node contributions go / come from
 
-struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x 216
+template <int TB>
+struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x win_cells
+    static constexpr int W = TileGeo<TB>::win_edge, WC = TileGeo<TB>::win_cells;
     int *lds;
     int l0;
     __device__ __forceinline__ void add4(const Node3 &, const AxisNode &, const AxisNode &, int gx,
                                          int gy, int gz, int32_t m, int32_t x, int32_t y,
                                          int32_t z) const {
-        const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
+        const int l = l0 + (gx * W + gy) * W + gz;
         atomicAdd(&lds[l], m);
-        atomicAdd(&lds[kWinCells + l], x);
-        atomicAdd(&lds[2 * kWinCells + l], y);
-        atomicAdd(&lds[3 * kWinCells + l], z);
+        atomicAdd(&lds[WC + l], x);
+        atomicAdd(&lds[2 * WC + l], y);
+        atomicAdd(&lds[3 * WC + l], z);
     }
 };
 
@@ -163,19 +168,21 @@ struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
     }
 };
 
-struct WinSink2 {  // p2g_2: decoded masses [0, 216), momentum ints [216, 864)
+template <int TB>
+struct WinSink2 {  // p2g_2: decoded masses [0, WC), momentum ints [WC, 4 WC)
+    static constexpr int W = TileGeo<TB>::win_edge, WC = TileGeo<TB>::win_cells;
     int *lds;
     int l0;
     __device__ __forceinline__ float mass(const Node3 &, const AxisNode &, const AxisNode &, int gx,
                                           int gy, int gz) const {
-        return reinterpret_cast<const float *>(lds)[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
+        return reinterpret_cast<const float *>(lds)[l0 + (gx * W + gy) * W + gz];
     }
     __device__ __forceinline__ void add3(const Node3 &, const AxisNode &, const AxisNode &, int gx,
                                          int gy, int gz, int32_t x, int32_t y, int32_t z) const {
-        const int l = l0 + (gx * kWinEdge + gy) * kWinEdge + gz;
-        atomicAdd(&lds[kWinCells + l], x);
-        atomicAdd(&lds[2 * kWinCells + l], y);
-        atomicAdd(&lds[3 * kWinCells + l], z);
+        const int l = l0 + (gx * W + gy) * W + gz;
+        atomicAdd(&lds[WC + l], x);
+        atomicAdd(&lds[2 * WC + l], y);
+        atomicAdd(&lds[3 * WC + l], z);
     }
 };
 
@@ -202,12 +209,14 @@ struct GridSink2 {  // masses from the completed grid, momentum to the wild accu
     }
 };
 
+template <int TB>
 struct WinSinkG {  // g2p: updated, decoded velocities staged in LDS
+    static constexpr int W = TileGeo<TB>::win_edge;
     const float4 *lvel;
     int l0;
     __device__ __forceinline__ void vel(const Node3 &, const AxisNode &, const AxisNode &, int gx,
                                         int gy, int gz, float cv[3]) const {
-        const float4 o = lvel[l0 + (gx * kWinEdge + gy) * kWinEdge + gz];
+        const float4 o = lvel[l0 + (gx * W + gy) * W + gz];
         cv[0] = o.x;
         cv[1] = o.y;
         cv[2] = o.z;
@@ -245,9 +254,10 @@ struct GridSinkG {  // g2p straight from the decoded velocities
 
 constexpr uint32_t kRankFinal = 0xffffffffu;
 
+template <int TB>
 __device__ __forceinline__ uint32_t bin_key(const BatchArgs &a, const SceneDesc *sd,
                                             const float x[3], const int from[3], uint32_t *dir) {
-    return bin_key_of(a.tile_rank, sd, x, from, dir);
+    return bin_key_of<TB>(a.tile_rank, sd, x, from, dir);
 }
 
 // Counts one particle per active lane into count[key] and returns the lane's
@@ -274,6 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t *p, uint32_t n) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) p[i] = 0;
 }
 
+template <int TB>
 __global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a) {
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
         const Chunk ch = a.chunks[c];
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_count(BatchArgs a) {
             const int nowhere[3] = {-4, -4, -4};
             uint32_t dir;
             load_x(a.state, a.stride, ch.begin + j, x);
-            key = bin_key(a, sd, x, nowhere, &dir);
+            key = bin_key<TB>(a, sd, x, nowhere, &dir);
         }
         const uint32_t r = count_rank(a.count, key, active);
         if (active) {
@@ -600,11 +611,13 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather_sparse(BatchArg
 // rarely meet on one LDS address (same-address atomics of one instruction
 // serialise).  The position of the k-th particle of cell c is
 // base[k] + #{cells c' < c holding more than k}.  Integer adds commute, so
-// the order changes no result.
+// the order changes no result.  With 8^3 tiles (512 cells, too many for the
+// 64-bit level masks) the chunk's particles keep their bin order: a wave's
+// 64 particles then spread over 8 times as many cells.
 
-template <int PPT>
+template <int PPT, int TB>
 struct LevelOrder {
-    uint32_t cnt[kTileCells];      // particles per cell of the tile
+    uint32_t cnt[TileGeo<TB>::cells];  // particles per cell of the tile
     uint32_t base[kBlock * PPT];   // first position of each level
     uint64_t mask[kBlock * PPT];   // cells present in each level
     uint16_t perm[kBlock * PPT];   // position -> slot within the chunk
@@ -613,9 +626,25 @@ struct LevelOrder {
 };
 
 // Fills o.perm for window chunk ch; every thread of the block calls it.
-template <int PPT>
+template <int PPT, int TB>
 __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk &ch,
-                                            LevelOrder<PPT> &o) {
+                                            LevelOrder<PPT, TB> &o) {
+    if constexpr (TB == 3) {
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
+            if (j < ch.w) {
+                const uint32_t i = a.perm[ch.z + j];
+                o.slot[j] = i;
+                float x[3];
+                load_x(a.state, a.stride, i, x);
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) o.x[ax][j] = x[ax];
+                o.perm[j] = (uint16_t)j;
+            }
+        }
+        __syncthreads();
+    } else {
     const int lane = threadIdx.x & (kWave - 1);
     if (threadIdx.x < kTileCells) o.cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -663,6 +692,7 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
         }
     }
     __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -676,11 +706,12 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
 template <int PPT>
 constexpr int kChunkWaves = PPT == 2 ? 6 : 1;  // amdgpu_waves_per_eu minimum (1: no constraint)
 
-template <int PPT>
+template <int PPT, int TB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
 void k_p2g1(BatchArgs a) {
-    __shared__ int lds[4 * kWinCells];  // [mass | vx | vy | vz]
-    __shared__ LevelOrder<PPT> ord;
+    constexpr int WC = TileGeo<TB>::win_cells;
+    __shared__ int lds[4 * WC];  // [mass | vx | vy | vz]
+    __shared__ LevelOrder<PPT, TB> ord;
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
         const TileChunk ch = a.tchunks[w.c];
@@ -690,8 +721,8 @@ void k_p2g1(BatchArgs a) {
         }
         const SceneDesc *sd = a.scenes + ch.x;
         const float mult = sd->prm.mult;
-        for (int l = threadIdx.x; l < 4 * kWinCells; l += kBlock) lds[l] = 0;
-        level_order<PPT>(a, ch, ord);  // its barriers also publish the cleared window
+        for (int l = threadIdx.x; l < 4 * WC; l += kBlock) lds[l] = 0;
+        level_order<PPT, TB>(a, ch, ord);  // its barriers also publish the cleared window
         float x[PPT][3], v[PPT][3], C[PPT][3][3];
         bool valid[PPT];
 #pragma unroll
@@ -713,13 +744,12 @@ void k_p2g1(BatchArgs a) {
             if (!valid[k]) continue;
             Node3 n;
             node3_init(x[k], n);
-            p2g1_particle(n, v[k], C[k], mult, WinSink1{lds, win_l0(x[k])});
+            p2g1_particle(n, v[k], C[k], mult, WinSink1<TB>{lds, win_l0<TB>(x[k])});
         }
         __syncthreads();
-        int4 *out = a.partial + (size_t)w.c * kWinCells;
-        for (int l = threadIdx.x; l < kWinCells; l += kBlock)
-            out[l] = make_int4(lds[kWinCells + l], lds[2 * kWinCells + l], lds[3 * kWinCells + l],
-                               lds[l]);
+        int4 *out = a.partial + (size_t)w.c * WC;
+        for (int l = threadIdx.x; l < WC; l += kBlock)
+            out[l] = make_int4(lds[WC + l], lds[2 * WC + l], lds[3 * WC + l], lds[l]);
         __syncthreads();
     }
     // Wild chunks (rare): global atomics into the wild accumulator.  A loop of
@@ -758,6 +788,7 @@ struct CellPlan {
     uint32_t rounds;  // chunks of the fullest neighbour (wave-uniform)
 };
 
+template <int TB>
 __device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b, const int l[3],
                                                   bool cell_ok) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -776,10 +807,10 @@ __device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b
     for (int s = 0; s < 8; ++s) {
         int q;
         bool valid;
-        const int widx = cover_slot(l, s, &q, &valid);
+        const int widx = cover_slot<TB>(l, s, &q, &valid);
         valid = valid && cell_ok;
         const uint32_t f = __shfl(nfirst, q, kWave), c = __shfl(ncnt, q, kWave);
-        p.base[s] = f * (uint32_t)kWinCells + (uint32_t)widx;
+        p.base[s] = f * (uint32_t)TileGeo<TB>::win_cells + (uint32_t)widx;
         p.cnt[s] = valid ? c : 0u;
         mc = max(mc, p.cnt[s]);
     }
@@ -788,13 +819,14 @@ __device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b
 }
 
 // Sum of the planned partials: all slots' loads of one round in flight at once.
+template <int TB>
 __device__ __forceinline__ int4 sum_partials(const BatchArgs &a, const CellPlan &p) {
     int4 acc = make_int4(0, 0, 0, 0);
     for (uint32_t j = 0; j < p.rounds; ++j) {
         int4 v[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s)
-            v[s] = j < p.cnt[s] ? a.partial[(size_t)p.base[s] + (size_t)j * kWinCells]
+            v[s] = j < p.cnt[s] ? a.partial[(size_t)p.base[s] + (size_t)j * TileGeo<TB>::win_cells]
                                 : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int s = 0; s < 8; ++s) acc = add4(acc, v[s]);
@@ -809,18 +841,25 @@ __device__ __forceinline__ void take_wild(const BatchArgs &a, uint64_t gi, int4 
     a.wild[gi] = make_int4(0, 0, 0, 0);
 }
 
-template <int PHASE>
-__global__ __launch_bounds__(kTileWaves * kWave) void k_reduce(BatchArgs a) {
+// One lane per cell: a 4^3 tile is one wave (a block holds 4 tiles); an 8^3
+// tile is a whole block of 8 waves, whose "anything to write" decision is
+// taken for the tile as a whole (__syncthreads_or), so its waves agree on
+// the tile's dirty flag.
+template <int PHASE, int TB>
+__global__ __launch_bounds__(ReduceGeo<TB>::threads) void k_reduce(BatchArgs a) {
+    constexpr int E = TileGeo<TB>::edge;
+    constexpr uint32_t kCells = (uint32_t)TileGeo<TB>::cells;
     const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t b = reduce_tile(a);
-    if (b >= a.stiles) return;
+    const uint32_t cell = TB == 2 ? lane : threadIdx.x;  // cell of the tile this lane owns
+    const uint32_t b = reduce_tile<TB>(a);
+    if (b >= a.stiles) return;  // (8^3: the whole block)
     const uint4 ti = a.tile_info[b];
     const bool dirty = a.dirty[b] != 0;
     // no particle near, nothing left to clear, no wild particle in the scene:
     // the tile's cells are empty and already written as such
     if (!a.near[b] && !dirty) return;  // (its count is zero too)
     const SceneDesc *sd = a.scenes + ti.w;
-    if (PHASE == 2 && lane == 0) {  // ready for the next counting pass
+    if (PHASE == 2 && cell == 0) {  // ready for the next counting pass
         a.count[sd->koff + (b - sd->soff)] = 0;
         if (b == sd->soff) a.count[sd->koff + sd->ntiles] = 0;  // the scene's wild key
     }
@@ -828,7 +867,7 @@ __global__ __launch_bounds__(kTileWaves * kWave) void k_reduce(BatchArgs a) {
     const Params prm = sd->prm;
     const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
     if (!sd->fast) {  // every particle is wild: cells in linear order
-        const uint32_t li = (b - sd->soff) * kTileCells + lane;
+        const uint32_t li = (b - sd->soff) * kCells + cell;
         if (li >= sd->G) return;
         const uint64_t gi = (uint64_t)sd->goff + li;
         if (PHASE == 1) {
@@ -845,31 +884,31 @@ __global__ __launch_bounds__(kTileWaves * kWave) void k_reduce(BatchArgs a) {
         }
         return;
     }
-    const int l[3] = {(int)lane >> 4, ((int)lane >> 2) & 3, (int)lane & 3};
-    const int cx = (int)ti.x * kTileEdge + l[0], cy = (int)ti.y * kTileEdge + l[1],
-              cz = (int)ti.z * kTileEdge + l[2];
+    const int l[3] = {(int)(cell >> (2 * TB)), (int)(cell >> TB) & (E - 1), (int)cell & (E - 1)};
+    const int cx = (int)ti.x * E + l[0], cy = (int)ti.y * E + l[1], cz = (int)ti.z * E + l[2];
     const bool cell_ok = in_dims(cx, cy, cz, sd);
     const uint64_t gi = (uint64_t)sd->goff + cell_linear(cx, cy, cz, sd->uni.Y, sd->uni.Z);
-    const CellPlan plan = plan_partials(a, b, l, cell_ok);
+    const CellPlan plan = plan_partials<TB>(a, b, l, cell_ok);
     // No particle near the tile: its cells are empty.  If they were already
     // empty after the last substep (not dirty), nothing needs writing.
-    const bool empty = plan.rounds == 0 && !has_wild;
+    bool empty = plan.rounds == 0 && !has_wild;
+    if constexpr (TB == 3) empty = !__syncthreads_or(empty ? 0 : 1);
     if (PHASE == 1) {
         if (empty && !dirty) return;
-        int4 acc = sum_partials(a, plan);
+        int4 acc = sum_partials<TB>(a, plan);
         if (!cell_ok) return;
         if (has_wild) take_wild(a, gi, acc, true);
         a.grid[gi] = acc;
     } else {
         if (empty) {
             if (dirty && cell_ok) a.vel[gi] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (lane == 0 && dirty) a.dirty[b] = 0;
+            if (cell == 0 && dirty) a.dirty[b] = 0;
             return;
         }
         int4 g = make_int4(0, 0, 0, 0);
         if (cell_ok) g = a.grid[gi];
-        const int4 acc = sum_partials(a, plan);
-        if (lane == 0 && !dirty) a.dirty[b] = 1;
+        const int4 acc = sum_partials<TB>(a, plan);
+        if (cell == 0 && !dirty) a.dirty[b] = 1;
         if (!cell_ok) return;
         g = add4(g, make_int4(acc.x, acc.y, acc.z, 0));
         if (has_wild) take_wild(a, gi, g, false);
@@ -883,9 +922,10 @@ __global__ __launch_bounds__(kTileWaves * kWave) void k_reduce(BatchArgs a) {
 // ---------------------------------------------------------------------------
 // p2g_2: density, pressure, viscous stress momentum (mls-mpm/p2g_2.wgsl:30-94)
 
-template <int PPT>
+template <int PPT, int TB>
 __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
-    __shared__ int lds[4 * kWinCells];  // [decoded mass (f32) | vx | vy | vz]
+    constexpr int WC = TileGeo<TB>::win_cells;
+    __shared__ int lds[4 * WC];  // [decoded mass (f32) | vx | vy | vz]
     float *dm = reinterpret_cast<float *>(lds);
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
@@ -910,12 +950,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
         }
         int t[3];
         tile_coords(ch.y, sd, t);
-        for (int l = threadIdx.x; l < kWinCells; l += kBlock) {
-            const int64_t gi = win_cell_index(l, t, sd);
+        for (int l = threadIdx.x; l < WC; l += kBlock) {
+            const int64_t gi = win_cell_index<TB>(l, t, sd);
             dm[l] = gi >= 0 ? decode(a.grid[gi].w, prm.mult) : 0.0f;
-            lds[kWinCells + l] = 0;
-            lds[2 * kWinCells + l] = 0;
-            lds[3 * kWinCells + l] = 0;
+            lds[WC + l] = 0;
+            lds[2 * WC + l] = 0;
+            lds[3 * WC + l] = 0;
         }
         __syncthreads();
 #pragma unroll
@@ -923,12 +963,12 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
             if (!valid[k]) continue;
             Node3 n;
             node3_init(x[k], n);
-            p2g2_particle(n, C[k], prm, WinSink2{lds, win_l0(x[k])});
+            p2g2_particle(n, C[k], prm, WinSink2<TB>{lds, win_l0<TB>(x[k])});
         }
         __syncthreads();
-        int4 *out = a.partial + (size_t)w.c * kWinCells;
-        for (int l = threadIdx.x; l < kWinCells; l += kBlock)
-            out[l] = make_int4(lds[kWinCells + l], lds[2 * kWinCells + l], lds[3 * kWinCells + l], 0);
+        int4 *out = a.partial + (size_t)w.c * WC;
+        for (int l = threadIdx.x; l < WC; l += kBlock)
+            out[l] = make_int4(lds[WC + l], lds[2 * WC + l], lds[3 * WC + l], 0);
         __syncthreads();
     }
     // Wild chunks (rare), in a loop of their own like k_p2g1's.
@@ -1008,10 +1048,11 @@ __device__ __forceinline__ void finish_chunk_counts(const BatchArgs &a, const Sc
     __syncthreads();
 }
 
-template <int PPT>
+template <int PPT, int TB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
 void k_g2p(BatchArgs a) {
-    __shared__ float4 lvel[kWinCells];
+    constexpr int WC = TileGeo<TB>::win_cells;
+    __shared__ float4 lvel[WC];
     __shared__ uint32_t lcnt[kDirWild + 2];  // per-direction counts, wild base
     // whether the (single) scene has wild chunks at all, read up front
     const bool any_wild =
@@ -1040,8 +1081,8 @@ void k_g2p(BatchArgs a) {
         }
         int t[3];
         tile_coords(ch.y, sd, t);
-        for (int l = threadIdx.x; l < kWinCells; l += kBlock) {
-            const int64_t gi = win_cell_index(l, t, sd);
+        for (int l = threadIdx.x; l < WC; l += kBlock) {
+            const int64_t gi = win_cell_index<TB>(l, t, sd);
             lvel[l] = gi >= 0 ? a.vel[gi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         __syncthreads();
@@ -1054,10 +1095,10 @@ void k_g2p(BatchArgs a) {
                 float v[3], B[3][3], xo[3], Cn[3][3];
                 Node3 n;
                 node3_init(x[k], n);
-                g2p_gather(n, WinSinkG{lvel, win_l0(x[k])}, v, B);
+                g2p_gather(n, WinSinkG<TB>{lvel, win_l0<TB>(x[k])}, v, B);
                 g2p_finish(x[k], rb, dt, v, B, xo, Cn);
                 g2p_store(a, i, oid[k], xo, v, Cn);
-                key = bin_key(a, sd, xo, t, &dir);
+                key = bin_key<TB>(a, sd, xo, t, &dir);
             }
             wild[k] = valid[k] && dir == kDirWild;
             // (a wave whose k-th task holds no particle skips the ranking together)
@@ -1099,7 +1140,7 @@ void k_g2p(BatchArgs a) {
                 g2p_gather<true>(n, sink, v, B);
                 g2p_finish(x, rb, dt, v, B, xo, Cn);
                 g2p_store(a, i, a.id[si], xo, v, Cn);
-                key = bin_key(a, sd, xo, nowhere, &dir);
+                key = bin_key<TB>(a, sd, xo, nowhere, &dir);
                 dir = kDirFar;
             }
             if (a.count_next) rank_next(a, lcnt, w.c, i, key, dir, active);
@@ -1135,23 +1176,48 @@ inline unsigned walk_blocks(const BatchArgs &a) {
     return ((nb + kXcds - 1) / kXcds) * kXcds;
 }
 
-template <int PPT>
+template <int PPT, int TB>
 hipError_t launch_ppt(int which, const BatchArgs &a, hipStream_t st) {
     const unsigned nb = walk_blocks(a);
     switch (which) {
-    case 1: launch_timed(k_p2g1<PPT>, nb, kBlock, st, a); break;
-    case 2: launch_timed(k_p2g2<PPT>, nb, kBlock, st, a); break;
-    case 3: launch_timed(k_g2p<PPT>, nb, kBlock, st, a); break;
+    case 1: launch_timed(k_p2g1<PPT, TB>, nb, kBlock, st, a); break;
+    case 2: launch_timed(k_p2g2<PPT, TB>, nb, kBlock, st, a); break;
+    case 3: launch_timed(k_g2p<PPT, TB>, nb, kBlock, st, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_by_ppt(int which, const BatchArgs &a, hipStream_t st) {
+template <int TB>
+hipError_t launch_by_ppt_tb(int which, const BatchArgs &a, hipStream_t st) {
     switch (a.ppt) {
-    case 1: return launch_ppt<1>(which, a, st);
-    case 2: return launch_ppt<2>(which, a, st);
-    case 4: return launch_ppt<4>(which, a, st);
+    case 1: return launch_ppt<1, TB>(which, a, st);
+    case 2: return launch_ppt<2, TB>(which, a, st);
+    case 4: return launch_ppt<4, TB>(which, a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_by_ppt(int which, const BatchArgs &a, hipStream_t st) {
+    switch (a.tile_bits) {
+    case 2: return launch_by_ppt_tb<2>(which, a, st);
+    case 3: return launch_by_ppt_tb<3>(which, a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int PHASE, int TB>
+hipError_t launch_reduce_tb(const BatchArgs &a, hipStream_t st) {
+    if (a.stiles)
+        launch_timed(k_reduce<PHASE, TB>, tile_blocks<TB>(a.stiles), ReduceGeo<TB>::threads, st, a);
+    return hipGetLastError();
+}
+
+template <int PHASE>
+hipError_t launch_reduce(const BatchArgs &a, hipStream_t st) {
+    switch (a.tile_bits) {
+    case 2: return launch_reduce_tb<PHASE, 2>(a, st);
+    case 3: return launch_reduce_tb<PHASE, 3>(a, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1167,7 +1233,12 @@ hipError_t launch_bin_gather(const BatchArgs &a, const BinBuffers &b, bool count
         const unsigned zn = std::max(1u, std::min(blocks_for(b.nkeys, kBlock), 1024u));
         k_zero_u32<<<zn, kBlock, 0, st>>>(b.count, b.nkeys);
         k_zero_u32<<<1, kBlock, 0, st>>>(reinterpret_cast<uint32_t *>(b.sup), 2 * b.nsup);
-        if (a.nchunks) k_bin_count<<<nbin, kBlock, 0, st>>>(a);
+        if (a.nchunks) {
+            if (a.tile_bits == 3)
+                k_bin_count<3><<<nbin, kBlock, 0, st>>>(a);
+            else
+                k_bin_count<2><<<nbin, kBlock, 0, st>>>(a);
+        }
     }
     // many more keys than particle chunks: the sparse variant (measured on the
     // 64-scene mix); otherwise one wave per key
@@ -1214,15 +1285,9 @@ hipError_t launch_p2g1(const BatchArgs &a, hipStream_t st) { return launch_by_pp
 hipError_t launch_p2g2(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(2, a, st); }
 hipError_t launch_g2p(const BatchArgs &a, hipStream_t st) { return launch_by_ppt(3, a, st); }
 
-hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st) {
-    if (a.stiles) launch_timed(k_reduce<1>, tile_blocks(a.stiles), kTileWaves * kWave, st, a);
-    return hipGetLastError();
-}
+hipError_t launch_reduce1(const BatchArgs &a, hipStream_t st) { return launch_reduce<1>(a, st); }
 
-hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st) {
-    if (a.stiles) launch_timed(k_reduce<2>, tile_blocks(a.stiles), kTileWaves * kWave, st, a);
-    return hipGetLastError();
-}
+hipError_t launch_reduce2(const BatchArgs &a, hipStream_t st) { return launch_reduce<2>(a, st); }
 
 void arm_kernel_timer(hipEvent_t start, hipEvent_t stop) {
     t_timer_start = start;

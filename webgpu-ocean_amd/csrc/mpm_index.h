@@ -12,9 +12,17 @@
 
 namespace mpm {
 
-// One-wave-per-tile launches (the reduce kernels) pack kTileWaves waves, i.e.
-// consecutive tiles, per block: fewer, larger blocks dispatch faster.
+// Reduce launches: one lane per cell of a tile.  A 4^3 tile is one wave, and
+// a block packs kTileWaves waves, i.e. consecutive tiles (fewer, larger
+// blocks dispatch faster); an 8^3 tile is a block of 8 waves.
 constexpr uint32_t kTileWaves = 4;
+
+template <int TB>
+struct ReduceGeo {
+    static constexpr uint32_t waves_per_tile = (uint32_t)TileGeo<TB>::cells / 64u;  // 1 | 8
+    static constexpr uint32_t tiles_per_block = TB == 2 ? kTileWaves : 1u;
+    static constexpr uint32_t threads = tiles_per_block * waves_per_tile * 64u;   // 256 | 512
+};
 
 // Runs of consecutive work items dealt round robin over the XCDs (see
 // tile_walk_of).
@@ -24,28 +32,32 @@ constexpr uint32_t kDirs = 27;              // near directions (dx+1)*9 + (dy+1)
 constexpr uint32_t kDirWild = kDirs;        // became wild: block-level global atomic
 constexpr uint32_t kDirFar = kDirs + 1;     // moved further: per-particle global atomic
 
-// Blocks of a one-wave-per-tile launch, padded to a multiple of 8 runs
-// (tiles >= stiles are padding).
+// Blocks of a reduce launch, padded to a multiple of 8 runs (tiles >= stiles
+// are padding).
+template <int TB = 2>
 __host__ __device__ __forceinline__ uint32_t tile_blocks(uint32_t stiles) {
-    const uint32_t nb = (stiles + kTileWaves - 1) / kTileWaves;
+    constexpr uint32_t per = ReduceGeo<TB>::tiles_per_block;
+    const uint32_t nb = (stiles + per - 1) / per;
     return (nb + kXcds * kXcdRun - 1) / (kXcds * kXcdRun) * (kXcds * kXcdRun);
 }
 
-// Spatial tile of wave `wave` of block `block` of a one-wave-per-tile launch.
+// Spatial tile of wave `wave` of block `block` of a reduce launch.
 // The dispatcher puts block b on XCD b mod 8; blocks take Hilbert-consecutive
 // tiles in runs of kXcdRun blocks per XCD turn, so neighbouring partial
 // windows mostly meet in one L2 while every XCD gets a share of the fluid's
 // dense and sparse regions (with xcd_remap: one contiguous range of tiles per
 // XCD).  Both are bijections of the padded block range.
+template <int TB = 2>
 __host__ __device__ __forceinline__ uint32_t reduce_tile_of(uint32_t block, uint32_t wave,
                                                            uint32_t stiles, int xcd_remap) {
     const uint32_t x = block % kXcds, j = block / kXcds;
     uint32_t blk;
     if (xcd_remap)
-        blk = x * (tile_blocks(stiles) / kXcds) + j;
+        blk = x * (tile_blocks<TB>(stiles) / kXcds) + j;
     else
         blk = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
-    return blk * kTileWaves + wave;
+    constexpr uint32_t per = ReduceGeo<TB>::tiles_per_block;
+    return blk * per + (per > 1 ? wave : 0u);
 }
 
 __host__ __device__ __forceinline__ uint32_t min_u32(uint32_t a, uint32_t b) { return a < b ? a : b; }
@@ -103,29 +115,33 @@ __host__ __device__ __forceinline__ bool in_dims(int x, int y, int z, const Scen
            (uint32_t)z < (uint32_t)sd->kdim[2];
 }
 
-// Grid cell of window cell l (0..215) of the tile with coordinates t, or -1
-// outside the scene's cells.
+// Grid cell of window cell l (0..win_cells-1) of the tile with coordinates
+// t, or -1 outside the scene's cells.
+template <int TB = 2>
 __host__ __device__ __forceinline__ int64_t win_cell_index(int l, const int t[3],
                                                           const SceneDesc *sd) {
-    const int x = t[0] * kTileEdge - 1 + l / (kWinEdge * kWinEdge);
-    const int y = t[1] * kTileEdge - 1 + (l / kWinEdge) % kWinEdge;
-    const int z = t[2] * kTileEdge - 1 + l % kWinEdge;
+    constexpr int E = TileGeo<TB>::edge, W = TileGeo<TB>::win_edge;
+    const int x = t[0] * E - 1 + l / (W * W);
+    const int y = t[1] * E - 1 + (l / W) % W;
+    const int z = t[2] * E - 1 + l % W;
     if (!in_dims(x, y, z, sd)) return -1;
     return (int64_t)sd->goff + cell_linear(x, y, z, sd->uni.Y, sd->uni.Z);
 }
 
 // LDS cell of node (0,0,0) of a particle with base cell `x` in its tile's
 // window (the window starts one cell before the tile).
+template <int TB = 2>
 __host__ __device__ __forceinline__ int win_l0(const float x[3]) {
-    const int m = kTileEdge - 1;
+    constexpr int m = TileGeo<TB>::edge - 1, W = TileGeo<TB>::win_edge;
     const int cx = (int)floorf(x[0]), cy = (int)floorf(x[1]), cz = (int)floorf(x[2]);
-    return ((cx & m) * kWinEdge + (cy & m)) * kWinEdge + (cz & m);
+    return ((cx & m) * W + (cy & m)) * W + (cz & m);
 }
 
 // Bin key of a particle at x: its tile's key, or the scene's wild key when
 // the 3x3x3 stencil is not inside [0, kdim) of an integral box (NaN fails
 // every comparison).  *dir receives the direction from tile `from` (kDirFar
 // when further away or when `from` is not a tile, kDirWild when wild).
+template <int TB = 2>
 __host__ __device__ __forceinline__ uint32_t bin_key_of(const uint32_t *tile_rank,
                                                        const SceneDesc *sd, const float x[3],
                                                        const int from[3], uint32_t *dir) {
@@ -135,7 +151,7 @@ __host__ __device__ __forceinline__ uint32_t bin_key_of(const uint32_t *tile_ran
     for (int ax = 0; ax < 3; ++ax) {
         const float f = floorf(x[ax]);
         tame = tame && f >= 1.0f && f <= (float)(sd->kdim[ax] - 2);
-        t[ax] = tame ? (int32_t)f >> kTileBits : 0;
+        t[ax] = tame ? (int32_t)f >> TB : 0;
     }
     if (!tame) {
         *dir = kDirWild;
@@ -155,23 +171,24 @@ __host__ __device__ __forceinline__ uint32_t chunks_of(uint32_t n, uint32_t cap)
 
 // The covering slot s (0..7) of tile-local cell l[3] in a per-tile reduce:
 // per axis bit s>>ax selects the tile itself or its neighbour below (local
-// coordinate 0) or above (3).  *q receives the neighbour (27-entry order) and
-// the return value the cell's index in that neighbour's 6^3 window; *valid is
-// false when the slot needs a neighbour the cell does not touch.
+// coordinate 0) or above (edge - 1).  *q receives the neighbour (27-entry
+// order) and the return value the cell's index in that neighbour's window;
+// *valid is false when the slot needs a neighbour the cell does not touch.
+template <int TB = 2>
 __host__ __device__ __forceinline__ int cover_slot(const int l[3], int s, int *q, bool *valid) {
+    constexpr int E = TileGeo<TB>::edge, W = TileGeo<TB>::win_edge;
     bool ok = true;
     int d[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        const int ext = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
+        const int ext = l[ax] == 0 ? -1 : (l[ax] == E - 1 ? 1 : 0);
         const bool bit = (s >> ax) & 1;
         d[ax] = bit ? ext : 0;
         ok = ok && !(bit && ext == 0);
     }
     *q = (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1);
     *valid = ok;
-    return ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) * kWinEdge +
-           (l[2] + 1 - kTileEdge * d[2]);
+    return ((l[0] + 1 - E * d[0]) * W + (l[1] + 1 - E * d[1])) * W + (l[2] + 1 - E * d[2]);
 }
 
 // Slot where range x (1..kXcds-1) of the XCD split of `total` particles

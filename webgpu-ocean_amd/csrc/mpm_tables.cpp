@@ -86,20 +86,22 @@ void append_tile_tables(const SceneDesc &d, uint32_t s, const uint32_t *ranks,
     }
 }
 
-void scene_key_geometry(SceneDesc &d) {
+void scene_key_geometry(SceneDesc &d, int tile_bits) {
+    const int edge = 1 << tile_bits;
     uint64_t nt = 1;
     bool integral = true;
     for (int a = 0; a < 3; ++a) {
         const float b = d.uni.init_box[a];
         const int32_t dim = (int32_t)std::ceil(b);
         d.kdim[a] = dim < 1 ? 1 : dim;
-        d.knt[a] = (uint32_t)((d.kdim[a] + kTileEdge - 1) >> kTileBits);
+        d.knt[a] = (uint32_t)((d.kdim[a] + edge - 1) >> tile_bits);
         nt *= d.knt[a];
         integral = integral && std::floor(b) == b;
     }
     d.ntiles = (uint32_t)nt;
     d.nkeys = (uint32_t)(nt + 1);  // one key per tile plus the wild key
     d.fast = integral ? 1u : 0u;
+    d.tile_bits = (uint32_t)tile_bits;
 }
 
 }  // namespace mpm

@@ -52,6 +52,7 @@ class Tuning(ctypes.Structure):
         ("particles_per_thread", ctypes.c_int),
         ("grid_blocks", ctypes.c_int),
         ("xcd_remap", ctypes.c_int),
+        ("tile_edge", ctypes.c_int),
     ]
 
 
