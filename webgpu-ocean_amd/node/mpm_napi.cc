@@ -1,7 +1,9 @@
-// mpm_napi.cc -- Node N-API addon over libmpm355's C ABI (include/mpm.h).
+// mpm_napi.cc -- Node N-API addon over libmpm355's C ABIs (include/mpm.h,
+// include/sph.h).
 //
-// Binds exactly the entry points the JS MLSMPMSimulator needs; a failing
-// call throws a JS Error carrying mpm_last_error().  Built with g++ against
+// Binds exactly the entry points the JS MLSMPMSimulator and SPHSimulator
+// need; a failing call throws a JS Error carrying mpm_last_error() /
+// sph_last_error().  Built with g++ against
 // the system Node headers (N-API >= 6 for BigInt device pointers).
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -11,6 +13,7 @@
 #include <string>
 
 #include "mpm.h"
+#include "sph.h"
 
 namespace {
 
@@ -368,6 +371,238 @@ napi_value AbiVersion(napi_env env, napi_callback_info) {
     return r;
 }
 
+// ---- SPH (include/sph.h) ----------------------------------------------------
+
+napi_value throw_sph(napi_env env, int rc) {
+    std::string msg = std::string("sph error ") + std::to_string(rc) + ": " + sph_last_error();
+    napi_throw_error(env, nullptr, msg.c_str());
+    return nullptr;
+}
+
+#define SPH_CALL(env, call)                                   \
+    do {                                                      \
+        int rc_ = (call);                                     \
+        if (rc_ != SPH_OK) return throw_sph((env), rc_);      \
+    } while (0)
+
+struct SphHolder {
+    sph_sim *sim;
+};
+
+bool get_sph(napi_env env, napi_value v, SphHolder **holder, sph_sim **out) {
+    void *p = nullptr;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, nullptr, "expected an SPH simulator handle");
+        return false;
+    }
+    *holder = static_cast<SphHolder *>(p);
+    if (out) {
+        if (!(*holder)->sim) {
+            napi_throw_error(env, nullptr, "simulator was destroyed");
+            return false;
+        }
+        *out = (*holder)->sim;
+    }
+    return true;
+}
+
+// SPH parameters: JS numbers (doubles) as the reference's constants are.
+bool get_sph_params(napi_env env, napi_value v, sph_params *p) {
+    sph_default_params(p);
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t != napi_object) return true;
+    const char *names[7] = {"kernelRadius", "mass", "dt", "stiffness", "nearStiffness",
+                            "restDensity", "viscosity"};
+    double *fields[7] = {&p->kernel_radius, &p->mass, &p->dt, &p->stiffness,
+                         &p->near_stiffness, &p->rest_density, &p->viscosity};
+    for (int i = 0; i < 7; ++i) {
+        bool has = false;
+        napi_has_named_property(env, v, names[i], &has);
+        if (!has) continue;
+        napi_value e;
+        napi_get_named_property(env, v, names[i], &e);
+        if (napi_get_value_double(env, e, fields[i]) != napi_ok) {
+            napi_throw_type_error(env, nullptr, "params fields must be numbers");
+            return false;
+        }
+    }
+    return true;
+}
+
+void finalize_sph(napi_env, void *data, void *) {
+    SphHolder *h = static_cast<SphHolder *>(data);
+    sph_destroy(h->sim);
+    delete h;
+}
+
+// sphCreate(params|null, particles|null, n, initHalfBox[3]) -> handle
+napi_value SphCreate(napi_env env, napi_callback_info info) {
+    napi_value argv[4];
+    size_t argc;
+    if (!get_args(env, info, argv, &argc) || argc < 4) {
+        napi_throw_type_error(env, nullptr, "sphCreate(params, particles, n, initHalfBox)");
+        return nullptr;
+    }
+    sph_params p;
+    void *data;
+    size_t len;
+    uint32_t n;
+    float box[3];
+    if (!get_sph_params(env, argv[0], &p) || !get_bytes(env, argv[1], &data, &len) ||
+        !get_u32(env, argv[2], &n) || !get_f3(env, argv[3], box))
+        return nullptr;
+    if ((size_t)n * SPH_PARTICLE_STRUCT_SIZE > len) {
+        napi_throw_range_error(env, nullptr, "particle buffer shorter than n x 64 bytes");
+        return nullptr;
+    }
+    sph_sim *sim = nullptr;
+    SPH_CALL(env, sph_create(&p, data, n, box, &sim));
+    SphHolder *holder = new SphHolder{sim};
+    napi_value h;
+    if (napi_create_external(env, holder, finalize_sph, nullptr, &h) != napi_ok) {
+        sph_destroy(sim);
+        delete holder;
+        napi_throw_error(env, nullptr, "napi_create_external failed");
+        return nullptr;
+    }
+    return h;
+}
+
+// sphReset(h, particles, n, initHalfBox)
+napi_value SphReset(napi_env env, napi_callback_info info) {
+    napi_value argv[4];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    void *data;
+    size_t len;
+    uint32_t n;
+    float box[3];
+    if (!get_args(env, info, argv, &argc) || argc < 4 || !get_sph(env, argv[0], &hd, &sim) ||
+        !get_bytes(env, argv[1], &data, &len) || !get_u32(env, argv[2], &n) ||
+        !get_f3(env, argv[3], box))
+        return nullptr;
+    if ((size_t)n * SPH_PARTICLE_STRUCT_SIZE > len) {
+        napi_throw_range_error(env, nullptr, "particle buffer shorter than n x 64 bytes");
+        return nullptr;
+    }
+    SPH_CALL(env, sph_reset(sim, data, n, box));
+    return Undefined(env);
+}
+
+// sphSetBox(h, realHalfBox) / sphStep(h, substeps) / sphSynchronize(h)
+napi_value SphSetBox(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    float box[3];
+    if (!get_args(env, info, argv, &argc) || argc < 2 || !get_sph(env, argv[0], &hd, &sim) ||
+        !get_f3(env, argv[1], box))
+        return nullptr;
+    SPH_CALL(env, sph_set_box(sim, box));
+    return Undefined(env);
+}
+
+napi_value SphStep(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    uint32_t k;
+    if (!get_args(env, info, argv, &argc) || argc < 2 || !get_sph(env, argv[0], &hd, &sim) ||
+        !get_u32(env, argv[1], &k))
+        return nullptr;
+    SPH_CALL(env, sph_step(sim, k));
+    return Undefined(env);
+}
+
+napi_value SphSynchronize(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    if (!get_args(env, info, argv, &argc) || argc < 1 || !get_sph(env, argv[0], &hd, &sim))
+        return nullptr;
+    SPH_CALL(env, sph_synchronize(sim));
+    return Undefined(env);
+}
+
+// sphReadPosvel(h, dst) / sphReadParticles(h, dst) / sphWriteParticles(h, src)
+napi_value SphTransfer(napi_env env, napi_callback_info info, int which) {
+    napi_value argv[2];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    void *data;
+    size_t len;
+    if (!get_args(env, info, argv, &argc) || argc < 2 || !get_sph(env, argv[0], &hd, &sim) ||
+        !get_bytes(env, argv[1], &data, &len))
+        return nullptr;
+    uint32_t n = 0;
+    SPH_CALL(env, sph_get_counts(sim, &n, nullptr));
+    const size_t rec = which == 0 ? SPH_POSVEL_STRUCT_SIZE : SPH_PARTICLE_STRUCT_SIZE;
+    if ((size_t)n * rec > len) {
+        napi_throw_range_error(env, nullptr, "buffer shorter than numParticles records");
+        return nullptr;
+    }
+    if (which == 0) SPH_CALL(env, sph_read_posvel(sim, data, n));
+    if (which == 1) SPH_CALL(env, sph_read_particles(sim, data, n));
+    if (which == 2) SPH_CALL(env, sph_write_particles(sim, data, n));
+    return Undefined(env);
+}
+napi_value SphReadPosvel(napi_env env, napi_callback_info info) { return SphTransfer(env, info, 0); }
+napi_value SphReadParticles(napi_env env, napi_callback_info info) { return SphTransfer(env, info, 1); }
+napi_value SphWriteParticles(napi_env env, napi_callback_info info) { return SphTransfer(env, info, 2); }
+
+// sphCounts(h) -> { numParticles, gridCount }
+napi_value SphCounts(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    if (!get_args(env, info, argv, &argc) || argc < 1 || !get_sph(env, argv[0], &hd, &sim))
+        return nullptr;
+    uint32_t n = 0;
+    uint64_t g = 0;
+    SPH_CALL(env, sph_get_counts(sim, &n, &g));
+    napi_value o, vn, vg;
+    NAPI_CALL(env, napi_create_object(env, &o));
+    NAPI_CALL(env, napi_create_uint32(env, n, &vn));
+    NAPI_CALL(env, napi_create_double(env, (double)g, &vg));
+    NAPI_CALL(env, napi_set_named_property(env, o, "numParticles", vn));
+    NAPI_CALL(env, napi_set_named_property(env, o, "gridCount", vg));
+    return o;
+}
+
+napi_value SphPosvelDevicePtr(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc;
+    SphHolder *hd;
+    sph_sim *sim;
+    if (!get_args(env, info, argv, &argc) || argc < 1 || !get_sph(env, argv[0], &hd, &sim))
+        return nullptr;
+    void *p = nullptr;
+    SPH_CALL(env, sph_posvel_device_ptr(sim, &p));
+    napi_value b;
+    NAPI_CALL(env, napi_create_bigint_uint64(env, (uint64_t)(uintptr_t)p, &b));
+    return b;
+}
+
+// sphDestroy(h): releases the simulator now; later calls on h throw.
+napi_value SphDestroy(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc;
+    SphHolder *hd;
+    if (!get_args(env, info, argv, &argc) || argc < 1 || !get_sph(env, argv[0], &hd, nullptr))
+        return nullptr;
+    sph_sim *sim = hd->sim;
+    hd->sim = nullptr;
+    SPH_CALL(env, sph_destroy(sim));
+    return Undefined(env);
+}
+
 napi_value Init(napi_env env, napi_value exports) {
     struct {
         const char *name;
@@ -380,6 +615,12 @@ napi_value Init(napi_env env, napi_value exports) {
         {"writeParticles", WriteParticles}, {"counts", Counts},
         {"posvelDevicePtr", PosvelDevicePtr}, {"timeSteps", TimeSteps},
         {"destroy", Destroy},         {"abiVersion", AbiVersion},
+        {"sphCreate", SphCreate},     {"sphReset", SphReset},
+        {"sphSetBox", SphSetBox},     {"sphStep", SphStep},
+        {"sphSynchronize", SphSynchronize}, {"sphReadPosvel", SphReadPosvel},
+        {"sphReadParticles", SphReadParticles}, {"sphWriteParticles", SphWriteParticles},
+        {"sphCounts", SphCounts},     {"sphPosvelDevicePtr", SphPosvelDevicePtr},
+        {"sphDestroy", SphDestroy},
     };
     for (auto &f : fns) {
         napi_value fn;
