@@ -310,7 +310,9 @@ __device__ __forceinline__ float quad_accumulate(float acc, float t) {
 }
 
 // computeDensity -- sph/density.wgsl:41-51 (kernels), :65-106, over the sorted
-// slots.
+// slots.  (Listing the interacting candidates first, as the force walk does,
+// was slower here: 88 -> 109 us at 200k particles; the density terms are too
+// cheap to pay for the second pass.)
 __global__ void __launch_bounds__(kNbrBlock) k_density(Bufs b) {
     const uint32_t g = blockIdx.x * kNbrBlock + threadIdx.x;
     const uint32_t k = g / kQuad, q = g % kQuad;
@@ -351,11 +353,29 @@ __global__ void __launch_bounds__(kNbrBlock) k_density(Bufs b) {
 // over the sorted slots.  |pos_j - pos_i| is the r of the pair: the two
 // differences are negations of each other, exactly, so dot(dir, dir) == r2
 // bit for bit and normalize's length is sqrt(r2).
+//
+// Two passes per quad: the cheap test (densities non-zero, 0 < r2 < h^2)
+// over the candidate stream appends the interacting slots, in stream order,
+// to the quad's list in LDS (quad ballot + prefix); the pair terms (two
+// square roots, twelve IEEE divisions) are then evaluated over the list, four
+// lanes at a time, and added in list order through the DPP quad broadcasts.
+// Only about one candidate in six interacts, so evaluating the terms over
+// the whole stream left most lanes of a wave idle in the expensive part.
+// The dropped candidates would have added +0.0: the sums are unchanged, bit
+// for bit.  When any list of a wave could overflow (kListCap entries), all of
+// the wave's quads evaluate their lists before the test continues: a drain
+// taken by one quad at a time would run the pair terms with most of the wave
+// masked off.
+constexpr uint32_t kListCap = 128;
+
 __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
+    __shared__ uint32_t s_list[kNbrBlock / kQuad][kListCap];
     const uint32_t g = blockIdx.x * kNbrBlock + threadIdx.x;
     const uint32_t k = g / kQuad, q = g % kQuad;
     const Uniforms &u = *b.u;
     if (k >= b.n || k >= b.start[u.G]) return;  // the whole quad leaves together
+    uint32_t *list = s_list[threadIdx.x / kQuad];
+    const uint32_t quad_shift = (threadIdx.x & 63u) & ~3u;
     const float4 p = b.spos[k];
     float4 v = b.svel[k];
     const float2 dni = b.sdn[k];
@@ -366,6 +386,66 @@ __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
     const float pressure_i = u.stiff * (density_i - u.rest);
     const float near_pressure_i = u.near_stiff * near_density_i;
     float fpx = 0.0f, fpy = 0.0f, fpz = 0.0f, fvx = 0.0f, fvy = 0.0f, fvz = 0.0f;
+    uint32_t cnt = 0;  // entries in the quad's list (quad-uniform)
+    // Evaluates the listed pairs in order and empties the list.
+    auto drain = [&]() {
+        // the list entries were written by the other lanes of this quad (same
+        // wave, earlier in program order); keep the compiler from moving the
+        // reads above those writes
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t base = 0; base < cnt; base += kQuad) {
+            const uint32_t x = base + q;
+            const bool valid = x < cnt;
+            const uint32_t j = valid ? list[x] : k;
+            const float4 c = b.spos[j];
+            const float2 dnj = b.sdn[j];
+            const float4 w = b.svel[j];
+            float ax = 0.0f, ay = 0.0f, az = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+            float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+            if (valid) {
+                const float ex = p.x - c.x, ey = p.y - c.y, ez = p.z - c.z;
+                const float r = sqrtf(dot3(ex, ey, ez, ex, ey, ez));
+                const float pressure_j = u.stiff * (dnj.x - u.rest);
+                const float near_pressure_j = u.near_stiff * dnj.y;
+                const float shared = (pressure_i + pressure_j) / 2.0f;
+                const float near_shared = (near_pressure_i + near_pressure_j) / 2.0f;
+                const float dirx = (c.x - p.x) / r, diry = (c.y - p.y) / r,
+                            dirz = (c.z - p.z) / r;
+                const float dg = u.h - r;
+                const float grad = (scale_g * dg) * dg;
+                const float near_grad = (scale_ng * dg) * dg;
+                const float lap = scale_l * dg;
+                const float sp = -u.mass * shared, nsp = -u.mass * near_shared;
+                ax = ((sp * dirx) * grad) / dnj.x;
+                ay = ((sp * diry) * grad) / dnj.x;
+                az = ((sp * dirz) * grad) / dnj.x;
+                nx = ((nsp * dirx) * near_grad) / dnj.y;
+                ny = ((nsp * diry) * near_grad) / dnj.y;
+                nz = ((nsp * dirz) * near_grad) / dnj.y;
+                vx = ((u.mass * (w.x - v.x)) * lap) / dnj.x;
+                vy = ((u.mass * (w.y - v.y)) * lap) / dnj.x;
+                vz = ((u.mass * (w.z - v.z)) * lap) / dnj.x;
+            }
+            // per pair, in order: fPress += pressure term, then += near-pressure
+            // term (force.wgsl:113-114); fVisc += (:116)
+            fpx = fpx + quad_lane<0>(ax); fpx = fpx + quad_lane<0>(nx);
+            fpx = fpx + quad_lane<1>(ax); fpx = fpx + quad_lane<1>(nx);
+            fpx = fpx + quad_lane<2>(ax); fpx = fpx + quad_lane<2>(nx);
+            fpx = fpx + quad_lane<3>(ax); fpx = fpx + quad_lane<3>(nx);
+            fpy = fpy + quad_lane<0>(ay); fpy = fpy + quad_lane<0>(ny);
+            fpy = fpy + quad_lane<1>(ay); fpy = fpy + quad_lane<1>(ny);
+            fpy = fpy + quad_lane<2>(ay); fpy = fpy + quad_lane<2>(ny);
+            fpy = fpy + quad_lane<3>(ay); fpy = fpy + quad_lane<3>(ny);
+            fpz = fpz + quad_lane<0>(az); fpz = fpz + quad_lane<0>(nz);
+            fpz = fpz + quad_lane<1>(az); fpz = fpz + quad_lane<1>(nz);
+            fpz = fpz + quad_lane<2>(az); fpz = fpz + quad_lane<2>(nz);
+            fpz = fpz + quad_lane<3>(az); fpz = fpz + quad_lane<3>(nz);
+            fvx = quad_accumulate(fvx, vx);
+            fvy = quad_accumulate(fvy, vy);
+            fvz = quad_accumulate(fvz, vz);
+        }
+        cnt = 0;
+    };
     const Stream st = stream_of(row_ranges(b, u, rows_of(u, p)));
     const uint32_t total = st.pre[9];
     for (uint32_t base = 0; base < total; base += kQuad) {
@@ -374,51 +454,15 @@ __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
         const uint32_t j = valid ? slot_at(st, x) : k;
         const float4 c = b.spos[j];
         const float2 dnj = b.sdn[j];
-        const float4 w = b.svel[j];
         const float ex = p.x - c.x, ey = p.y - c.y, ez = p.z - c.z;
         const float r2 = dot3(ex, ey, ez, ex, ey, ez);
-        float ax = 0.0f, ay = 0.0f, az = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-        float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-        if (valid && dnj.x != 0.0f && dnj.y != 0.0f && r2 < u.h2 && 0.0f < r2) {
-            const float r = sqrtf(r2);
-            const float pressure_j = u.stiff * (dnj.x - u.rest);
-            const float near_pressure_j = u.near_stiff * dnj.y;
-            const float shared = (pressure_i + pressure_j) / 2.0f;
-            const float near_shared = (near_pressure_i + near_pressure_j) / 2.0f;
-            const float dirx = (c.x - p.x) / r, diry = (c.y - p.y) / r, dirz = (c.z - p.z) / r;
-            const float dg = u.h - r;
-            const float grad = (scale_g * dg) * dg;
-            const float near_grad = (scale_ng * dg) * dg;
-            const float lap = scale_l * dg;
-            const float sp = -u.mass * shared, nsp = -u.mass * near_shared;
-            ax = ((sp * dirx) * grad) / dnj.x;
-            ay = ((sp * diry) * grad) / dnj.x;
-            az = ((sp * dirz) * grad) / dnj.x;
-            nx = ((nsp * dirx) * near_grad) / dnj.y;
-            ny = ((nsp * diry) * near_grad) / dnj.y;
-            nz = ((nsp * dirz) * near_grad) / dnj.y;
-            vx = ((u.mass * (w.x - v.x)) * lap) / dnj.x;
-            vy = ((u.mass * (w.y - v.y)) * lap) / dnj.x;
-            vz = ((u.mass * (w.z - v.z)) * lap) / dnj.x;
-        }
-        // per candidate, in stream order: fPress += pressure term, then
-        // += near-pressure term (force.wgsl:113-114); fVisc += (:116)
-        fpx = fpx + quad_lane<0>(ax); fpx = fpx + quad_lane<0>(nx);
-        fpx = fpx + quad_lane<1>(ax); fpx = fpx + quad_lane<1>(nx);
-        fpx = fpx + quad_lane<2>(ax); fpx = fpx + quad_lane<2>(nx);
-        fpx = fpx + quad_lane<3>(ax); fpx = fpx + quad_lane<3>(nx);
-        fpy = fpy + quad_lane<0>(ay); fpy = fpy + quad_lane<0>(ny);
-        fpy = fpy + quad_lane<1>(ay); fpy = fpy + quad_lane<1>(ny);
-        fpy = fpy + quad_lane<2>(ay); fpy = fpy + quad_lane<2>(ny);
-        fpy = fpy + quad_lane<3>(ay); fpy = fpy + quad_lane<3>(ny);
-        fpz = fpz + quad_lane<0>(az); fpz = fpz + quad_lane<0>(nz);
-        fpz = fpz + quad_lane<1>(az); fpz = fpz + quad_lane<1>(nz);
-        fpz = fpz + quad_lane<2>(az); fpz = fpz + quad_lane<2>(nz);
-        fpz = fpz + quad_lane<3>(az); fpz = fpz + quad_lane<3>(nz);
-        fvx = quad_accumulate(fvx, vx);
-        fvy = quad_accumulate(fvy, vy);
-        fvz = quad_accumulate(fvz, vz);
+        const bool hit = valid && dnj.x != 0.0f && dnj.y != 0.0f && r2 < u.h2 && 0.0f < r2;
+        const uint32_t qm = (uint32_t)(__ballot(hit) >> quad_shift) & 0xFu;
+        if (__any(cnt + kQuad > kListCap)) drain();  // wave-uniform
+        if (hit) list[cnt + (uint32_t)__popc(qm & ((1u << q) - 1u))] = j;
+        cnt += (uint32_t)__popc(qm);
     }
+    drain();
     if (q != 0) return;
     const uint32_t i = b.order2[k];
     fvx = fvx * u.visc;
