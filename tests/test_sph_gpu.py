@@ -165,3 +165,16 @@ def test_timing_hooks_cover_every_stage(sphmod):
         st = sim.time_stages(4)
         assert set(st) == set(sphmod.STAGES) and all(v > 0 for v in st.values())
         assert sim.time_steps(10) > 0
+
+
+def test_particle_cap_200k_two_frames_bit_exact(sphmod):
+    # numParticlesMax (common.ts:11) in the largest setting's box: long
+    # neighbour lists (the force walk's list drains mid-stream)
+    half = sphmod.scenes.INIT_HALF_BOXES[3]
+    p = sphmod.scenes.dam_break(3, seed=21, num_particles=200000)
+    with sphmod.SPHSimulator(particles=p, init_half_box_size=half) as sim:
+        sc = osph.Scene(p, half)
+        for frame in range(2):
+            sim.execute()
+            sc.execute(2, threads=16)
+        _compare(sim, sc, "200k")
