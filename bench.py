@@ -643,8 +643,9 @@ def main() -> int:
         return 0
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not "
-              "supported (scenes would shard by x-slab with halo exchange; see DESIGN.md)",
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a scene would shard by x-slabs of the grid, exchanging the boundary tiles' "
+              "partial windows and halo velocities with the neighbouring GPU)",
               file=sys.stderr)
         return 2
 

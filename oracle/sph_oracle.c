@@ -19,7 +19,7 @@
  *     cell, particles are sorted by decreasing index.
  * Particles step exactly n (the reference writes n into the u32 field as the
  * bits of a float, sph.ts:293-296, and so also steps stale "ghost" records
- * up to its buffer's size; see DESIGN.md).  Linear cell ids outside
+ * up to its buffer's size).  Linear cell ids outside
  * [0, gridCount) are not binned (the reference only tests the upper bound,
  * gridBuild.wgsl:57, leaving negative ids to WebGPU's robust access).
  */
