@@ -695,16 +695,12 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
                 e = hipMemcpyAsync(state_, snap, sb, hipMemcpyDeviceToDevice, st);
                 if (e == hipSuccess) e = hipMemcpyAsync(id_, snap + sb, ib, hipMemcpyDeviceToDevice, st);
                 keys_ready_ = false;
-                // eager launches: graph capture per candidate would dominate
-                for (unsigned k = 0; k < 2 && e == hipSuccess; ++k) {
-                    e = launch_substep(st, nullptr);
-                    after_substep(st);
-                }
+                // timed as production runs: graph-replayed substeps (the
+                // warm-up substeps capture this candidate's graphs)
+                for (unsigned k = 0; k < 3 && e == hipSuccess; ++k) e = substep(st);
                 if (e == hipSuccess) e = hipEventRecord(e0, st);
-                for (unsigned k = 0; k < substeps_per_trial && e == hipSuccess; ++k) {
-                    e = launch_substep(st, nullptr);
-                    after_substep(st);
-                }
+                for (unsigned k = 0; k < substeps_per_trial && e == hipSuccess; ++k)
+                    e = substep(st);
                 if (e == hipSuccess) e = hipEventRecord(e1, st);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float ms = 0.0f;
