@@ -412,3 +412,20 @@ def test_switching_the_tile_edge_mid_rollout_keeps_results():
             sim.execute()
             ref.execute(2)
             assert_same(sim, ref, ("particles", "posvel"), ctx=f"frame {frame} edge {edge}")
+
+
+def test_autotune_restores_the_state_and_keeps_results():
+    """mpm_autotune times graph-replayed candidates (both tile edges) on a
+    snapshot and restores it: the simulation then continues exactly as the
+    oracle's from the same state."""
+    rec, box = scenes.config0()
+    sim, ref = pair(rec, box)
+    with sim:
+        sim.execute()
+        ref.execute(2)
+        chosen = sim.autotune(3)
+        assert chosen["tile_edge"] in (4, 8) and chosen["particles_per_thread"] in (1, 2, 4)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="after autotune")
+        sim.execute()
+        ref.execute(2)
+        assert_same(sim, ref, ("particles", "posvel"), ctx="frame after autotune")
