@@ -309,22 +309,54 @@ __device__ __forceinline__ float quad_accumulate(float acc, float t) {
     return acc;
 }
 
+// Value of lane l ^ 4 within the same 8-lane group (DPP row_shl:4 for the
+// lower quad, row_shr:4 for the upper; groups of 8 lie inside one 16-lane
+// DPP row).
+__device__ __forceinline__ float other_quad(float v, bool upper) {
+    const int i = __float_as_int(v);
+    const int from_above = __builtin_amdgcn_update_dpp(0, i, 0x104, 0xF, 0xF, false);  // row_shl:4
+    const int from_below = __builtin_amdgcn_update_dpp(0, i, 0x114, 0xF, 0xF, false);  // row_shr:4
+    return __int_as_float(upper ? from_below : from_above);
+}
+
+// acc += the 8 terms of the lane's 8-lane group in order (lower quad's lanes
+// 0..3, then the upper quad's), bit for bit the sequential sum.
+__device__ __forceinline__ float oct_accumulate(float acc, float t, bool upper) {
+    const float o = other_quad(t, upper);
+    const float lo = upper ? o : t, hi = upper ? t : o;
+    acc = quad_accumulate(acc, lo);
+    return quad_accumulate(acc, hi);
+}
+
+constexpr unsigned kOct = 8;
+
+// Below this many particles a quad per particle leaves fewer than 4 waves per
+// SIMD of the chip (256 CUs x 4 SIMDs), and the density walk takes 8 lanes
+// per particle instead (40k: 24.1 -> 20.6 us; 200k, where quads already fill
+// the chip: 88 -> 99.5 us with 8 lanes).
+constexpr uint32_t kOctBelow = 65536;
+
 // computeDensity -- sph/density.wgsl:41-51 (kernels), :65-106, over the sorted
-// slots.  (Listing the interacting candidates first, as the force walk does,
-// was slower here: 88 -> 109 us at 200k particles; the density terms are too
-// cheap to pay for the second pass.)
+// slots.  L = 4 or 8 lanes per particle: lane l evaluates stream entries
+// L m + l, and every lane adds the group's L terms in stream order (8 lanes:
+// the lower quad's four, then the upper quad's).  (Listing the interacting
+// candidates first, as the force walk does, was slower here: 88 -> 109 us at
+// 200k particles; the density terms are too cheap to pay for a second pass.)
+template <unsigned L>
 __global__ void __launch_bounds__(kNbrBlock) k_density(Bufs b) {
+    static_assert(L == kQuad || L == kOct, "4 or 8 lanes per particle");
     const uint32_t g = blockIdx.x * kNbrBlock + threadIdx.x;
-    const uint32_t k = g / kQuad, q = g % kQuad;
+    const uint32_t k = g / L, q = g % L;
+    const bool upper = q >= kQuad;
     const Uniforms &u = *b.u;
-    if (k >= b.n || k >= b.start[u.G]) return;  // the whole quad leaves together
+    if (k >= b.n || k >= b.start[u.G]) return;  // the whole group leaves together
     const float4 p = b.spos[k];
     const float scale_d = 315.0f / ((float)(64. * 3.1415926535) * u.h9);
     const float scale_n = 15.0f / ((float)3.1415926535 * u.h6);
     float density = 0.0f, near_density = 0.0f;
     const Stream st = stream_of(row_ranges(b, u, rows_of(u, p)));
     const uint32_t total = st.pre[9];
-    for (uint32_t base = 0; base < total; base += kQuad) {
+    for (uint32_t base = 0; base < total; base += L) {
         const uint32_t x = base + q;
         const bool valid = x < total;
         const float4 c = b.spos[valid ? slot_at(st, x) : k];
@@ -338,8 +370,13 @@ __global__ void __launch_bounds__(kNbrBlock) k_density(Bufs b) {
             const float d = u.h - r;
             tn = u.mass * (((scale_n * d) * d) * d);
         }
-        density = quad_accumulate(density, td);
-        near_density = quad_accumulate(near_density, tn);
+        if constexpr (L == kOct) {
+            density = oct_accumulate(density, td, upper);
+            near_density = oct_accumulate(near_density, tn, upper);
+        } else {
+            density = quad_accumulate(density, td);
+            near_density = quad_accumulate(near_density, tn);
+        }
     }
     if (q != 0) return;
     const uint32_t i = b.order2[k];
@@ -592,13 +629,16 @@ hipError_t launch_stage(int stage, const Bufs &b, unsigned ntiles, hipStream_t s
                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const unsigned pb = blocks_for(b.n);
     const unsigned nb = (unsigned)(((uint64_t)b.n * kQuad + kNbrBlock - 1) / kNbrBlock);
+    const unsigned ob = (unsigned)(((uint64_t)b.n * kOct + kNbrBlock - 1) / kNbrBlock);
     switch (stage) {
     case SPH_STAGE_COUNT: return launch_kernel(k_count, pb, b, st, e0, e1);
     case SPH_STAGE_SCAN_REDUCE: return launch_kernel(k_scan_reduce, ntiles, b, st, e0, e1);
     case SPH_STAGE_SCAN: return launch_kernel(k_scan, ntiles, b, st, e0, e1);
     case SPH_STAGE_SCATTER: return launch_kernel(k_scatter, pb, b, st, e0, e1);
     case SPH_STAGE_SORT_GATHER: return launch_kernel(k_sort_gather, pb, b, st, e0, e1);
-    case SPH_STAGE_DENSITY: return launch_kernel(k_density, nb, b, st, e0, e1, kNbrBlock);
+    case SPH_STAGE_DENSITY:
+        return b.n < kOctBelow ? launch_kernel(k_density<kOct>, ob, b, st, e0, e1, kNbrBlock)
+                               : launch_kernel(k_density<kQuad>, nb, b, st, e0, e1, kNbrBlock);
     case SPH_STAGE_FORCE: return launch_kernel(k_force, nb, b, st, e0, e1, kNbrBlock);
     default: return hipErrorInvalidValue;
     }
