@@ -391,28 +391,76 @@ __global__ void __launch_bounds__(kNbrBlock) k_density(Bufs b) {
 // differences are negations of each other, exactly, so dot(dir, dir) == r2
 // bit for bit and normalize's length is sqrt(r2).
 //
-// Two passes per quad: the cheap test (densities non-zero, 0 < r2 < h^2)
-// over the candidate stream appends the interacting slots, in stream order,
-// to the quad's list in LDS (quad ballot + prefix); the pair terms (two
-// square roots, twelve IEEE divisions) are then evaluated over the list, four
-// lanes at a time, and added in list order through the DPP quad broadcasts.
+// Two passes per particle's lane group: the cheap test (densities non-zero,
+// 0 < r2 < h^2) over the candidate stream appends the interacting slots, in
+// stream order, to the group's list in LDS (group ballot + prefix); the pair
+// terms (two square roots, twelve IEEE divisions) are then evaluated over the
+// list, one pair per lane, and added in list order through DPP broadcasts.
 // Only about one candidate in six interacts, so evaluating the terms over
 // the whole stream left most lanes of a wave idle in the expensive part.
 // The dropped candidates would have added +0.0: the sums are unchanged, bit
 // for bit.  When any list of a wave could overflow (kListCap entries), all of
-// the wave's quads evaluate their lists before the test continues: a drain
-// taken by one quad at a time would run the pair terms with most of the wave
-// masked off.
+// the wave's groups evaluate their lists before the test continues: a drain
+// taken by one group at a time would run the pair terms with most of the
+// wave masked off.
 constexpr uint32_t kListCap = 128;
 
+// L = 4 or 8 lanes per particle: the group's list is filled L candidates per
+// step and evaluated L pairs per step; with 8 lanes the lower quad's pairs
+// come before the upper quad's.  The walk runs with 8 lanes at every size
+// (dam break, 40k particles: 37.2 -> 28.3 us; 200k: 183 -> 160 us against
+// quads).  16 lanes (one DPP row, row_newbcast) measured slower: 30.9 and
+// 177 us; every lane adds all the group's terms, and that share grows with
+// the group.
+template <unsigned L>
+__device__ __forceinline__ void lanes_of_pair(float t, bool upper, float &lo, float &hi) {
+    if constexpr (L == kOct) {
+        const float o = other_quad(t, upper);
+        lo = upper ? o : t;
+        hi = upper ? t : o;
+    } else {
+        lo = t;
+        hi = 0.0f;
+    }
+}
+
+// acc += a[0], n[0], a[1], n[1], ... over the group's pairs in order (the
+// pressure term, then the near-pressure term, of each pair).
+template <unsigned L>
+__device__ __forceinline__ float add_pairs(float acc, float a, float n, bool upper) {
+    float alo, ahi, nlo, nhi;
+    lanes_of_pair<L>(a, upper, alo, ahi);
+    lanes_of_pair<L>(n, upper, nlo, nhi);
+    acc = acc + quad_lane<0>(alo); acc = acc + quad_lane<0>(nlo);
+    acc = acc + quad_lane<1>(alo); acc = acc + quad_lane<1>(nlo);
+    acc = acc + quad_lane<2>(alo); acc = acc + quad_lane<2>(nlo);
+    acc = acc + quad_lane<3>(alo); acc = acc + quad_lane<3>(nlo);
+    if constexpr (L == kOct) {
+        acc = acc + quad_lane<0>(ahi); acc = acc + quad_lane<0>(nhi);
+        acc = acc + quad_lane<1>(ahi); acc = acc + quad_lane<1>(nhi);
+        acc = acc + quad_lane<2>(ahi); acc = acc + quad_lane<2>(nhi);
+        acc = acc + quad_lane<3>(ahi); acc = acc + quad_lane<3>(nhi);
+    }
+    return acc;
+}
+
+template <unsigned L>
+__device__ __forceinline__ float add_terms(float acc, float t, bool upper) {
+    if constexpr (L == kOct) return oct_accumulate(acc, t, upper);
+    else return quad_accumulate(acc, t);
+}
+
+template <unsigned L>
 __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
-    __shared__ uint32_t s_list[kNbrBlock / kQuad][kListCap];
+    static_assert(L == kQuad || L == kOct, "4 or 8 lanes per particle");
+    __shared__ uint32_t s_list[kNbrBlock / L][kListCap];
     const uint32_t g = blockIdx.x * kNbrBlock + threadIdx.x;
-    const uint32_t k = g / kQuad, q = g % kQuad;
+    const uint32_t k = g / L, q = g % L;
+    const bool upper = q >= kQuad;
     const Uniforms &u = *b.u;
-    if (k >= b.n || k >= b.start[u.G]) return;  // the whole quad leaves together
-    uint32_t *list = s_list[threadIdx.x / kQuad];
-    const uint32_t quad_shift = (threadIdx.x & 63u) & ~3u;
+    if (k >= b.n || k >= b.start[u.G]) return;  // the whole group leaves together
+    uint32_t *list = s_list[threadIdx.x / L];
+    const uint32_t group_shift = (threadIdx.x & 63u) & ~(L - 1u);
     const float4 p = b.spos[k];
     float4 v = b.svel[k];
     const float2 dni = b.sdn[k];
@@ -423,14 +471,14 @@ __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
     const float pressure_i = u.stiff * (density_i - u.rest);
     const float near_pressure_i = u.near_stiff * near_density_i;
     float fpx = 0.0f, fpy = 0.0f, fpz = 0.0f, fvx = 0.0f, fvy = 0.0f, fvz = 0.0f;
-    uint32_t cnt = 0;  // entries in the quad's list (quad-uniform)
+    uint32_t cnt = 0;  // entries in the group's list (group-uniform)
     // Evaluates the listed pairs in order and empties the list.
     auto drain = [&]() {
-        // the list entries were written by the other lanes of this quad (same
+        // the list entries were written by the other lanes of this group (same
         // wave, earlier in program order); keep the compiler from moving the
         // reads above those writes
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t base = 0; base < cnt; base += kQuad) {
+        for (uint32_t base = 0; base < cnt; base += L) {
             const uint32_t x = base + q;
             const bool valid = x < cnt;
             const uint32_t j = valid ? list[x] : k;
@@ -465,27 +513,18 @@ __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
             }
             // per pair, in order: fPress += pressure term, then += near-pressure
             // term (force.wgsl:113-114); fVisc += (:116)
-            fpx = fpx + quad_lane<0>(ax); fpx = fpx + quad_lane<0>(nx);
-            fpx = fpx + quad_lane<1>(ax); fpx = fpx + quad_lane<1>(nx);
-            fpx = fpx + quad_lane<2>(ax); fpx = fpx + quad_lane<2>(nx);
-            fpx = fpx + quad_lane<3>(ax); fpx = fpx + quad_lane<3>(nx);
-            fpy = fpy + quad_lane<0>(ay); fpy = fpy + quad_lane<0>(ny);
-            fpy = fpy + quad_lane<1>(ay); fpy = fpy + quad_lane<1>(ny);
-            fpy = fpy + quad_lane<2>(ay); fpy = fpy + quad_lane<2>(ny);
-            fpy = fpy + quad_lane<3>(ay); fpy = fpy + quad_lane<3>(ny);
-            fpz = fpz + quad_lane<0>(az); fpz = fpz + quad_lane<0>(nz);
-            fpz = fpz + quad_lane<1>(az); fpz = fpz + quad_lane<1>(nz);
-            fpz = fpz + quad_lane<2>(az); fpz = fpz + quad_lane<2>(nz);
-            fpz = fpz + quad_lane<3>(az); fpz = fpz + quad_lane<3>(nz);
-            fvx = quad_accumulate(fvx, vx);
-            fvy = quad_accumulate(fvy, vy);
-            fvz = quad_accumulate(fvz, vz);
+            fpx = add_pairs<L>(fpx, ax, nx, upper);
+            fpy = add_pairs<L>(fpy, ay, ny, upper);
+            fpz = add_pairs<L>(fpz, az, nz, upper);
+            fvx = add_terms<L>(fvx, vx, upper);
+            fvy = add_terms<L>(fvy, vy, upper);
+            fvz = add_terms<L>(fvz, vz, upper);
         }
         cnt = 0;
     };
     const Stream st = stream_of(row_ranges(b, u, rows_of(u, p)));
     const uint32_t total = st.pre[9];
-    for (uint32_t base = 0; base < total; base += kQuad) {
+    for (uint32_t base = 0; base < total; base += L) {
         const uint32_t x = base + q;
         const bool valid = x < total;
         const uint32_t j = valid ? slot_at(st, x) : k;
@@ -494,8 +533,8 @@ __global__ void __launch_bounds__(kNbrBlock) k_force(Bufs b) {
         const float ex = p.x - c.x, ey = p.y - c.y, ez = p.z - c.z;
         const float r2 = dot3(ex, ey, ez, ex, ey, ez);
         const bool hit = valid && dnj.x != 0.0f && dnj.y != 0.0f && r2 < u.h2 && 0.0f < r2;
-        const uint32_t qm = (uint32_t)(__ballot(hit) >> quad_shift) & 0xFu;
-        if (__any(cnt + kQuad > kListCap)) drain();  // wave-uniform
+        const uint32_t qm = (uint32_t)(__ballot(hit) >> group_shift) & ((1u << L) - 1u);
+        if (__any(cnt + L > kListCap)) drain();  // wave-uniform
         if (hit) list[cnt + (uint32_t)__popc(qm & ((1u << q) - 1u))] = j;
         cnt += (uint32_t)__popc(qm);
     }
@@ -639,7 +678,7 @@ hipError_t launch_stage(int stage, const Bufs &b, unsigned ntiles, hipStream_t s
     case SPH_STAGE_DENSITY:
         return b.n < kOctBelow ? launch_kernel(k_density<kOct>, ob, b, st, e0, e1, kNbrBlock)
                                : launch_kernel(k_density<kQuad>, nb, b, st, e0, e1, kNbrBlock);
-    case SPH_STAGE_FORCE: return launch_kernel(k_force, nb, b, st, e0, e1, kNbrBlock);
+    case SPH_STAGE_FORCE: return launch_kernel(k_force<kOct>, ob, b, st, e0, e1, kNbrBlock);
     default: return hipErrorInvalidValue;
     }
 }
