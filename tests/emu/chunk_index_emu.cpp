@@ -227,7 +227,16 @@ struct Emu {
                 std::vector<uint32_t> visits(ntchunks, 0);
                 const uint32_t nn = std::min<uint32_t>(ntchunks, (uint32_t)cap_tchunks);
                 for (uint32_t block = 0; block < grid; ++block) {
+                    // the descriptor a block loads before it knows the count
+                    const uint32_t first = walk_first_of(block, grid);
+                    const uint32_t cap = (uint32_t)cap_tchunks;
+                    CHECK(spec_chunk_of(first, cap) < cap_tchunks,
+                          "speculative descriptor %u of %llu", spec_chunk_of(first, cap),
+                          (unsigned long long)cap_tchunks);
+                    CHECK(spec_chunk_of(first, 0) == 0, "speculative descriptor at capacity 0");
                     const TileWalk w = tile_walk_of(block, grid, nn, xcd_first.data(), remap);
+                    if (!remap) CHECK(w.c == first, "first chunk of block %u", block);
+                    CHECK(w.end <= nn, "walk end %u beyond %u", w.end, nn);
                     for (uint32_t c = w.c; c < w.end; c += w.step) {
                         CHECK(c < ntchunks, "walk reaches chunk %u of %u", c, ntchunks);
                         ++visits[c];

@@ -135,9 +135,10 @@ hipError_t Batch::ensure_capacity(uint32_t total_n, uint64_t cells, uint64_t key
     TRY(hipMemset(sup_, 0, sizeof(uint2) * 2 * nsup));
     sup_cap_ = nsup;
     TRY(alloc(&tchunks_, sizeof(TileChunk) * tc));
-    // chunk count, then the kXcds + 1 XCD range boundaries of the chunk walk
-    TRY(alloc(&ntchunks_, sizeof(uint32_t) * (2 + kXcds)));
-    TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t) * (2 + kXcds)));
+    // chunk count, wild chunks of the last key, then the kXcds + 1 XCD range
+    // boundaries of the chunk walk
+    TRY(alloc(&ntchunks_, sizeof(uint32_t) * kWalkHeadWords));
+    TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t) * kWalkHeadWords));
     TRY(alloc(&tile_chunk_, sizeof(uint2) * k));
     TRY(hipMemset(count_, 0, sizeof(uint32_t) * k));
     TRY(hipMemset(ntchunks_, 0, sizeof(uint32_t)));
@@ -355,7 +356,7 @@ BatchArgs Batch::args() const {
     a.nchunks = (uint32_t)h_chunks_.size();
     a.tchunks = tchunks_;
     a.ntchunks = ntchunks_;
-    a.xcd_first = ntchunks_ + 1;
+    a.xcd_first = ntchunks_ + 2;
     a.total_slots = total_n_;
     a.tchunk_cap = tchunk_cap();
     a.tile_chunk = tile_chunk_;
@@ -408,7 +409,7 @@ BinBuffers Batch::bin_buffers() const {
     b.tchunks = tchunks_;
     b.tchunk_cap = tchunk_cap();
     b.ntchunks = ntchunks_;
-    b.xcd_first = ntchunks_ + 1;
+    b.xcd_first = ntchunks_ + 2;
     b.tile_chunk = tile_chunk_;
     return b;
 }

@@ -47,6 +47,9 @@ constexpr uint32_t kSegBits = 4;   // 16 keys per block of the count-path totals
 constexpr uint32_t kSupBits = 8;   // binning scan totals per 256 keys (super-segment; g2p
                                    // adds to them with atomics, so they must not be too few)
 constexpr uint32_t kWildTile = 0xffffffffu;
+// Words of the walk head buffer: chunk count, chunks of the last key, then
+// the kXcds + 1 XCD range boundaries (BatchArgs::ntchunks, ::xcd_first).
+constexpr uint32_t kWalkHeadWords = 2 + kXcds + 1;
 constexpr uint32_t kNoTile = 0xffffffffu;  // neighbour outside the grid
 
 // Per-scene constants, device resident.
@@ -85,7 +88,8 @@ struct BatchArgs {
     const Chunk *chunks;         // static slot chunks
     uint32_t nchunks;
     const TileChunk *tchunks;    // tile chunks of the current binning
-    const uint32_t *ntchunks;    // their number (device)
+    const uint32_t *ntchunks;    // [0]: their number, [1]: chunks of the last key (a
+                                 // single scene's wild key), both written by the scan
     const uint32_t *xcd_first;   // kXcds + 1 chunk indices: XCD x walks chunks
                                  // [xcd_first[x], xcd_first[x+1]) (equal particle shares)
     uint32_t total_slots;        // particles of all scenes
@@ -123,6 +127,8 @@ struct BatchArgs {
     int count_next;              // g2p also counts the keys of its output
     int ppt;                     // particles per thread: chunk <= kBlock * ppt
     int grid_blocks;             // persistent blocks of the chunk kernels (0: default)
+    uint32_t walk_grid;          // blocks of the chunk kernel being launched (= gridDim.x;
+                                 // set by its launcher, a kernel argument loaded with the rest)
     int xcd_remap;               // contiguous chunk ranges per XCD
     int tile_bits;               // 2: 4^3 tiles, 3: 8^3 tiles
 };
@@ -136,7 +142,7 @@ struct BinBuffers {
     uint32_t *perm_out;        // bin-order slot -> slot of the state (the scatter writes it)
     TileChunk *tchunks;
     uint32_t tchunk_cap;
-    uint32_t *ntchunks;
+    uint32_t *ntchunks;        // [0] count, [1] chunks of the last key
     uint32_t *xcd_first;       // kXcds + 1 entries, written by the scan
     uint2 *tile_chunk;
 };

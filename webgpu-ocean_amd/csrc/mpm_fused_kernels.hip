@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -70,7 +71,9 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 
 template <int TB>
 __device__ __forceinline__ uint32_t reduce_tile(const BatchArgs &a) {
-    return reduce_tile_of<TB>(blockIdx.x, threadIdx.x / kWave, a.stiles, a.xcd_remap);
+    // (the wave index is wave-uniform: say so, so that the tile is too)
+    return reduce_tile_of<TB>(blockIdx.x, __builtin_amdgcn_readfirstlane(threadIdx.x / kWave),
+                              a.stiles, a.xcd_remap);
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
@@ -111,6 +114,66 @@ __device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
     return tile_walk_of(blockIdx.x, gridDim.x, min(*a.ntchunks, a.tchunk_cap), a.xcd_first,
                         a.xcd_remap);
 }
+
+// Loads through the constant address space: scalar loads, which the compiler
+// may also move out of loops.  Only for data that no thread writes while the
+// kernel runs (scene descriptors; the chunk list and counts of the binning,
+// which the scan kernel wrote).
+template <typename T>
+__device__ __forceinline__ T load_const(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T *)(uintptr_t)p;
+#else
+    return *p;  // (the host pass of a device function; never executed)
+#endif
+}
+
+// A chunk kernel block's walk with the descriptor of its current chunk.  In
+// the default walk the first chunk's index does not depend on the chunk
+// count, so the descriptor, the count and the wild-chunk count (and, for a
+// batch of one scene, the scene's descriptor) are loaded together and waited
+// for once; each iteration loads the next descriptor before its own particle
+// loads.
+struct ChunkWalk {
+    TileWalk w;
+    TileChunk ch;    // descriptor of chunk w.c (meaningful while w.c < w.end)
+    uint32_t wild;   // tile chunks of the last key (a single scene's wild key)
+
+    template <bool kSingle>
+    __device__ __forceinline__ ChunkWalk(const BatchArgs &a, SceneDesc &scene,
+                                         std::integral_constant<bool, kSingle>) {
+        if constexpr (kSingle) scene = load_const(a.scenes);
+        const uint32_t grid = a.walk_grid;  // = gridDim.x
+        w.c = walk_first_of(blockIdx.x, grid);
+        ch = load_const(a.tchunks + spec_chunk_of(w.c, a.tchunk_cap));
+        const uint2 head = load_const(reinterpret_cast<const uint2 *>(a.ntchunks));
+        // Wait for these loads together, here: left to itself the compiler
+        // issues each at its first use, the descriptor's after the count's
+        // wait (an empty asm that needs the values, not a load).
+        if constexpr (kSingle)
+            asm volatile("" ::"s"(head.x), "s"(head.y), "s"(ch.x), "s"(ch.y), "s"(ch.z),
+                         "s"(ch.w), "s"(scene.goff), "s"(scene.uni.Y), "s"(scene.uni.Z),
+                         "s"(scene.knt[1]), "s"(scene.knt[2]), "s"(scene.kdim[0]),
+                         "s"(scene.kdim[1]), "s"(scene.kdim[2]), "s"(scene.prm.mult));
+        else
+            asm volatile("" ::"s"(head.x), "s"(head.y), "s"(ch.x), "s"(ch.y), "s"(ch.z),
+                         "s"(ch.w));
+        wild = head.y;
+        w.end = min(head.x, a.tchunk_cap);
+        w.step = grid;
+        if (a.xcd_remap && (grid % kXcds) == 0) {  // XCD ranges: another round of loads
+            w = tile_walk_of(blockIdx.x, grid, w.end, a.xcd_first, 1);
+            ch = load_const(a.tchunks + spec_chunk_of(w.c, a.tchunk_cap));
+        }
+    }
+    // Returns the current chunk's descriptor and starts loading the next one.
+    __device__ __forceinline__ TileChunk take(const BatchArgs &a) {
+        const TileChunk cur = ch;
+        const uint32_t nc = w.c + w.step;
+        if (nc < w.end) ch = load_const(a.tchunks + nc);
+        return cur;
+    }
+};
 
 __device__ __forceinline__ void load_x(const float *__restrict__ st, uint32_t stride, uint32_t i,
                                        float x[3]) {
@@ -423,7 +486,8 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
             }
         }
         if (k == b.nkeys - 1) {
-            *b.ntchunks = cb + nch;
+            b.ntchunks[0] = cb + nch;
+            b.ntchunks[1] = nch;  // the last key is the last scene's wild key
             b.xcd_first[0] = 0;
             b.xcd_first[kXcds] = cb + nch;
         }
@@ -625,21 +689,31 @@ struct LevelOrder {
     float x[3][kBlock * PPT];      // position of each slot within the chunk (read once)
 };
 
-// Fills o.perm for window chunk ch; every thread of the block calls it.
+// Fills o.perm for window chunk ch and clears the LDS window win (nwin ints,
+// published by the barriers here); every thread of the block calls it.  The
+// state slots of all the thread's particles are loaded first, then all their
+// positions: two rounds of loads in flight, not two per particle.
 template <int PPT, int TB>
 __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk &ch,
-                                            LevelOrder<PPT, TB> &o) {
+                                            LevelOrder<PPT, TB> &o, int *win, int nwin) {
+    // (slots past the chunk's end load its last particle: no branch around
+    // the loads; nothing is stored for them)
+    uint32_t src[PPT];
+    float xr[PPT][3];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k)
+        src[k] = a.perm[ch.z + min(threadIdx.x + (uint32_t)k * kBlock, ch.w - 1)];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) load_x(a.state, a.stride, src[k], xr[k]);
+    for (int l = threadIdx.x; l < nwin; l += kBlock) win[l] = 0;
     if constexpr (TB == 3) {
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
             if (j < ch.w) {
-                const uint32_t i = a.perm[ch.z + j];
-                o.slot[j] = i;
-                float x[3];
-                load_x(a.state, a.stride, i, x);
+                o.slot[j] = src[k];
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) o.x[ax][j] = x[ax];
+                for (int ax = 0; ax < 3; ++ax) o.x[ax][j] = xr[k][ax];
                 o.perm[j] = (uint16_t)j;
             }
         }
@@ -654,16 +728,13 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
         const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
         cell[k] = lev[k] = 0;
         if (j < ch.w) {
-            const uint32_t i = a.perm[ch.z + j];
-            o.slot[j] = i;
-            float x[3];
-            load_x(a.state, a.stride, i, x);
+            o.slot[j] = src[k];
 #pragma unroll
-            for (int ax = 0; ax < 3; ++ax) o.x[ax][j] = x[ax];
+            for (int ax = 0; ax < 3; ++ax) o.x[ax][j] = xr[k][ax];
             const int m = kTileEdge - 1;
-            const int cx = (int)floorf(x[0]) & m;
-            const int cy = (int)floorf(x[1]) & m;
-            const int cz = (int)floorf(x[2]) & m;
+            const int cx = (int)floorf(xr[k][0]) & m;
+            const int cy = (int)floorf(xr[k][1]) & m;
+            const int cz = (int)floorf(xr[k][2]) & m;
             cell[k] = (uint32_t)((cx * kTileEdge + cy) * kTileEdge + cz);
             lev[k] = atomicAdd(&o.cnt[cell[k]], 1u);
         }
@@ -706,38 +777,38 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
 template <int PPT>
 constexpr int kChunkWaves = PPT == 2 ? 6 : 1;  // amdgpu_waves_per_eu minimum (1: no constraint)
 
-template <int PPT, int TB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
-void k_p2g1(BatchArgs a) {
+// The walk of a p2g_1 block.  kSingle: a batch of one scene, whose
+// descriptor is then known before the first chunk's.
+template <int PPT, int TB, bool kSingle>
+__device__ __forceinline__ void p2g1_walk(const BatchArgs &a, int *lds, LevelOrder<PPT, TB> &ord) {
     constexpr int WC = TileGeo<TB>::win_cells;
-    __shared__ int lds[4 * WC];  // [mass | vx | vy | vz]
-    __shared__ LevelOrder<PPT, TB> ord;
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
-    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
-        const TileChunk ch = a.tchunks[w.c];
+    SceneDesc scene;
+    for (ChunkWalk cw(a, scene, std::integral_constant<bool, kSingle>{}); cw.w.c < cw.w.end;
+         cw.w.c += cw.w.step) {
+        const TileChunk ch = cw.take(a);
         if (ch.y == kWildTile) {  // second loop
             wild_seen = true;
             continue;
         }
-        const SceneDesc *sd = a.scenes + ch.x;
-        const float mult = sd->prm.mult;
-        for (int l = threadIdx.x; l < 4 * WC; l += kBlock) lds[l] = 0;
-        level_order<PPT, TB>(a, ch, ord);  // its barriers also publish the cleared window
+        if constexpr (!kSingle) scene = load_const(a.scenes + ch.x);
+        const float mult = scene.prm.mult;
+        level_order<PPT, TB>(a, ch, ord, lds, 4 * WC);
+        // positions past the chunk's end load its last particle (no branch
+        // around the loads; those lanes store and add nothing)
         float x[PPT][3], v[PPT][3], C[PPT][3][3];
         bool valid[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
             valid[k] = pos < ch.w;
-            if (valid[k]) {
-                const uint32_t j = ord.perm[pos];
-                const uint32_t i = ord.slot[j];
-                a.order[ch.z + pos] = i;  // p2g_2 takes the same order
+            const uint32_t j = ord.perm[min(pos, ch.w - 1)];
+            const uint32_t i = ord.slot[j];
+            if (valid[k]) a.order[ch.z + pos] = i;  // p2g_2 takes the same order
 #pragma unroll
-                for (int ax = 0; ax < 3; ++ax) x[k][ax] = ord.x[ax][j];
-                load_v(a.state, a.stride, i, v[k]);
-                load_C(a.state, a.stride, i, C[k]);
-            }
+            for (int ax = 0; ax < 3; ++ax) x[k][ax] = ord.x[ax][j];
+            load_v(a.state, a.stride, i, v[k]);
+            load_C(a.state, a.stride, i, C[k]);
         }
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
@@ -747,7 +818,7 @@ void k_p2g1(BatchArgs a) {
             p2g1_particle(n, v[k], C[k], mult, WinSink1<TB>{lds, win_l0<TB>(x[k])});
         }
         __syncthreads();
-        int4 *out = a.partial + (size_t)w.c * WC;
+        int4 *out = a.partial + (size_t)cw.w.c * WC;
         for (int l = threadIdx.x; l < WC; l += kBlock)
             out[l] = make_int4(lds[WC + l], lds[2 * WC + l], lds[3 * WC + l], lds[l]);
         __syncthreads();
@@ -773,6 +844,18 @@ void k_p2g1(BatchArgs a) {
     }
 }
 
+template <int PPT, int TB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
+void k_p2g1(BatchArgs a) {
+    constexpr int WC = TileGeo<TB>::win_cells;
+    __shared__ int lds[4 * WC];  // [mass | vx | vy | vz]
+    __shared__ LevelOrder<PPT, TB> ord;
+    if (a.nscenes == 1)
+        p2g1_walk<PPT, TB, true>(a, lds, ord);
+    else
+        p2g1_walk<PPT, TB, false>(a, lds, ord);
+}
+
 // ---------------------------------------------------------------------------
 // reduce: one wave per spatial tile (tiles in Hilbert order).  Lanes 0..26
 // fetch the chunk ranges of the 27 neighbour tiles; then each lane owns one
@@ -788,19 +871,17 @@ struct CellPlan {
     uint32_t rounds;  // chunks of the fullest neighbour (wave-uniform)
 };
 
+// kt: lane q < 27 holds the key of neighbour q of the tile (a.nbr).
 template <int TB>
-__device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t b, const int l[3],
+__device__ __forceinline__ CellPlan plan_partials(const BatchArgs &a, uint32_t kt, const int l[3],
                                                   bool cell_ok) {
-    const int lane = threadIdx.x & (kWave - 1);
-    uint32_t nfirst = 0, ncnt = 0;
-    if (lane < 27) {
-        const uint32_t kt = a.nbr[(size_t)b * 27 + lane];
-        if (kt != kNoTile) {
-            const uint2 tc = a.tile_chunk[kt];
-            nfirst = tc.x;
-            ncnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
-        }
-    }
+    // (no branch around the load: a missing neighbour, or a lane past 26,
+    // whose kt is kNoTile, reads key 0 and takes no chunks)
+    const uint2 tc = a.tile_chunk[kt != kNoTile ? kt : 0u];
+    const bool has = kt != kNoTile;
+    const uint32_t nfirst = has ? tc.x : 0u;
+    const uint32_t ncnt =
+        has && tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
     CellPlan p;
     uint32_t mc = 0;
 #pragma unroll
@@ -841,29 +922,54 @@ __device__ __forceinline__ void take_wild(const BatchArgs &a, uint64_t gi, int4 
     a.wild[gi] = make_int4(0, 0, 0, 0);
 }
 
-// One lane per cell: a 4^3 tile is one wave (a block holds 4 tiles); an 8^3
-// tile is a whole block of 8 waves, whose "anything to write" decision is
-// taken for the tile as a whole (__syncthreads_or), so its waves agree on
-// the tile's dirty flag.
-template <int PHASE, int TB>
-__global__ __launch_bounds__(ReduceGeo<TB>::threads) void k_reduce(BatchArgs a) {
+// The reduce of one tile.  kSingle: a batch of one scene, whose descriptor is
+// then known before the tile's.
+template <int PHASE, int TB, bool kSingle>
+__device__ __forceinline__ void reduce_run(const BatchArgs &a) {
     constexpr int E = TileGeo<TB>::edge;
     constexpr uint32_t kCells = (uint32_t)TileGeo<TB>::cells;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t cell = TB == 2 ? lane : threadIdx.x;  // cell of the tile this lane owns
+    // the kernel arguments the first loads need, loaded as one batch
+    asm volatile("" ::"s"(a.scenes), "s"(a.ntchunks), "s"(a.tile_info), "s"(a.dirty),
+                 "s"(a.near), "s"(a.nbr), "s"(a.stiles), "s"(a.xcd_remap));
     const uint32_t b = reduce_tile<TB>(a);
     if (b >= a.stiles) return;  // (8^3: the whole block)
-    const uint4 ti = a.tile_info[b];
-    const bool dirty = a.dirty[b] != 0;
+    // Everything the tile's first decisions need is loaded together: its
+    // flags and coordinates, its neighbours' keys, and for a batch of one
+    // scene the scene and whether it has wild particles.
+    // (lanes past 26 load neighbour 26 again: no branch around the load)
+    const uint32_t ktl = a.nbr[(size_t)b * 27 + min(lane, 26u)];
+    const uint32_t kt = lane < 27 ? ktl : kNoTile;
+    SceneDesc scene;
+    uint2 head = make_uint2(0u, 0u);
+    if constexpr (kSingle) {
+        scene = load_const(a.scenes);
+        head = load_const(reinterpret_cast<const uint2 *>(a.ntchunks));
+    }
+    const uint4 ti = load_const(a.tile_info + b);
+    const uint32_t dirty_raw = a.dirty[b], near_raw = a.near[b];
+    // (an empty asm that needs the values: their loads complete here
+    // together; the scheduling barrier keeps every use below all the loads)
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (kSingle)
+        asm volatile("" ::"s"(ti.x), "s"(ti.y), "s"(ti.z), "s"(head.y), "s"(scene.goff),
+                     "s"(scene.uni.Y), "s"(scene.uni.Z), "s"(scene.kdim[0]), "s"(scene.kdim[1]),
+                     "s"(scene.kdim[2]), "v"(ktl), "v"(dirty_raw), "v"(near_raw));
+    else
+        asm volatile("" ::"s"(ti.w), "v"(ktl), "v"(dirty_raw), "v"(near_raw));
+    const bool dirty = dirty_raw != 0, near = near_raw != 0;
     // no particle near, nothing left to clear, no wild particle in the scene:
     // the tile's cells are empty and already written as such
-    if (!a.near[b] && !dirty) return;  // (its count is zero too)
-    const SceneDesc *sd = a.scenes + ti.w;
+    if (!near && !dirty) return;  // (its count is zero too)
+    if constexpr (!kSingle) scene = load_const(a.scenes + ti.w);
+    const SceneDesc *sd = &scene;
     if (PHASE == 2 && cell == 0) {  // ready for the next counting pass
         a.count[sd->koff + (b - sd->soff)] = 0;
         if (b == sd->soff) a.count[sd->koff + sd->ntiles] = 0;  // the scene's wild key
     }
-    const bool has_wild = a.tile_chunk[sd->toff + sd->ntiles].y != 0;
+    const bool has_wild =
+        kSingle ? head.y != 0 : a.tile_chunk[sd->toff + sd->ntiles].y != 0;
     const Params prm = sd->prm;
     const int32_t wall_hi[3] = {sd->uni.wall_hi[0], sd->uni.wall_hi[1], sd->uni.wall_hi[2]};
     if (!sd->fast) {  // every particle is wild: cells in linear order
@@ -888,7 +994,7 @@ __global__ __launch_bounds__(ReduceGeo<TB>::threads) void k_reduce(BatchArgs a) 
     const int cx = (int)ti.x * E + l[0], cy = (int)ti.y * E + l[1], cz = (int)ti.z * E + l[2];
     const bool cell_ok = in_dims(cx, cy, cz, sd);
     const uint64_t gi = (uint64_t)sd->goff + cell_linear(cx, cy, cz, sd->uni.Y, sd->uni.Z);
-    const CellPlan plan = plan_partials<TB>(a, b, l, cell_ok);
+    const CellPlan plan = plan_partials<TB>(a, kt, l, cell_ok);
     // No particle near the tile: its cells are empty.  If they were already
     // empty after the last substep (not dirty), nothing needs writing.
     bool empty = plan.rounds == 0 && !has_wild;
@@ -919,34 +1025,52 @@ __global__ __launch_bounds__(ReduceGeo<TB>::threads) void k_reduce(BatchArgs a) 
     }
 }
 
+// One lane per cell: a 4^3 tile is one wave (a block holds 4 tiles); an 8^3
+// tile is a whole block of 8 waves, whose "anything to write" decision is
+// taken for the tile as a whole (__syncthreads_or), so its waves agree on
+// the tile's dirty flag.
+template <int PHASE, int TB>
+__global__ __launch_bounds__(ReduceGeo<TB>::threads) void k_reduce(BatchArgs a) {
+    if (a.nscenes == 1)
+        reduce_run<PHASE, TB, true>(a);
+    else
+        reduce_run<PHASE, TB, false>(a);
+}
+
 // ---------------------------------------------------------------------------
 // p2g_2: density, pressure, viscous stress momentum (mls-mpm/p2g_2.wgsl:30-94)
 
-template <int PPT, int TB>
-__global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
+template <int PPT, int TB, bool kSingle>
+__device__ __forceinline__ void p2g2_walk(const BatchArgs &a, int *lds) {
     constexpr int WC = TileGeo<TB>::win_cells;
-    __shared__ int lds[4 * WC];  // [decoded mass (f32) | vx | vy | vz]
-    float *dm = reinterpret_cast<float *>(lds);
+    float *dm = reinterpret_cast<float *>(lds);  // [decoded mass (f32) | vx | vy | vz]
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
-    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
-        const TileChunk ch = a.tchunks[w.c];
+    SceneDesc scene;
+    const SceneDesc *sd = &scene;
+    for (ChunkWalk cw(a, scene, std::integral_constant<bool, kSingle>{}); cw.w.c < cw.w.end;
+         cw.w.c += cw.w.step) {
+        const TileChunk ch = cw.take(a);
         if (ch.y == kWildTile) {  // second loop
             wild_seen = true;
             continue;
         }
-        const SceneDesc *sd = a.scenes + ch.x;
+        if constexpr (!kSingle) scene = load_const(a.scenes + ch.x);
         const Params prm = sd->prm;
-        float x[PPT][3], C[PPT][3][3];
+        // the level order p2g_1 recorded: all slots, then all particle loads
+        // (positions past the chunk's end load its last particle)
+        uint32_t oi[PPT];
         bool valid[PPT];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const uint32_t pos = threadIdx.x + (uint32_t)k * kBlock;
             valid[k] = pos < ch.w;
-            if (valid[k]) {  // the level order p2g_1 recorded
-                const uint32_t i = a.order[ch.z + pos];
-                load_x(a.state, a.stride, i, x[k]);
-                load_C(a.state, a.stride, i, C[k]);
-            }
+            oi[k] = a.order[ch.z + min(pos, ch.w - 1)];
+        }
+        float x[PPT][3], C[PPT][3][3];
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            load_x(a.state, a.stride, oi[k], x[k]);
+            load_C(a.state, a.stride, oi[k], C[k]);
         }
         int t[3];
         tile_coords(ch.y, sd, t);
@@ -966,7 +1090,7 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
             p2g2_particle(n, C[k], prm, WinSink2<TB>{lds, win_l0<TB>(x[k])});
         }
         __syncthreads();
-        int4 *out = a.partial + (size_t)w.c * WC;
+        int4 *out = a.partial + (size_t)cw.w.c * WC;
         for (int l = threadIdx.x; l < WC; l += kBlock)
             out[l] = make_int4(lds[WC + l], lds[2 * WC + l], lds[3 * WC + l], 0);
         __syncthreads();
@@ -990,6 +1114,15 @@ __global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
             p2g2_particle<true>(n, C, prm, sink);
         }
     }
+}
+
+template <int PPT, int TB>
+__global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
+    __shared__ int lds[4 * TileGeo<TB>::win_cells];
+    if (a.nscenes == 1)
+        p2g2_walk<PPT, TB, true>(a, lds);
+    else
+        p2g2_walk<PPT, TB, false>(a, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1048,36 +1181,38 @@ __device__ __forceinline__ void finish_chunk_counts(const BatchArgs &a, const Sc
     __syncthreads();
 }
 
-template <int PPT, int TB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
-void k_g2p(BatchArgs a) {
+template <int PPT, int TB, bool kSingle>
+__device__ __forceinline__ void g2p_walk(const BatchArgs &a, float4 *lvel, uint32_t *lcnt) {
     constexpr int WC = TileGeo<TB>::win_cells;
-    __shared__ float4 lvel[WC];
-    __shared__ uint32_t lcnt[kDirWild + 2];  // per-direction counts, wild base
-    // whether the (single) scene has wild chunks at all, read up front
-    const bool any_wild =
-        a.nscenes != 1 || a.tile_chunk[a.scenes[0].toff + a.scenes[0].ntiles].y != 0;
-    for (TileWalk w = tile_walk(a); w.c < w.end; w.c += w.step) {
-        const TileChunk ch = a.tchunks[w.c];
+    SceneDesc scene;
+    const SceneDesc *sd = &scene;
+    ChunkWalk cw(a, scene, std::integral_constant<bool, kSingle>{});
+    // whether the binning has wild chunks at all (a batch: assume it has)
+    const bool any_wild = !kSingle || cw.wild != 0;
+    for (; cw.w.c < cw.w.end; cw.w.c += cw.w.step) {
+        const TileChunk ch = cw.take(a);
         if (ch.y == kWildTile) continue;  // second loop
-        const SceneDesc *sd = a.scenes + ch.x;
+        if constexpr (!kSingle) scene = load_const(a.scenes + ch.x);
         const float dt = sd->prm.dt;
         const float rb[3] = {sd->uni.real_box[0], sd->uni.real_box[1], sd->uni.real_box[2]};
         if (threadIdx.x < kDirWild + 1) lcnt[threadIdx.x] = 0;
-        float x[PPT][3];
-        uint32_t oid[PPT];  // caller index of each particle
+        // contiguous slots per task: coalesced state stores.  All state slots
+        // first, then all positions and caller indices (slots past the
+        // chunk's end load its last particle and store nothing).
+        uint32_t src[PPT];
         bool valid[PPT];
-        // contiguous slots per task: coalesced state stores
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const uint32_t j = task_of(k) * 64 + (threadIdx.x & 63);
             valid[k] = j < ch.w;
-            oid[k] = 0;
-            if (valid[k]) {
-                const uint32_t src = a.perm[ch.z + j];
-                load_x(a.state, a.stride, src, x[k]);
-                oid[k] = a.id[src];
-            }
+            src[k] = a.perm[ch.z + min(j, ch.w - 1)];
+        }
+        float x[PPT][3];
+        uint32_t oid[PPT];  // caller index of each particle
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            load_x(a.state, a.stride, src[k], x[k]);
+            oid[k] = a.id[src[k]];
         }
         int t[3];
         tile_coords(ch.y, sd, t);
@@ -1102,10 +1237,10 @@ void k_g2p(BatchArgs a) {
             }
             wild[k] = valid[k] && dir == kDirWild;
             // (a wave whose k-th task holds no particle skips the ranking together)
-            if (a.count_next && __ballot(valid[k])) rank_next(a, lcnt, w.c, i, key, dir, valid[k]);
+            if (a.count_next && __ballot(valid[k])) rank_next(a, lcnt, cw.w.c, i, key, dir, valid[k]);
         }
         if (a.count_next) {
-            finish_chunk_counts(a, sd, lcnt, w.c);
+            finish_chunk_counts(a, sd, lcnt, cw.w.c);
 #pragma unroll
             for (int k = 0; k < PPT; ++k)
                 if (wild[k]) {
@@ -1148,6 +1283,17 @@ void k_g2p(BatchArgs a) {
     }
 }
 
+template <int PPT, int TB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
+void k_g2p(BatchArgs a) {
+    __shared__ float4 lvel[TileGeo<TB>::win_cells];
+    __shared__ uint32_t lcnt[kDirWild + 2];  // per-direction counts, wild base
+    if (a.nscenes == 1)
+        g2p_walk<PPT, TB, true>(a, lvel, lcnt);
+    else
+        g2p_walk<PPT, TB, false>(a, lvel, lcnt);
+}
+
 __global__ void k_set_scene_uniforms(SceneDesc *d, Uniforms u) { d->uni = u; }
 
 // Events armed for the next chunk/tile kernel launch (time_stages): the
@@ -1177,8 +1323,10 @@ inline unsigned walk_blocks(const BatchArgs &a) {
 }
 
 template <int PPT, int TB>
-hipError_t launch_ppt(int which, const BatchArgs &a, hipStream_t st) {
-    const unsigned nb = walk_blocks(a);
+hipError_t launch_ppt(int which, const BatchArgs &args, hipStream_t st) {
+    const unsigned nb = walk_blocks(args);
+    BatchArgs a = args;
+    a.walk_grid = nb;
     switch (which) {
     case 1: launch_timed(k_p2g1<PPT, TB>, nb, kBlock, st, a); break;
     case 2: launch_timed(k_p2g2<PPT, TB>, nb, kBlock, st, a); break;

@@ -73,6 +73,18 @@ struct TileWalk {
     uint32_t c, end, step;
 };
 
+// First chunk of block `block` in the default walk (runs of kXcdRun chunks
+// dealt round robin over the XCDs; a bijection of [0, grid) when grid is a
+// multiple of kXcds * kXcdRun).  It does not depend on the chunk count, so a
+// block can load its first chunk's descriptor beside the count.
+__host__ __device__ __forceinline__ uint32_t walk_first_of(uint32_t block, uint32_t grid) {
+    if ((grid % (kXcds * kXcdRun)) == 0) {
+        const uint32_t x = block % kXcds, j = block / kXcds;
+        return ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
+    }
+    return block;
+}
+
 __host__ __device__ __forceinline__ TileWalk tile_walk_of(uint32_t block, uint32_t grid,
                                                          uint32_t n, const uint32_t *xcd_first,
                                                          int xcd_remap) {
@@ -83,15 +95,19 @@ __host__ __device__ __forceinline__ TileWalk tile_walk_of(uint32_t block, uint32
         w.end = min_u32(xcd_first[x + 1], n);
         w.step = grid / kXcds;
     } else {
-        w.c = block;
-        if ((grid % (kXcds * kXcdRun)) == 0) {  // a bijection of [0, grid)
-            const uint32_t x = block % kXcds, j = block / kXcds;
-            w.c = ((j / kXcdRun) * kXcds + x) * kXcdRun + j % kXcdRun;
-        }
+        w.c = walk_first_of(block, grid);
         w.end = n;
         w.step = grid;
     }
     return w;
+}
+
+// Descriptor slot a block reads for chunk c before it knows whether c is
+// below the chunk count: c itself, or the last slot of the cap descriptors
+// the walk may use (slot 0 when cap is 0; the buffer always holds one).  The
+// value read is used only for c < count <= cap.
+__host__ __device__ __forceinline__ uint32_t spec_chunk_of(uint32_t c, uint32_t cap) {
+    return c < cap ? c : (cap ? cap - 1u : 0u);
 }
 
 // Linear cell index of integer cell coordinates, i32 wrap-around like WGSL
