@@ -416,20 +416,26 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
     const uint32_t c = in ? b.count[k] : 0u;
     const uint32_t xm = in ? a.tile_xmaj[k] : 0u;
     if (part != 0) {
+        // Every lane loads its key's neighbour keys and their counts, whether
+        // or not it needs them: two rounds of loads with no branch around
+        // them (the wild key and lanes past the last key load the entries of
+        // a valid tile; missing neighbours read key 0).
+        const uint32_t kk = min(k, b.nkeys - 1);
+        const SceneDesc sd = load_const(a.scenes + (a.nscenes == 1 ? 0u : a.key_scene[kk]));
+        const uint32_t t = sd.soff + min(kk - sd.koff, sd.ntiles - 1);  // spatial tile
+        const uint32_t *nb = a.nbr + (size_t)t * kDirs;
+        uint32_t nk[kReachPer], cn[kReachPer];
+#pragma unroll
+        for (int j = 0; j < kReachPer; ++j)
+            nk[j] = nb[min((uint32_t)((part - 1) + kReachWaves * j), kDirs - 1)];
+#pragma unroll
+        for (int j = 0; j < kReachPer; ++j) cn[j] = b.count[nk[j] != kNoTile ? nk[j] : 0u];
         bool any = false;
-        if (in && xm != kWildTile && c == 0) {
-            const SceneDesc *sd = a.scenes + key_scene_of(a, k);
-            if (sd->fast) {
-                const uint32_t *nb = a.nbr + (size_t)(sd->soff + (k - sd->koff)) * kDirs;
-                uint32_t nk[kReachPer];
+        if (in && xm != kWildTile && c == 0 && sd.fast) {
 #pragma unroll
-                for (int j = 0; j < kReachPer; ++j) {
-                    const uint32_t d = (uint32_t)((part - 1) + kReachWaves * j);
-                    nk[j] = d < kDirs ? nb[d] : kNoTile;
-                }
-#pragma unroll
-                for (int j = 0; j < kReachPer; ++j)
-                    any = any || (nk[j] != kNoTile && b.count[nk[j]] != 0);
+            for (int j = 0; j < kReachPer; ++j) {
+                const uint32_t d = (uint32_t)((part - 1) + kReachWaves * j);
+                any = any || (d < kDirs && nk[j] != kNoTile && cn[j] != 0);
             }
         }
         s_any[part - 1][lane] = any ? 1u : 0u;
@@ -512,7 +518,12 @@ __global__ __launch_bounds__(kScanWaves * kWave) void k_scan_keys(BatchArgs a, B
 // range (the check only guards the buffers against inconsistent ranks).
 __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs a, BinBuffers b) {
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x) {
-        const Chunk ch = a.chunks[c];
+        // a batch of one scene: static chunk c is slots [256 c, 256 c + 256) of
+        // its n = total_slots (Batch::upload_chunks), known without a load
+        const Chunk ch =
+            a.nscenes == 1
+                ? Chunk{0u, c * kBlock, min((uint32_t)kBlock, a.total_slots - c * kBlock), 0u}
+                : load_const(a.chunks + c);
         const SceneDesc *sd = a.scenes + ch.scene;
         const uint32_t lo = sd->poff, hi = sd->poff + sd->n;
         for (uint32_t j = threadIdx.x; j < ch.count; j += kBlock) {
@@ -543,21 +554,17 @@ __global__ __launch_bounds__(kBlock) void k_gather_binned(BatchArgs a, float *ds
 // (chunk, direction) entry belongs to exactly one destination tile.  Every
 // lane of the wave calls this with the same key; returns the key's total,
 // which lane 0 also stores.
+// s: lane q < 27 holds the key of neighbour q of key k's tile (a.nbr), the
+// source tile T + d of direction q; other lanes hold kNoTile.
 __device__ __forceinline__ uint32_t gather_key(const BatchArgs &a, const BinBuffers &bb,
-                                               uint32_t k, const SceneDesc *sd, uint32_t rank,
-                                               uint32_t total) {
+                                               uint32_t k, uint32_t s, uint32_t total) {
     const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t b = sd->soff + rank;  // spatial tile of key k
-    uint32_t first = 0, cnt = 0, dir = 0;
-    if (lane < (int)kDirs) {
-        const uint32_t s = a.nbr[(size_t)b * kDirs + lane];  // source tile T + d
-        if (s != kNoTile) {
-            const uint2 tc = a.tile_chunk[s];
-            first = tc.x;
-            cnt = tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x ? tc.y : 0u;  // guard
-        }
-        dir = kDirs - 1 - lane;  // from the source, T lies at -d
-    }
+    // (no branch around the load: a missing source reads key 0 and counts nothing)
+    const uint2 tc = a.tile_chunk[s != kNoTile ? s : 0u];
+    const uint32_t first = s != kNoTile ? tc.x : 0u;
+    const uint32_t cnt = s != kNoTile && tc.x <= a.tchunk_cap && tc.y <= a.tchunk_cap - tc.x
+                             ? tc.y : 0u;  // (guard)
+    const uint32_t dir = lane < (int)kDirs ? kDirs - 1 - lane : 0u;  // from the source, T lies at -d
     const uint32_t rounds = wave_max(cnt);
     for (uint32_t j = 0; j < rounds; ++j) {
         const bool has = j < cnt;
@@ -600,17 +607,27 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather(BatchArgs a, Bi
         const uint32_t per = gridDim.x / kXcds;
         sg = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     }
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave),
+              lane = threadIdx.x & (kWave - 1);
     const uint32_t k = sg * kSegKeys + (uint32_t)w;
     uint32_t total = 0;
     if (sg < nseg && k < bb.nkeys) {
-        const SceneDesc *sd = a.scenes + key_scene_of(a, k);
-        const uint32_t rank = k - sd->koff;
-        total = bb.count[k];
+        // the key's count, its tile's reach flag and its 27 source keys are
+        // loaded in one round (a batch of one scene needs no scene lookup
+        // first; lanes past 26 and the wild key load a valid neighbour entry)
+        const SceneDesc sd = load_const(a.scenes + (a.nscenes == 1 ? 0u : a.key_scene[k]));
+        const uint32_t rank = k - sd.koff;
+        const uint32_t b = sd.soff + min(rank, sd.ntiles - 1);  // spatial tile of key k
+        const uint32_t cnt_raw = bb.count[k];
+        const uint32_t near_raw = a.near_prev[b];
+        const uint32_t sl = a.nbr[(size_t)b * kDirs + min((uint32_t)lane, kDirs - 1)];
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::"v"(cnt_raw), "v"(near_raw), "v"(sl));
+        total = cnt_raw;
         // a tile no particle came near last substep has no source chunk
-        const bool sources = rank < sd->ntiles && a.near_prev[sd->soff + rank] != 0;
-        if (!final_counts && sd->fast && rank < sd->ntiles && sources)
-            total = gather_key(a, bb, k, sd, rank, total);
+        const bool sources = rank < sd.ntiles && near_raw != 0;
+        if (!final_counts && sd.fast && rank < sd.ntiles && sources)
+            total = gather_key(a, bb, k, lane < (int)kDirs ? sl : kNoTile, total);
     }
     if (lane == 0) {
         s_cnt[w] = total;
@@ -654,7 +671,10 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather_sparse(BatchArg
         const int i = __ffsll((unsigned long long)pending) - 1;
         const uint32_t kk = __shfl(k, i, kWave), rr = __shfl(rank, i, kWave);
         const uint32_t ss = __shfl(sc, i, kWave), t0 = __shfl(total, i, kWave);
-        const uint32_t t = gather_key(a, bb, kk, a.scenes + ss, rr, t0);
+        const SceneDesc *sd = a.scenes + ss;
+        const uint32_t b = sd->soff + rr;  // spatial tile of key kk
+        const uint32_t sk = lane < (int)kDirs ? a.nbr[(size_t)b * kDirs + lane] : kNoTile;
+        const uint32_t t = gather_key(a, bb, kk, sk, t0);
         if (lane == i) total = t;
     }
     const uint32_t wt = wave_sum(mine ? total : 0u);
