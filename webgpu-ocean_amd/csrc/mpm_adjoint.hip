@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "mpm_batch.h"
+#include "mpm_index.h"
 #include "mpm_particle.h"
 
 #pragma clang fp contract(off)
@@ -166,6 +167,7 @@ struct AdjArgs {
     Params prm;
     Uniforms uni;
     int xcd_remap;
+    uint32_t walk_grid;            // blocks of the chunk kernels (= their gridDim.x)
 };
 
 // G2P adjoint of one particle, in two phases: (a) the forward recomputation
@@ -562,13 +564,6 @@ __device__ __forceinline__ int64_t adj_win_cell(const AdjArgs &q, int l, const i
                      (uint32_t)y * (uint32_t)q.uni.Z + (uint32_t)z);
 }
 
-__device__ __forceinline__ void adj_tile_coords(const AdjArgs &q, uint32_t xm, int t[3]) {
-    const uint32_t n1 = q.sd->knt[1], n2 = q.sd->knt[2];
-    t[0] = (int)(xm / (n1 * n2));
-    t[1] = (int)((xm / n2) % n1);
-    t[2] = (int)(xm % n2);
-}
-
 // Persistent walk over the taped tile chunks: runs of 16 consecutive chunks
 // dealt round robin over the XCDs (block b runs on XCD b mod 8), or with
 // xcd_remap one contiguous range per XCD.
@@ -584,16 +579,57 @@ __device__ __forceinline__ AdjWalk adj_walk(const AdjArgs &q) {
         w.end = min(n, (x + 1) * per);
         w.step = gridDim.x / 8;
     } else {
-        w.c = blockIdx.x;
-        if ((gridDim.x % (8 * 16)) == 0) {  // a bijection of [0, gridDim)
-            const uint32_t x = blockIdx.x % 8, j = blockIdx.x / 8;
-            w.c = ((j / 16) * 8 + x) * 16 + j % 16;
-        }
+        w.c = walk_first_of(blockIdx.x, gridDim.x);
         w.end = n;
         w.step = gridDim.x;
     }
     return w;
 }
+
+// The walk of a chunk kernel's window loop with the descriptor of its current
+// chunk.  In the default walk the first chunk's index does not depend on the
+// chunk count, so the count, that descriptor and the scene's tile counts are
+// loaded together and waited for once (scalar loads: the tape and the scene
+// are not written while the kernel runs); each iteration loads the next
+// descriptor before its own loads.
+struct AdjChunkWalk {
+    AdjWalk w;
+    TileChunk ch;   // descriptor of chunk w.c (meaningful while w.c < w.end)
+    uint32_t knt1, knt2;
+
+    __device__ __forceinline__ explicit AdjChunkWalk(const AdjArgs &q) {
+        const uint32_t grid = q.walk_grid;  // = gridDim.x
+        w.c = walk_first_of(blockIdx.x, grid);
+        ch = load_const(q.tchunks + spec_chunk_of(w.c, q.tchunk_cap));
+        const uint32_t n = load_const(q.ntchunks);
+        knt1 = load_const(&q.sd->knt[1]);
+        knt2 = load_const(&q.sd->knt[2]);
+        // (an empty asm that needs the values: their loads complete here together)
+        asm volatile("" ::"s"(n), "s"(ch.x), "s"(ch.y), "s"(ch.z), "s"(ch.w), "s"(knt1),
+                     "s"(knt2));
+        w.end = min(n, q.tchunk_cap);
+        w.step = grid;
+        if (q.xcd_remap && (grid % 8) == 0) {  // contiguous ranges: another round of loads
+            const uint32_t per = (w.end + 7) / 8, x = blockIdx.x % 8;
+            w.c = x * per + blockIdx.x / 8;
+            w.end = min(w.end, (x + 1) * per);
+            w.step = grid / 8;
+            ch = load_const(q.tchunks + spec_chunk_of(w.c, q.tchunk_cap));
+        }
+    }
+    // Returns the current chunk's descriptor and starts loading the next one.
+    __device__ __forceinline__ TileChunk take(const AdjArgs &q) {
+        const TileChunk cur = ch;
+        const uint32_t nc = w.c + w.step;
+        if (nc < w.end) ch = load_const(q.tchunks + nc);
+        return cur;
+    }
+    __device__ __forceinline__ void tile_coords(uint32_t xm, int t[3]) const {
+        t[0] = (int)(xm / (knt1 * knt2));
+        t[1] = (int)((xm / knt2) % knt1);
+        t[2] = (int)(xm % knt2);
+    }
+};
 
 // ---- chunk kernels ---------------------------------------------------------------
 
@@ -607,14 +643,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
     __shared__ float s_sum[3 * kWinCells];
     __shared__ uint32_t s_max;
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
-    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
-        const TileChunk ch = q.tchunks[w.c];
+    for (AdjChunkWalk cw(q); cw.w.c < cw.w.end; cw.w.c += cw.w.step) {
+        const TileChunk ch = cw.take(q);
         if (ch.y == kWildTile) {  // second loop
             wild_seen = true;
             continue;
         }
         int t[3];
-        adj_tile_coords(q, ch.y, t);
+        cw.tile_coords(ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
             const int64_t gi = adj_win_cell(q, l, t);
             float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -652,7 +688,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_g2p_t(AdjArgs q) {
             if (threadIdx.x == 0) s_max = 0u;
             __syncthreads();
         }
-        float4 *out = q.partial + (size_t)w.c * kWinCells;
+        float4 *out = q.partial + (size_t)cw.w.c * kWinCells;
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock)
             out[l] = make_float4(s_sum[l], s_sum[kWinCells + l], s_sum[2 * kWinCells + l], 0.0f);
         __syncthreads();
@@ -682,14 +718,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
     __shared__ float s_red[2][kAdjBlock / 64];
     float gk = 0.0f, gmu = 0.0f;
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
-    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
-        const TileChunk ch = q.tchunks[w.c];
+    for (AdjChunkWalk cw(q); cw.w.c < cw.w.end; cw.w.c += cw.w.step) {
+        const TileChunk ch = cw.take(q);
         if (ch.y == kWildTile) {  // second loop
             wild_seen = true;
             continue;
         }
         int t[3];
-        adj_tile_coords(q, ch.y, t);
+        cw.tile_coords(ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
             const int64_t gi = adj_win_cell(q, l, t);
             float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -721,7 +757,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
             if (threadIdx.x == 0) s_max = 0u;
             __syncthreads();
         }
-        float4 *out = q.partial + (size_t)w.c * kWinCells;
+        float4 *out = q.partial + (size_t)cw.w.c * kWinCells;
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock)
             out[l] = make_float4(s_sum[l], 0.0f, 0.0f, 0.0f);
         __syncthreads();
@@ -765,14 +801,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g2_t(AdjArgs q) {
 __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
     __shared__ float4 s_w4[kWinCells];
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
-    for (AdjWalk w = adj_walk(q); w.c < w.end; w.c += w.step) {
-        const TileChunk ch = q.tchunks[w.c];
+    for (AdjChunkWalk cw(q); cw.w.c < cw.w.end; cw.w.c += cw.w.step) {
+        const TileChunk ch = cw.take(q);
         if (ch.y == kWildTile) {  // second loop
             wild_seen = true;
             continue;
         }
         int t[3];
-        adj_tile_coords(q, ch.y, t);
+        cw.tile_coords(ch.y, t);
         for (int l = threadIdx.x; l < kWinCells; l += kAdjBlock) {
             const int64_t gi = adj_win_cell(q, l, t);
             float4 c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -807,22 +843,19 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_p2g1_t(AdjArgs q) {
 
 // Sum of the partial windows covering each of a tile's 64 cells (one wave
 // per tile; the same cover as the forward reduce).
-__device__ __forceinline__ float4 adj_sum_partials(const AdjArgs &q, uint32_t b, const int l[3],
+// kt: lane q < 27 holds the key of neighbour q of the tile (q.nbr), other
+// lanes kNoTile.  All covering slots' loads of one round are in flight at once.
+__device__ __forceinline__ float4 adj_sum_partials(const AdjArgs &q, uint32_t kt, const int l[3],
                                                    bool cell_ok) {
-    const int lane = threadIdx.x & 63;
-    uint32_t nfirst = 0, ncnt = 0;
-    if (lane < 27) {
-        const uint32_t kt = q.nbr[(size_t)b * 27 + lane];
-        if (kt != kNoTile) {
-            const uint2 tc = q.tile_chunk[kt];
-            nfirst = tc.x;
-            ncnt = tc.x <= q.tchunk_cap && tc.y <= q.tchunk_cap - tc.x ? tc.y : 0u;
-        }
-    }
+    // (no branch around the load: a missing neighbour reads key 0 and takes no chunks)
+    const uint2 tc = q.tile_chunk[kt != kNoTile ? kt : 0u];
+    const uint32_t nfirst = kt != kNoTile ? tc.x : 0u;
+    const uint32_t ncnt =
+        kt != kNoTile && tc.x <= q.tchunk_cap && tc.y <= q.tchunk_cap - tc.x ? tc.y : 0u;
     int ext[3];
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) ext[ax] = l[ax] == 0 ? -1 : (l[ax] == kTileEdge - 1 ? 1 : 0);
-    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t base[8], cnt[8], rounds = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         bool valid = cell_ok;
@@ -838,15 +871,47 @@ __device__ __forceinline__ float4 adj_sum_partials(const AdjArgs &q, uint32_t b,
         const int widx = ((l[0] + 1 - kTileEdge * d[0]) * kWinEdge + (l[1] + 1 - kTileEdge * d[1])) *
                              kWinEdge +
                          (l[2] + 1 - kTileEdge * d[2]);
-        if (!valid) continue;
-        for (uint32_t j = 0; j < c; ++j) {
-            const float4 v = q.partial[(size_t)(f + j) * kWinCells + widx];
-            acc.x += v.x;
-            acc.y += v.y;
-            acc.z += v.z;
-        }
+        base[s] = f * (uint32_t)kWinCells + (uint32_t)widx;
+        cnt[s] = valid ? c : 0u;
+        rounds = max(rounds, cnt[s]);
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) rounds = max(rounds, (uint32_t)__shfl_xor(rounds, o, 64));
+    // per slot (its chunks in order), then the slots in order: a fixed order,
+    // so the result does not depend on the launch
+    float4 sl[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) sl[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t j = 0; j < rounds; ++j) {
+        float4 v[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            v[s] = j < cnt[s] ? q.partial[(size_t)base[s] + (size_t)j * kWinCells]
+                              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (j < cnt[s]) {
+                sl[s].x += v[s].x;
+                sl[s].y += v[s].y;
+                sl[s].z += v[s].z;
+            }
+    }
+    float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        acc.x += sl[s].x;
+        acc.y += sl[s].y;
+        acc.z += sl[s].z;
     }
     return acc;
+}
+
+// The neighbour keys of tile b, one per lane (lanes past 26: kNoTile), loaded
+// without a branch around the load.
+__device__ __forceinline__ uint32_t adj_nbr_keys(const AdjArgs &q, uint32_t b) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t k = q.nbr[(size_t)b * 27 + min(lane, 26u)];
+    return lane < 27 ? k : kNoTile;
 }
 
 // Grid-update adjoint per tile: grid-velocity adjoint = global scatter (wild
@@ -854,18 +919,21 @@ __device__ __forceinline__ float4 adj_sum_partials(const AdjArgs &q, uint32_t b,
 __global__ __launch_bounds__(64) void k_adj_update_t(AdjArgs q) {
     const uint32_t b = blockIdx.x;
     if (b >= q.stiles) return;
-    const SceneDesc *sd = q.sd;
+    // the scene, the tile and its neighbour keys in one round of loads
+    const SceneDesc sdv = load_const(q.sd);
+    const SceneDesc *sd = &sdv;
+    const uint4 ti = load_const(q.tile_info + b);
+    const uint32_t kt = adj_nbr_keys(q, b);
     int l[3] = {(int)threadIdx.x >> 4, ((int)threadIdx.x >> 2) & 3, (int)threadIdx.x & 3};
     int64_t gi = -1;
     float4 part = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (sd->fast) {
-        const uint4 ti = q.tile_info[b];
         const int c[3] = {(int)ti.x * kTileEdge + l[0], (int)ti.y * kTileEdge + l[1],
                           (int)ti.z * kTileEdge + l[2]};
         const bool ok = (uint32_t)c[0] < (uint32_t)sd->kdim[0] &&
                         (uint32_t)c[1] < (uint32_t)sd->kdim[1] &&
                         (uint32_t)c[2] < (uint32_t)sd->kdim[2];
-        part = adj_sum_partials(q, b, l, ok);
+        part = adj_sum_partials(q, kt, l, ok);
         if (ok)
             gi = (int64_t)((uint32_t)c[0] * (uint32_t)q.uni.Y * (uint32_t)q.uni.Z +
                            (uint32_t)c[1] * (uint32_t)q.uni.Z + (uint32_t)c[2]);
@@ -900,15 +968,19 @@ __global__ __launch_bounds__(64) void k_adj_update_t(AdjArgs q) {
 // Adds the P2G-2 partials (mass adjoints) into gm, per tile.
 __global__ __launch_bounds__(64) void k_adj_reduce_gm_t(AdjArgs q) {
     const uint32_t b = blockIdx.x;
-    if (b >= q.stiles || !q.sd->fast) return;
-    const SceneDesc *sd = q.sd;
+    if (b >= q.stiles) return;
+    // the scene, the tile and its neighbour keys in one round of loads
+    const SceneDesc sdv = load_const(q.sd);
+    const SceneDesc *sd = &sdv;
+    const uint4 ti = load_const(q.tile_info + b);
+    const uint32_t kt = adj_nbr_keys(q, b);
+    if (!sd->fast) return;
     const int l[3] = {(int)threadIdx.x >> 4, ((int)threadIdx.x >> 2) & 3, (int)threadIdx.x & 3};
-    const uint4 ti = q.tile_info[b];
     const int c[3] = {(int)ti.x * kTileEdge + l[0], (int)ti.y * kTileEdge + l[1],
                       (int)ti.z * kTileEdge + l[2]};
     const bool ok = (uint32_t)c[0] < (uint32_t)sd->kdim[0] &&
                     (uint32_t)c[1] < (uint32_t)sd->kdim[1] && (uint32_t)c[2] < (uint32_t)sd->kdim[2];
-    const float4 part = adj_sum_partials(q, b, l, ok);
+    const float4 part = adj_sum_partials(q, kt, l, ok);
     if (!ok) return;
     const uint64_t gi = (uint32_t)c[0] * (uint32_t)q.uni.Y * (uint32_t)q.uni.Z +
                         (uint32_t)c[1] * (uint32_t)q.uni.Z + (uint32_t)c[2];
@@ -1158,6 +1230,7 @@ hipError_t Batch::backward(const AdjointConfig &cfg, hipStream_t st, AdjointResu
         q.uni = d.uni;
         q.xcd_remap = tun_.xcd_remap;
         const unsigned nchunk_blocks = 2048;
+        q.walk_grid = nchunk_blocks;
         for (int32_t sgi = (int32_t)nseg - 1; sgi >= 0 && e == hipSuccess; --sgi) {
             const uint32_t t0 = (uint32_t)sgi * K, t1 = std::min(T, t0 + K);
             if (sgi != (int32_t)nseg - 1) {  // (the last segment's tape is the forward's)

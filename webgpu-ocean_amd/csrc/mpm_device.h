@@ -30,6 +30,19 @@ struct Params {
     float stiffness, rest_density, viscosity, dt, mult;
 };
 
+// Loads through the constant address space: scalar loads, which the compiler
+// may also move out of loops.  Only for data that no thread writes while the
+// kernel runs (scene descriptors; the chunk list and counts of a binning,
+// written by an earlier kernel).
+template <typename T>
+__device__ __forceinline__ T load_const(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T *)(uintptr_t)p;
+#else
+    return *p;  // (the host pass of a device function; never executed)
+#endif
+}
+
 // i32(f32): truncate toward zero, saturating, NaN -> 0.  That is exactly
 // what gfx950's V_CVT_I32_F32 does, so the conversion is that one
 // instruction (C's (int)f leaves out-of-range values undefined and would

@@ -115,19 +115,6 @@ __device__ __forceinline__ TileWalk tile_walk(const BatchArgs &a) {
                         a.xcd_remap);
 }
 
-// Loads through the constant address space: scalar loads, which the compiler
-// may also move out of loops.  Only for data that no thread writes while the
-// kernel runs (scene descriptors; the chunk list and counts of the binning,
-// which the scan kernel wrote).
-template <typename T>
-__device__ __forceinline__ T load_const(const T *p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(4))) T *)(uintptr_t)p;
-#else
-    return *p;  // (the host pass of a device function; never executed)
-#endif
-}
-
 // A chunk kernel block's walk with the descriptor of its current chunk.  In
 // the default walk the first chunk's index does not depend on the chunk
 // count, so the descriptor, the count and the wild-chunk count (and, for a
