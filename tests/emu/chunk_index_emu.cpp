@@ -52,6 +52,66 @@ long g_checks = 0;
         }                                                            \
     } while (0)
 
+// The P2G kernels' level order of a chunk with cnt[c] particles in tile cell
+// c (k_p2g1's wave-0 computation and level_position): every particle's
+// position lies in [0, total) and no two particles share one.
+void check_level_order(const uint32_t cnt[kTileCells]) {
+    uint32_t cmax = 0, total = 0;
+    for (uint32_t c = 0; c < kTileCells; ++c) {
+        cmax = std::max(cmax, cnt[c]);
+        total += cnt[c];
+    }
+    const uint32_t levels = std::min(cmax, kLevelCap);
+    uint32_t base[kLevelCap] = {0}, exoff[kTileCells] = {0};
+    uint64_t mask[kLevelCap] = {0};
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < levels; ++k) {
+        uint64_t m = 0;
+        for (uint32_t c = 0; c < kTileCells; ++c)
+            if (cnt[c] > k) m |= 1ull << c;
+        mask[k] = m;
+        base[k] = run;
+        run += (uint32_t)__builtin_popcountll(m);
+    }
+    uint32_t ex_run = run;  // exclusive scan of the excess, cell order
+    for (uint32_t c = 0; c < kTileCells; ++c) {
+        exoff[c] = ex_run;
+        ex_run += cnt[c] > kLevelCap ? cnt[c] - kLevelCap : 0u;
+    }
+    CHECK(ex_run == total, "level order covers %u of %u positions", ex_run, total);
+    std::vector<uint8_t> used(total, 0);
+    for (uint32_t c = 0; c < kTileCells; ++c)
+        for (uint32_t lev = 0; lev < cnt[c]; ++lev) {
+            const uint32_t q = level_position(lev, c, base, mask, exoff);
+            CHECK(q < total, "level position %u of %u (cell %u level %u)", q, total, c, lev);
+            CHECK(!used[q], "level position %u taken twice", q);
+            used[q] = 1;
+        }
+}
+
+// Synthetic count patterns: all cells below the level cap, one crowded cell,
+// several crowded cells, every particle in one cell.
+void check_level_order_patterns(uint32_t cap_particles) {
+    uint32_t cnt[kTileCells];
+    for (uint32_t pat = 0; pat < 6; ++pat) {
+        uint32_t left = cap_particles;
+        for (uint32_t c = 0; c < kTileCells; ++c) {
+            uint32_t want = 0;
+            switch (pat) {
+            case 0: want = 8; break;
+            case 1: want = c == 5 ? cap_particles : 3; break;
+            case 2: want = (c % 9 == 0) ? 60 : 2; break;
+            case 3: want = c == 63 ? cap_particles : 0; break;
+            case 4: want = c == 0 ? kLevelCap + 1 : (c == 1 ? kLevelCap : 1); break;
+            default: want = (c * 7) % 41; break;
+            }
+            cnt[c] = std::min(want, left);
+            left -= cnt[c];
+        }
+        check_level_order(cnt);
+    }
+}
+
 struct Emu {
     SceneDesc d{};
     std::vector<uint32_t> rank_of, xmaj, nbr;
@@ -297,6 +357,7 @@ struct Emu {
             uint32_t positions = 0;
             for (uint32_t q = 0; q < kTileCells; ++q) positions += cells[q];
             CHECK(positions == ch.w && positions <= cap, "level order positions");
+            check_level_order(cells);
             for (int l = 0; l < kWinCells; ++l) {
                 const int64_t gi = win_cell_index(l, t, &d);
                 CHECK(gi >= -1 && gi < (int64_t)d.G, "window cell %d -> %lld", l, (long long)gi);
@@ -431,6 +492,7 @@ int main(int argc, char **argv) {
         return 1;
     }
     std::fclose(f);
+    for (uint32_t cp : {256u, 512u, 1024u}) check_level_order_patterns(cp);
     Emu e;
     e.init(n, box, ppt);
     const oracle_params prm = {3.0f, 4.0f, 0.1f, 0.2f, 1e7f};

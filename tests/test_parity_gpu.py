@@ -198,12 +198,7 @@ def test_config2_two_million_mass_and_one_substep():
         assert_same(sim, ref, ("particles", "posvel"), ctx="cfg2 substep 1")
 
 
-@pytest.mark.parametrize("ppt", [1, 2, 4])
-def test_dense_cells_match_the_oracle(ppt):
-    """Hundreds of particles per cell (the pile-up at a wall after a box
-    shrink): tiles split into several chunks, the P2G level order runs to
-    hundreds of levels, and the i32 mass accumulators wrap.  Integer adds
-    wrap the same on both sides, so results stay bit-exact."""
+def _dense_scene():
     rng = np.random.default_rng(11)
     box = [24.0, 20.0, 24.0]
     dense = rng.uniform([9.0, 6.0, 9.0], [11.0, 8.0, 11.0], size=(1800, 3))  # 8 cells
@@ -212,7 +207,18 @@ def test_dense_cells_match_the_oracle(ppt):
     spread = rng.uniform(3.5, 16.5, size=(500, 3))
     pos = np.concatenate([dense, plane, spread]).astype(np.float32)
     vel = rng.normal(0.0, 0.3, size=pos.shape).astype(np.float32)
-    rec = scenes.records(pos, vel)
+    return scenes.records(pos, vel), box
+
+
+@pytest.mark.parametrize("ppt", [1, 2, 4])
+def test_dense_cells_match_the_oracle(ppt):
+    """Hundreds of particles per cell (the pile-up at a wall after a box
+    shrink): tiles split into several chunks, the P2G level order runs past
+    its level-major part into the crowded cells' cell-major tail (whole waves
+    of one cell, added through the window replicas), and the i32 mass
+    accumulators wrap.  Integer adds wrap the same on both sides, so results
+    stay bit-exact."""
+    rec, box = _dense_scene()
     sim, ref = pair(rec, box)
     with sim:
         sim.set_tuning(particles_per_thread=ppt)
@@ -220,6 +226,19 @@ def test_dense_cells_match_the_oracle(ppt):
             sim.execute()
             ref.execute(2)
             assert_same(sim, ref, ("particles", "posvel"), ctx=f"ppt {ppt} frame {frame}")
+
+
+@pytest.mark.parametrize("edge", [4, 8])
+def test_dense_cells_production_grid_accumulators_bit_exact(edge):
+    """The raw P2G accumulators of crowded chunks (cells of hundreds of
+    particles: replicated LDS windows on 4^3 tiles) equal the oracle's, bit
+    for bit, after a frame and on the next two substeps."""
+    rec, box = _dense_scene()
+    with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+        sim.set_tuning(tile_edge=edge)
+        sim.execute(2)
+        for k in range(2):
+            _production_step_matches_oracle_stages(sim, box, box, f"dense edge {edge} substep {k}")
 
 
 def test_particles_outside_the_grid_are_dropped_like_the_oracle():

@@ -185,6 +185,30 @@ __device__ __forceinline__ void load_C(const float *__restrict__ st, uint32_t st
 // ---------------------------------------------------------------------------
 // sinks: where a particle's 27 node contributions go / come from
 
+// LDS replicas of the P2G windows, used for a "crowded" chunk (a tile cell
+// with more than kLevelCap particles; such a cell's particles past the level
+// order's levels fill whole waves of one cell).  Lane L then adds into
+// replica L mod count, so the lanes of a wave that all add to one node meet
+// `count` addresses, 64/count lanes each, instead of one address 64 times
+// over (same-address LDS atomics of one instruction serialise); the flush
+// sums the replicas.  Other chunks use replica 0 only and clear and flush
+// nothing else.  Each replica is offset by 8 more banks than a whole number
+// of windows.  (8^3 tiles: one replica, the window is already 16 KB.)
+template <int TB>
+struct WinRep {
+    static constexpr int count = TB == 2 ? 4 : 1;
+    static constexpr int WC = TileGeo<TB>::win_cells;
+    static constexpr int stride1 = 4 * WC + 8;  // p2g_1: [mass | vx | vy | vz] per replica
+    static constexpr int stride2 = 3 * WC + 8;  // p2g_2: [vx | vy | vz] per replica
+    __device__ __forceinline__ static int of_lane(bool crowded) {
+        return crowded ? (int)(threadIdx.x & (uint32_t)(count - 1)) : 0;
+    }
+};
+
+// p2g_1 marks a crowded chunk in bit 31 of the first entry of its level order
+// for p2g_2 (state slots are below 2^31).
+constexpr uint32_t kCrowdedBit = 0x80000000u;
+
 template <int TB>
 struct WinSink1 {  // p2g_1 into the LDS window: [mass | vx | vy | vz] x win_cells
     static constexpr int W = TileGeo<TB>::win_edge, WC = TileGeo<TB>::win_cells;
@@ -219,20 +243,21 @@ struct GridSink1 {  // p2g_1 straight to the grid (reference-shaped atomics)
 };
 
 template <int TB>
-struct WinSink2 {  // p2g_2: decoded masses [0, WC), momentum ints [WC, 4 WC)
+struct WinSink2 {  // p2g_2: decoded masses dm[WC]; momentum ints [vx | vy | vz] x WC
     static constexpr int W = TileGeo<TB>::win_edge, WC = TileGeo<TB>::win_cells;
-    int *lds;
+    const float *dm;
+    int *mom;
     int l0;
     __device__ __forceinline__ float mass(const Node3 &, const AxisNode &, const AxisNode &, int gx,
                                           int gy, int gz) const {
-        return reinterpret_cast<const float *>(lds)[l0 + (gx * W + gy) * W + gz];
+        return dm[l0 + (gx * W + gy) * W + gz];
     }
     __device__ __forceinline__ void add3(const Node3 &, const AxisNode &, const AxisNode &, int gx,
                                          int gy, int gz, int32_t x, int32_t y, int32_t z) const {
         const int l = l0 + (gx * W + gy) * W + gz;
-        atomicAdd(&lds[WC + l], x);
-        atomicAdd(&lds[2 * WC + l], y);
-        atomicAdd(&lds[3 * WC + l], z);
+        atomicAdd(&mom[l], x);
+        atomicAdd(&mom[WC + l], y);
+        atomicAdd(&mom[2 * WC + l], z);
     }
 };
 
@@ -681,28 +706,36 @@ __global__ __launch_bounds__(kSegKeys * kWave) void k_bin_gather_sparse(BatchArg
 // mostly hold 64 different cells: their ds_add_u32 to one stencil node then
 // rarely meet on one LDS address (same-address atomics of one instruction
 // serialise).  The position of the k-th particle of cell c is
-// base[k] + #{cells c' < c holding more than k}.  Integer adds commute, so
-// the order changes no result.  With 8^3 tiles (512 cells, too many for the
+// base[k] + #{cells c' < c holding more than k} for the first kLevelCap levels
+// (a crowded cell's particles past them follow cell-major: level_position;
+// the window replicas bound the cost of such a cell's adds).  Integer adds
+// commute, so the order changes no result.  With 8^3 tiles (512 cells, too many for the
 // 64-bit level masks) the chunk's particles keep their bin order: a wave's
 // 64 particles then spread over 8 times as many cells.
 
 template <int PPT, int TB>
 struct LevelOrder {
-    uint32_t cnt[TileGeo<TB>::cells];  // particles per cell of the tile
-    uint32_t base[kBlock * PPT];   // first position of each level
-    uint64_t mask[kBlock * PPT];   // cells present in each level
+    uint32_t cnt[TileGeo<TB>::cells];    // particles per cell of the tile
+    uint32_t base[kLevelCap];            // first position of each level
+    uint64_t mask[kLevelCap];            // cells present in each level
+    uint32_t exoff[TileGeo<TB>::cells];  // first position of each cell's particles past
+                                         // the last level (level_position)
+    uint32_t crowded;                    // a cell holds more than kLevelCap particles
     uint16_t perm[kBlock * PPT];   // position -> slot within the chunk
     uint32_t slot[kBlock * PPT];   // slot within the chunk -> state slot (a.perm)
     float x[3][kBlock * PPT];      // position of each slot within the chunk (read once)
 };
 
-// Fills o.perm for window chunk ch and clears the LDS window win (nwin ints,
-// published by the barriers here); every thread of the block calls it.  The
-// state slots of all the thread's particles are loaded first, then all their
-// positions: two rounds of loads in flight, not two per particle.
+// Fills o.perm and o.crowded for window chunk ch and clears the LDS window
+// win: replica 0 (nwin ints), and for a crowded chunk the other replicas
+// (nextra ints after it); the barriers here publish it.  Every thread of the
+// block calls it.  The state slots of all the thread's particles are loaded
+// first, then all their positions: two rounds of loads in flight, not two per
+// particle.
 template <int PPT, int TB>
 __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk &ch,
-                                            LevelOrder<PPT, TB> &o, int *win, int nwin) {
+                                            LevelOrder<PPT, TB> &o, int *win, int nwin,
+                                            int nextra) {
     // (slots past the chunk's end load its last particle: no branch around
     // the loads; nothing is stored for them)
     uint32_t src[PPT];
@@ -724,6 +757,7 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
                 o.perm[j] = (uint16_t)j;
             }
         }
+        if (threadIdx.x == 0) o.crowded = 0;
         __syncthreads();
     } else {
     const int lane = threadIdx.x & (kWave - 1);
@@ -749,8 +783,8 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
     __syncthreads();
     if (threadIdx.x < kWave) {  // one wave: lane = cell
         const uint32_t c = o.cnt[lane];
-        const uint32_t levels = wave_max(c);
-        uint32_t run = 0;
+        const uint32_t cmax = wave_max(c), levels = min(cmax, kLevelCap);
+        uint32_t run = 0;  // (wave-uniform)
         for (uint32_t k = 0; k < levels; ++k) {
             const uint64_t m = __ballot(c > k);
             if (lane == 0) {
@@ -759,16 +793,20 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
             }
             run += (uint32_t)__popcll(m);
         }
+        if (cmax > kLevelCap) {  // (wave-uniform) cell-major past the levels
+            const uint32_t ex = c > kLevelCap ? c - kLevelCap : 0u;
+            o.exoff[lane] = run + wave_incl_scan(ex, lane) - ex;
+        }
+        if (lane == 0) o.crowded = cmax > kLevelCap ? 1u : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const uint32_t j = threadIdx.x + (uint32_t)k * kBlock;
-        if (j < ch.w) {
-            const uint64_t below = (1ull << cell[k]) - 1ull;
-            o.perm[o.base[lev[k]] + (uint32_t)__popcll(o.mask[lev[k]] & below)] = (uint16_t)j;
-        }
+        if (j < ch.w) o.perm[level_position(lev[k], cell[k], o.base, o.mask, o.exoff)] = (uint16_t)j;
     }
+    if (o.crowded)  // (block-uniform)
+        for (int l = threadIdx.x; l < nextra; l += kBlock) win[nwin + l] = 0;
     __syncthreads();
     }
 }
@@ -777,10 +815,10 @@ __device__ __forceinline__ void level_order(const BatchArgs &a, const TileChunk 
 // p2g_1: mass and APIC momentum (mls-mpm/p2g_1.wgsl:24-65) into tile partials
 
 // Register budget of the chunk kernels at the default 2 particles per thread:
-// 80 VGPRs (6 waves per SIMD).  At this budget k_p2g1 spills a few bytes and
-// k_g2p spills only in its wild-chunk loop; measured 6% (p2g_1) and 8% (g2p)
-// faster than the compiler's own ~92-VGPR choice.  k_p2g2 gains nothing from
-// it and keeps the compiler's budget.
+// 80 VGPRs (6 waves per SIMD).  At this budget k_g2p spills only in its
+// wild-chunk loop; measured 6% (p2g_1) and 8% (g2p) faster than the
+// compiler's own ~92-VGPR choice, and p2g_2 (85 VGPRs on its own, i.e. 5
+// waves per SIMD) about 5% faster.
 template <int PPT>
 constexpr int kChunkWaves = PPT == 2 ? 6 : 1;  // amdgpu_waves_per_eu minimum (1: no constraint)
 
@@ -800,7 +838,9 @@ __device__ __forceinline__ void p2g1_walk(const BatchArgs &a, int *lds, LevelOrd
         }
         if constexpr (!kSingle) scene = load_const(a.scenes + ch.x);
         const float mult = scene.prm.mult;
-        level_order<PPT, TB>(a, ch, ord, lds, 4 * WC);
+        level_order<PPT, TB>(a, ch, ord, lds, WinRep<TB>::stride1,
+                             (WinRep<TB>::count - 1) * WinRep<TB>::stride1);
+        const bool crowded = ord.crowded != 0;
         // positions past the chunk's end load its last particle (no branch
         // around the loads; those lanes store and add nothing)
         float x[PPT][3], v[PPT][3], C[PPT][3][3];
@@ -811,7 +851,8 @@ __device__ __forceinline__ void p2g1_walk(const BatchArgs &a, int *lds, LevelOrd
             valid[k] = pos < ch.w;
             const uint32_t j = ord.perm[min(pos, ch.w - 1)];
             const uint32_t i = ord.slot[j];
-            if (valid[k]) a.order[ch.z + pos] = i;  // p2g_2 takes the same order
+            if (valid[k])  // p2g_2 takes the same order (and the crowded mark)
+                a.order[ch.z + pos] = i | (pos == 0 && crowded ? kCrowdedBit : 0u);
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) x[k][ax] = ord.x[ax][j];
             load_v(a.state, a.stride, i, v[k]);
@@ -822,12 +863,27 @@ __device__ __forceinline__ void p2g1_walk(const BatchArgs &a, int *lds, LevelOrd
             if (!valid[k]) continue;
             Node3 n;
             node3_init(x[k], n);
-            p2g1_particle(n, v[k], C[k], mult, WinSink1<TB>{lds, win_l0<TB>(x[k])});
+            p2g1_particle(n, v[k], C[k], mult,
+                          WinSink1<TB>{lds + WinRep<TB>::of_lane(crowded) * WinRep<TB>::stride1,
+                                       win_l0<TB>(x[k])});
         }
         __syncthreads();
         int4 *out = a.partial + (size_t)cw.w.c * WC;
-        for (int l = threadIdx.x; l < WC; l += kBlock)
-            out[l] = make_int4(lds[WC + l], lds[2 * WC + l], lds[3 * WC + l], lds[l]);
+        auto flush1 = [&](auto nrep) {
+            for (int l = threadIdx.x; l < WC; l += kBlock) {
+                int4 s = make_int4(0, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < decltype(nrep)::value; ++r) {
+                    const int *w = lds + r * WinRep<TB>::stride1;
+                    s = add4(s, make_int4(w[WC + l], w[2 * WC + l], w[3 * WC + l], w[l]));
+                }
+                out[l] = s;
+            }
+        };
+        if (crowded)  // (block-uniform)
+            flush1(std::integral_constant<int, WinRep<TB>::count>{});
+        else
+            flush1(std::integral_constant<int, 1>{});
         __syncthreads();
     }
     // Wild chunks (rare): global atomics into the wild accumulator.  A loop of
@@ -855,7 +911,7 @@ template <int PPT, int TB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
 void k_p2g1(BatchArgs a) {
     constexpr int WC = TileGeo<TB>::win_cells;
-    __shared__ int lds[4 * WC];  // [mass | vx | vy | vz]
+    __shared__ int lds[WinRep<TB>::count * WinRep<TB>::stride1];  // [mass | vx | vy | vz] x replica
     __shared__ LevelOrder<PPT, TB> ord;
     if (a.nscenes == 1)
         p2g1_walk<PPT, TB, true>(a, lds, ord);
@@ -1048,9 +1104,12 @@ __global__ __launch_bounds__(ReduceGeo<TB>::threads) void k_reduce(BatchArgs a) 
 // p2g_2: density, pressure, viscous stress momentum (mls-mpm/p2g_2.wgsl:30-94)
 
 template <int PPT, int TB, bool kSingle>
-__device__ __forceinline__ void p2g2_walk(const BatchArgs &a, int *lds) {
+__device__ __forceinline__ void p2g2_walk(const BatchArgs &a, int *lds, uint32_t &s_crowded) {
     constexpr int WC = TileGeo<TB>::win_cells;
-    float *dm = reinterpret_cast<float *>(lds);  // [decoded mass (f32) | vx | vy | vz]
+    using Rep = WinRep<TB>;
+    // [decoded mass (f32)] then per replica [vx | vy | vz]
+    float *dm = reinterpret_cast<float *>(lds);
+    int *mom = lds + WC;
     bool wild_seen = false;  // block-uniform: a wild chunk among this block's
     SceneDesc scene;
     const SceneDesc *sd = &scene;
@@ -1073,6 +1132,9 @@ __device__ __forceinline__ void p2g2_walk(const BatchArgs &a, int *lds) {
             valid[k] = pos < ch.w;
             oi[k] = a.order[ch.z + min(pos, ch.w - 1)];
         }
+        if (threadIdx.x == 0) s_crowded = oi[0] >> 31;  // (position 0 carries the mark)
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) oi[k] &= ~kCrowdedBit;
         float x[PPT][3], C[PPT][3][3];
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
@@ -1084,22 +1146,41 @@ __device__ __forceinline__ void p2g2_walk(const BatchArgs &a, int *lds) {
         for (int l = threadIdx.x; l < WC; l += kBlock) {
             const int64_t gi = win_cell_index<TB>(l, t, sd);
             dm[l] = gi >= 0 ? decode(a.grid[gi].w, prm.mult) : 0.0f;
-            lds[WC + l] = 0;
-            lds[2 * WC + l] = 0;
-            lds[3 * WC + l] = 0;
         }
+        for (int l = threadIdx.x; l < Rep::stride2; l += kBlock) mom[l] = 0;
         __syncthreads();
+        const bool crowded = s_crowded != 0;
+        if (crowded) {  // (block-uniform) the other replicas too
+            for (int l = threadIdx.x; l < (Rep::count - 1) * Rep::stride2; l += kBlock)
+                mom[Rep::stride2 + l] = 0;
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             if (!valid[k]) continue;
             Node3 n;
             node3_init(x[k], n);
-            p2g2_particle(n, C[k], prm, WinSink2<TB>{lds, win_l0<TB>(x[k])});
+            p2g2_particle(n, C[k], prm,
+                          WinSink2<TB>{dm, mom + Rep::of_lane(crowded) * Rep::stride2,
+                                       win_l0<TB>(x[k])});
         }
         __syncthreads();
         int4 *out = a.partial + (size_t)cw.w.c * WC;
-        for (int l = threadIdx.x; l < WC; l += kBlock)
-            out[l] = make_int4(lds[WC + l], lds[2 * WC + l], lds[3 * WC + l], 0);
+        auto flush2 = [&](auto nrep) {
+            for (int l = threadIdx.x; l < WC; l += kBlock) {
+                int4 s = make_int4(0, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < decltype(nrep)::value; ++r) {
+                    const int *w = mom + r * Rep::stride2;
+                    s = add4(s, make_int4(w[l], w[WC + l], w[2 * WC + l], 0));
+                }
+                out[l] = s;
+            }
+        };
+        if (crowded)  // (block-uniform)
+            flush2(std::integral_constant<int, Rep::count>{});
+        else
+            flush2(std::integral_constant<int, 1>{});
         __syncthreads();
     }
     // Wild chunks (rare), in a loop of their own like k_p2g1's.
@@ -1124,12 +1205,14 @@ __device__ __forceinline__ void p2g2_walk(const BatchArgs &a, int *lds) {
 }
 
 template <int PPT, int TB>
-__global__ __launch_bounds__(kBlock) void k_p2g2(BatchArgs a) {
-    __shared__ int lds[4 * TileGeo<TB>::win_cells];
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
+void k_p2g2(BatchArgs a) {
+    __shared__ int lds[TileGeo<TB>::win_cells + WinRep<TB>::count * WinRep<TB>::stride2];
+    __shared__ uint32_t s_crowded;  // the current chunk is crowded (p2g_1's mark)
     if (a.nscenes == 1)
-        p2g2_walk<PPT, TB, true>(a, lds);
+        p2g2_walk<PPT, TB, true>(a, lds, s_crowded);
     else
-        p2g2_walk<PPT, TB, false>(a, lds);
+        p2g2_walk<PPT, TB, false>(a, lds, s_crowded);
 }
 
 // ---------------------------------------------------------------------------

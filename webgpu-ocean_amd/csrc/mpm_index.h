@@ -207,6 +207,24 @@ __host__ __device__ __forceinline__ int cover_slot(const int l[3], int s, int *q
     return ((l[0] + 1 - E * d[0]) * W + (l[1] + 1 - E * d[1])) * W + (l[2] + 1 - E * d[2]);
 }
 
+// Level order of a 4^3-tile chunk (the P2G kernels' lane assignment).  With
+// cnt[c] particles in tile cell c, levels 0..kLevelCap-1 are level-major: level
+// k holds the k-th particle of every cell with more than k, cells in index
+// order (mask[k]: those cells, base[k]: the level's first position).  A
+// crowded cell's particles beyond level kLevelCap-1 follow cell-major after
+// all the levels, from exoff[c].
+constexpr uint32_t kLevelCap = 32;
+
+// Position of the particle at level `lev` of tile cell `cell` (0..63).
+__host__ __device__ __forceinline__ uint32_t level_position(uint32_t lev, uint32_t cell,
+                                                           const uint32_t *base,
+                                                           const uint64_t *mask,
+                                                           const uint32_t *exoff) {
+    if (lev >= kLevelCap) return exoff[cell] + (lev - kLevelCap);
+    const uint64_t below = (1ull << cell) - 1ull;
+    return base[lev] + (uint32_t)__builtin_popcountll(mask[lev] & below);
+}
+
 // Slot where range x (1..kXcds-1) of the XCD split of `total` particles
 // starts: ceil(x * total / kXcds).
 __host__ __device__ __forceinline__ uint32_t xcd_split_slot(uint32_t x, uint32_t total) {
