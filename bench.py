@@ -253,8 +253,9 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
 # kernel behind each production time_stages slot (default tuning: 2 particles per thread;
 # G2P in the gather counting mode, which the 300k / 80^3 scene uses)
 SLOT_KERNEL = {"bin_gather": "k_bin_gather", "bin_scan": "k_scan_keys",
-               "bin_scatter": "k_bin_scatter", "p2g_1": "k_p2g1<2>", "reduce_1": "k_reduce<1>",
-               "p2g_2": "k_p2g2<2>", "reduce_2": "k_reduce<2>", "g2p": "k_g2p<2>"}
+               "bin_scatter": "k_bin_scatter", "p2g_1": "k_p2g1<2, 2>",
+               "reduce_1": "k_reduce<1, 2>", "p2g_2": "k_p2g2<2, 2>", "reduce_2": "k_reduce<2, 2>",
+               "g2p": "k_g2p<2, 2>"}
 
 
 def measured_traffic(slot: str):
