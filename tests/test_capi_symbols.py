@@ -101,3 +101,51 @@ def test_sph_no_device_is_reported_not_faked():
     with pytest.raises(sph.SPHError) as ei:
         sph.SPHSimulator()
     assert ei.value.status == 5
+
+
+RENDER_HEADER = os.path.join(ROOT, "include", "render.h")
+
+
+def render_declared_functions():
+    text = open(RENDER_HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(render_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_render_header_binding_and_library_agree():
+    import render
+    from render import _capi as render_capi
+    assert render_declared_functions() == sorted(render_capi.EXPORTED)
+    L = render.load()
+    out = subprocess.run(["nm", "-D", "--defined-only", render_capi.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(render_\w+)", out))
+    for name in render_declared_functions():
+        assert hasattr(L, name) and name in exported, name
+
+
+def test_render_arguments_are_checked_before_the_device():
+    import render
+    from render import _capi as render_capi
+    L = render.load()
+    out = ctypes.c_void_p()
+    prm = render_capi.Params(64, 48, 0.6, 0.785)
+    assert L.render_create(None, None, 0, ctypes.byref(out)) == 1
+    assert b"NULL" in L.render_last_error()
+    assert L.render_create(ctypes.byref(render_capi.Params(0, 48, 0.6, 0.785)), None, 0,
+                           ctypes.byref(out)) == 1
+    assert L.render_create(ctypes.byref(render_capi.Params(64, 48, 0.6, 4.0)), None, 0,
+                           ctypes.byref(out)) == 1
+    assert L.render_create(ctypes.byref(prm), None, 0, ctypes.byref(out)) == 1   # no cube map
+    assert L.render_execute(None, None, 0, None, 0) == 1
+    assert L.render_read_image(None, 0, None) == 1
+    assert L.render_destroy(None) == 0
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_render_no_device_is_reported_not_faked():
+    import numpy as np
+    import render
+    with pytest.raises(render._capi.RenderError) as ei:
+        render.FluidRenderer(64, 48, cubemap=np.ones((6, 2, 2, 4), np.float32))
+    assert ei.value.status == 5

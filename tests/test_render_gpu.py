@@ -1,0 +1,193 @@
+"""GPU parity of the screen-space fluid renderer (include/render.h, the
+render_* entry points of libmpm355) against the CPU oracle
+(oracle/render_oracle.c), pass by pass:
+
+  depth map            bit-exact (same fragments, same winner, same f32 math)
+  thickness (r16float) within one f16 ulp (float atomics add in another order)
+  filtered depth       the oracle's 4 x bilateral on the GPU's depth map,
+                       relative 1e-5 (expf of the device library vs glibc)
+  filtered thickness   the oracle's gaussian on the GPU's thickness, one f16 ulp
+  colour               the oracle's fluid pass on the GPU's filtered maps,
+                       absolute 2e-4 (expf / powf)
+  sphere mode colour   absolute 1e-6
+
+Scenes: the reference's startup (MLS-MPM dam break of 40,000 and 70,000
+particles with its camera, main.ts:128-196), rendered straight from the
+simulator's device PosVel buffer after some frames; a random cloud; empty and
+off-screen / behind-the-camera / NaN particles."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import render as R
+
+pytestmark = pytest.mark.gpu
+
+FOV = 45 * math.pi / 180
+RADIUS = 0.6
+
+
+@pytest.fixture(scope="module")
+def rmod():
+    import render
+    render.load()
+    return render
+
+
+def reference_camera(rmod, box, distance, w, h):
+    cam = rmod.Camera()
+    cam.reset(w, h, distance, [box[0] / 2, box[1] / 4, box[2] / 2], FOV)
+    return cam.uniforms(w, h, 2 * RADIUS)
+
+
+def f16_ulp(x):
+    return np.spacing(np.abs(x).astype(np.float16)).astype(np.float32)
+
+
+def check_frame(r, pv, n, u, cube, sphere=False, what=""):
+    """The GPU frame in `r` (already executed) against the oracle."""
+    w, h = r.width, r.height
+    c = R.consts(w, h, r.radius, r.fov)
+    want = R.render_frame(pv, n, u, c, cube, sphere, threads=16)
+    depth = r.read_image("depth")
+    bad = np.count_nonzero(depth.view(np.uint32) != want["depth"].view(np.uint32))
+    assert bad == 0, f"{what}: {bad} depth pixels differ"
+    color = r.read_image("color")
+    if sphere:
+        np.testing.assert_allclose(color, want["color"], rtol=0, atol=1e-6, err_msg=what)
+        return
+    thick = r.read_image("thickness")
+    assert np.all(np.abs(thick - want["thickness"]) <= f16_ulp(want["thickness"])), what
+    fd = r.read_image("filtered_depth")
+    ref = depth
+    for _ in range(4):
+        ref = R.bilateral(R.bilateral(ref, (1, 0), c, 16), (0, 1), c, 16)
+    np.testing.assert_allclose(fd, ref, rtol=1e-5, atol=0, err_msg=what + " filtered depth")
+    ft = r.read_image("filtered_thickness")
+    ref_t = R.gaussian(R.gaussian(thick, (1, 0), 16), (0, 1), 16)
+    assert np.all(np.abs(ft - ref_t) <= f16_ulp(ref_t)), what + " filtered thickness"
+    ref_c = R.fluid(u, fd, ft, cube, 16)
+    err = np.abs(color - ref_c)
+    assert err.max() <= 2e-4, (what, float(err.max()), np.unravel_index(err.argmax(), err.shape))
+    # end to end: the oracle's own frame, through its own filters
+    e2e = np.abs(color - want["color"])
+    assert np.mean(e2e > 1e-2) < 1e-3, (what, float(np.mean(e2e > 1e-2)))
+    return depth
+
+
+@pytest.mark.parametrize("preset", [0, 1])
+def test_reference_dam_break_rendered_from_the_simulator(rmod, preset):
+    from mlsmpm import MLSMPMSimulator
+    n = [40_000, 70_000][preset]
+    box = [[35, 25, 55], [40, 30, 60]][preset]
+    dist = [60, 70][preset]
+    w, h = 400, 240
+    cube = rmod.synthetic_cubemap(32)
+    u = reference_camera(rmod, box, dist, w, h)
+    with MLSMPMSimulator(init_box_size=box) as sim, \
+            rmod.FluidRenderer(w, h, RADIUS, FOV, cube) as r:
+        sim.reset(n, box)
+        for _ in range(20):
+            sim.execute(2)
+        sim.synchronize()
+        pv = sim.read_posvel()
+        r.execute(sim.posvel_device_ptr(), n, u)
+        depth = check_frame(r, pv, n, u, cube, what=f"dam break {n}")
+        assert np.count_nonzero(depth) > 0.05 * w * h
+        r.execute(sim.posvel_device_ptr(), n, u, sphere_render=True)
+        check_frame(r, pv, n, u, cube, sphere=True, what=f"dam break {n} spheres")
+
+
+def test_random_cloud_and_camera_orbit(rmod):
+    rng = np.random.default_rng(5)
+    n = 6000
+    pv = np.zeros((n, 8), dtype=np.float32)
+    pv[:, 0:3] = rng.uniform([2, 2, 2], [30, 20, 45], (n, 3))
+    pv[:, 4:7] = rng.normal(0, 1.0, (n, 3))
+    w, h = 320, 200
+    cube = rmod.synthetic_cubemap(16)
+    cam = rmod.Camera()
+    cam.reset(w, h, 60, [17.5, 6.25, 27.5], FOV)
+    with rmod.FluidRenderer(w, h, RADIUS, FOV, cube) as r:
+        for dx, dy in ((0, 0), (180, -40), (-400, 150)):
+            cam.orbit(dx, dy)
+            u = cam.uniforms(w, h, 2 * RADIUS)
+            r.execute(pv, n, u)
+            check_frame(r, pv, n, u, cube, what=f"cloud orbit {dx},{dy}")
+            r.execute(pv, n, u, sphere_render=True)
+            check_frame(r, pv, n, u, cube, sphere=True, what=f"cloud spheres {dx},{dy}")
+
+
+def test_no_particles_and_particles_that_draw_nothing(rmod):
+    w, h = 128, 96
+    cube = rmod.synthetic_cubemap(8)
+    cam = rmod.Camera()
+    cam.reset(w, h, 50, [0, 0, 0], FOV)
+    u = cam.uniforms(w, h, 2 * RADIUS)
+    V = cam.view.reshape(4, 4).T
+    eye = (np.linalg.inv(V) @ np.array([0, 0, 0, 1.0]))[:3]
+    fwd = -V[2, :3]
+    pos = [eye - 5 * fwd,                  # behind the camera
+           eye + 0.05 * fwd,               # nearer than the near plane
+           eye + 600 * fwd,                # beyond the far plane
+           [1e4, 0, 0], [0, -1e4, 0],      # off screen
+           [np.nan, 0, 0], [np.inf, 1, 1]]
+    pv = np.zeros((len(pos), 8), dtype=np.float32)
+    pv[:, 0:3] = np.array(pos, dtype=np.float32)
+    with rmod.FluidRenderer(w, h, RADIUS, FOV, cube) as r:
+        for n in (0, len(pos)):
+            r.execute(pv, n, u)
+            assert not r.read_image("depth").any() and not r.read_image("thickness").any()
+            col = r.read_image("color")
+            assert (col[..., :3] == np.float32(0.8)).all() and (col[..., 3] == 1).all()
+            check_frame(r, pv, n, u, cube, what=f"nothing drawn n={n}")
+        # one visible particle among them
+        pv2 = np.concatenate([pv, np.zeros((1, 8), np.float32)])
+        r.execute(pv2, len(pv2), u)
+        assert np.count_nonzero(r.read_image("depth")) > 0
+        check_frame(r, pv2, len(pv2), u, cube, what="one visible")
+
+
+def test_device_and_host_inputs_give_the_same_frame(rmod):
+    """The same records from the simulator's device buffer and from host
+    memory give the same frame; the output image keeps its device address."""
+    from mlsmpm import MLSMPMSimulator
+    box = [35, 25, 55]
+    w, h = 200, 120
+    cube = rmod.synthetic_cubemap(16)
+    u = reference_camera(rmod, box, 60, w, h)
+    with MLSMPMSimulator(init_box_size=box) as sim, \
+            rmod.FluidRenderer(w, h, RADIUS, FOV, cube) as r:
+        sim.reset(5000, box)
+        sim.execute(2)
+        sim.synchronize()
+        pv = sim.read_posvel()
+        r.execute(pv, len(pv), u)
+        a = {k: r.read_image(k) for k in ("depth", "filtered_depth", "thickness", "color")}
+        ptr = r.color_device_ptr()
+        r.execute(sim.posvel_device_ptr(), len(pv), u)
+        assert r.color_device_ptr() == ptr != 0
+        for k, v in a.items():
+            got = r.read_image(k)
+            if k in ("depth", "filtered_depth"):
+                np.testing.assert_array_equal(got, v, err_msg=k)
+            else:   # the float-atomic sums may round differently
+                np.testing.assert_allclose(got, v, rtol=0, atol=2e-3, err_msg=k)
+        assert r._L.render_execute(r._h, sim.posvel_device_ptr() + 4, 1, u.ctypes.data, 0) == 1
+        assert b"aligned" in r._L.render_last_error()
+
+
+def test_invalid_arguments_are_errors_not_crashes(rmod):
+    from render import _capi
+    u = np.zeros(68, np.float32)
+    with rmod.FluidRenderer(64, 48) as r:
+        buf = np.zeros(64 * 48 * 4, np.float32)
+        assert r._L.render_read_image(r._h, 9, buf.ctypes.data) == 1
+        assert b"image" in r._L.render_last_error()
+        assert r._L.render_execute(r._h, None, 5, u.ctypes.data, 0) == 1
+        assert r._L.render_execute(r._h, None, 0, None, 0) == 1
+        with pytest.raises(ValueError):
+            r.execute(np.zeros((2, 8), np.float32), 3, u)
+    with pytest.raises(_capi.RenderError):
+        rmod.FluidRenderer(0, 10)

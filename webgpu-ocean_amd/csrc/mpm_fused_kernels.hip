@@ -910,7 +910,6 @@ __device__ __forceinline__ void p2g1_walk(const BatchArgs &a, int *lds, LevelOrd
 template <int PPT, int TB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChunkWaves<PPT>)))
 void k_p2g1(BatchArgs a) {
-    constexpr int WC = TileGeo<TB>::win_cells;
     __shared__ int lds[WinRep<TB>::count * WinRep<TB>::stride1];  // [mass | vx | vy | vz] x replica
     __shared__ LevelOrder<PPT, TB> ord;
     if (a.nscenes == 1)
