@@ -26,12 +26,20 @@
  * impostors' depth map with a less-than depth test, 4 x (bilateral filter
  * along x, then y), the additive thickness map (r16float), 1 x (gaussian filter
  * along x, then y), and the fluid shading pass; or, in sphere mode, the
- * sphere pass.  Rasterisation is done as a per-particle splat: a fragment is
- * a pixel whose centre lies in the impostor's screen rectangle, left and top
- * edges included (the rectangle's top-left rule), with its uv interpolated
- * linearly (the quad is parallel to the image plane, so perspective-correct
- * interpolation is linear); ties of the depth test go to the lower particle
- * index (the earlier-drawn instance, as a raster pipeline orders them).
+ * sphere pass.  The impostors are rasterised in screen tiles, each tile
+ * walking its impostors in draw order: a fragment is a pixel whose centre
+ * lies in the impostor's screen rectangle, left and top edges included (the
+ * rectangle's top-left rule), with its uv interpolated linearly (the quad is
+ * parallel to the image plane, so perspective-correct interpolation is
+ * linear); ties of the depth test go to the lower particle index (the
+ * earlier-drawn instance); the thickness target blends one fragment at a
+ * time in draw order, rounding to f16 after each, as a raster pipeline's
+ * blend unit does.  Frames are deterministic.
+ *
+ * render_execute waits once per frame for the device (the number of
+ * (tile, impostor) pairs sizes the tile lists); the rest of the frame runs
+ * asynchronously.  A frame whose impostors touch more than 2^31 tile slots
+ * (a camera right at the particles) fails with RENDER_ERR_OUT_OF_MEMORY.
  */
 #ifndef RENDER355_H
 #define RENDER355_H
@@ -95,8 +103,9 @@ int render_destroy(render_ctx *r);
  * (fluidRender.ts:332): renders the first n PosVel records of the device
  * buffer `posvel_device` (n * 32 bytes, 16-byte aligned, e.g.
  * mpm_posvel_device_ptr) with the given uniforms.  Asynchronous on the
- * handle's stream; the buffer must be complete when called (synchronize its
- * producer first, e.g. mpm_synchronize). */
+ * handle's stream after one wait for the device (see above); the buffer must
+ * be complete when called (synchronize its producer first, e.g.
+ * mpm_synchronize). */
 int render_execute(render_ctx *r, const void *posvel_device, uint32_t n,
                    const render_uniforms *uniforms, int sphere_mode);
 

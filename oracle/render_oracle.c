@@ -16,9 +16,9 @@
  *     earlier draw); frag_depth is clamped to [0, 1]; impostors outside
  *     the clip volume (clip z not in [0, w]: nearer than the near plane or
  *     beyond the far plane) are not drawn;
- *   - the thickness map (r16float, additive blending) is summed in f32 in
- *     instance order and rounded to f16 once per pixel (a raster pipeline
- *     rounds after every blend);
+ *   - the thickness map (r16float target, additive blending) blends each
+ *     fragment in draw order: the f32 sum of the stored value and the
+ *     fragment's output, rounded to f16 (round to nearest even) and stored;
  *   - textureLoad coordinates are clamped into the image (Tint's
  *     robustness), u32 conversions of negative coordinates give 0;
  *   - the cubemap sample is the major-axis face, bilinear, clamp to edge.
@@ -438,7 +438,7 @@ void oracle_render_frame(const float *posvel, uint32_t n, const oracle_render_un
                     who[pi] = i;
                     out->depth[pi] = rvp[2];
                 }
-                thick[pi] += 0.05f * sqrtf(1.0f - r2);
+                thick[pi] = oracle_f16_round(thick[pi] + 0.05f * sqrtf(1.0f - r2));
             }
     }
     if (sphere_mode) {
@@ -480,7 +480,7 @@ void oracle_render_frame(const float *posvel, uint32_t n, const oracle_render_un
         oracle_bilateral_pass(tmp, out->filtered_depth, w, h, dy, c, threads);
     }
     /* thickness (r16float) and 1 x (gaussian x, gaussian y) */
-    for (size_t pi = 0; pi < np; ++pi) out->thickness[pi] = oracle_f16_round(thick[pi]);
+    memcpy(out->thickness, thick, np * sizeof(float));
     oracle_gaussian_pass(out->thickness, tmp, w, h, dx, threads);
     oracle_gaussian_pass(tmp, out->filtered_thickness, w, h, dy, threads);
     oracle_fluid_pass(u, out->filtered_depth, out->filtered_thickness, cubemap, cube_size, w, h,

@@ -186,6 +186,33 @@ def test_depth_test_keeps_the_nearest_and_ties_go_to_the_first():
     assert not np.array_equal(solo_slow["color"], solo_fast["color"])
 
 
+def test_thickness_blends_in_f16_one_fragment_at_a_time():
+    """The r16float target rounds after every additive blend: K fragments of
+    thickness c at one pixel give the sequential f16 sum, which falls short
+    of f16(K c) once the f16 spacing nears c."""
+    D, K = 5.0, 300
+    u = identity_uniforms()
+    c = R.consts(W, H, 0.6, FOV)
+    im = R.render_frame(records(np.tile([[0.0, 0.0, -D]], (K, 1))), K, u, c, face_cube())
+    P = u[20:36].reshape(4, 4).T.astype(np.float64)
+    half_w = 0.5 * 1.2 * P[0, 0] / D * W / 2
+    half_h = 0.5 * 1.2 * P[1, 1] / D * H / 2
+    y, x = H // 2, W // 2
+    r2 = (0.5 / half_w) ** 2 + (0.5 / half_h) ** 2
+    frag = 0.05 * math.sqrt(1 - r2)
+
+    def sequential(cval):
+        t = np.float16(0)
+        for _ in range(K):
+            t = np.float16(np.float32(t) + np.float32(cval))
+        return np.float32(t)
+
+    want = sequential(frag)
+    assert want == sequential(frag * (1 + 1e-6)) == sequential(frag * (1 - 1e-6))
+    assert im["thickness"][y, x] == want
+    assert want < np.float32(np.float16(K * frag)) - 0.25
+
+
 def test_sphere_mode_colours_by_speed_with_lambert_shading():
     u = identity_uniforms()
     c = R.consts(W, H, 0.6, FOV)

@@ -3,19 +3,24 @@
 // C ABI of include/render.h.
 //
 // A frame (FluidRenderer.execute, fluidRender.ts:424-484):
-//   k_clear      depth keys, thickness accumulators
-//   k_splat      one thread per particle walks the pixels of its impostor
-//                rectangle: every fragment adds its thickness (the additive
-//                thickness pass: no depth test, fluidRender.ts:130-149) with a
-//                float atomic, and enters the depth test as a 64-bit key
-//                (depth bits << 32 | particle index) through atomicMin -- the
-//                lowest depth wins, ties to the earlier instance, as the
-//                raster pipeline's less-than test in draw order
-//   k_resolve    per pixel: the winner's view-space z (the depth map, r32float)
-//                or, in sphere mode, its shaded colour; the thickness rounded
-//                to f16 (the r16float target)
-//   k_bilateral  4 x (along x, along y) on the depth map
-//   k_gaussian   1 x (along x, along y) on the thickness, r16float
+//   k_project    per particle: its impostor's screen rectangle and the number
+//                of kTile^2 screen tiles it touches
+//   scan         pair offsets (rocPRIM exclusive scan); the host reads the
+//                total to size the pair buffers (one wait per frame)
+//   k_emit       (tile, particle) pairs in particle order
+//   sort         stable radix sort of the pairs by tile (rocPRIM): every
+//                tile's list stays in particle (= draw) order
+//   k_ranges     each tile's run of the sorted pairs
+//   k_raster     one block per 8x8-pixel tile, one lane per pixel, over the
+//                tile's list: the depth test (lowest depth, ties to the
+//                earlier instance) and the additive r16float thickness blend
+//                in draw order (the depth map and thickness map passes draw
+//                the same quads, fluidRender.ts:424-448); in sphere mode the
+//                winner's shaded colour
+//   k_bilateral  4 x (along x, along y) on the depth map (spatial weights
+//                from a table made once by k_filter_weights)
+//   k_gaussian   1 x (along x, along y) on the thickness, r16float (weights
+//                from the same table)
 //   k_fluid      per pixel shading with the cubemap
 // Fragment rules (the rasteriser's choices, fixed here and in the oracle):
 // pixel centres inside [xl, xr) x [yt, yb) of the impostor's screen
@@ -29,6 +34,9 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "render.h"
 
@@ -53,7 +61,6 @@ int fail(int status, const std::string &msg) {
     } while (0)
 
 constexpr int kThreads = 256;
-constexpr uint64_t kNoFragment = ~0ull;
 
 struct Consts {
     int width, height;
@@ -197,87 +204,275 @@ __device__ __forceinline__ void value_to_color(float value, float out[3]) {
     for (int i = 0; i < 3; ++i) out[i] = col[k][i] * (1.0f - t) + col[k + 1][i] * t;
 }
 
-// ---- kernels ----
+// ---- impostor raster ----
+//
+// The depth map and thickness map passes draw the same quads (depthMap.wgsl,
+// thicknessMap.wgsl), so one raster serves both.  Screen tiles of kTile^2
+// pixels; each drawn impostor is listed in every tile its pixel rectangle
+// touches, the lists in particle order (a stable sort by tile of pairs
+// emitted in particle order), and one thread per pixel walks its tile's list:
+// the depth test keeps the first of the lowest depths, the thickness blends
+// in list order, i.e. draw order, as the raster pipeline's blend unit does.
 
-__global__ __launch_bounds__(kThreads) void k_clear(uint64_t *key, float *thick, size_t np) {
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < np;
-         i += (size_t)gridDim.x * kThreads) {
-        key[i] = kNoFragment;
-        thick[i] = 0.0f;
-    }
+constexpr int kTile = 8;
+constexpr int kTileShift = 3;
+static_assert(kTile * kTile == 64, "a raster tile is one wave's lanes");
+
+// Lower bound of the depth of any fragment of an impostor, less a margin, for
+// a projection whose clip z and w depend on view z alone (the perspective
+// matrix: entries 2, 3, 6, 7 zero, w = -z): a fragment's view z is at most
+// view z + radius (its normal's z is at most 1), and the depth decreases with
+// view z, up to a few ulps of rounding; the margin 2^-20 covers them.  With
+// another projection the bound is -1 (it never skips).
+__device__ __forceinline__ float depth_floor(const render_uniforms &u, const Impostor &im,
+                                             int standard_projection) {
+    if (!standard_projection) return -1.0f;
+    const float nxy[2] = {0.0f, 0.0f};
+    float rvp[3], nrm[3];
+    return frag_depth(u, im, nxy, 0.0f, rvp, nrm) - 0x1p-20f;
 }
 
-// Both impostor passes over the same fragments: the thickness adds (no depth
-// test in the reference's thickness pipeline) and the depth test.
-__global__ __launch_bounds__(kThreads) void k_splat(const float4 *__restrict__ posvel, uint32_t n,
-                                                    render_uniforms u, int w, int h,
-                                                    uint64_t *key, float *thick) {
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+struct TileRect {
+    int tx0, ty0, nx;
+    uint32_t count;
+};
+
+__device__ __forceinline__ TileRect tile_rect(const Impostor &im, bool drawn) {
+    TileRect r;
+    r.tx0 = im.px0 >> kTileShift;
+    r.ty0 = im.py0 >> kTileShift;
+    r.nx = ((im.px1 - 1) >> kTileShift) - r.tx0 + 1;
+    r.count = drawn ? (uint32_t)r.nx * (uint32_t)(((im.py1 - 1) >> kTileShift) - r.ty0 + 1) : 0u;
+    return r;
+}
+
+// Impostor record of a particle: its screen rectangle, its view position and
+// its depth floor, and the number of tiles its pixel rectangle touches
+// (count[n] = 0: the scan's total lands in offset[n]).
+__global__ __launch_bounds__(kThreads) void k_project(const float4 *__restrict__ posvel, uint32_t n,
+                                                      render_uniforms u, int w, int h,
+                                                      int standard_projection,
+                                                      float4 *__restrict__ rec,
+                                                      uint32_t *__restrict__ count) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i > n) return;
+    Impostor im{};
+    bool drawn = false;
+    if (i < n) {
         const float4 pv = posvel[2 * (size_t)i];
         const float p[3] = {pv.x, pv.y, pv.z};
-        Impostor im;
-        if (!impostor(u, p, w, h, im)) continue;
-        for (int py = im.py0; py < im.py1; ++py)
-            for (int px = im.px0; px < im.px1; ++px) {
-                float nxy[2], r2;
-                if (!fragment(im, px, py, nxy, r2)) continue;
-                const size_t pi = (size_t)py * w + px;
-                float rvp[3], nrm[3];
-                const float d = frag_depth(u, im, nxy, r2, rvp, nrm);
-                const uint64_t k = ((uint64_t)__float_as_uint(d) << 32) | i;
-                if (k < __hip_atomic_load(&key[pi], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    atomicMin((unsigned long long *)&key[pi], (unsigned long long)k);
-                unsafeAtomicAdd(&thick[pi], 0.05f * sqrtf(1.0f - r2));
-            }
+        drawn = impostor(u, p, w, h, im);
+        if (drawn) {
+            rec[2 * (size_t)i] = make_float4(im.xl, im.xr, im.yt, im.yb);
+            rec[2 * (size_t)i + 1] = make_float4(im.view[0], im.view[1], im.view[2],
+                                                 depth_floor(u, im, standard_projection));
+        }
+    }
+    count[i] = tile_rect(im, drawn).count;
+}
+
+// (tile, particle) pairs of every drawn impostor at its offset, tiles in
+// row-major order: the pair array is in particle order.
+__global__ __launch_bounds__(kThreads) void k_emit(const float4 *__restrict__ posvel, uint32_t n,
+                                                   render_uniforms u, int w, int h, int tiles_x,
+                                                   const uint64_t *__restrict__ offset,
+                                                   uint32_t *__restrict__ keys,
+                                                   uint32_t *__restrict__ vals) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    uint64_t o = offset[i];
+    if (offset[i + 1] == o) return;
+    const float4 pv = posvel[2 * (size_t)i];
+    const float p[3] = {pv.x, pv.y, pv.z};
+    Impostor im{};
+    impostor(u, p, w, h, im);
+    for (int ty = im.py0 >> kTileShift; ty <= (im.py1 - 1) >> kTileShift; ++ty)
+        for (int tx = im.px0 >> kTileShift; tx <= (im.px1 - 1) >> kTileShift; ++tx) {
+            keys[o] = (uint32_t)(ty * tiles_x + tx);
+            vals[o] = i;
+            ++o;
+        }
+}
+
+__global__ __launch_bounds__(kThreads) void k_clear_ranges(uint2 *__restrict__ range, uint32_t ntiles) {
+    for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < ntiles; t += gridDim.x * kThreads)
+        range[t] = make_uint2(0u, 0u);
+}
+
+// [begin, end) of every tile's run in the sorted pairs
+__global__ __launch_bounds__(kThreads) void k_ranges(const uint32_t *__restrict__ keys,
+                                                     uint32_t total, uint2 *__restrict__ range) {
+    for (uint32_t q = blockIdx.x * kThreads + threadIdx.x; q < total; q += gridDim.x * kThreads) {
+        const uint32_t k = keys[q];
+        if (q == 0 || keys[q - 1] != k) range[k].x = q;
+        if (q + 1 == total || keys[q + 1] != k) range[k].y = q + 1;
     }
 }
 
-// Per pixel: the depth map (the winner's view z; 0 = no fragment) or the
-// sphere pass's colour, and the thickness map as r16float.
-__global__ __launch_bounds__(kThreads) void k_resolve(const float4 *__restrict__ posvel,
-                                                      render_uniforms u, int w, int h,
-                                                      const uint64_t *key, const float *thick,
-                                                      float *depth, float *thick16, float4 *color,
-                                                      int sphere_mode) {
-    const size_t np = (size_t)w * h;
-    for (size_t pi = (size_t)blockIdx.x * kThreads + threadIdx.x; pi < np;
-         pi += (size_t)gridDim.x * kThreads) {
-        const uint64_t k = key[pi];
-        float z = 0.0f;
-        float4 c = make_float4(0.8f, 0.8f, 0.8f, 1.0f);
-        if (k != kNoFragment) {
-            const uint32_t i = (uint32_t)(k & 0xffffffffu);
-            const float4 pv = posvel[2 * (size_t)i];
-            const float p[3] = {pv.x, pv.y, pv.z};
-            Impostor im;
-            impostor(u, p, w, h, im);
-            float nxy[2], r2, rvp[3], nrm[3];
-            fragment(im, (int)(pi % w), (int)(pi / w), nxy, r2);
-            frag_depth(u, im, nxy, r2, rvp, nrm);
-            z = rvp[2];
-            if (sphere_mode) {
-                const float4 vv = posvel[2 * (size_t)i + 1];
-                const float v[3] = {vv.x, vv.y, vv.z};
-                const float speed = sqrtf(dot3(v, v));
-                const float one[3] = {1.0f, 1.0f, 1.0f};
-                float ln[3], col[3];
-                normalize3(one, ln);
-                const float dd = dot3(nrm, ln);
-                const float diffuse = dd > 0.0f ? dd : 0.0f;
-                value_to_color(speed / 1.5f, col);
-                c = make_float4(unorm(col[0] * diffuse), unorm(col[1] * diffuse),
-                                unorm(col[2] * diffuse), 1.0f);
-            }
+// Raster blocks: kRasterWaves waves per tile, every lane of a wave one pixel.
+// A tile's list (in particle order = draw order) is walked in chunks of kChunk
+// entries: each wave computes the fragments of its kChunkPer entries
+// (thickness contributions into LDS, 0 where an entry has no fragment, and its
+// own depth winner per pixel); then wave 0 blends the chunk's contributions in
+// list order.  The waves' winners merge at the end as (depth, particle index)
+// keys: the lowest depth, ties to the earlier instance, whichever wave saw it.
+constexpr int kRasterWaves = 8;
+constexpr int kRasterThreads = 64 * kRasterWaves;
+constexpr int kChunkPer = 32;
+constexpr int kChunk = kRasterWaves * kChunkPer;
+constexpr int kCStride = kChunk + 1;  // odd row stride: conflict-free in both phases
+
+// Fragment of record (rect, view) at pixel (px, py): 0 without a fragment,
+// else its thickness contribution; d, z, normal when the depth test may pass
+// (floor not above best_d), else d = 2.
+__device__ __forceinline__ float raster_fragment(const render_uniforms &u, const float4 rc,
+                                                 const float4 vw, int px, int py, float cx,
+                                                 float cy, float best_d, float &d, float &z,
+                                                 float nrm[3]) {
+    d = 2.0f;
+    if (!(cx >= rc.x && cx < rc.y && cy >= rc.z && cy < rc.w)) return 0.0f;
+    Impostor im;
+    im.xl = rc.x;
+    im.xr = rc.y;
+    im.yt = rc.z;
+    im.yb = rc.w;
+    float nxy[2], r2;
+    if (!fragment(im, px, py, nxy, r2)) return 0.0f;
+    const float nz = sqrtf(1.0f - r2);
+    if (!(vw.w > best_d)) {
+        im.view[0] = vw.x;
+        im.view[1] = vw.y;
+        im.view[2] = vw.z;
+        float rvp[3];
+        d = frag_depth(u, im, nxy, r2, rvp, nrm);
+        z = rvp[2];
+    }
+    return 0.05f * nz;
+}
+
+// One block per tile: the depth map (the winner's view z; 0 = no fragment) or
+// the sphere pass's colour, and the thickness map.
+__global__ __launch_bounds__(kRasterThreads) void k_raster(
+    const float4 *__restrict__ posvel, const float4 *__restrict__ rec,
+    const uint32_t *__restrict__ vals, const uint2 *__restrict__ range, render_uniforms u, int w,
+    int h, int tiles_x, float *__restrict__ depth, float *__restrict__ thick16,
+    float4 *__restrict__ color, int sphere_mode) {
+    __shared__ float s_c[64 * kCStride];
+    __shared__ float4 s_rect[kChunk], s_view[kChunk];
+    __shared__ uint32_t s_idx[kChunk];
+    __shared__ unsigned long long s_key[64];
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int tx = (int)(blockIdx.x % (uint32_t)tiles_x), ty = (int)(blockIdx.x / (uint32_t)tiles_x);
+    const int px = tx * kTile + (lane & (kTile - 1));
+    const int py = ty * kTile + (lane >> kTileShift);
+    const bool on = px < w && py < h;
+    const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
+    const uint2 rg = range[blockIdx.x];
+    if (wave == 0) s_key[lane] = ~0ull;
+    float best_d = 2.0f, t = 0.0f;
+    uint32_t best_i = 0xffffffffu;
+    for (uint32_t base = rg.x; base < rg.y; base += kChunk) {
+        const int m = rg.y - base < (uint32_t)kChunk ? (int)(rg.y - base) : kChunk;
+        __syncthreads();  // the previous chunk's blend is done with s_c
+        if ((int)threadIdx.x < m) {
+            const uint32_t i = vals[base + threadIdx.x];
+            s_idx[threadIdx.x] = i;
+            s_rect[threadIdx.x] = rec[2 * (size_t)i];
+            s_view[threadIdx.x] = rec[2 * (size_t)i + 1];
         }
-        depth[pi] = z;
-        thick16[pi] = sphere_mode ? 0.0f : f16_round(thick[pi]);
-        if (sphere_mode) color[pi] = c;
+        __syncthreads();
+        const int k1 = (wave + 1) * kChunkPer < m ? (wave + 1) * kChunkPer : m;
+        for (int k = wave * kChunkPer; k < k1; ++k) {
+            float d, z, nrm[3];
+            const float c = raster_fragment(u, s_rect[k], s_view[k], px, py, cx, cy, best_d, d,
+                                            z, nrm);
+            if (d < best_d) {
+                best_d = d;
+                best_i = s_idx[k];
+            }
+            s_c[lane * kCStride + k] = c;
+        }
+        __syncthreads();
+        if (wave == 0 && !sphere_mode) {
+            // the thickness target's blends, one fragment at a time in list order
+            // (adding 0 leaves an f16 value unchanged)
+            const float *row = s_c + lane * kCStride;
+            for (int k = 0; k < m; ++k) t = f16_round(t + row[k]);
+        }
+    }
+    if (best_i != 0xffffffffu)
+        atomicMin(&s_key[lane], ((unsigned long long)__float_as_uint(best_d) << 32) | best_i);
+    __syncthreads();
+    if (wave != 0 || !on) return;
+    const size_t pi = (size_t)py * w + px;
+    const unsigned long long key = s_key[lane];
+    float z = 0.0f, nrm[3] = {0.0f, 0.0f, 0.0f};
+    const uint32_t wi = (uint32_t)(key & 0xffffffffu);
+    if (key != ~0ull) {
+        // the winner's fragment again, for its view z and normal
+        float d;
+        raster_fragment(u, rec[2 * (size_t)wi], rec[2 * (size_t)wi + 1], px, py, cx, cy, 2.0f, d,
+                        z, nrm);
+    }
+    depth[pi] = z;
+    thick16[pi] = sphere_mode ? 0.0f : t;
+    if (sphere_mode) {
+        float4 c = make_float4(0.8f, 0.8f, 0.8f, 1.0f);
+        if (key != ~0ull) {
+            // sphere.wgsl: colour by speed, Lambert term towards normalize(1, 1, 1)
+            const float4 vv = posvel[2 * (size_t)wi + 1];
+            const float v[3] = {vv.x, vv.y, vv.z};
+            const float speed = sqrtf(dot3(v, v));
+            const float one[3] = {1.0f, 1.0f, 1.0f};
+            float ln[3], col[3];
+            normalize3(one, ln);
+            const float dd = dot3(nrm, ln);
+            const float diffuse = dd > 0.0f ? dd : 0.0f;
+            value_to_color(speed / 1.5f, col);
+            c = make_float4(unorm(col[0] * diffuse), unorm(col[1] * diffuse),
+                            unorm(col[2] * diffuse), 1.0f);
+        }
+        color[pi] = c;
+    }
+}
+
+// The filters' data-independent weights, evaluated once with the same
+// expressions (and so the same bits) as inline: the bilateral's spatial weight
+// expf(-rr / two_sigma) per filter size (1..100) and tap distance, the
+// gaussian's 61 weights (sigma 10) and their sum in tap order.
+constexpr int kMaxFilter = 100;
+constexpr int kBilateralTable = (kMaxFilter + 1) * (kMaxFilter + 1);
+constexpr int kGaussHalf = 30;
+constexpr int kGaussTable = 2 * kGaussHalf + 2;
+
+__global__ __launch_bounds__(kThreads) void k_filter_weights(float *__restrict__ bw,
+                                                             float *__restrict__ gw) {
+    for (int q = (int)threadIdx.x; q < kBilateralTable; q += kThreads) {
+        const int fs = q / (kMaxFilter + 1), k = q % (kMaxFilter + 1);
+        const float sigma = (float)fs / 3.0f;
+        const float two_sigma = 2.0f * sigma * sigma;
+        const float co = (float)k;
+        const float rr = co * co + co * co;
+        bw[q] = fs ? expf(-rr / two_sigma) : 0.0f;
+    }
+    const float sigma = (float)kGaussHalf / 3.0f;
+    const float two_sigma = 2.0f * sigma * sigma;
+    for (int k = (int)threadIdx.x; k <= 2 * kGaussHalf; k += kThreads) {
+        const float co = (float)(k - kGaussHalf);
+        gw[k] = expf(-co * co / two_sigma);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float wsum = 0.0f;
+        for (int k = 0; k <= 2 * kGaussHalf; ++k) wsum += gw[k];
+        gw[2 * kGaussHalf + 1] = wsum;
     }
 }
 
 // bilateral.wgsl:20-58 along `dir`
 __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict__ src,
                                                         float *__restrict__ dst, Consts c,
+                                                        const float *__restrict__ bw,
                                                         float dirx, float diry) {
     const int w = c.width, h = c.height;
     const size_t np = (size_t)w * h;
@@ -294,16 +489,14 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
         const int fsz = q >= 2147483648.0f ? 2147483647 : (int)q;
         const int mfs = (int)c.max_filter_size;
         const int filter_size = mfs < fsz ? mfs : fsz;
-        const float sigma = (float)filter_size / 3.0f;
-        const float two_sigma = 2.0f * sigma * sigma;
         const float sigma_depth = c.depth_threshold / 3.0f;
         const float two_sigma_depth = 2.0f * sigma_depth * sigma_depth;
+        const float *wrow = bw + filter_size * (kMaxFilter + 1);
         float sum = 0.0f, wsum = 0.0f;
         for (int k = -filter_size; k <= filter_size; ++k) {
             const float co = (float)k;
             const float sampled = fabsf(src[tex_index(iu + co * dirx, iv + co * diry, w, h)]);
-            const float rr = co * co + co * co;
-            const float wt = expf(-rr / two_sigma);
+            const float wt = wrow[k < 0 ? -k : k];
             const float r_depth = sampled - depth;
             const float wd = expf(-r_depth * r_depth / two_sigma_depth);
             sum += sampled * wt * wd;
@@ -316,8 +509,10 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
 // gaussian.wgsl:15-42 along `dir` (r16float target)
 __global__ __launch_bounds__(kThreads) void k_gaussian(const float *__restrict__ src,
                                                        float *__restrict__ dst, int w, int h,
+                                                       const float *__restrict__ gw,
                                                        float dirx, float diry) {
     const size_t np = (size_t)w * h;
+    const float wsum = gw[2 * kGaussHalf + 1];
     for (size_t pi = (size_t)blockIdx.x * kThreads + threadIdx.x; pi < np;
          pi += (size_t)gridDim.x * kThreads) {
         const int x = (int)(pi % w), y = (int)(pi / w);
@@ -327,16 +522,11 @@ __global__ __launch_bounds__(kThreads) void k_gaussian(const float *__restrict__
             dst[pi] = 0.0f;
             continue;
         }
-        const int filter_size = 30;
-        const float sigma = (float)filter_size / 3.0f;
-        const float two_sigma = 2.0f * sigma * sigma;
-        float sum = 0.0f, wsum = 0.0f;
-        for (int k = -filter_size; k <= filter_size; ++k) {
+        float sum = 0.0f;
+        for (int k = -kGaussHalf; k <= kGaussHalf; ++k) {
             const float co = (float)k;
             const float sampled = src[tex_index(iu + dirx * co, iv + diry * co, w, h)];
-            const float wt = expf(-co * co / two_sigma);
-            sum += sampled * wt;
-            wsum += wt;
+            sum += sampled * gw[k + kGaussHalf];
         }
         dst[pi] = f16_round(sum / wsum);
     }
@@ -492,45 +682,146 @@ struct render_ctx {
     hipStream_t stream = nullptr;
     Consts c{};
     int cube_size = 0;
+    int tiles_x = 0, tiles_y = 0;
+    int key_bits = 1;
     float4 *cube = nullptr;
-    uint64_t *key = nullptr;
-    float *thick = nullptr, *depth = nullptr, *fdepth = nullptr, *tmp = nullptr;
+    float *depth = nullptr, *fdepth = nullptr, *tmp = nullptr;
     float *thick16 = nullptr, *fthick = nullptr;
     float4 *color = nullptr;
-    float4 *staging = nullptr;  // host PosVel records (render_execute_host)
+    float *bilateral_w = nullptr;  // spatial weights per filter size and tap
+    float *gauss_w = nullptr;      // the gaussian's 61 weights and their sum
+    uint2 *range = nullptr;        // per tile [begin, end) in the sorted pairs
+    uint64_t *total = nullptr;     // pinned host word: the frame's pair count
+    // per particle (grown on demand)
+    float4 *rec = nullptr;         // 2 per particle
+    uint32_t *count = nullptr;     // n + 1
+    uint64_t *offset = nullptr;    // n + 1
+    uint32_t part_cap = 0;
+    // (tile, particle) pairs (grown on demand)
+    uint32_t *keys = nullptr, *vals = nullptr, *keys_sorted = nullptr, *vals_sorted = nullptr;
+    size_t pair_cap = 0;
+    void *scratch = nullptr;       // rocPRIM temporary storage
+    size_t scratch_bytes = 0;
+    float4 *staging = nullptr;     // host PosVel records (render_execute_host)
     uint32_t staging_cap = 0;
 };
 
 namespace {
 
+// Upper bound of a frame's (tile, particle) pairs: 2^31 (16 bytes each).
+constexpr uint64_t kMaxPairs = 1ull << 31;
+
+// The radix sort always takes rocPRIM's onesweep path (no merge sort below a
+// size limit): two 8-bit passes for up to 65536 tiles.
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+
 void release(render_ctx *r) {
-    void *ptrs[] = {r->cube, r->key, r->thick, r->depth, r->fdepth, r->tmp, r->thick16,
-                    r->fthick, r->color, r->staging};
+    void *ptrs[] = {r->cube, r->depth, r->fdepth, r->tmp, r->thick16, r->fthick, r->color,
+                    r->bilateral_w, r->gauss_w, r->range, r->rec, r->count, r->offset, r->keys,
+                    r->vals, r->keys_sorted, r->vals_sorted, r->scratch, r->staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (r->total) (void)hipHostFree(r->total);
     if (r->stream) (void)hipStreamDestroy(r->stream);
+}
+
+// Grows a device buffer to `bytes` (contents not kept).
+int grow(void **p, size_t bytes) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    R_TRY(hipMalloc(p, bytes));
+    return RENDER_OK;
+}
+
+#define R_CHECK(expr)                  \
+    do {                               \
+        const int rc_ = (expr);        \
+        if (rc_ != RENDER_OK) return rc_; \
+    } while (0)
+
+int ensure_scratch(render_ctx *r, size_t bytes) {
+    if (bytes <= r->scratch_bytes) return RENDER_OK;
+    R_TRY(hipStreamSynchronize(r->stream));  // the previous frame may still use it
+    r->scratch_bytes = 0;
+    R_CHECK(grow(&r->scratch, bytes));
+    r->scratch_bytes = bytes;
+    return RENDER_OK;
 }
 
 int frame(render_ctx *r, const float4 *pv, uint32_t n, const render_uniforms &u, int sphere) {
     const int w = r->c.width, h = r->c.height;
     const size_t np = (size_t)w * h;
     const unsigned pb = blocks_for(np);
+    const uint32_t ntiles = (uint32_t)(r->tiles_x * r->tiles_y);
     hipStream_t st = r->stream;
-    k_clear<<<pb, kThreads, 0, st>>>(r->key, r->thick, np);
-    if (n) k_splat<<<blocks_for(n), kThreads, 0, st>>>(pv, n, u, w, h, r->key, r->thick);
-    k_resolve<<<pb, kThreads, 0, st>>>(pv, u, w, h, r->key, r->thick, r->depth, r->thick16,
-                                       r->color, sphere);
+    if (n + 1 > r->part_cap) {
+        R_TRY(hipStreamSynchronize(st));
+        r->part_cap = 0;
+        R_CHECK(grow((void **)&r->rec, sizeof(float4) * 2 * ((size_t)n + 1)));
+        R_CHECK(grow((void **)&r->count, sizeof(uint32_t) * ((size_t)n + 1)));
+        R_CHECK(grow((void **)&r->offset, sizeof(uint64_t) * ((size_t)n + 1)));
+        r->part_cap = n + 1;
+    }
+    // The perspective form for which the depth floor holds (column-major):
+    // clip z and w do not depend on view x, y; w = -z exactly; depth
+    // -P10 - P14 / z decreases with z (P14 < 0); |P10| <= 2 bounds the rounding
+    // of each depth evaluation by 2.4e-7, within the floor's margin 2^-20.
+    const float *P = u.projection;
+    const int standard = P[2] == 0.0f && P[6] == 0.0f && P[3] == 0.0f && P[7] == 0.0f &&
+                         P[11] == -1.0f && P[15] == 0.0f && P[14] < 0.0f &&
+                         fabsf(P[10]) <= 2.0f;
+    // impostors, their tile counts and the pairs' offsets
+    k_project<<<n / kThreads + 1, kThreads, 0, st>>>(pv, n, u, w, h, standard, r->rec, r->count);
+    size_t bytes = 0;
+    R_TRY(rocprim::exclusive_scan(nullptr, bytes, r->count, r->offset, (uint64_t)0, (size_t)n + 1,
+                                  rocprim::plus<uint64_t>(), st));
+    R_CHECK(ensure_scratch(r, bytes));
+    R_TRY(rocprim::exclusive_scan(r->scratch, bytes, r->count, r->offset, (uint64_t)0,
+                                  (size_t)n + 1, rocprim::plus<uint64_t>(), st));
+    R_TRY(hipMemcpyAsync(r->total, r->offset + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    R_TRY(hipStreamSynchronize(st));
+    const uint64_t total = *r->total;
+    if (total > kMaxPairs)
+        return fail(RENDER_ERR_OUT_OF_MEMORY,
+                    "the impostors cover more than 2^31 tile slots (camera too close to the particles)");
+    if (total > r->pair_cap) {
+        const size_t cap = (size_t)total + total / 4 + 4096;
+        r->pair_cap = 0;
+        R_CHECK(grow((void **)&r->keys, 4 * cap));
+        R_CHECK(grow((void **)&r->vals, 4 * cap));
+        R_CHECK(grow((void **)&r->keys_sorted, 4 * cap));
+        R_CHECK(grow((void **)&r->vals_sorted, 4 * cap));
+        r->pair_cap = cap;
+    }
+    k_clear_ranges<<<blocks_for(ntiles), kThreads, 0, st>>>(r->range, ntiles);
+    if (total) {
+        k_emit<<<(n + kThreads - 1) / kThreads, kThreads, 0, st>>>(pv, n, u, w, h, r->tiles_x,
+                                                                   r->offset, r->keys, r->vals);
+        bytes = 0;
+        R_TRY(rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, r->keys, r->keys_sorted,
+                                                    r->vals, r->vals_sorted, (size_t)total, 0,
+                                                    r->key_bits, st));
+        R_CHECK(ensure_scratch(r, bytes));
+        R_TRY(rocprim::radix_sort_pairs<SortConfig>(r->scratch, bytes, r->keys, r->keys_sorted,
+                                                    r->vals, r->vals_sorted, (size_t)total, 0,
+                                                    r->key_bits, st));
+        k_ranges<<<blocks_for(total), kThreads, 0, st>>>(r->keys_sorted, (uint32_t)total, r->range);
+    }
+    k_raster<<<ntiles, kRasterThreads, 0, st>>>(pv, r->rec, r->vals_sorted, r->range, u, w, h,
+                                               r->tiles_x, r->depth, r->thick16, r->color, sphere);
     if (!sphere) {
         // 4 x (bilateral x, bilateral y) (fluidRender.ts:432-444)
         const float *src = r->depth;
         for (int it = 0; it < 4; ++it) {
-            k_bilateral<<<pb, kThreads, 0, st>>>(src, r->tmp, r->c, 1.0f, 0.0f);
-            k_bilateral<<<pb, kThreads, 0, st>>>(r->tmp, r->fdepth, r->c, 0.0f, 1.0f);
+            k_bilateral<<<pb, kThreads, 0, st>>>(src, r->tmp, r->c, r->bilateral_w, 1.0f, 0.0f);
+            k_bilateral<<<pb, kThreads, 0, st>>>(r->tmp, r->fdepth, r->c, r->bilateral_w, 0.0f,
+                                                 1.0f);
             src = r->fdepth;
         }
         // 1 x (gaussian x, gaussian y) (fluidRender.ts:450-462)
-        k_gaussian<<<pb, kThreads, 0, st>>>(r->thick16, r->tmp, w, h, 1.0f, 0.0f);
-        k_gaussian<<<pb, kThreads, 0, st>>>(r->tmp, r->fthick, w, h, 0.0f, 1.0f);
+        k_gaussian<<<pb, kThreads, 0, st>>>(r->thick16, r->tmp, w, h, r->gauss_w, 1.0f, 0.0f);
+        k_gaussian<<<pb, kThreads, 0, st>>>(r->tmp, r->fthick, w, h, r->gauss_w, 0.0f, 1.0f);
         k_fluid<<<pb, kThreads, 0, st>>>(r->fdepth, r->fthick, r->cube, r->cube_size, u, w, h,
                                          r->color);
     } else {
@@ -579,16 +870,28 @@ int render_create(const render_params *params, const float *cubemap, uint32_t cu
         if (e == hipSuccess) e = hipMalloc(p, bytes);
     };
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    r->tiles_x = (int)((params->width + kTile - 1) / kTile);
+    r->tiles_y = (int)((params->height + kTile - 1) / kTile);
+    const uint32_t ntiles = (uint32_t)(r->tiles_x * r->tiles_y);
     alloc((void **)&r->cube, cube_bytes);
-    alloc((void **)&r->key, np * sizeof(uint64_t));
-    alloc((void **)&r->thick, np * sizeof(float));
+    while (r->key_bits < 32 && (ntiles - 1) >> r->key_bits) ++r->key_bits;
+    alloc((void **)&r->range, ntiles * sizeof(uint2));
+    alloc((void **)&r->bilateral_w, sizeof(float) * kBilateralTable);
+    alloc((void **)&r->gauss_w, sizeof(float) * kGaussTable);
     alloc((void **)&r->depth, np * sizeof(float));
     alloc((void **)&r->fdepth, np * sizeof(float));
     alloc((void **)&r->tmp, np * sizeof(float));
     alloc((void **)&r->thick16, np * sizeof(float));
     alloc((void **)&r->fthick, np * sizeof(float));
     alloc((void **)&r->color, np * sizeof(float4));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&r->total, sizeof(uint64_t), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemcpy(r->cube, cubemap, cube_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_filter_weights<<<1, kThreads, 0, r->stream>>>(r->bilateral_w, r->gauss_w);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
     if (e != hipSuccess) {
         release(r);
         delete r;
