@@ -115,7 +115,13 @@ def calibration_copy(nbytes: int = CALIBRATION_BYTES) -> None:
 
 
 def _short_kernel(name: str) -> str:
-    n = name.replace("mpm::(anonymous namespace)::", "").replace("mpm::", "")
+    if "rocprim" in name:  # library kernels: named by the algorithm they serve
+        for part in ("onesweep_config", "radix_sort_block_sort", "merge_sort", "scan_config",
+                     "init_lookback", "histogram", "transform"):
+            if part in name:
+                return "rocprim " + part.replace("_config", "")
+        return "rocprim"
+    n = name.replace("(anonymous namespace)::", "").replace("mpm::", "")
     return n.split("(")[0].replace("void ", "").strip()
 
 
@@ -604,6 +610,71 @@ def bench_sph(frames: int = 500, cpu: bool = True) -> dict:
     return out
 
 
+def bench_render(frames: int = 200, cpu: bool = True) -> dict:
+    """Screen-space renderer (render/fluidRender.ts): the reference's startup
+    scene (MLS-MPM dam break of 70,000 particles in a 40 x 30 x 60 box, camera
+    at distance 70 looking at (20, 7.5, 30), fov 45 deg, main.ts:128-196) and
+    the 300k dam break of the headline workload (camera at distance 140), each
+    after 100 simulated frames, rendered `frames` times from the simulator's
+    device PosVel buffer (fluid and sphere modes)."""
+    import math
+
+    import numpy as np
+
+    import render
+    from mlsmpm import MLSMPMSimulator, scenes
+
+    fov = 45 * math.pi / 180
+    cube = render.synthetic_cubemap(64)
+    out = {}
+    cases = (("ref70k", 70000, [40.0, 30.0, 60.0], 70.0), ("dam300k", 300000, None, 140.0))
+    for label, count, box, dist in cases:
+        if box is None:
+            rec, box = scenes.config1(n=count)
+        else:
+            rec = scenes.init_dambreak(box, count)
+        with MLSMPMSimulator(particles=rec, init_box_size=box) as sim:
+            for _ in range(100):
+                sim.execute()
+            sim.synchronize()
+            ptr = sim.posvel_device_ptr()
+            pv = sim.read_posvel() if (cpu and label == "ref70k") else None
+            for w, h in ((1280, 720), (1920, 1080)):
+                cam = render.Camera()
+                cam.reset(w, h, dist, [box[0] / 2, box[1] / 4, box[2] / 2], fov)
+                u = cam.uniforms(w, h, 1.2)
+                key = f"{label} {w}x{h}"
+                with render.FluidRenderer(w, h, 0.6, fov, cube) as r:
+                    entry = {}
+                    for mode, sphere in (("fluid", False), ("spheres", True)):
+                        r.time_frames(ptr, count, u, 10, sphere)
+                        ms = r.time_frames(ptr, count, u, frames, sphere) / frames
+                        entry[mode] = {"ms_per_frame": ms, "frames_per_s": 1e3 / ms}
+                    r.execute(ptr, count, u)
+                    d = r.read_image("depth")
+                    entry["covered_pixel_fraction"] = float(np.count_nonzero(d) / d.size)
+                    out[key] = entry
+                if pv is not None and (w, h) == (1280, 720):
+                    from oracle import render as orender
+                    c = orender.consts(w, h, 0.6, fov)
+                    threads, _ = cpu_threads()
+                    reps = []
+                    for _ in range(3):
+                        t0 = time.perf_counter()
+                        orender.render_frame(pv, count, u, c, cube, False, threads)
+                        reps.append(time.perf_counter() - t0)
+                    sec = sorted(reps)[1]
+                    out[key]["cpu_baseline"] = {
+                        "value": 1.0 / sec, "unit": "frames/s", "cores": threads, "kind": "port",
+                        "sample": f"median of 3 oracle frames of the same scene at {w}x{h} "
+                                  f"(fluid mode; the impostor splat serial, the filters and "
+                                  f"shading on {threads} threads)"}
+    out["workload"] = ("frames of FluidRenderer.execute from the MLS-MPM PosVel buffer: tiled "
+                       "impostor raster (depth test + r16float thickness blend in draw order), "
+                       "4 x bilateral, 1 x gaussian, fluid shading; synthetic cube map")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -618,6 +689,8 @@ def main() -> int:
                          "for profiling them)")
     ap.add_argument("--sph-only", action="store_true",
                     help="run only the SPH section (--steps frames; one JSON line)")
+    ap.add_argument("--render-only", action="store_true",
+                    help="run only the renderer section (--steps frames; one JSON line)")
     ap.add_argument("--stage-substeps", type=int, default=0,
                     help="substeps of the per-kernel timing replay (default: all of the timed "
                          "frames)")
@@ -658,6 +731,10 @@ def main() -> int:
     if args.sph_only:
         print(json.dumps({"sph": bench_sph(frames=args.steps, cpu=not args.no_cpu_baseline)}),
               flush=True)
+        return 0
+    if args.render_only:
+        print(json.dumps({"render": bench_render(frames=args.steps,
+                                                 cpu=not args.no_cpu_baseline)}), flush=True)
         return 0
     if args.extras_only:
         print(json.dumps({"batched": bench_batched(), "adjoint": bench_adjoint()}), flush=True)
@@ -753,6 +830,7 @@ def main() -> int:
         out["batched"] = bench_batched()
         out["adjoint"] = bench_adjoint()
         out["sph"] = bench_sph(cpu=not args.no_cpu_baseline)
+        out["render"] = bench_render(cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rec, box)
     print(json.dumps(out), flush=True)
