@@ -12,9 +12,10 @@ render_* entry points of libmpm355) against the CPU oracle
   sphere mode colour   absolute 1e-6
 
 Scenes: the reference's startup (MLS-MPM dam break of 40,000 and 70,000
-particles with its camera, main.ts:128-196), rendered straight from the
-simulator's device PosVel buffer after some frames; a random cloud; empty and
-off-screen / behind-the-camera / NaN particles."""
+particles with its camera, main.ts:128-196) and its SPH settings (the
+second renderer, radius 0.04), rendered straight from the simulators' device
+PosVel buffers after some frames; a random cloud; other projections; crowded
+tiles; empty and off-screen / behind-the-camera / NaN particles."""
 import math
 
 import numpy as np
@@ -98,6 +99,35 @@ def test_reference_dam_break_rendered_from_the_simulator(rmod, preset):
         assert np.count_nonzero(depth) > 0.05 * w * h
         r.execute(sim.posvel_device_ptr(), n, u, sphere_render=True)
         check_frame(r, pv, n, u, cube, sphere=True, what=f"dam break {n} spheres")
+
+
+@pytest.mark.parametrize("setting", [0, 3])
+def test_sph_frame_rendered_from_the_simulator(rmod, setting):
+    """The reference's second renderer instance (radius 0.04, main.ts:142-149)
+    over the SPH simulator's device PosVel buffer, its camera as on the SPH
+    buttons (distance 2.6-3.8, target (0, -half_y + 0.1, 0), main.ts:231-234),
+    sphere size = the SPH diameter 0.08."""
+    import sph
+    half = sph.scenes.INIT_HALF_BOXES[setting]
+    dist = [2.6, 3.0, 3.4, 3.8][setting]
+    rec = sph.scenes.dam_break(setting, seed=1)
+    w, h = 320, 200
+    cube = rmod.synthetic_cubemap(16)
+    cam = rmod.Camera()
+    cam.reset(w, h, dist, [0.0, -half[1] + 0.1, 0.0], FOV)
+    u = cam.uniforms(w, h, 0.08)
+    with sph.SPHSimulator(particles=rec, init_half_box_size=half) as sim, \
+            rmod.FluidRenderer(w, h, 0.04, FOV, cube) as r:
+        for _ in range(20):
+            sim.execute()
+        sim.synchronize()
+        pv = sim.read_posvel()
+        n = len(pv)
+        r.execute(sim.posvel_device_ptr(), n, u)
+        depth = check_frame(r, pv, n, u, cube, what=f"sph setting {setting}")
+        assert np.count_nonzero(depth) > 0.02 * w * h
+        r.execute(sim.posvel_device_ptr(), n, u, sphere_render=True)
+        check_frame(r, pv, n, u, cube, sphere=True, what=f"sph setting {setting} spheres")
 
 
 def test_random_cloud_and_camera_orbit(rmod):

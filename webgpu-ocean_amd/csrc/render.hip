@@ -39,6 +39,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "render.h"
+#include "render_math.h"
 
 #pragma clang fp contract(off)
 
@@ -147,17 +148,6 @@ __device__ __forceinline__ bool impostor(const render_uniforms &u, const float p
     im.py0 = (int)c2;
     im.py1 = (int)d;
     return true;
-}
-
-__device__ __forceinline__ bool fragment(const Impostor &im, int px, int py, float nxy[2],
-                                         float &r2) {
-    const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
-    const float uvx = (cx - im.xl) / (im.xr - im.xl);
-    const float uvy = (im.yb - cy) / (im.yb - im.yt);
-    nxy[0] = uvx * 2.0f - 1.0f;
-    nxy[1] = uvy * 2.0f - 1.0f;
-    r2 = nxy[0] * nxy[0] + nxy[1] * nxy[1];
-    return !(r2 > 1.0f);
 }
 
 __device__ __forceinline__ float frag_depth(const render_uniforms &u, const Impostor &im,
@@ -322,24 +312,34 @@ constexpr int kChunkPer = 32;
 constexpr int kChunk = kRasterWaves * kChunkPer;
 constexpr int kCStride = kChunk + 1;  // odd row stride: conflict-free in both phases
 
-// Fragment of record (rect, view) at pixel (px, py): 0 without a fragment,
-// else its thickness contribution; d, z, normal when the depth test may pass
-// (floor not above best_d), else d = 2.
+// The rectangle's extents and their rounded reciprocals, (xr - xl, 1 / that,
+// yb - yt, 1 / that), for div_rn.
+__device__ __forceinline__ float4 rect_extents(const float4 rc) {
+    const float bx = rc.y - rc.x, by = rc.w - rc.z;
+    return make_float4(bx, 1.0f / bx, by, 1.0f / by);
+}
+
+// Fragment of record (rect, view) at pixel centre (cx, cy): 0 without a
+// fragment, else its thickness contribution; d, z, normal when the depth test
+// may pass (floor not above bound), else d = 2.  uv = ((cx - xl) / (xr - xl),
+// (yb - cy) / (yb - yt)), the quotients correctly rounded (div_rn from the
+// extents' reciprocals); n.xy = 2 uv - 1; no fragment where |n.xy|^2 > 1.
 __device__ __forceinline__ float raster_fragment(const render_uniforms &u, const float4 rc,
-                                                 const float4 vw, int px, int py, float cx,
-                                                 float cy, float best_d, float &d, float &z,
+                                                 const float4 ext, const float4 vw, float cx,
+                                                 float cy, float bound, float &d, float &z,
                                                  float nrm[3]) {
     d = 2.0f;
     if (!(cx >= rc.x && cx < rc.y && cy >= rc.z && cy < rc.w)) return 0.0f;
-    Impostor im;
-    im.xl = rc.x;
-    im.xr = rc.y;
-    im.yt = rc.z;
-    im.yb = rc.w;
-    float nxy[2], r2;
-    if (!fragment(im, px, py, nxy, r2)) return 0.0f;
+    const float uvx = div_rn(cx - rc.x, ext.x, ext.y);
+    const float uvy = div_rn(rc.w - cy, ext.z, ext.w);
+    float nxy[2];
+    nxy[0] = uvx * 2.0f - 1.0f;
+    nxy[1] = uvy * 2.0f - 1.0f;
+    const float r2 = nxy[0] * nxy[0] + nxy[1] * nxy[1];
+    if (r2 > 1.0f) return 0.0f;
     const float nz = sqrtf(1.0f - r2);
-    if (!(vw.w > best_d)) {
+    Impostor im;
+    if (!(vw.w > bound)) {
         im.view[0] = vw.x;
         im.view[1] = vw.y;
         im.view[2] = vw.z;
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(
     int h, int tiles_x, float *__restrict__ depth, float *__restrict__ thick16,
     float4 *__restrict__ color, int sphere_mode) {
     __shared__ float s_c[64 * kCStride];
-    __shared__ float4 s_rect[kChunk], s_view[kChunk];
+    __shared__ float4 s_rect[kChunk], s_ext[kChunk], s_view[kChunk];
     __shared__ uint32_t s_idx[kChunk];
     __shared__ unsigned long long s_key[64];
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
@@ -369,30 +369,42 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(
     const float cx = (float)px + 0.5f, cy = (float)py + 0.5f;
     const uint2 rg = range[blockIdx.x];
     if (wave == 0) s_key[lane] = ~0ull;
-    float best_d = 2.0f, t = 0.0f;
+    // best_d, best_i: this wave's winner (its entries in list order, so the
+    // strict test keeps the first of equal depths); bound: the lowest depth
+    // any wave has found, for the depth floor's skip only (a skipped
+    // fragment is deeper than a found one: it can neither win nor tie)
+    float best_d = 2.0f, bound = 2.0f, t = 0.0f;
     uint32_t best_i = 0xffffffffu;
     for (uint32_t base = rg.x; base < rg.y; base += kChunk) {
         const int m = rg.y - base < (uint32_t)kChunk ? (int)(rg.y - base) : kChunk;
         __syncthreads();  // the previous chunk's blend is done with s_c
         if ((int)threadIdx.x < m) {
             const uint32_t i = vals[base + threadIdx.x];
+            const float4 rc = rec[2 * (size_t)i];
             s_idx[threadIdx.x] = i;
-            s_rect[threadIdx.x] = rec[2 * (size_t)i];
+            s_rect[threadIdx.x] = rc;
+            s_ext[threadIdx.x] = rect_extents(rc);
             s_view[threadIdx.x] = rec[2 * (size_t)i + 1];
         }
         __syncthreads();
         const int k1 = (wave + 1) * kChunkPer < m ? (wave + 1) * kChunkPer : m;
         for (int k = wave * kChunkPer; k < k1; ++k) {
             float d, z, nrm[3];
-            const float c = raster_fragment(u, s_rect[k], s_view[k], px, py, cx, cy, best_d, d,
-                                            z, nrm);
+            const float c = raster_fragment(u, s_rect[k], s_ext[k], s_view[k], cx, cy,
+                                            fminf(best_d, bound), d, z, nrm);
             if (d < best_d) {
                 best_d = d;
                 best_i = s_idx[k];
             }
             s_c[lane * kCStride + k] = c;
         }
+        if (best_i != 0xffffffffu)
+            atomicMin(&s_key[lane], ((unsigned long long)__float_as_uint(best_d) << 32) | best_i);
         __syncthreads();
+        {
+            const unsigned long long kb = s_key[lane];
+            if (kb != ~0ull) bound = fminf(bound, __uint_as_float((uint32_t)(kb >> 32)));
+        }
         if (wave == 0 && !sphere_mode) {
             // the thickness target's blends, one fragment at a time in list order
             // (adding 0 leaves an f16 value unchanged)
@@ -410,9 +422,10 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(
     const uint32_t wi = (uint32_t)(key & 0xffffffffu);
     if (key != ~0ull) {
         // the winner's fragment again, for its view z and normal
+        const float4 rc = rec[2 * (size_t)wi];
         float d;
-        raster_fragment(u, rec[2 * (size_t)wi], rec[2 * (size_t)wi + 1], px, py, cx, cy, 2.0f, d,
-                        z, nrm);
+        raster_fragment(u, rc, rect_extents(rc), rec[2 * (size_t)wi + 1], cx, cy, 2.0f, d, z,
+                        nrm);
     }
     depth[pi] = z;
     thick16[pi] = sphere_mode ? 0.0f : t;
@@ -435,6 +448,11 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(
         color[pi] = c;
     }
 }
+
+// Per-pixel kernels: 2D blocks of kPixW x kPixH pixels (x, y straight from the
+// block and thread indices).
+constexpr int kPixW = 64, kPixH = 4;
+static_assert(kPixW * kPixH == kThreads, "per-pixel blocks");
 
 // The filters' data-independent weights, evaluated once with the same
 // expressions (and so the same bits) as inline: the bilateral's spatial weight
@@ -475,35 +493,35 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
                                                         const float *__restrict__ bw,
                                                         float dirx, float diry) {
     const int w = c.width, h = c.height;
-    const size_t np = (size_t)w * h;
-    for (size_t pi = (size_t)blockIdx.x * kThreads + threadIdx.x; pi < np;
-         pi += (size_t)gridDim.x * kThreads) {
-        const int x = (int)(pi % w), y = (int)(pi / w);
-        const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
-        const float depth = fabsf(src[tex_index(iu, iv, w, h)]);
-        if (depth >= 1e4f || depth <= 0.0f) {
-            dst[pi] = depth;
-            continue;
-        }
-        const float q = ceilf(c.projected_particle_constant / depth);
-        const int fsz = q >= 2147483648.0f ? 2147483647 : (int)q;
-        const int mfs = (int)c.max_filter_size;
-        const int filter_size = mfs < fsz ? mfs : fsz;
-        const float sigma_depth = c.depth_threshold / 3.0f;
-        const float two_sigma_depth = 2.0f * sigma_depth * sigma_depth;
-        const float *wrow = bw + filter_size * (kMaxFilter + 1);
-        float sum = 0.0f, wsum = 0.0f;
-        for (int k = -filter_size; k <= filter_size; ++k) {
-            const float co = (float)k;
-            const float sampled = fabsf(src[tex_index(iu + co * dirx, iv + co * diry, w, h)]);
-            const float wt = wrow[k < 0 ? -k : k];
-            const float r_depth = sampled - depth;
-            const float wd = expf(-r_depth * r_depth / two_sigma_depth);
-            sum += sampled * wt * wd;
-            wsum += wt * wd;
-        }
-        dst[pi] = sum / wsum;
+    const int x = (int)(blockIdx.x * kPixW + threadIdx.x);
+    const int y = (int)(blockIdx.y * kPixH + threadIdx.y);
+    if (x >= w || y >= h) return;
+    const size_t pi = (size_t)y * w + x;
+    const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
+    const float depth = fabsf(src[tex_index(iu, iv, w, h)]);
+    if (depth >= 1e4f || depth <= 0.0f) {
+        dst[pi] = depth;
+        return;
     }
+    const float q = ceilf(c.projected_particle_constant / depth);
+    const int fsz = q >= 2147483648.0f ? 2147483647 : (int)q;
+    const int mfs = (int)c.max_filter_size;
+    const int filter_size = mfs < fsz ? mfs : fsz;
+    const float sigma_depth = c.depth_threshold / 3.0f;
+    const float two_sigma_depth = 2.0f * sigma_depth * sigma_depth;
+    const float *wrow = bw + filter_size * (kMaxFilter + 1);
+    float sum = 0.0f, wsum = 0.0f;
+#pragma unroll 4
+    for (int k = -filter_size; k <= filter_size; ++k) {
+        const float co = (float)k;
+        const float sampled = fabsf(src[tex_index(iu + co * dirx, iv + co * diry, w, h)]);
+        const float wt = wrow[k < 0 ? -k : k];
+        const float r_depth = sampled - depth;
+        const float wd = expf(-r_depth * r_depth / two_sigma_depth);
+        sum += sampled * wt * wd;
+        wsum += wt * wd;
+    }
+    dst[pi] = sum / wsum;
 }
 
 // gaussian.wgsl:15-42 along `dir` (r16float target)
@@ -511,25 +529,25 @@ __global__ __launch_bounds__(kThreads) void k_gaussian(const float *__restrict__
                                                        float *__restrict__ dst, int w, int h,
                                                        const float *__restrict__ gw,
                                                        float dirx, float diry) {
-    const size_t np = (size_t)w * h;
-    const float wsum = gw[2 * kGaussHalf + 1];
-    for (size_t pi = (size_t)blockIdx.x * kThreads + threadIdx.x; pi < np;
-         pi += (size_t)gridDim.x * kThreads) {
-        const int x = (int)(pi % w), y = (int)(pi / w);
-        const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
-        const float t = src[tex_index(iu, iv, w, h)];
-        if (t == 0.0f) {
-            dst[pi] = 0.0f;
-            continue;
-        }
-        float sum = 0.0f;
-        for (int k = -kGaussHalf; k <= kGaussHalf; ++k) {
-            const float co = (float)k;
-            const float sampled = src[tex_index(iu + dirx * co, iv + diry * co, w, h)];
-            sum += sampled * gw[k + kGaussHalf];
-        }
-        dst[pi] = f16_round(sum / wsum);
+    const int x = (int)(blockIdx.x * kPixW + threadIdx.x);
+    const int y = (int)(blockIdx.y * kPixH + threadIdx.y);
+    if (x >= w || y >= h) return;
+    const size_t pi = (size_t)y * w + x;
+    const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
+    const float t = src[tex_index(iu, iv, w, h)];
+    if (t == 0.0f) {
+        dst[pi] = 0.0f;
+        return;
     }
+    const float wsum = gw[2 * kGaussHalf + 1];
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = -kGaussHalf; k <= kGaussHalf; ++k) {
+        const float co = (float)k;
+        const float sampled = src[tex_index(iu + dirx * co, iv + diry * co, w, h)];
+        sum += sampled * gw[k + kGaussHalf];
+    }
+    dst[pi] = f16_round(sum / wsum);
 }
 
 // textureSampleLevel(envmap, linear sampler, dir, 0): the major-axis face,
@@ -593,81 +611,80 @@ __global__ __launch_bounds__(kThreads) void k_fluid(const float *__restrict__ de
                                                     const float4 *__restrict__ cube, int cube_size,
                                                     render_uniforms u, int w, int h,
                                                     float4 *__restrict__ color) {
-    const size_t np = (size_t)w * h;
-    for (size_t pi = (size_t)blockIdx.x * kThreads + threadIdx.x; pi < np;
-         pi += (size_t)gridDim.x * kThreads) {
-        const int x = (int)(pi % w), y = (int)(pi / w);
-        const float bg[3] = {0.8f, 0.8f, 0.8f};
-        const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
-        const float uvx = iu / (float)w, uvy = iv / (float)h;
-        const float depth = fabsf(depth_map[tex_index(iu, iv, w, h)]);
-        if (depth >= 1e4f || depth <= 0.0f) {
-            color[pi] = make_float4(bg[0], bg[1], bg[2], 1.0f);
-            continue;
-        }
-        float vp[3];
-        view_pos_from_uv_depth(u, uvx, uvy, depth, vp);
-        const float tsx = u.texel_size[0], tsy = u.texel_size[1];
-        float p[3], ddx[3], ddy[3], ddx2[3], ddy2[3];
-        view_pos_from_uv_depth(u, uvx + tsx, uvy + 0.0f,
-                               fabsf(depth_map[tex_index(iu + 1.0f, iv + 0.0f, w, h)]), p);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ddx[i] = p[i] - vp[i];
-        view_pos_from_uv_depth(u, uvx + 0.0f, uvy + tsy,
-                               fabsf(depth_map[tex_index(iu + 0.0f, iv + 1.0f, w, h)]), p);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ddy[i] = p[i] - vp[i];
-        view_pos_from_uv_depth(u, uvx + -tsx, uvy + 0.0f,
-                               fabsf(depth_map[tex_index(iu + -1.0f, iv + 0.0f, w, h)]), p);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ddx2[i] = vp[i] - p[i];
-        view_pos_from_uv_depth(u, uvx + 0.0f, uvy + -tsy,
-                               fabsf(depth_map[tex_index(iu + 0.0f, iv + -1.0f, w, h)]), p);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ddy2[i] = vp[i] - p[i];
-        if (fabsf(ddx[2]) > fabsf(ddx2[2]))
-            for (int i = 0; i < 3; ++i) ddx[i] = ddx2[i];
-        if (fabsf(ddy[2]) > fabsf(ddy2[2]))
-            for (int i = 0; i < 3; ++i) ddy[i] = ddy2[i];
-        const float cr[3] = {ddx[1] * ddy[2] - ddx[2] * ddy[1], ddx[2] * ddy[0] - ddx[0] * ddy[2],
-                             ddx[0] * ddy[1] - ddx[1] * ddy[0]};
-        float normal[3];
-        normalize3(cr, normal);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) normal[i] = -normal[i];
-        float ray[3];
-        normalize3(vp, ray);
-        const float ld4[4] = {0.0f, 0.0f, -1.0f, 0.0f};
-        float l4[4], light[3];
-        mv4(u.view, ld4, l4);
-        normalize3(l4, light);
-        const float hv[3] = {light[0] - ray[0], light[1] - ray[1], light[2] - ray[2]};
-        float H[3];
-        normalize3(hv, H);
-        const float hn = dot3(H, normal);
-        const float specular = powf(hn > 0.0f ? hn : 0.0f, 250.0f);
-        const float density = 1.5f;
-        const float thickness = thick[tex_index(iu, iv, w, h)];
-        const float dc[3] = {0.085f, 0.6375f, 0.9f};
-        float refr[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) refr[i] = bg[i] * expf(-density * thickness * (1.0f - dc[i]));
-        const float F0 = 0.02f;
-        const float nr[3] = {-ray[0], -ray[1], -ray[2]};
-        float fr = F0 + (1.0f - F0) * powf(1.0f - dot3(normal, nr), 5.0f);
-        fr = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
-        const float dn = dot3(normal, ray);
-        const float refl[4] = {ray[0] - 2.0f * dn * normal[0], ray[1] - 2.0f * dn * normal[1],
-                               ray[2] - 2.0f * dn * normal[2], 0.0f};
-        float rw[4];
-        mv4(u.inv_view, refl, rw);
-        float rc[3];
-        cube_sample(cube, cube_size, rw, rc);
-        color[pi] = make_float4(unorm(1.0f * specular + (refr[0] * (1.0f - fr) + rc[0] * fr)),
-                                unorm(1.0f * specular + (refr[1] * (1.0f - fr) + rc[1] * fr)),
-                                unorm(1.0f * specular + (refr[2] * (1.0f - fr) + rc[2] * fr)),
-                                1.0f);
+    const int x = (int)(blockIdx.x * kPixW + threadIdx.x);
+    const int y = (int)(blockIdx.y * kPixH + threadIdx.y);
+    if (x >= w || y >= h) return;
+    const size_t pi = (size_t)y * w + x;
+    const float bg[3] = {0.8f, 0.8f, 0.8f};
+    const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
+    const float uvx = iu / (float)w, uvy = iv / (float)h;
+    const float depth = fabsf(depth_map[tex_index(iu, iv, w, h)]);
+    if (depth >= 1e4f || depth <= 0.0f) {
+        color[pi] = make_float4(bg[0], bg[1], bg[2], 1.0f);
+        return;
     }
+    float vp[3];
+    view_pos_from_uv_depth(u, uvx, uvy, depth, vp);
+    const float tsx = u.texel_size[0], tsy = u.texel_size[1];
+    float p[3], ddx[3], ddy[3], ddx2[3], ddy2[3];
+    view_pos_from_uv_depth(u, uvx + tsx, uvy + 0.0f,
+                           fabsf(depth_map[tex_index(iu + 1.0f, iv + 0.0f, w, h)]), p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ddx[i] = p[i] - vp[i];
+    view_pos_from_uv_depth(u, uvx + 0.0f, uvy + tsy,
+                           fabsf(depth_map[tex_index(iu + 0.0f, iv + 1.0f, w, h)]), p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ddy[i] = p[i] - vp[i];
+    view_pos_from_uv_depth(u, uvx + -tsx, uvy + 0.0f,
+                           fabsf(depth_map[tex_index(iu + -1.0f, iv + 0.0f, w, h)]), p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ddx2[i] = vp[i] - p[i];
+    view_pos_from_uv_depth(u, uvx + 0.0f, uvy + -tsy,
+                           fabsf(depth_map[tex_index(iu + 0.0f, iv + -1.0f, w, h)]), p);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ddy2[i] = vp[i] - p[i];
+    if (fabsf(ddx[2]) > fabsf(ddx2[2]))
+        for (int i = 0; i < 3; ++i) ddx[i] = ddx2[i];
+    if (fabsf(ddy[2]) > fabsf(ddy2[2]))
+        for (int i = 0; i < 3; ++i) ddy[i] = ddy2[i];
+    const float cr[3] = {ddx[1] * ddy[2] - ddx[2] * ddy[1], ddx[2] * ddy[0] - ddx[0] * ddy[2],
+                         ddx[0] * ddy[1] - ddx[1] * ddy[0]};
+    float normal[3];
+    normalize3(cr, normal);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) normal[i] = -normal[i];
+    float ray[3];
+    normalize3(vp, ray);
+    const float ld4[4] = {0.0f, 0.0f, -1.0f, 0.0f};
+    float l4[4], light[3];
+    mv4(u.view, ld4, l4);
+    normalize3(l4, light);
+    const float hv[3] = {light[0] - ray[0], light[1] - ray[1], light[2] - ray[2]};
+    float H[3];
+    normalize3(hv, H);
+    const float hn = dot3(H, normal);
+    const float specular = powf(hn > 0.0f ? hn : 0.0f, 250.0f);
+    const float density = 1.5f;
+    const float thickness = thick[tex_index(iu, iv, w, h)];
+    const float dc[3] = {0.085f, 0.6375f, 0.9f};
+    float refr[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) refr[i] = bg[i] * expf(-density * thickness * (1.0f - dc[i]));
+    const float F0 = 0.02f;
+    const float nr[3] = {-ray[0], -ray[1], -ray[2]};
+    float fr = F0 + (1.0f - F0) * powf(1.0f - dot3(normal, nr), 5.0f);
+    fr = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
+    const float dn = dot3(normal, ray);
+    const float refl[4] = {ray[0] - 2.0f * dn * normal[0], ray[1] - 2.0f * dn * normal[1],
+                           ray[2] - 2.0f * dn * normal[2], 0.0f};
+    float rw[4];
+    mv4(u.inv_view, refl, rw);
+    float rc[3];
+    cube_sample(cube, cube_size, rw, rc);
+    color[pi] = make_float4(unorm(1.0f * specular + (refr[0] * (1.0f - fr) + rc[0] * fr)),
+                            unorm(1.0f * specular + (refr[1] * (1.0f - fr) + rc[1] * fr)),
+                            unorm(1.0f * specular + (refr[2] * (1.0f - fr) + rc[2] * fr)),
+                            1.0f);
 }
 
 unsigned blocks_for(size_t count) {
@@ -752,7 +769,8 @@ int ensure_scratch(render_ctx *r, size_t bytes) {
 int frame(render_ctx *r, const float4 *pv, uint32_t n, const render_uniforms &u, int sphere) {
     const int w = r->c.width, h = r->c.height;
     const size_t np = (size_t)w * h;
-    const unsigned pb = blocks_for(np);
+    const dim3 pgrid((unsigned)((w + kPixW - 1) / kPixW), (unsigned)((h + kPixH - 1) / kPixH));
+    const dim3 pblock(kPixW, kPixH);
     const uint32_t ntiles = (uint32_t)(r->tiles_x * r->tiles_y);
     hipStream_t st = r->stream;
     if (n + 1 > r->part_cap) {
@@ -814,15 +832,15 @@ int frame(render_ctx *r, const float4 *pv, uint32_t n, const render_uniforms &u,
         // 4 x (bilateral x, bilateral y) (fluidRender.ts:432-444)
         const float *src = r->depth;
         for (int it = 0; it < 4; ++it) {
-            k_bilateral<<<pb, kThreads, 0, st>>>(src, r->tmp, r->c, r->bilateral_w, 1.0f, 0.0f);
-            k_bilateral<<<pb, kThreads, 0, st>>>(r->tmp, r->fdepth, r->c, r->bilateral_w, 0.0f,
+            k_bilateral<<<pgrid, pblock, 0, st>>>(src, r->tmp, r->c, r->bilateral_w, 1.0f, 0.0f);
+            k_bilateral<<<pgrid, pblock, 0, st>>>(r->tmp, r->fdepth, r->c, r->bilateral_w, 0.0f,
                                                  1.0f);
             src = r->fdepth;
         }
         // 1 x (gaussian x, gaussian y) (fluidRender.ts:450-462)
-        k_gaussian<<<pb, kThreads, 0, st>>>(r->thick16, r->tmp, w, h, r->gauss_w, 1.0f, 0.0f);
-        k_gaussian<<<pb, kThreads, 0, st>>>(r->tmp, r->fthick, w, h, r->gauss_w, 0.0f, 1.0f);
-        k_fluid<<<pb, kThreads, 0, st>>>(r->fdepth, r->fthick, r->cube, r->cube_size, u, w, h,
+        k_gaussian<<<pgrid, pblock, 0, st>>>(r->thick16, r->tmp, w, h, r->gauss_w, 1.0f, 0.0f);
+        k_gaussian<<<pgrid, pblock, 0, st>>>(r->tmp, r->fthick, w, h, r->gauss_w, 0.0f, 1.0f);
+        k_fluid<<<pgrid, pblock, 0, st>>>(r->fdepth, r->fthick, r->cube, r->cube_size, u, w, h,
                                          r->color);
     } else {
         R_TRY(hipMemcpyAsync(r->fdepth, r->depth, np * sizeof(float), hipMemcpyDeviceToDevice, st));
