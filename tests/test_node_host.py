@@ -170,3 +170,150 @@ def test_sph_headless_frame_loop_matches_oracle(tmp_path, slider):
     pv = np.fromfile(vfile, dtype=np.float32).reshape(-1, 8)
     cols = [0, 1, 2, 4, 5, 6]
     assert np.array_equal(pv[:, cols].view(np.int32), ref.posvel[:, cols].view(np.int32))
+
+
+# ---- the renderer on Node (fluid-render.js, camera.js) ----
+
+def test_renderer_and_camera_surface():
+    r = node("""
+const f = require('./fluid-render.js');
+const c = require('./camera.js');
+console.log(JSON.stringify({
+  addon: Object.keys(f.addon).filter(k => k.startsWith('render')).sort(),
+  renderer: Object.getOwnPropertyNames(f.FluidRenderer.prototype).sort(),
+  camera: Object.getOwnPropertyNames(c.Camera.prototype).sort(),
+  bytes: c.renderUniformsValues.byteLength,
+}));""")
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out["addon"] == ["renderColorDevicePtr", "renderCreate", "renderDestroy",
+                            "renderExecute", "renderReadImage", "renderTimeFrames"]
+    assert {"execute", "readColor", "readImage", "destroy"} <= set(out["renderer"])
+    assert {"reset", "recalculateView", "drag", "wheel"} <= set(out["camera"])
+    assert out["bytes"] == 272
+
+
+def test_js_camera_uniforms_match_the_python_camera():
+    """camera.js and render/camera.py restate the same camera.ts and
+    wgpu-matrix arithmetic: the 68 uniform floats agree to f32 rounding after
+    a reset, two drags and two wheel events (JS drag(dx, dy) is a mouse move
+    of (-dx, -dy))."""
+    import math
+
+    from render import camera
+    r = node("""
+const c = require('./camera.js');
+const cam = new c.Camera(null);
+const out = [];
+cam.reset({width: 400, height: 240}, 70, [20, 7.5, 30], 45 * Math.PI / 180, 1.5);
+c.renderUniformsViews.texel_size.set([1 / 400, 1 / 240]);
+c.renderUniformsViews.sphere_size.set([1.2]);
+out.push(Array.from(new Float32Array(c.renderUniformsValues)));
+cam.drag(37, -12); out.push(Array.from(new Float32Array(c.renderUniformsValues)));
+cam.drag(-400, 900); out.push(Array.from(new Float32Array(c.renderUniformsValues)));
+cam.wheel(-3); cam.wheel(-3); out.push(Array.from(new Float32Array(c.renderUniformsValues)));
+console.log(JSON.stringify(out));""")
+    assert r.returncode == 0, r.stderr
+    js = np.array(json.loads(r.stdout), dtype=np.float32)
+    cam = camera.Camera()
+    cam.reset(400, 240, 70, [20, 7.5, 30], 45 * math.pi / 180, 1.5)
+    py = [cam.uniforms(400, 240, 1.2)]
+    cam.orbit(-37, 12)
+    py.append(cam.uniforms(400, 240, 1.2))
+    cam.orbit(400, -900)
+    py.append(cam.uniforms(400, 240, 1.2))
+    cam.zoom(-3)
+    cam.zoom(-3)
+    py.append(cam.uniforms(400, 240, 1.2))
+    np.testing.assert_allclose(js, np.array(py), rtol=1e-6, atol=1e-6)
+    assert cam.y_theta == 0.0   # the second drag clamps at the horizon
+    assert cam.distance == pytest.approx(67.0)
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
+def test_renderer_throws_without_device():
+    r = node("""
+const f = require('./fluid-render.js');
+const c = require('./camera.js');
+try { new f.FluidRenderer(null, {width: 64, height: 48}, null, 0.6, 0.78, null,
+                          c.renderUniformsValues, {size: 2, data: new Float32Array(96)});
+      console.log('no-throw'); }
+catch (e) { console.log(e.message); }""")
+    assert "render error 5" in r.stdout
+
+
+@pytest.mark.gpu
+def test_node_frame_loop_renders_like_the_oracle(tmp_path):
+    """main.ts's frame loop on Node: MLSMPMSimulator steps, the camera fills
+    the uniform buffer, FluidRenderer renders from the simulator's device
+    buffer (fluid, then spheres); the depth and thickness maps equal the
+    renderer oracle's on the same PosVel and uniforms, bit for bit."""
+    import math
+
+    from oracle import render as orender
+    from render import camera
+    cube = camera.synthetic_cubemap(16)
+    cube.tofile(str(tmp_path / "cube.f32"))
+    w, h = 240, 160
+    r = node("""
+const fs = require('fs');
+const m = require('./mls-mpm.js');
+const f = require('./fluid-render.js');
+const c = require('./camera.js');
+const dir = process.argv[1];
+let k = 0;
+const sim = new m.MLSMPMSimulator(null, null, 1.2, {random: () => ((k++ % 89) / 89)});
+sim.reset(40000, [35, 25, 55]);
+for (let i = 0; i < 10; ++i) sim.execute(null);
+const cam = new c.Camera(null);
+cam.reset({width: WIDTH, height: HEIGHT}, 60, [35 / 2, 25 / 4, 55 / 2], 45 * Math.PI / 180, 1.5);
+c.renderUniformsViews.texel_size.set([1 / WIDTH, 1 / HEIGHT]);
+c.renderUniformsViews.sphere_size.set([1.2]);
+const cubeData = new Float32Array(fs.readFileSync(dir + '/cube.f32').buffer.slice(0));
+const ren = new f.FluidRenderer(null, {width: WIDTH, height: HEIGHT}, null, 0.6, 45 * Math.PI / 180,
+                                sim, c.renderUniformsValues, {size: 16, data: cubeData});
+ren.execute(null, null, sim.numParticles, false);
+fs.writeFileSync(dir + '/depth.f32', Buffer.from(ren.readImage('depth').buffer));
+fs.writeFileSync(dir + '/thick.f32', Buffer.from(ren.readImage('thickness').buffer));
+ren.execute(null, null, sim.numParticles, true);
+fs.writeFileSync(dir + '/sphere.f32', Buffer.from(ren.readColor().buffer));
+fs.writeFileSync(dir + '/posvel.f32', Buffer.from(sim.readPositions()));
+fs.writeFileSync(dir + '/uniforms.f32', Buffer.from(c.renderUniformsValues));
+console.log(sim.numParticles);
+ren.destroy();
+sim.destroy();""".replace("WIDTH", str(w)).replace("HEIGHT", str(h)), str(tmp_path), timeout=300)
+    assert r.returncode == 0, r.stderr
+    n = int(r.stdout.strip())
+    pv = np.fromfile(str(tmp_path / "posvel.f32"), dtype=np.float32).reshape(n, 8)
+    u = np.fromfile(str(tmp_path / "uniforms.f32"), dtype=np.float32)
+    c = orender.consts(w, h, 0.6, 45 * math.pi / 180)
+    want = orender.render_frame(pv, n, u, c, cube, False, 16)
+    depth = np.fromfile(str(tmp_path / "depth.f32"), dtype=np.float32).reshape(h, w)
+    thick = np.fromfile(str(tmp_path / "thick.f32"), dtype=np.float32).reshape(h, w)
+    assert np.count_nonzero(depth) > 0.05 * w * h
+    np.testing.assert_array_equal(depth.view(np.uint32), want["depth"].view(np.uint32))
+    np.testing.assert_array_equal(thick.view(np.uint32), want["thickness"].view(np.uint32))
+    sph = orender.render_frame(pv, n, u, c, cube, True, 16)
+    got = np.fromfile(str(tmp_path / "sphere.f32"), dtype=np.float32).reshape(h, w, 4)
+    np.testing.assert_allclose(got, sph["color"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("solver", ["mlsmpm", "sph"])
+def test_headless_frame_loop_renders_every_frame(tmp_path, solver):
+    """headless.js --render: main.ts's frame loop with the renderer's pass over
+    the simulator's buffer each frame (fluid, then spheres); the last frame
+    shows the fluid over the 0.8 grey background, every value in [0, 1]."""
+    w, h = 200, 120
+    for spheres in (0, 1):
+        out = tmp_path / f"color{spheres}.f32"
+        r = subprocess.run(["node", os.path.join(NODE_DIR, "headless.js"), "--solver", solver,
+                            "--preset", "0", "--frames", "8", "--render", f"{w}x{h}",
+                            "--spheres", str(spheres), "--color", str(out)],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert json.loads(r.stdout)["rendered"] == [w, h]
+        img = np.fromfile(str(out), dtype=np.float32).reshape(h, w, 4)
+        assert np.isfinite(img).all() and img.min() >= 0.0 and img.max() <= 1.0
+        fluid = np.any(img[..., :3] != np.float32(0.8), axis=-1)
+        assert fluid.mean() > 0.02, fluid.mean()

@@ -1,9 +1,9 @@
 // mpm_napi.cc -- Node N-API addon over libmpm355's C ABIs (include/mpm.h,
-// include/sph.h).
+// include/sph.h, include/render.h).
 //
-// Binds exactly the entry points the JS MLSMPMSimulator and SPHSimulator
-// need; a failing call throws a JS Error carrying mpm_last_error() /
-// sph_last_error().  Built with g++ against
+// Binds exactly the entry points the JS MLSMPMSimulator, SPHSimulator and
+// FluidRenderer need; a failing call throws a JS Error carrying
+// mpm_last_error() / sph_last_error() / render_last_error().  Built with g++ against
 // the system Node headers (N-API >= 6 for BigInt device pointers).
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -13,6 +13,7 @@
 #include <string>
 
 #include "mpm.h"
+#include "render.h"
 #include "sph.h"
 
 namespace {
@@ -603,6 +604,203 @@ napi_value SphDestroy(napi_env env, napi_callback_info info) {
     return Undefined(env);
 }
 
+// ---- renderer (include/render.h) ----
+
+#define RENDER_CALL(env, call)                                                          \
+    do {                                                                                \
+        int rc_ = (call);                                                               \
+        if (rc_ != RENDER_OK) {                                                         \
+            std::string msg = std::string("render error ") + std::to_string(rc_) + ": " + \
+                              render_last_error();                                      \
+            napi_throw_error((env), nullptr, msg.c_str());                              \
+            return nullptr;                                                             \
+        }                                                                               \
+    } while (0)
+
+bool is_exception_pending(napi_env env) {
+    bool pending = false;
+    napi_is_exception_pending(env, &pending);
+    return pending;
+}
+
+struct RenderHolder {
+    render_ctx *r;
+    uint32_t width, height;
+};
+
+bool get_render(napi_env env, napi_value v, RenderHolder **out) {
+    void *p = nullptr;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, nullptr, "expected a renderer handle");
+        return false;
+    }
+    *out = static_cast<RenderHolder *>(p);
+    if (!(*out)->r) {
+        napi_throw_error(env, nullptr, "renderer was destroyed");
+        return false;
+    }
+    return true;
+}
+
+void finalize_render(napi_env, void *data, void *) {
+    RenderHolder *h = static_cast<RenderHolder *>(data);
+    render_destroy(h->r);
+    delete h;
+}
+
+// The 68 floats of the render uniform buffer (common.ts:1-9).
+bool get_uniforms(napi_env env, napi_value v, render_uniforms *u) {
+    void *data;
+    size_t len;
+    if (!get_bytes(env, v, &data, &len)) return false;
+    if (len < sizeof(render_uniforms)) {
+        napi_throw_range_error(env, nullptr, "render uniforms must hold 68 floats");
+        return false;
+    }
+    std::memcpy(u, data, sizeof(render_uniforms));
+    return true;
+}
+
+// renderCreate(width, height, radius, fov, cubemap Float32Array, cubeSize) -> handle
+napi_value RenderCreate(napi_env env, napi_callback_info info) {
+    napi_value argv[6];
+    size_t argc;
+    if (!get_args(env, info, argv, &argc) || argc < 6) {
+        napi_throw_type_error(env, nullptr, "renderCreate(width, height, radius, fov, cubemap, size)");
+        return nullptr;
+    }
+    render_params p;
+    uint32_t size;
+    void *cube;
+    size_t len;
+    if (!get_u32(env, argv[0], &p.width) || !get_u32(env, argv[1], &p.height) ||
+        napi_get_value_double(env, argv[2], &p.radius) != napi_ok ||
+        napi_get_value_double(env, argv[3], &p.fov) != napi_ok ||
+        !get_bytes(env, argv[4], &cube, &len) || !get_u32(env, argv[5], &size)) {
+        if (!is_exception_pending(env)) napi_throw_type_error(env, nullptr, "bad renderCreate arguments");
+        return nullptr;
+    }
+    if ((size_t)size * size * 6 * 4 * sizeof(float) > len) {
+        napi_throw_range_error(env, nullptr, "cubemap shorter than 6 x size^2 RGBA floats");
+        return nullptr;
+    }
+    render_ctx *r = nullptr;
+    RENDER_CALL(env, render_create(&p, static_cast<const float *>(cube), size, &r));
+    RenderHolder *holder = new RenderHolder{r, p.width, p.height};
+    napi_value h;
+    if (napi_create_external(env, holder, finalize_render, nullptr, &h) != napi_ok) {
+        render_destroy(r);
+        delete holder;
+        napi_throw_error(env, nullptr, "napi_create_external failed");
+        return nullptr;
+    }
+    return h;
+}
+
+// renderExecute(h, posvel (BigInt device address | ArrayBuffer of host
+// records), n, uniforms Float32Array(68), sphere)
+napi_value RenderExecute(napi_env env, napi_callback_info info) {
+    napi_value argv[5];
+    size_t argc;
+    RenderHolder *hd;
+    uint32_t n;
+    render_uniforms u;
+    bool sphere = false;
+    if (!get_args(env, info, argv, &argc) || argc < 5 || !get_render(env, argv[0], &hd) ||
+        !get_u32(env, argv[2], &n) || !get_uniforms(env, argv[3], &u))
+        return nullptr;
+    napi_get_value_bool(env, argv[4], &sphere);
+    napi_valuetype t;
+    napi_typeof(env, argv[1], &t);
+    if (t == napi_bigint) {
+        uint64_t addr = 0;
+        bool lossless = false;
+        NAPI_CALL(env, napi_get_value_bigint_uint64(env, argv[1], &addr, &lossless));
+        RENDER_CALL(env, render_execute(hd->r, (const void *)(uintptr_t)addr, n, &u, sphere));
+    } else {
+        void *data;
+        size_t len;
+        if (!get_bytes(env, argv[1], &data, &len)) return nullptr;
+        if ((size_t)n * 32 > len) {
+            napi_throw_range_error(env, nullptr, "PosVel buffer shorter than n x 32 bytes");
+            return nullptr;
+        }
+        RENDER_CALL(env, render_execute_host(hd->r, data, n, &u, sphere));
+    }
+    return Undefined(env);
+}
+
+// renderReadImage(h, image, dst ArrayBuffer|Float32Array)
+napi_value RenderReadImage(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    size_t argc;
+    RenderHolder *hd;
+    uint32_t image;
+    void *data;
+    size_t len;
+    if (!get_args(env, info, argv, &argc) || argc < 3 || !get_render(env, argv[0], &hd) ||
+        !get_u32(env, argv[1], &image) || !get_bytes(env, argv[2], &data, &len))
+        return nullptr;
+    const size_t need = (size_t)hd->width * hd->height * sizeof(float) *
+                        (image == RENDER_IMAGE_COLOR ? 4 : 1);
+    if (len < need) {
+        napi_throw_range_error(env, nullptr, "destination shorter than the image");
+        return nullptr;
+    }
+    RENDER_CALL(env, render_read_image(hd->r, (int)image, static_cast<float *>(data)));
+    return Undefined(env);
+}
+
+// renderColorDevicePtr(h) -> BigInt
+napi_value RenderColorDevicePtr(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc;
+    RenderHolder *hd;
+    if (!get_args(env, info, argv, &argc) || argc < 1 || !get_render(env, argv[0], &hd))
+        return nullptr;
+    void *p = nullptr;
+    RENDER_CALL(env, render_color_device_ptr(hd->r, &p));
+    napi_value b;
+    NAPI_CALL(env, napi_create_bigint_uint64(env, (uint64_t)(uintptr_t)p, &b));
+    return b;
+}
+
+// renderTimeFrames(h, posvel BigInt, n, uniforms, sphere, frames) -> ms
+napi_value RenderTimeFrames(napi_env env, napi_callback_info info) {
+    napi_value argv[6];
+    size_t argc;
+    RenderHolder *hd;
+    uint32_t n, frames;
+    render_uniforms u;
+    bool sphere = false, lossless = false;
+    uint64_t addr = 0;
+    if (!get_args(env, info, argv, &argc) || argc < 6 || !get_render(env, argv[0], &hd) ||
+        !get_u32(env, argv[2], &n) || !get_uniforms(env, argv[3], &u) ||
+        !get_u32(env, argv[5], &frames))
+        return nullptr;
+    NAPI_CALL(env, napi_get_value_bigint_uint64(env, argv[1], &addr, &lossless));
+    napi_get_value_bool(env, argv[4], &sphere);
+    float ms = 0.0f;
+    RENDER_CALL(env, render_time_frames(hd->r, (const void *)(uintptr_t)addr, n, &u, sphere,
+                                        frames, &ms));
+    napi_value out;
+    NAPI_CALL(env, napi_create_double(env, ms, &out));
+    return out;
+}
+
+// renderDestroy(h): releases the renderer now; later calls on h throw.
+napi_value RenderDestroy(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc;
+    RenderHolder *hd;
+    if (!get_args(env, info, argv, &argc) || argc < 1 || !get_render(env, argv[0], &hd))
+        return nullptr;
+    render_ctx *r = hd->r;
+    hd->r = nullptr;
+    RENDER_CALL(env, render_destroy(r));
+    return Undefined(env);
+}
+
 napi_value Init(napi_env env, napi_value exports) {
     struct {
         const char *name;
@@ -621,6 +819,9 @@ napi_value Init(napi_env env, napi_value exports) {
         {"sphReadParticles", SphReadParticles}, {"sphWriteParticles", SphWriteParticles},
         {"sphCounts", SphCounts},     {"sphPosvelDevicePtr", SphPosvelDevicePtr},
         {"sphDestroy", SphDestroy},
+        {"renderCreate", RenderCreate}, {"renderExecute", RenderExecute},
+        {"renderReadImage", RenderReadImage}, {"renderColorDevicePtr", RenderColorDevicePtr},
+        {"renderTimeFrames", RenderTimeFrames}, {"renderDestroy", RenderDestroy},
     };
     for (auto &f : fns) {
         napi_value fn;

@@ -53,7 +53,8 @@ def inverse(m: np.ndarray) -> np.ndarray:
 
 class Camera:
     """camera.ts's orbit camera without the mouse handlers: reset() and
-    recalculate_view() as there; orbit(dx, dy) applies a drag."""
+    recalculate_view() as there; orbit(dx, dy) applies a drag, zoom(delta_y)
+    a wheel event."""
 
     def __init__(self):
         self.x_theta = math.pi / 4
@@ -88,6 +89,12 @@ class Camera:
         self.x_theta += self.sensitivity * -dx
         self.y_theta = min(max(self.y_theta + self.sensitivity * -dy, self.min_y_theta),
                            self.max_y_theta)
+        self.recalculate_view()
+
+    def zoom(self, delta_y: float):
+        """A wheel event of deltaY (camera.ts:27-34)."""
+        self.distance += (1 if delta_y > 0 else -1) * self.zoom_rate
+        self.distance = min(max(self.distance, self.min_distance), self.max_distance)
         self.recalculate_view()
 
     def recalculate_view(self):
