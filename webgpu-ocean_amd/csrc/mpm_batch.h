@@ -188,6 +188,10 @@ private:
     static constexpr int kMaxGraphs = 6;
     GraphEntry graphs_[kMaxGraphs] = {};
     int ngraphs_ = 0;
+    // the stream the cached graphs were launched on: drop_graphs() waits for
+    // it, so no executable graph is destroyed while a launch of it may run
+    hipStream_t graph_stream_ = nullptr;
+    bool graph_launched_ = false;
 };
 
 }  // namespace mpm

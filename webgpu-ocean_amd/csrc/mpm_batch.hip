@@ -531,6 +531,8 @@ hipError_t Batch::substep(hipStream_t st) {
         graphs_[ngraphs_] = GraphEntry{a, b, v, tun_.path, keys_ready_, exec};
         g = &graphs_[ngraphs_++];
     }
+    graph_stream_ = st;
+    graph_launched_ = true;
     TRY(hipGraphLaunch(g->exec, st));
     after_substep(st);
     return hipSuccess;
@@ -738,6 +740,8 @@ hipError_t Batch::autotune(hipStream_t st, unsigned substeps_per_trial, Tuning *
 }
 
 void Batch::drop_graphs() {
+    if (ngraphs_ && graph_launched_) (void)hipStreamSynchronize(graph_stream_);
+    graph_launched_ = false;
     for (int i = 0; i < ngraphs_; ++i) (void)hipGraphExecDestroy(graphs_[i].exec);
     ngraphs_ = 0;
 }
