@@ -1,4 +1,4 @@
-// Checks div_rn (webgpu-ocean_amd/csrc/render_math.h) against IEEE single
+// Checks div_rn (webgpu-ocean_amd/csrc/div_rn.h) against IEEE single
 // division, bit for bit: random operands over the ranges the raster uses
 // (pixel-centre offsets a in [0, 2^14], rectangle extents b in [2^-10, 2^14]),
 // random significands over whole binades, and structured cases (a = 0,
@@ -9,9 +9,10 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 
-#include "render_math.h"
+#include "div_rn.h"
 
 static uint64_t g_state = 0x9e3779b97f4a7c15ull;
 static uint64_t next_u64() {  // splitmix64

@@ -1,4 +1,4 @@
-"""The raster's correctly rounded division (div_rn, render_math.h: two
+"""The raster's correctly rounded division (div_rn, div_rn.h: two
 Markstein corrections from RN(1/b)) equals IEEE single division bit for bit
 on 60 million quotients over the raster's operand ranges and whole binades
 (tests/emu/div_rn_check.cpp), under AddressSanitizer/UBSan; a planted

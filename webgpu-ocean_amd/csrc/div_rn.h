@@ -1,13 +1,13 @@
-// render_math.h -- arithmetic helpers of the renderer's raster, shared by the
-// gfx950 kernels (render.hip) and the host test that checks them against
-// IEEE division (tests/emu/div_rn_check.cpp).
+// div_rn.h -- correctly rounded single division from a precomputed
+// reciprocal, used by the gfx950 renderer's raster (render.hip) and by the
+// host test that checks it against IEEE division (tests/emu/div_rn_check.cpp).
 #pragma once
 #include <cmath>
 
 #if defined(__HIPCC__)
-#define RENDER_HD __host__ __device__
+#define DIV_RN_HD __host__ __device__
 #else
-#define RENDER_HD
+#define DIV_RN_HD
 #endif
 
 // The correctly rounded quotient RN(a / b) from y = RN(1 / b), by two
@@ -18,7 +18,7 @@
 // r1 = a - b q1 is exact in one fma and q = RN(q1 + r1 y) is RN(a / b).
 // Valid without underflow or overflow in the steps: the raster calls it with
 // a in [0, 2^24] (pixel-centre offsets) and b > 0 a normal number.
-RENDER_HD inline float div_rn(float a, float b, float y) {
+DIV_RN_HD inline float div_rn(float a, float b, float y) {
     const float q0 = a * y;
     const float r0 = fmaf(-q0, b, a);
     const float q1 = fmaf(r0, y, q0);

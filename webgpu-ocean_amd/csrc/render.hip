@@ -39,7 +39,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include "render.h"
-#include "render_math.h"
+#include "div_rn.h"
 
 #pragma clang fp contract(off)
 
