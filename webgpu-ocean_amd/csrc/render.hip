@@ -509,6 +509,10 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
     const int filter_size = mfs < fsz ? mfs : fsz;
     const float sigma_depth = c.depth_threshold / 3.0f;
     const float two_sigma_depth = 2.0f * sigma_depth * sigma_depth;
+    // every tap divides by two_sigma_depth: the correctly rounded quotient
+    // from its rounded reciprocal (div_rn; the numerators are 0 or squares
+    // of depth differences, >= ~1e-14)
+    const float inv_two_sigma_depth = 1.0f / two_sigma_depth;
     const float *wrow = bw + filter_size * (kMaxFilter + 1);
     float sum = 0.0f, wsum = 0.0f;
 #pragma unroll 4
@@ -517,7 +521,8 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
         const float sampled = fabsf(src[tex_index(iu + co * dirx, iv + co * diry, w, h)]);
         const float wt = wrow[k < 0 ? -k : k];
         const float r_depth = sampled - depth;
-        const float wd = expf(-r_depth * r_depth / two_sigma_depth);
+        const float wd =
+            expf(div_rn(-r_depth * r_depth, two_sigma_depth, inv_two_sigma_depth));
         sum += sampled * wt * wd;
         wsum += wt * wd;
     }
