@@ -311,6 +311,7 @@ constexpr int kRasterThreads = 64 * kRasterWaves;
 constexpr int kChunkPer = 32;
 constexpr int kChunk = kRasterWaves * kChunkPer;
 constexpr int kCStride = kChunk + 1;  // odd row stride: conflict-free in both phases
+static_assert(kChunk <= kRasterThreads - 64, "waves 1.. stage a whole chunk during the blend");
 
 // The rectangle's extents and their rounded reciprocals, (xr - xl, 1 / that,
 // yb - yt, 1 / that), for div_rn.
@@ -375,18 +376,24 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(
     // fragment is deeper than a found one: it can neither win nor tie)
     float best_d = 2.0f, bound = 2.0f, t = 0.0f;
     uint32_t best_i = 0xffffffffu;
+    // Chunk records are staged by the waves that are idle while wave 0 blends:
+    // the first chunk before the loop, each next one during the blend of the
+    // current one (phase 2 reads only s_c).
+    auto stage = [&](uint32_t base, int tid) {
+        const int m = rg.y - base < (uint32_t)kChunk ? (int)(rg.y - base) : kChunk;
+        if (tid >= 0 && tid < m) {
+            const uint32_t i = vals[base + tid];
+            const float4 rc = rec[2 * (size_t)i];
+            s_idx[tid] = i;
+            s_rect[tid] = rc;
+            s_ext[tid] = rect_extents(rc);
+            s_view[tid] = rec[2 * (size_t)i + 1];
+        }
+    };
+    if (rg.x < rg.y) stage(rg.x, (int)threadIdx.x);
     for (uint32_t base = rg.x; base < rg.y; base += kChunk) {
         const int m = rg.y - base < (uint32_t)kChunk ? (int)(rg.y - base) : kChunk;
-        __syncthreads();  // the previous chunk's blend is done with s_c
-        if ((int)threadIdx.x < m) {
-            const uint32_t i = vals[base + threadIdx.x];
-            const float4 rc = rec[2 * (size_t)i];
-            s_idx[threadIdx.x] = i;
-            s_rect[threadIdx.x] = rc;
-            s_ext[threadIdx.x] = rect_extents(rc);
-            s_view[threadIdx.x] = rec[2 * (size_t)i + 1];
-        }
-        __syncthreads();
+        __syncthreads();  // this chunk's records are staged; the last blend is done with s_c
         const int k1 = (wave + 1) * kChunkPer < m ? (wave + 1) * kChunkPer : m;
         for (int k = wave * kChunkPer; k < k1; ++k) {
             float d, z, nrm[3];
@@ -400,16 +407,20 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(
         }
         if (best_i != 0xffffffffu)
             atomicMin(&s_key[lane], ((unsigned long long)__float_as_uint(best_d) << 32) | best_i);
-        __syncthreads();
+        __syncthreads();  // s_c complete; the records are free for the next chunk
         {
             const unsigned long long kb = s_key[lane];
             if (kb != ~0ull) bound = fminf(bound, __uint_as_float((uint32_t)(kb >> 32)));
         }
-        if (wave == 0 && !sphere_mode) {
-            // the thickness target's blends, one fragment at a time in list order
-            // (adding 0 leaves an f16 value unchanged)
-            const float *row = s_c + lane * kCStride;
-            for (int k = 0; k < m; ++k) t = f16_round(t + row[k]);
+        if (wave == 0) {
+            if (!sphere_mode) {
+                // the thickness target's blends, one fragment at a time in list
+                // order (adding 0 leaves an f16 value unchanged)
+                const float *row = s_c + lane * kCStride;
+                for (int k = 0; k < m; ++k) t = f16_round(t + row[k]);
+            }
+        } else if (base + kChunk < rg.y) {
+            stage(base + kChunk, (int)threadIdx.x - 64);
         }
     }
     if (best_i != 0xffffffffu)
