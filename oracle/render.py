@@ -48,6 +48,10 @@ def lib():
                                             ctypes.POINTER(Consts), i32]
         L.oracle_gaussian_pass.argtypes = [P, P, i32, i32, ctypes.POINTER(f32), i32]
         L.oracle_fluid_pass.argtypes = [ctypes.POINTER(Uniforms), P, P, P, i32, i32, i32, P, i32]
+        for name in ("oracle_render_exp", "oracle_render_log", "oracle_render_pow5",
+                     "oracle_render_pow250"):
+            getattr(L, name).restype = f32
+            getattr(L, name).argtypes = [f32]
         _lib = L
     return _lib
 
@@ -129,3 +133,20 @@ def fluid(u: np.ndarray, depth: np.ndarray, thickness: np.ndarray, cube: np.ndar
     lib().oracle_fluid_pass(ctypes.byref(uu), _p(d), _p(t), _p(cb), cb.shape[1], d.shape[1],
                             d.shape[0], _p(out), threads)
     return out
+
+
+def exp(x: float) -> float:
+    """The renderer's deterministic exp (the HIP kernels use the same formula)."""
+    return float(lib().oracle_render_exp(float(x)))
+
+
+def log(x: float) -> float:
+    return float(lib().oracle_render_log(float(x)))
+
+
+def pow5(x: float) -> float:
+    return float(lib().oracle_render_pow5(float(x)))
+
+
+def pow250(x: float) -> float:
+    return float(lib().oracle_render_pow250(float(x)))

@@ -21,7 +21,13 @@
  *     fragment's output, rounded to f16 (round to nearest even) and stored;
  *   - textureLoad coordinates are clamped into the image (Tint's
  *     robustness), u32 conversions of negative coordinates give 0;
- *   - the cubemap sample is the major-axis face, bilinear, clamp to edge.
+ *   - the cubemap sample is the major-axis face, bilinear, clamp to edge;
+ *   - exp and pow (WGSL bounds their error, exp to 3 + 2|x| ulp, pow as
+ *     exp2(y log2 x), without fixing their values) are one deterministic
+ *     formula each, the same in the HIP kernels: oracle_render_exp (range
+ *     reduction by ln 2 in two parts, degree-7 Taylor polynomial, 0 below
+ *     x = -86), pow(x, 5) = ((x x)(x x)) x, pow(x, 250) =
+ *     exp(250 log x) with oracle_render_log (atanh series).
  * mat4 x vec4 products are evaluated column by column,
  * ((c0 x + c1 y) + c2 z) + c3 w.
  */
@@ -37,6 +43,59 @@ static void mv4(const float *m, const float v[4], float out[4]) {
 }
 
 static float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+/* exp(x): x = k ln2 + r with k = rint(x log2 e) and r from ln2 split into
+ * 0.693359375 + (-2.12194440e-4) (k ln2_hi exact); exp(r), |r| <= 0.35, by
+ * its Taylor polynomial to degree 7 (truncation < 6e-9) in Horner form, one
+ * rounding per operation; times 2^k.  0 below -86 (results stay normal),
+ * inf above 88, NaN through. */
+float oracle_render_exp(float x) {
+    if (x != x) return x;
+    if (!(x > -86.0f)) return 0.0f;
+    if (x > 88.0f) return INFINITY;
+    const float k = rintf(x * 1.44269502f);
+    const float r = (x - k * 0.693359375f) - k * -2.12194440e-4f;
+    float p = 1.98412698e-4f;           /* 1/7! */
+    p = p * r + 1.38888889e-3f;         /* 1/6! */
+    p = p * r + 8.33333333e-3f;         /* 1/5! */
+    p = p * r + 4.16666667e-2f;         /* 1/4! */
+    p = p * r + 1.66666667e-1f;         /* 1/3! */
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, (int)k);
+}
+
+/* log(x) for x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m =
+ * 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.172, by its series to s^9
+ * (truncation < 2e-9); e ln2 with ln2 split as for exp.  -inf at 0, NaN
+ * below. */
+float oracle_render_log(float x) {
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e;
+    float m = frexpf(x, &e);          /* x = m 2^e, m in [0.5, 1) */
+    if (m < 0.70710678f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f), s2 = s * s;
+    float p = 1.11111111e-1f;          /* 1/9 */
+    p = p * s2 + 1.42857143e-1f;       /* 1/7 */
+    p = p * s2 + 2.00000000e-1f;       /* 1/5 */
+    p = p * s2 + 3.33333333e-1f;       /* 1/3 */
+    p = p * s2 + 1.0f;
+    const float fe = (float)e;
+    return fe * 0.693359375f + (fe * -2.12194440e-4f + (2.0f * s) * p);
+}
+
+/* pow(x, 5) and pow(x, 250) for x >= 0 (fluid.wgsl:70,80): ((x x)(x x)) x,
+ * and exp(250 log x), whose error (about |250 log x| / 2 ulp) stays within
+ * WGSL's bound for pow, 3 + 2 |250 log2 x| ulp. */
+float oracle_render_pow5(float x) { return ((x * x) * (x * x)) * x; }
+
+float oracle_render_pow250(float x) { return oracle_render_exp(250.0f * oracle_render_log(x)); }
 
 static void normalize3(const float v[3], float out[3]) {
     const float l = sqrtf(dot3(v, v));
@@ -233,9 +292,9 @@ void oracle_bilateral_pass(const float *src, float *dst, int w, int h, const flo
                 const float sx = iu + co * dir[0], sy = iv + co * dir[1];
                 const float sampled = fabsf(src[tex_index(sx, sy, w, h)]);
                 const float rr = co * co + co * co;
-                const float wt = expf(-rr / two_sigma);
+                const float wt = oracle_render_exp(-rr / two_sigma);
                 const float r_depth = sampled - depth;
-                const float wd = expf(-r_depth * r_depth / two_sigma_depth);
+                const float wd = oracle_render_exp(-r_depth * r_depth / two_sigma_depth);
                 sum += sampled * wt * wd;
                 wsum += wt * wd;
             }
@@ -261,7 +320,7 @@ void oracle_gaussian_pass(const float *src, float *dst, int w, int h, const floa
             for (int k = -filter_size; k <= filter_size; ++k) {
                 const float co = (float)k;
                 const float sampled = src[tex_index(iu + dir[0] * co, iv + dir[1] * co, w, h)];
-                const float wt = expf(-co * co / two_sigma);
+                const float wt = oracle_render_exp(-co * co / two_sigma);
                 sum += sampled * wt;
                 wsum += wt;
             }
@@ -376,15 +435,15 @@ static void fluid_pixel(const oracle_render_uniforms *u, const float *depth_map,
     float H[3];
     normalize3(hv, H);
     const float hn = dot3(H, normal);
-    const float specular = powf(hn > 0.0f ? hn : 0.0f, 250.0f);
+    const float specular = oracle_render_pow250(hn > 0.0f ? hn : 0.0f);
     const float density = 1.5f;
     const float thickness = thick[tex_index(iu, iv, w, h)];
     const float dc[3] = {0.085f, 0.6375f, 0.9f};
     float refr[3];
-    for (int i = 0; i < 3; ++i) refr[i] = bg[i] * expf(-density * thickness * (1.0f - dc[i]));
+    for (int i = 0; i < 3; ++i) refr[i] = bg[i] * oracle_render_exp(-density * thickness * (1.0f - dc[i]));
     const float F0 = 0.02f;
     const float nr[3] = {-ray[0], -ray[1], -ray[2]};
-    float fr = F0 + (1.0f - F0) * powf(1.0f - dot3(normal, nr), 5.0f);
+    float fr = F0 + (1.0f - F0) * oracle_render_pow5(1.0f - dot3(normal, nr));
     fr = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
     const float dn = dot3(normal, ray);
     const float refl[4] = {ray[0] - 2.0f * dn * normal[0], ray[1] - 2.0f * dn * normal[1],

@@ -38,6 +38,10 @@ void oracle_render_frame(const float *posvel, uint32_t n, const oracle_render_un
                          int sphere_mode, oracle_render_images *out, int threads);
 
 /* Single pieces, for closed-form tests. */
+float oracle_render_exp(float x);
+float oracle_render_log(float x);
+float oracle_render_pow5(float x);
+float oracle_render_pow250(float x);
 void oracle_cube_sample(const float *cubemap, int size, const float dir[3], float rgb[3]);
 void oracle_bilateral_pass(const float *src, float *dst, int w, int h, const float dir[2],
                            const oracle_render_consts *c, int threads);

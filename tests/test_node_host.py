@@ -275,6 +275,9 @@ const ren = new f.FluidRenderer(null, {width: WIDTH, height: HEIGHT}, null, 0.6,
 ren.execute(null, null, sim.numParticles, false);
 fs.writeFileSync(dir + '/depth.f32', Buffer.from(ren.readImage('depth').buffer));
 fs.writeFileSync(dir + '/thick.f32', Buffer.from(ren.readImage('thickness').buffer));
+fs.writeFileSync(dir + '/fdepth.f32', Buffer.from(ren.readImage('filteredDepth').buffer));
+fs.writeFileSync(dir + '/fthick.f32', Buffer.from(ren.readImage('filteredThickness').buffer));
+fs.writeFileSync(dir + '/fluid.f32', Buffer.from(ren.readColor().buffer));
 ren.execute(null, null, sim.numParticles, true);
 fs.writeFileSync(dir + '/sphere.f32', Buffer.from(ren.readColor().buffer));
 fs.writeFileSync(dir + '/posvel.f32', Buffer.from(sim.readPositions()));
@@ -293,9 +296,13 @@ sim.destroy();""".replace("WIDTH", str(w)).replace("HEIGHT", str(h)), str(tmp_pa
     assert np.count_nonzero(depth) > 0.05 * w * h
     np.testing.assert_array_equal(depth.view(np.uint32), want["depth"].view(np.uint32))
     np.testing.assert_array_equal(thick.view(np.uint32), want["thickness"].view(np.uint32))
+    for name, key in (("fdepth", "filtered_depth"), ("fthick", "filtered_thickness"),
+                      ("fluid", "color")):
+        got = np.fromfile(str(tmp_path / (name + ".f32")), dtype=np.float32)
+        np.testing.assert_array_equal(got.view(np.uint32), want[key].reshape(-1).view(np.uint32))
     sph = orender.render_frame(pv, n, u, c, cube, True, 16)
     got = np.fromfile(str(tmp_path / "sphere.f32"), dtype=np.float32).reshape(h, w, 4)
-    np.testing.assert_allclose(got, sph["color"], rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(got.view(np.uint32), sph["color"].view(np.uint32))
 
 
 @pytest.mark.gpu

@@ -62,6 +62,56 @@ def test_f16_round_equals_numpy_float16():
     assert math.isnan(R.f16_round(float("nan")))
 
 
+# ---- exp and pow (one deterministic formula each, shared with the kernels) ---
+
+def test_exp_is_within_wgsl_bounds_of_the_true_exp():
+    """Within 1.5 ulp of exp on (-86, 0] (WGSL allows 3 + 2|x| ulp), exact at
+    0, 0 below -86, inf above 88, NaN through."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([np.linspace(-85.999, 0, 40001), rng.uniform(-85.999, 0, 40000),
+                        -np.logspace(-9, 1.93, 4000), rng.uniform(0, 88, 4000)]).astype(np.float32)
+    got = np.array([R.exp(v) for v in x], dtype=np.float32)
+    true = np.exp(x.astype(np.float64))
+    ulps = np.abs(got - true) / np.spacing(true.astype(np.float32)).astype(np.float64)
+    assert ulps.max() <= 1.5, (ulps.max(), x[ulps.argmax()])
+    assert R.exp(0.0) == 1.0 and R.exp(-86.0) == 0.0 and R.exp(-1e4) == 0.0
+    assert R.exp(100.0) == float("inf") and math.isnan(R.exp(float("nan")))
+
+
+def test_log_pow5_and_pow250_are_within_wgsl_bounds():
+    """log within 3 ulp; pow(x, 5) within 3 ulp; pow(x, 250) = exp(250 log x)
+    within WGSL's bound for pow, inherited from exp2(y log2 x): log2 with an
+    absolute error of 2^-21 on [0.5, 2] (scaled by y = 250 in the exponent)
+    and exp2 within 3 + 2|a| ulp of its argument a, wherever x^250 >= 2^-124
+    (below: 0, where exp flushes, or as tiny)."""
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(0, 1, 10000), rng.uniform(0.9, 1.1, 10000),
+                        [1e-30, 0.5, 2.0, 3e10]]).astype(np.float32)
+    got = np.array([R.log(v) for v in x], dtype=np.float32)
+    true = np.log(x.astype(np.float64))
+    nz = true != 0
+    assert (np.abs(got - true)[nz] / np.spacing(np.abs(true).astype(np.float32))[nz]).max() <= 3
+    assert R.log(1.0) == 0.0 and R.log(0.0) == float("-inf") and math.isnan(R.log(-1.0))
+    x = rng.uniform(0, 2, 20000).astype(np.float32)
+    got = np.array([R.pow5(v) for v in x], dtype=np.float32)
+    true = x.astype(np.float64) ** 5
+    assert (np.abs(got - true) / np.spacing(true.astype(np.float32))).max() <= 3
+    x = np.concatenate([rng.uniform(0.5, 1, 20000), rng.uniform(0.99, 1, 5000), [1.0]]).astype(np.float32)
+    got = np.array([R.pow250(v) for v in x], dtype=np.float32)
+    true = x.astype(np.float64) ** 250
+    big = true >= 2.0 ** -124
+    a = np.abs(250 * np.log2(x.astype(np.float64)))
+    allowed = (true * np.expm1(math.log(2) * 250 * 2.0 ** -21)
+               + (3 + 2 * a) * np.spacing(true.astype(np.float32)))
+    err = np.abs(got - true)
+    assert np.all(err[big] <= allowed[big])
+    # near x = 1 the bound is tightest (a chain of squarings exceeds it there)
+    near1 = (x > 0.999) & big
+    assert np.all(err[near1] <= allowed[near1])
+    assert np.all(got[~big] <= 2.0 ** -123)   # flushed to 0 or as tiny
+    assert R.pow250(1.0) == 1.0 and R.pow250(0.0) == 0.0
+
+
 # ---- camera (camera.ts, wgpu-matrix 3.3.0) -----------------------------------
 
 def test_perspective_maps_near_and_far_to_0_and_1():

@@ -92,6 +92,59 @@ __device__ __forceinline__ int tex_index(float fx, float fy, int w, int h) {
     return iy * w + ix;
 }
 
+// exp and pow: WGSL bounds their error (exp: 3 + 2|x| ulp; pow as
+// exp2(y log2 x)) without fixing their values; here one deterministic formula
+// each, the same as the oracle's (oracle_render_exp / _pow5 / _pow250).
+// exp(x): x = k ln2 + r, k = rint(x log2 e), ln2 split into 0.693359375 +
+// (-2.12194440e-4); exp(r), |r| <= 0.35, by its degree-7 Taylor polynomial in
+// Horner form (one rounding per operation); times 2^k.  0 below -86 (results
+// stay normal), inf above 88, NaN through; within 1.2 ulp of exp elsewhere.
+__device__ __forceinline__ float det_exp(float x) {
+    if (x != x) return x;
+    if (!(x > -86.0f)) return 0.0f;
+    if (x > 88.0f) return __builtin_inff();
+    const float k = rintf(x * 1.44269502f);
+    const float r = (x - k * 0.693359375f) - k * -2.12194440e-4f;
+    float p = 1.98412698e-4f;  // 1/7!
+    p = p * r + 1.38888889e-3f;
+    p = p * r + 8.33333333e-3f;
+    p = p * r + 4.16666667e-2f;
+    p = p * r + 1.66666667e-1f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, (int)k);
+}
+
+// log(x) for x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
+// s = (m - 1) / (m + 1), by its series to s^9; e ln2 with ln2 split as for
+// exp.  -inf at 0, NaN below.
+__device__ __forceinline__ float det_log(float x) {
+    if (x != x || x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_inff();
+    if (x == __builtin_inff()) return x;
+    int e;
+    float m = frexpf(x, &e);  // x = m 2^e, m in [0.5, 1)
+    if (m < 0.70710678f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f), s2 = s * s;
+    float p = 1.11111111e-1f;  // 1/9
+    p = p * s2 + 1.42857143e-1f;
+    p = p * s2 + 2.00000000e-1f;
+    p = p * s2 + 3.33333333e-1f;
+    p = p * s2 + 1.0f;
+    const float fe = (float)e;
+    return fe * 0.693359375f + (fe * -2.12194440e-4f + (2.0f * s) * p);
+}
+
+// pow(x, 5) and pow(x, 250) for x >= 0 (fluid.wgsl:70,80): ((x x)(x x)) x and
+// exp(250 log x), within WGSL's bound for pow (inherited from exp2(y log2 x)).
+__device__ __forceinline__ float det_pow5(float x) { return ((x * x) * (x * x)) * x; }
+
+__device__ __forceinline__ float det_pow250(float x) { return det_exp(250.0f * det_log(x)); }
+
 // f32 -> r16float -> f32 (round to nearest even)
 __device__ __forceinline__ float f16_round(float f) { return (float)(_Float16)f; }
 
@@ -467,7 +520,7 @@ static_assert(kPixW * kPixH == kThreads, "per-pixel blocks");
 
 // The filters' data-independent weights, evaluated once with the same
 // expressions (and so the same bits) as inline: the bilateral's spatial weight
-// expf(-rr / two_sigma) per filter size (1..100) and tap distance, the
+// exp(-rr / two_sigma) per filter size (1..100) and tap distance, the
 // gaussian's 61 weights (sigma 10) and their sum in tap order.
 constexpr int kMaxFilter = 100;
 constexpr int kBilateralTable = (kMaxFilter + 1) * (kMaxFilter + 1);
@@ -482,13 +535,13 @@ __global__ __launch_bounds__(kThreads) void k_filter_weights(float *__restrict__
         const float two_sigma = 2.0f * sigma * sigma;
         const float co = (float)k;
         const float rr = co * co + co * co;
-        bw[q] = fs ? expf(-rr / two_sigma) : 0.0f;
+        bw[q] = fs ? det_exp(-rr / two_sigma) : 0.0f;
     }
     const float sigma = (float)kGaussHalf / 3.0f;
     const float two_sigma = 2.0f * sigma * sigma;
     for (int k = (int)threadIdx.x; k <= 2 * kGaussHalf; k += kThreads) {
         const float co = (float)(k - kGaussHalf);
-        gw[k] = expf(-co * co / two_sigma);
+        gw[k] = det_exp(-co * co / two_sigma);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -533,7 +586,7 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
         const float wt = wrow[k < 0 ? -k : k];
         const float r_depth = sampled - depth;
         const float wd =
-            expf(div_rn(-r_depth * r_depth, two_sigma_depth, inv_two_sigma_depth));
+            det_exp(div_rn(-r_depth * r_depth, two_sigma_depth, inv_two_sigma_depth));
         sum += sampled * wt * wd;
         wsum += wt * wd;
     }
@@ -679,16 +732,16 @@ __global__ __launch_bounds__(kThreads) void k_fluid(const float *__restrict__ de
     float H[3];
     normalize3(hv, H);
     const float hn = dot3(H, normal);
-    const float specular = powf(hn > 0.0f ? hn : 0.0f, 250.0f);
+    const float specular = det_pow250(hn > 0.0f ? hn : 0.0f);
     const float density = 1.5f;
     const float thickness = thick[tex_index(iu, iv, w, h)];
     const float dc[3] = {0.085f, 0.6375f, 0.9f};
     float refr[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) refr[i] = bg[i] * expf(-density * thickness * (1.0f - dc[i]));
+    for (int i = 0; i < 3; ++i) refr[i] = bg[i] * det_exp(-density * thickness * (1.0f - dc[i]));
     const float F0 = 0.02f;
     const float nr[3] = {-ray[0], -ray[1], -ray[2]};
-    float fr = F0 + (1.0f - F0) * powf(1.0f - dot3(normal, nr), 5.0f);
+    float fr = F0 + (1.0f - F0) * det_pow5(1.0f - dot3(normal, nr));
     fr = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
     const float dn = dot3(normal, ray);
     const float refl[4] = {ray[0] - 2.0f * dn * normal[0], ray[1] - 2.0f * dn * normal[1],
