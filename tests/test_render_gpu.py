@@ -180,6 +180,41 @@ def test_orthographic_and_off_centre_projections(rmod):
                     assert np.count_nonzero(d) > 0.05 * w * h
 
 
+def test_close_up_filter_sizes(rmod):
+    """Particles close to the camera (filter sizes up to the cap of 100
+    taps) beside far ones (about 10 taps): the bilateral filter's blocks of
+    64 x 4 pixels whose largest filter exceeds the 40-pixel LDS apron gather
+    their taps from global memory, the others read them from LDS; both
+    kinds occur in this frame, and in blocks straddling the image edges."""
+    rng = np.random.default_rng(13)
+    n = 600
+    pv = np.zeros((n, 8), dtype=np.float32)
+    near = np.arange(n) < 12  # a few, in the upper left of the image
+    z = np.where(near, rng.uniform(-3.0, -1.5, n), rng.uniform(-14.0, -8.0, n))
+    t = math.tan(FOV / 2)
+    w, h = 240, 160
+    sx = np.where(near, rng.uniform(-1.1, -0.6, n), rng.uniform(-1.1, 1.1, n))
+    sy = np.where(near, rng.uniform(0.5, 1.1, n), rng.uniform(-1.1, 1.1, n))
+    pv[:, 0] = sx * -z * t * w / h
+    pv[:, 1] = sy * -z * t
+    pv[:, 2] = z
+    pv[:, 4:7] = rng.normal(0, 1.0, (n, 3))
+    cube = rmod.synthetic_cubemap(16)
+    cam = rmod.Camera()
+    cam.reset(w, h, 60, [0.0, 0.0, 0.0], FOV)
+    cam.view = np.eye(4).reshape(16)
+    u = cam.uniforms(w, h, 2 * RADIUS)
+    with rmod.FluidRenderer(w, h, RADIUS, FOV, cube) as r:
+        r.execute(pv, n, u)
+        d = check_frame(r, pv, n, u, cube, what="close-up")
+    ppc = 12 * 2 * RADIUS * 0.05 * (h / 2) / math.tan(FOV / 2)
+    depth = np.abs(d)
+    act = (depth > 0) & (depth < 1e4)
+    fs = np.where(act, np.minimum(100, np.ceil(ppc / np.where(act, depth, 1))), -1)
+    blocks = np.array([fs[i:i + 4, j:j + 64].max() for i in range(0, h, 4) for j in range(0, w, 64)])
+    assert np.any(blocks > 40) and np.any((blocks > 0) & (blocks <= 40))
+
+
 @pytest.mark.parametrize("n", [6000, 20000])
 def test_crowded_tiles(rmod, n):
     """Thousands of impostors over a few screen tiles: tile lists longer than

@@ -551,38 +551,21 @@ __global__ __launch_bounds__(kThreads) void k_filter_weights(float *__restrict__
     }
 }
 
-// bilateral.wgsl:20-58 along `dir`
-__global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict__ src,
-                                                        float *__restrict__ dst, Consts c,
-                                                        const float *__restrict__ bw,
-                                                        float dirx, float diry) {
-    const int w = c.width, h = c.height;
-    const int x = (int)(blockIdx.x * kPixW + threadIdx.x);
-    const int y = (int)(blockIdx.y * kPixH + threadIdx.y);
-    if (x >= w || y >= h) return;
-    const size_t pi = (size_t)y * w + x;
-    const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
-    const float depth = fabsf(src[tex_index(iu, iv, w, h)]);
-    if (depth >= 1e4f || depth <= 0.0f) {
-        dst[pi] = depth;
-        return;
-    }
-    const float q = ceilf(c.projected_particle_constant / depth);
-    const int fsz = q >= 2147483648.0f ? 2147483647 : (int)q;
-    const int mfs = (int)c.max_filter_size;
-    const int filter_size = mfs < fsz ? mfs : fsz;
-    const float sigma_depth = c.depth_threshold / 3.0f;
-    const float two_sigma_depth = 2.0f * sigma_depth * sigma_depth;
-    // every tap divides by two_sigma_depth: the correctly rounded quotient
-    // from its rounded reciprocal (div_rn; the numerators are 0 or squares
-    // of depth differences, >= ~1e-14)
-    const float inv_two_sigma_depth = 1.0f / two_sigma_depth;
-    const float *wrow = bw + filter_size * (kMaxFilter + 1);
+// Largest filter size whose taps are read from LDS: a block stages its rows
+// (along x) or columns (along y) with an apron of its own largest filter
+// size; blocks with a larger one gather from global memory.
+constexpr int kApron = 40;
+
+// One pixel of bilateral.wgsl:20-58: taps k = -fs..fs, sample(k) the
+// neighbour's |depth| (clamped into the image).
+template <typename Sample>
+__device__ __forceinline__ float bilateral_pixel(float depth, int fs, const float *wrow,
+                                                 float two_sigma_depth,
+                                                 float inv_two_sigma_depth, Sample sample) {
     float sum = 0.0f, wsum = 0.0f;
 #pragma unroll 4
-    for (int k = -filter_size; k <= filter_size; ++k) {
-        const float co = (float)k;
-        const float sampled = fabsf(src[tex_index(iu + co * dirx, iv + co * diry, w, h)]);
+    for (int k = -fs; k <= fs; ++k) {
+        const float sampled = sample(k);
         const float wt = wrow[k < 0 ? -k : k];
         const float r_depth = sampled - depth;
         const float wd =
@@ -590,7 +573,97 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
         sum += sampled * wt * wd;
         wsum += wt * wd;
     }
-    dst[pi] = sum / wsum;
+    return sum / wsum;
+}
+
+// bilateral.wgsl:20-58 along x (kAlongX) or y.  Tap k of pixel (x, y) reads
+// |src| at column clamp(x + k) (along x) or row clamp(y + k) (along y), which
+// is textureLoad's clamped coordinate of (x + 0.5 + k, y + 0.5).  A block
+// is one 64-pixel row segment per wave, 4 rows; every wave's lanes run the
+// loop of the lane with the largest filter, so the taps' latency is what
+// bounds the pass: staged through LDS, the block's loads are issued at once.
+template <bool kAlongX>
+__global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict__ src,
+                                                        float *__restrict__ dst, Consts c,
+                                                        const float *__restrict__ bw) {
+    constexpr int kRowS = kPixW + 2 * kApron;  // along x: LDS row stride
+    constexpr int kS = kAlongX ? kPixH * kRowS : (kPixH + 2 * kApron) * kPixW;
+    __shared__ float s[kS];
+    __shared__ int s_wmax[kPixH];
+    const int w = c.width, h = c.height;
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const int x0 = (int)blockIdx.x * kPixW, y0 = (int)blockIdx.y * kPixH;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool inb = x < w && y < h;
+    const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
+    const float depth = inb ? fabsf(src[tex_index(iu, iv, w, h)]) : 0.0f;
+    const bool active = inb && !(depth >= 1e4f || depth <= 0.0f);
+    int fs = 0;
+    if (active) {
+        const float q = ceilf(c.projected_particle_constant / depth);
+        const int fsz = q >= 2147483648.0f ? 2147483647 : (int)q;
+        const int mfs = (int)c.max_filter_size;
+        fs = mfs < fsz ? mfs : fsz;
+    }
+    // the block's largest filter size + 1 (0: no pixel to filter); wave ty
+    // is row ty of the block
+    int m = active ? fs + 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if (tx == 0) s_wmax[ty] = m;
+    __syncthreads();
+    int g = s_wmax[0];
+    for (int i = 1; i < kPixH; ++i) g = s_wmax[i] > g ? s_wmax[i] : g;
+    if (g == 0) {  // (block-uniform) background only
+        if (inb) dst[(size_t)y * w + x] = depth;
+        return;
+    }
+    const int F = g - 1;
+    const float sigma_depth = c.depth_threshold / 3.0f;
+    const float two_sigma_depth = 2.0f * sigma_depth * sigma_depth;
+    // every tap divides by two_sigma_depth: the correctly rounded quotient
+    // from its rounded reciprocal (div_rn; the numerators are 0 or squares
+    // of depth differences, >= ~1e-14)
+    const float inv_two_sigma_depth = 1.0f / two_sigma_depth;
+    const float *wrow = bw + fs * (kMaxFilter + 1);
+    if (F > kApron) {  // (block-uniform) gather from global memory
+        if (!active) {
+            if (inb) dst[(size_t)y * w + x] = depth;
+            return;
+        }
+        dst[(size_t)y * w + x] = bilateral_pixel(
+            depth, fs, wrow, two_sigma_depth, inv_two_sigma_depth, [&](int k) {
+                const float co = (float)k;
+                return kAlongX ? fabsf(src[tex_index(iu + co, iv, w, h)])
+                               : fabsf(src[tex_index(iu, iv + co, w, h)]);
+            });
+        return;
+    }
+    if (kAlongX) {  // row y0 + ty (clamped), columns x0 - F .. x0 + 63 + F (clamped)
+        const float *srow = src + (size_t)(y < h ? y : h - 1) * w;
+        for (int j = tx; j < kPixW + 2 * F; j += kPixW) {
+            const int cx = x0 - F + j;
+            s[ty * kRowS + j] = fabsf(srow[cx < 0 ? 0 : (cx >= w ? w - 1 : cx)]);
+        }
+    } else {  // rows y0 - F .. y0 + 3 + F (clamped), column x (clamped)
+        const int cx = x < w ? x : w - 1;
+        for (int j = ty; j < kPixH + 2 * F; j += kPixH) {
+            const int cy = y0 - F + j;
+            s[j * kPixW + tx] = fabsf(src[(size_t)(cy < 0 ? 0 : (cy >= h ? h - 1 : cy)) * w + cx]);
+        }
+    }
+    __syncthreads();
+    if (!active) {
+        if (inb) dst[(size_t)y * w + x] = depth;
+        return;
+    }
+    const float *centre = kAlongX ? s + ty * kRowS + tx + F : s + (ty + F) * kPixW + tx;
+    constexpr int kStep = kAlongX ? 1 : kPixW;
+    dst[(size_t)y * w + x] =
+        bilateral_pixel(depth, fs, wrow, two_sigma_depth, inv_two_sigma_depth,
+                        [&](int k) { return centre[k * kStep]; });
 }
 
 // gaussian.wgsl:15-42 along `dir` (r16float target)
@@ -901,9 +974,8 @@ int frame(render_ctx *r, const float4 *pv, uint32_t n, const render_uniforms &u,
         // 4 x (bilateral x, bilateral y) (fluidRender.ts:432-444)
         const float *src = r->depth;
         for (int it = 0; it < 4; ++it) {
-            k_bilateral<<<pgrid, pblock, 0, st>>>(src, r->tmp, r->c, r->bilateral_w, 1.0f, 0.0f);
-            k_bilateral<<<pgrid, pblock, 0, st>>>(r->tmp, r->fdepth, r->c, r->bilateral_w, 0.0f,
-                                                 1.0f);
+            k_bilateral<true><<<pgrid, pblock, 0, st>>>(src, r->tmp, r->c, r->bilateral_w);
+            k_bilateral<false><<<pgrid, pblock, 0, st>>>(r->tmp, r->fdepth, r->c, r->bilateral_w);
             src = r->fdepth;
         }
         // 1 x (gaussian x, gaussian y) (fluidRender.ts:450-462)
