@@ -666,30 +666,54 @@ __global__ __launch_bounds__(kThreads) void k_bilateral(const float *__restrict_
                         [&](int k) { return centre[k * kStep]; });
 }
 
-// gaussian.wgsl:15-42 along `dir` (r16float target)
+// gaussian.wgsl:15-42 along x (kAlongX) or y (r16float target).  Tap k of
+// pixel (x, y) reads src at column clamp(x + k) or row clamp(y + k)
+// (textureLoad's clamped coordinate of (x + 0.5 + k, y + 0.5)); a block with
+// any nonzero pixel stages its rows or columns with the 30-pixel apron in
+// LDS, read by the fixed 61-tap loop.
+template <bool kAlongX>
 __global__ __launch_bounds__(kThreads) void k_gaussian(const float *__restrict__ src,
                                                        float *__restrict__ dst, int w, int h,
-                                                       const float *__restrict__ gw,
-                                                       float dirx, float diry) {
-    const int x = (int)(blockIdx.x * kPixW + threadIdx.x);
-    const int y = (int)(blockIdx.y * kPixH + threadIdx.y);
-    if (x >= w || y >= h) return;
-    const size_t pi = (size_t)y * w + x;
-    const float iu = (float)x + 0.5f, iv = (float)y + 0.5f;
-    const float t = src[tex_index(iu, iv, w, h)];
-    if (t == 0.0f) {
-        dst[pi] = 0.0f;
+                                                       const float *__restrict__ gw) {
+    constexpr int kRowS = kPixW + 2 * kGaussHalf;  // along x: LDS row stride
+    constexpr int kS = kAlongX ? kPixH * kRowS : (kPixH + 2 * kGaussHalf) * kPixW;
+    __shared__ float s[kS];
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y;
+    const int x0 = (int)blockIdx.x * kPixW, y0 = (int)blockIdx.y * kPixH;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool inb = x < w && y < h;
+    const float t = inb ? src[tex_index((float)x + 0.5f, (float)y + 0.5f, w, h)] : 0.0f;
+    const bool active = inb && t != 0.0f;
+    if (!__syncthreads_or(active)) {  // (block-uniform) nothing to blur
+        if (inb) dst[(size_t)y * w + x] = 0.0f;
         return;
     }
+    if (kAlongX) {  // row y (clamped), columns x0 - 30 .. x0 + 93 (clamped)
+        const float *srow = src + (size_t)(y < h ? y : h - 1) * w;
+        for (int j = tx; j < kRowS; j += kPixW) {
+            const int cx = x0 - kGaussHalf + j;
+            s[ty * kRowS + j] = srow[cx < 0 ? 0 : (cx >= w ? w - 1 : cx)];
+        }
+    } else {  // rows y0 - 30 .. y0 + 33 (clamped), column x (clamped)
+        const int cx = x < w ? x : w - 1;
+        for (int j = ty; j < kPixH + 2 * kGaussHalf; j += kPixH) {
+            const int cy = y0 - kGaussHalf + j;
+            s[j * kPixW + tx] = src[(size_t)(cy < 0 ? 0 : (cy >= h ? h - 1 : cy)) * w + cx];
+        }
+    }
+    __syncthreads();
+    if (!active) {
+        if (inb) dst[(size_t)y * w + x] = 0.0f;
+        return;
+    }
+    const float *centre =
+        kAlongX ? s + ty * kRowS + tx + kGaussHalf : s + (ty + kGaussHalf) * kPixW + tx;
+    constexpr int kStep = kAlongX ? 1 : kPixW;
     const float wsum = gw[2 * kGaussHalf + 1];
     float sum = 0.0f;
 #pragma unroll
-    for (int k = -kGaussHalf; k <= kGaussHalf; ++k) {
-        const float co = (float)k;
-        const float sampled = src[tex_index(iu + dirx * co, iv + diry * co, w, h)];
-        sum += sampled * gw[k + kGaussHalf];
-    }
-    dst[pi] = f16_round(sum / wsum);
+    for (int k = -kGaussHalf; k <= kGaussHalf; ++k) sum += centre[k * kStep] * gw[k + kGaussHalf];
+    dst[(size_t)y * w + x] = f16_round(sum / wsum);
 }
 
 // textureSampleLevel(envmap, linear sampler, dir, 0): the major-axis face,
@@ -979,8 +1003,8 @@ int frame(render_ctx *r, const float4 *pv, uint32_t n, const render_uniforms &u,
             src = r->fdepth;
         }
         // 1 x (gaussian x, gaussian y) (fluidRender.ts:450-462)
-        k_gaussian<<<pgrid, pblock, 0, st>>>(r->thick16, r->tmp, w, h, r->gauss_w, 1.0f, 0.0f);
-        k_gaussian<<<pgrid, pblock, 0, st>>>(r->tmp, r->fthick, w, h, r->gauss_w, 0.0f, 1.0f);
+        k_gaussian<true><<<pgrid, pblock, 0, st>>>(r->thick16, r->tmp, w, h, r->gauss_w);
+        k_gaussian<false><<<pgrid, pblock, 0, st>>>(r->tmp, r->fthick, w, h, r->gauss_w);
         k_fluid<<<pgrid, pblock, 0, st>>>(r->fdepth, r->fthick, r->cube, r->cube_size, u, w, h,
                                          r->color);
     } else {
