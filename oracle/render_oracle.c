@@ -47,22 +47,22 @@ static float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[
 /* exp(x): x = k ln2 + r with k = rint(x log2 e) and r from ln2 split into
  * 0.693359375 + (-2.12194440e-4) (k ln2_hi exact); exp(r), |r| <= 0.35, by
  * its Taylor polynomial to degree 7 (truncation < 6e-9) in Horner form, one
- * rounding per operation; times 2^k.  0 below -86 (results stay normal),
+ * fused multiply-add (one rounding) per step; times 2^k.  0 below -86 (results stay normal),
  * inf above 88, NaN through. */
 float oracle_render_exp(float x) {
     if (x != x) return x;
     if (!(x > -86.0f)) return 0.0f;
     if (x > 88.0f) return INFINITY;
     const float k = rintf(x * 1.44269502f);
-    const float r = (x - k * 0.693359375f) - k * -2.12194440e-4f;
+    const float r = fmaf(k, 2.12194440e-4f, x - k * 0.693359375f);
     float p = 1.98412698e-4f;           /* 1/7! */
-    p = p * r + 1.38888889e-3f;         /* 1/6! */
-    p = p * r + 8.33333333e-3f;         /* 1/5! */
-    p = p * r + 4.16666667e-2f;         /* 1/4! */
-    p = p * r + 1.66666667e-1f;         /* 1/3! */
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    p = fmaf(p, r, 1.38888889e-3f);         /* 1/6! */
+    p = fmaf(p, r, 8.33333333e-3f);         /* 1/5! */
+    p = fmaf(p, r, 4.16666667e-2f);         /* 1/4! */
+    p = fmaf(p, r, 1.66666667e-1f);         /* 1/3! */
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
     return ldexpf(p, (int)k);
 }
 
@@ -82,12 +82,12 @@ float oracle_render_log(float x) {
     }
     const float s = (m - 1.0f) / (m + 1.0f), s2 = s * s;
     float p = 1.11111111e-1f;          /* 1/9 */
-    p = p * s2 + 1.42857143e-1f;       /* 1/7 */
-    p = p * s2 + 2.00000000e-1f;       /* 1/5 */
-    p = p * s2 + 3.33333333e-1f;       /* 1/3 */
-    p = p * s2 + 1.0f;
+    p = fmaf(p, s2, 1.42857143e-1f);       /* 1/7 */
+    p = fmaf(p, s2, 2.00000000e-1f);       /* 1/5 */
+    p = fmaf(p, s2, 3.33333333e-1f);       /* 1/3 */
+    p = fmaf(p, s2, 1.0f);
     const float fe = (float)e;
-    return fe * 0.693359375f + (fe * -2.12194440e-4f + (2.0f * s) * p);
+    return fe * 0.693359375f + fmaf(2.0f * s, p, fe * -2.12194440e-4f);
 }
 
 /* pow(x, 5) and pow(x, 250) for x >= 0 (fluid.wgsl:70,80): ((x x)(x x)) x,
